@@ -1,0 +1,235 @@
+"""ganit_v2_amd -- restarted-PDHG LP/QP engine for AMD Instinct MI355X (gfx950).
+
+Python view of the two native libraries built in-tree by ``ganit_v2_amd/build.py``:
+
+* ``libganit.so`` (include/ganit_host.h): MPS I/O, scaling, workload generators and the GPU solve, wrapped
+  here by :class:`Problem`;
+* ``libganit_hip.so`` (include/ganit_hip.h): the HIP kernels behind the C ABI, wrapped by
+  :mod:`ganit_v2_amd.hip`.
+
+There is no CPU fallback: if a library is missing, loading raises :class:`NativeLibraryMissing`.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+
+STATUS = ["OPTIMAL", "PRIMAL_INFEASIBLE", "ITERATION_LIMIT", "TIME_LIMIT", "NUMERICAL_ERROR"]
+PRECISION = {"double": 0, "mixed": 1, "dynamic": 2}
+VALUES = {"f32": 0, "bf16": 1}
+
+
+class NativeLibraryMissing(RuntimeError):
+    """A native library of this package has not been built (run ganit_v2_amd/build.py)."""
+
+
+def _load(name: str) -> ctypes.CDLL:
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise NativeLibraryMissing(f"{path} is missing: build it with `python ganit_v2_amd/build.py`")
+    return ctypes.CDLL(path, mode=ctypes.RTLD_GLOBAL)
+
+
+_host = None
+
+_i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_vp = ctypes.c_void_p
+
+
+def host_lib() -> ctypes.CDLL:
+    """libganit.so with its prototypes declared."""
+    global _host
+    if _host is not None:
+        return _host
+    lib = _load("libganit.so")
+    lib.ganit_last_error.restype = ctypes.c_char_p
+    lib.ganit_lp_read_mps.restype = _vp
+    lib.ganit_lp_read_mps.argtypes = [ctypes.c_char_p]
+    lib.ganit_lp_from_triplets.restype = _vp
+    lib.ganit_lp_from_triplets.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp, _vp, _vp, _vp,
+                                           _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int64, _vp, _vp, _vp]
+    lib.ganit_lp_generate.restype = _vp
+    lib.ganit_lp_generate.argtypes = [ctypes.c_int, _f64p, ctypes.c_uint64]
+    lib.ganit_lp_free.argtypes = [_vp]
+    lib.ganit_lp_dims.argtypes = [_vp, np.ctypeslib.ndpointer(dtype=np.int64)]
+    lib.ganit_lp_export.argtypes = [_vp] + [_vp] * 12
+    lib.ganit_lp_planted.argtypes = [_vp, _vp, _vp]
+    lib.ganit_lp_write_mps.argtypes = [_vp, ctypes.c_char_p]
+    lib.ganit_lp_scaled.argtypes = [_vp, ctypes.c_int, ctypes.c_int] + [_vp] * 19
+    lib.ganit_solve_gpu.argtypes = [_vp, _f64p, _f64p, _vp, _vp]
+    _host = lib
+    return lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check_host(rc: int, what: str) -> None:
+    if rc != 0:
+        raise RuntimeError(f"{what}: {host_lib().ganit_last_error().decode()}")
+
+
+@dataclass
+class CSR:
+    rows: int
+    cols: int
+    ptr: np.ndarray
+    idx: np.ndarray
+    val: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.ptr[-1]) if len(self.ptr) else 0
+
+    def to_dense(self) -> np.ndarray:
+        d = np.zeros((self.rows, self.cols))
+        for i in range(self.rows):
+            d[i, self.idx[self.ptr[i]:self.ptr[i + 1]]] += self.val[self.ptr[i]:self.ptr[i + 1]]
+        return d
+
+    def matvec(self, x: np.ndarray) -> np.ndarray:
+        rows = np.repeat(np.arange(self.rows), np.diff(self.ptr))
+        return np.bincount(rows, weights=self.val * x[self.idx], minlength=self.rows)
+
+
+class Problem:
+    """An LP/QP held by libganit.so (original space)."""
+
+    def __init__(self, handle: int):
+        if not handle:
+            raise RuntimeError(host_lib().ganit_last_error().decode())
+        self._h = ctypes.c_void_p(handle)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _host is not None:
+            _host.ganit_lp_free(h)
+            self._h = None
+
+    # ---- construction
+    @classmethod
+    def read_mps(cls, path: str) -> "Problem":
+        return cls(host_lib().ganit_lp_read_mps(os.fsencode(path)))
+
+    @classmethod
+    def generate(cls, kind: str, seed: int = 0, **p) -> "Problem":
+        kinds = {
+            "random_lp": (0, ["m", "n", "m_eq", "nnz_min", "nnz_max", "tighten_frac"],
+                          dict(m=2000, n=5000, m_eq=1000, nnz_min=10, nnz_max=10, tighten_frac=0.0)),
+            "blending": (1, ["crudes", "products", "specs", "periods"],
+                         dict(crudes=60, products=16, specs=5, periods=365)),
+            "portfolio": (2, ["assets", "factors", "gamma"], dict(assets=100000, factors=50, gamma=1.0)),
+            "diff_qp": (3, ["n", "m_eq", "m_le", "density", "qdensity"],
+                        dict(n=2000, m_eq=300, m_le=200, density=0.01, qdensity=0.001)),
+        }
+        k, names, defaults = kinds[kind]
+        vals = dict(defaults)
+        for key, v in p.items():
+            if key not in vals:
+                raise TypeError(f"unknown parameter {key!r} for {kind}")
+            vals[key] = v
+        arr = np.array([float(vals[n]) for n in names])
+        return cls(host_lib().ganit_lp_generate(k, arr, ctypes.c_uint64(seed)))
+
+    @classmethod
+    def from_triplets(cls, m, n, rows, cols, vals, c, l, u, lo, hi, obj_const=0.0, maximize=False,
+                      qrows=None, qcols=None, qvals=None) -> "Problem":
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        cc = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (c, l, u, lo, hi)]
+        if qvals is not None and len(qvals):
+            qr = np.ascontiguousarray(qrows, dtype=np.int32)
+            qc = np.ascontiguousarray(qcols, dtype=np.int32)
+            qv = np.ascontiguousarray(qvals, dtype=np.float64)
+            qn = len(qv)
+        else:
+            qr = qc = qv = None
+            qn = 0
+        h = host_lib().ganit_lp_from_triplets(int(m), int(n), len(v), _ptr(r), _ptr(cc), _ptr(v),
+                                              *[_ptr(a) for a in arrs], float(obj_const), int(bool(maximize)), qn,
+                                              _ptr(qr), _ptr(qc), _ptr(qv))
+        return cls(h)
+
+    # ---- inspection
+    def dims(self) -> dict:
+        d = np.zeros(5, dtype=np.int64)
+        host_lib().ganit_lp_dims(self._h, d)
+        return dict(m=int(d[0]), n=int(d[1]), nnz=int(d[2]), qnnz=int(d[3]), maximize=bool(d[4]))
+
+    def export(self) -> dict:
+        d = self.dims()
+        m, n = d["m"], d["n"]
+        A = CSR(m, n, np.zeros(m + 1, np.int32), np.zeros(d["nnz"], np.int32), np.zeros(d["nnz"]))
+        Q = CSR(n, n, np.zeros(n + 1, np.int32), np.zeros(d["qnnz"], np.int32), np.zeros(d["qnnz"]))
+        vec = {k: np.zeros(n) for k in ("c", "l", "u")}
+        vec.update({k: np.zeros(m) for k in ("lo", "hi")})
+        oc = np.zeros(1)
+        host_lib().ganit_lp_export(self._h, _ptr(A.ptr), _ptr(A.idx), _ptr(A.val), _ptr(Q.ptr), _ptr(Q.idx),
+                                   _ptr(Q.val), _ptr(vec["c"]), _ptr(vec["l"]), _ptr(vec["u"]), _ptr(vec["lo"]),
+                                   _ptr(vec["hi"]), _ptr(oc))
+        return dict(A=A, Q=Q, obj_const=float(oc[0]), maximize=d["maximize"], **vec)
+
+    def planted(self):
+        d = self.dims()
+        x0, y0 = np.zeros(d["n"]), np.zeros(d["m"])
+        _check_host(host_lib().ganit_lp_planted(self._h, _ptr(x0), _ptr(y0)), "planted")
+        return x0, y0
+
+    def write_mps(self, path: str) -> None:
+        _check_host(host_lib().ganit_lp_write_mps(self._h, os.fsencode(path)), "write_mps")
+
+    def scaled(self, ruiz_iters: int = 10, pock_chambolle: bool = True) -> dict:
+        """The scaled problem exactly as the solver uploads it (scale_lp + estimate_norm)."""
+        d = self.dims()
+        m, n = d["m"], d["n"]
+        scal = np.zeros(9)
+        # first call for the nnz of A^T and Q (same as A / Q)
+        A = CSR(m, n, np.zeros(m + 1, np.int32), np.zeros(d["nnz"], np.int32), np.zeros(d["nnz"]))
+        AT = CSR(n, m, np.zeros(n + 1, np.int32), np.zeros(d["nnz"], np.int32), np.zeros(d["nnz"]))
+        Q = CSR(n, n, np.zeros(n + 1, np.int32), np.zeros(d["qnnz"], np.int32), np.zeros(d["qnnz"]))
+        v = {k: np.zeros(n) for k in ("c", "l", "u", "C", "Cres")}
+        v.update({k: np.zeros(m) for k in ("lo", "hi", "R", "Rres")})
+        _check_host(host_lib().ganit_lp_scaled(
+            self._h, int(ruiz_iters), int(bool(pock_chambolle)), _ptr(scal), _ptr(A.ptr), _ptr(A.idx), _ptr(A.val),
+            _ptr(AT.ptr), _ptr(AT.idx), _ptr(AT.val), _ptr(Q.ptr), _ptr(Q.idx), _ptr(Q.val), _ptr(v["c"]),
+            _ptr(v["l"]), _ptr(v["u"]), _ptr(v["lo"]), _ptr(v["hi"]), _ptr(v["R"]), _ptr(v["C"]), _ptr(v["Rres"]),
+            _ptr(v["Cres"])), "scaled")
+        keys = ["bscale", "cscale", "normQ", "obj_const", "bnorm", "cnorm", "normA", "nnz_at", "maximize"]
+        out = dict(zip(keys, scal.tolist()))
+        out.update(A=A, AT=AT, Q=Q, m=m, n=n, **v)
+        return out
+
+    # ---- solve
+    def solve_gpu(self, tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10,
+                  pock_chambolle=True, verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12,
+                  stall_checks_l2=24, values="f32", device=0, return_solution=True) -> dict:
+        d = self.dims()
+        opts = np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
+                         infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device],
+                        dtype=np.float64)
+        out = np.zeros(18)
+        x = np.zeros(d["n"]) if return_solution else None
+        y = np.zeros(d["m"]) if return_solution else None
+        _check_host(host_lib().ganit_solve_gpu(self._h, opts, out, _ptr(x), _ptr(y)), "solve_gpu")
+        keys = ["status", "pobj", "dobj", "rel_p", "rel_d", "rel_gap", "iters", "fp32_iters", "l2_iters", "restarts",
+                "setup_s", "solve_s", "norm_a", "upload_s", "l1_s", "l2_s", "l3_s", "checks"]
+        r = dict(zip(keys, out.tolist()))
+        r["status"] = STATUS[int(out[0])]
+        for k in ("iters", "fp32_iters", "l2_iters", "restarts", "checks"):
+            r[k] = int(r[k])
+        if return_solution:
+            r["x"], r["y"] = x, y
+        return r
+
+
+__all__ = ["Problem", "CSR", "host_lib", "NativeLibraryMissing", "STATUS", "LIB_DIR", "BIN_DIR"]

@@ -1,0 +1,147 @@
+// Device-side building blocks shared by the PDHG kernels (gfx950, wave64).
+//
+//   * value types: FP64 / FP32 / BF16 matrix values, FP64 / FP32 vectors;
+//   * row groups: LPR lanes of one wavefront cooperate on one CSR row (LPR in {1..64});
+//   * deterministic reductions: per-block tree sums in a fixed order, partials in HBM, and the last
+//     block to arrive (agent-scope ticket) folds the partials in block order -- no FP atomics, so a
+//     reduction returns the same bits on every run;
+//   * XCD-aware block numbering: the 8 XCDs of an MI355X receive workgroups round-robin, so consecutive
+//     hardware block ids are remapped to give each XCD one contiguous slab of rows, keeping the slab's
+//     gathers of x inside that XCD's own L2.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ganit_hip {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;
+constexpr int kNumXcd = 8;
+
+// ---------------------------------------------------------------- BF16 storage
+struct bf16_t {
+    uint16_t bits;
+};
+
+__device__ __forceinline__ float bf16_to_float(bf16_t v) { return __uint_as_float(((uint32_t)v.bits) << 16); }
+
+// Widen a stored matrix value to the vector type V (BF16 goes through float, exactly as on the host).
+template <class V>
+__device__ __forceinline__ V widen(double v) { return (V)v; }
+template <class V>
+__device__ __forceinline__ V widen(float v) { return (V)v; }
+template <class V>
+__device__ __forceinline__ V widen(bf16_t v) { return (V)bf16_to_float(v); }
+
+// ---------------------------------------------------------------- CSR view
+template <class M>
+struct CsrView {
+    int rows;
+    const int* __restrict__ ptr;
+    const int* __restrict__ idx;
+    const M* __restrict__ val;
+};
+
+// ---------------------------------------------------------------- block numbering
+// Hardware dispatches block b to XCD b % 8.  Return a logical block id such that the blocks living on one
+// XCD cover a contiguous range of logical ids (a bijection on [0, gridDim.x)).
+__device__ __forceinline__ int xcd_block_id() {
+    const int G = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = G / kNumXcd, r = G % kNumXcd;
+    const int xcd = b % kNumXcd, k = b / kNumXcd;
+    return xcd * q + (xcd < r ? xcd : r) + k;
+}
+
+// ---------------------------------------------------------------- wave / group sums
+// Sum over the LPR lanes of a row group; the group's lane 0 holds the total.  Fixed tree order.
+template <int LPR, class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+    for (int off = LPR / 2; off > 0; off >>= 1) v += __shfl_down(v, off, LPR);
+    return v;
+}
+
+// Dot product of CSR row `row` with x, computed by the LPR lanes of a group, accumulated in V.
+template <int LPR, class M, class V>
+__device__ __forceinline__ V row_dot(const CsrView<M>& a, int row, int lane, const V* __restrict__ x) {
+    const int beg = a.ptr[row], end = a.ptr[row + 1];
+    V t = V(0);
+    for (int p = beg + lane; p < end; p += LPR) t += widen<V>(a.val[p]) * x[a.idx[p]];
+    return group_sum<LPR>(t);
+}
+
+// ---------------------------------------------------------------- deterministic grid reduction
+// Scratch for one reduction call site: `partials` holds gridDim.x * N doubles, `ticket` one counter that
+// is zero between launches (the last block resets it, so no memset is ever needed -- graph-safe).
+struct ReduceSlot {
+    double* partials;
+    unsigned int* ticket;
+    double* out;  // N doubles
+};
+
+__device__ __forceinline__ void store_coherent(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_coherent(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Block-wide sum of N values per thread; the result is valid in thread 0.  `lds` must hold
+// (blockDim.x / 64) * N doubles.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double* lds) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, kWave);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) lds[wave * N + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += lds[w * N + k];
+            v[k] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Grid-wide sum: every block calls this with its per-thread values; the last block writes slot.out.
+template <int N>
+__device__ void grid_sum(double (&v)[N], const ReduceSlot& slot, double* lds) {
+    block_sum<N>(v, lds);
+    __shared__ bool last;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) store_coherent(&slot.partials[(size_t)blockIdx.x * N + k], v[k]);
+        __threadfence();
+        const unsigned t = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // Fixed assignment of partials to threads, then the fixed block tree: deterministic.
+    double s[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) s[k] = 0.0;
+    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) s[k] += load_coherent(&slot.partials[(size_t)b * N + k]);
+    }
+    block_sum<N>(s, lds);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) slot.out[k] = s[k];
+        __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+}  // namespace ganit_hip

@@ -1,0 +1,652 @@
+// libganit_hip.so, part 2: the device-resident PDHG problem and its fused kernels.
+//
+// Per iteration (src/pdhg_impl.hpp:133-139) two kernels run instead of the reference's four or five:
+//   k_iter_primal : for each column j -- (A^T y)_j [+ (Q x)_j] by one row group of A^T (and Q), then the
+//                   primal prox, extrapolation and running sum (src/backend_cpu.cpp:72-79) in the same
+//                   thread: A^T y and Q x never touch HBM.
+//   k_iter_dual   : for each row i -- (A xbar)_i by one row group of A, then the dual prox and running sum
+//                   (src/backend_cpu.cpp:85-91).
+// A check (src/pdhg_impl.hpp:144-147 + :110-118 + :179) is one averaging kernel, one pass over A^T/Q and one
+// pass over A that evaluate the current point, the average and the infeasibility ray together, and reduce
+// all 19 sums deterministically; the host then reads them with a single copy.
+#include <algorithm>
+#include <cstdlib>
+#include <map>
+#include <tuple>
+
+#include "internal.hpp"
+#include "pdhg_math.hpp"
+
+namespace ganit_hip {
+
+constexpr int kCheckGridMax = 2048;
+constexpr int kNOutN = 12, kNOutM = 7;
+
+// ================================================================ iteration kernels
+template <int LPR, class M, class V, bool HASQ>
+__global__ __launch_bounds__(kBlock) void k_iter_primal(CsrView<M> AT, CsrView<M> Q, const V* __restrict__ y,
+                                                        const V* __restrict__ x, const V* __restrict__ c,
+                                                        const V* __restrict__ l, const V* __restrict__ u,
+                                                        const double* __restrict__ prm, V* __restrict__ xn,
+                                                        V* __restrict__ xbar, V* __restrict__ sumx) {
+    const int64_t gt = (int64_t)xcd_block_id() * kBlock + threadIdx.x;
+    const int j = (int)(gt / LPR), lane = (int)(gt % LPR);
+    if (j >= AT.rows) return;
+    const V aty = row_dot<LPR>(AT, j, lane, y);
+    const V qx = HASQ ? row_dot<LPR>(Q, j, lane, x) : V(0);
+    if (lane == 0) {
+        V s = sumx[j];
+        primal_update<V>(x[j], aty, qx, c[j], l[j], u[j], V(prm[0]), xn[j], xbar[j], s);
+        sumx[j] = s;
+    }
+}
+
+template <int LPR, class M, class V>
+__global__ __launch_bounds__(kBlock) void k_iter_dual(CsrView<M> A, const V* __restrict__ xbar,
+                                                      const V* __restrict__ y, const V* __restrict__ lo,
+                                                      const V* __restrict__ hi, const double* __restrict__ prm,
+                                                      V* __restrict__ yn, V* __restrict__ sumy) {
+    const int64_t gt = (int64_t)xcd_block_id() * kBlock + threadIdx.x;
+    const int i = (int)(gt / LPR), lane = (int)(gt % LPR);
+    if (i >= A.rows) return;
+    const V axb = row_dot<LPR>(A, i, lane, xbar);
+    if (lane == 0) {
+        V s = sumy[i];
+        dual_update<V>(y[i], axb, lo[i], hi[i], V(prm[1]), yn[i], s);
+        sumy[i] = s;
+    }
+}
+
+__global__ void k_set_steps(double* prm, double tau, double sigma) {
+    if (threadIdx.x == 0) {
+        prm[0] = tau;
+        prm[1] = sigma;
+    }
+}
+
+// ================================================================ check kernels
+// (xa, ya) = V(inv_k * double(sum)) over [0, n) then [0, m)  (src/backend_cuda.cu:135-139)
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_average(int n, int m, double inv_k, const V* __restrict__ sumx,
+                                                    const V* __restrict__ sumy, V* __restrict__ xa,
+                                                    V* __restrict__ ya) {
+    const int64_t total = (int64_t)n + m;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        if (t < n) xa[t] = V(inv_k * double(sumx[t]));
+        else ya[t - n] = V(inv_k * double(sumy[t - n]));
+    }
+}
+
+// Reduced-cost terms of one column (src/backend_cpu.cpp:120-127): z = cw c - aty + qx; a finite bound on
+// z's side contributes z * bound to the dual objective, an infinite one contributes z to the residual.
+__device__ __forceinline__ void dual_terms(double z, double lj, double uj, double cres, double& r2, double& ob) {
+    double r = 0.0, o = 0.0;
+    if (z > 0) { if (isfinite(lj)) o = z * lj; else r = z; }
+    else if (z < 0) { if (isfinite(uj)) o = z * uj; else r = z; }
+    r /= cres;
+    r2 = r * r;
+    ob = o;
+}
+
+// Row-bound dual objective term (src/backend_cpu.cpp:137-139).
+__device__ __forceinline__ double row_obj_term(double yi, double lw, double hg) {
+    if (yi > 0 && isfinite(lw)) return yi * lw;
+    if (yi < 0 && isfinite(hg)) return yi * hg;
+    return 0.0;
+}
+
+// One pass over A^T (and Q): out[0..11] as documented for ganit_hip_pdhg_check.
+template <int LPR, class M, class V, bool HASQ>
+__global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q, const V* __restrict__ x,
+                                                    const V* __restrict__ y, const V* __restrict__ xa,
+                                                    const V* __restrict__ ya, const V* __restrict__ xr,
+                                                    const V* __restrict__ yr, const V* __restrict__ c,
+                                                    const V* __restrict__ l, const V* __restrict__ u,
+                                                    const V* __restrict__ cres, int flags, ReduceSlot slot) {
+    __shared__ double lds[(kBlock / kWave) * kNOutN];
+    double acc[kNOutN];
+#pragma unroll
+    for (int k = 0; k < kNOutN; ++k) acc[k] = 0.0;
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
+    constexpr int kGroups = kBlock / LPR;
+    const int lane = threadIdx.x % LPR;
+    for (int64_t g = (int64_t)xcd_block_id() * kGroups + threadIdx.x / LPR; g < AT.rows;
+         g += (int64_t)gridDim.x * kGroups) {
+        const int j = (int)g;
+        V sc = V(0), sa = V(0), sr = V(0), qc = V(0), qa = V(0);
+        for (int p = AT.ptr[j] + lane; p < AT.ptr[j + 1]; p += LPR) {
+            const V a = widen<V>(AT.val[p]);
+            const int i = AT.idx[p];
+            sc += a * y[i];
+            if (avg) sa += a * ya[i];
+            if (ray) sr += a * V(y[i] - yr[i]);
+        }
+        if (HASQ) {
+            for (int p = Q.ptr[j] + lane; p < Q.ptr[j + 1]; p += LPR) {
+                const V a = widen<V>(Q.val[p]);
+                const int i = Q.idx[p];
+                qc += a * x[i];
+                if (avg) qa += a * xa[i];
+            }
+        }
+        sc = group_sum<LPR>(sc);
+        if (avg) sa = group_sum<LPR>(sa);
+        if (ray) sr = group_sum<LPR>(sr);
+        if (HASQ) {
+            qc = group_sum<LPR>(qc);
+            if (avg) qa = group_sum<LPR>(qa);
+        }
+        if (lane == 0) {
+            const double cj = double(c[j]), lj = double(l[j]), uj = double(u[j]), cr = double(cres[j]);
+            const double xj = double(x[j]);
+            double r2, ob;
+            dual_terms(cj - double(sc) + (HASQ ? double(qc) : 0.0), lj, uj, cr, r2, ob);
+            acc[0] += r2;
+            acc[1] += ob;
+            acc[2] += cj * xj;
+            if (HASQ) acc[3] += xj * double(qc);
+            if (avg) {
+                const double xaj = double(xa[j]), xrj = double(xr[j]);
+                dual_terms(cj - double(sa) + (HASQ ? double(qa) : 0.0), lj, uj, cr, r2, ob);
+                acc[4] += r2;
+                acc[5] += ob;
+                acc[6] += cj * xaj;
+                if (HASQ) acc[7] += xaj * double(qa);
+                const double d0 = xj - xrj, d1 = xaj - xrj;
+                acc[10] += d0 * d0;
+                acc[11] += d1 * d1;
+            }
+            if (ray) {  // homogeneous dual: cw = 0, no Qx (src/pdhg_impl.hpp:114)
+                dual_terms(0.0 * cj - double(sr) + 0.0, lj, uj, cr, r2, ob);
+                acc[8] += r2;
+                acc[9] += ob;
+            }
+        }
+    }
+    grid_sum<kNOutN>(acc, slot, lds);
+}
+
+// One pass over A: out[12..18] as documented for ganit_hip_pdhg_check.
+template <int LPR, class M, class V>
+__global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __restrict__ x, const V* __restrict__ xa,
+                                                    const V* __restrict__ y, const V* __restrict__ ya,
+                                                    const V* __restrict__ yr, const V* __restrict__ lo,
+                                                    const V* __restrict__ hi, const V* __restrict__ rres, int flags,
+                                                    ReduceSlot slot) {
+    __shared__ double lds[(kBlock / kWave) * kNOutM];
+    double acc[kNOutM];
+#pragma unroll
+    for (int k = 0; k < kNOutM; ++k) acc[k] = 0.0;
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
+    constexpr int kGroups = kBlock / LPR;
+    const int lane = threadIdx.x % LPR;
+    for (int64_t g = (int64_t)xcd_block_id() * kGroups + threadIdx.x / LPR; g < A.rows;
+         g += (int64_t)gridDim.x * kGroups) {
+        const int i = (int)g;
+        V sc = V(0), sa = V(0);
+        for (int p = A.ptr[i] + lane; p < A.ptr[i + 1]; p += LPR) {
+            const V a = widen<V>(A.val[p]);
+            const int j = A.idx[p];
+            sc += a * x[j];
+            if (avg) sa += a * xa[j];
+        }
+        sc = group_sum<LPR>(sc);
+        if (avg) sa = group_sum<LPR>(sa);
+        if (lane == 0) {
+            const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
+            auto pres = [&](double a) {  // src/backend_cpu.cpp:106-109
+                const double p = fmin(fmax(a, lw), hg);
+                const double r = (a - p) / rr;
+                return r * r;
+            };
+            acc[0] += pres(double(sc));
+            acc[1] += row_obj_term(yi, lw, hg);
+            if (avg) {
+                const double yai = double(ya[i]), yri = double(yr[i]);
+                acc[2] += pres(double(sa));
+                acc[3] += row_obj_term(yai, lw, hg);
+                const double d0 = yi - yri, d1 = yai - yri;
+                acc[5] += d0 * d0;
+                acc[6] += d1 * d1;
+            }
+            if (ray) acc[4] += row_obj_term(double(V(y[i] - yr[i])), lw, hg);
+        }
+    }
+    grid_sum<kNOutM>(acc, slot, lds);
+}
+
+// ================================================================ restart / hand-over kernels
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_restart(int n, int m, int adopt, V* __restrict__ x,
+                                                    const V* __restrict__ xa, V* __restrict__ xr,
+                                                    V* __restrict__ sumx, V* __restrict__ y, const V* __restrict__ ya,
+                                                    V* __restrict__ yr, V* __restrict__ sumy, int mode) {
+    // mode 0: restart (adopt?, then restart point = iterate, clear sums); mode 1: adopt only
+    const int64_t total = (int64_t)n + m;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        if (t < n) {
+            const V v = adopt ? xa[t] : x[t];
+            x[t] = v;
+            if (mode == 0) {
+                xr[t] = v;
+                sumx[t] = V(0);
+            }
+        } else {
+            const int64_t i = t - n;
+            const V v = adopt ? ya[i] : y[i];
+            y[i] = v;
+            if (mode == 0) {
+                yr[i] = v;
+                sumy[i] = V(0);
+            }
+        }
+    }
+}
+
+template <class S, class D>
+__global__ __launch_bounds__(kBlock) void k_handover(int64_t n, const S* __restrict__ s, D* __restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        d[i] = D(s[i]);
+}
+
+// ================================================================ host object
+struct VecSet {
+    bool on = false;
+    int vdt = 0;
+    void *c = nullptr, *l = nullptr, *u = nullptr, *lo = nullptr, *hi = nullptr, *rres = nullptr, *cres = nullptr;
+    void *X[2] = {nullptr, nullptr}, *Y[2] = {nullptr, nullptr};
+    void *xbar = nullptr, *sumx = nullptr, *sumy = nullptr, *xa = nullptr, *ya = nullptr, *xr = nullptr,
+         *yr = nullptr;
+    int cur = 0;
+};
+
+struct Pdhg {
+    Ctx* ctx = nullptr;
+    int m = 0, n = 0;
+    bool hasq = false;
+    Csr *A = nullptr, *AT = nullptr, *Q = nullptr;
+    VecSet vs[2];
+    double* prm = nullptr;
+    double* part_n = nullptr;
+    double* part_m = nullptr;
+    unsigned* tickets = nullptr;  // [2]
+    double* out = nullptr;        // [GANIT_CHECK_NOUT]
+    std::vector<void*> owned;
+    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
+    bool use_graphs = true;
+
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        GH_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+        owned.push_back(p);
+        return p;
+    }
+    VecSet& set(int vdt) {
+        GH_REQUIRE(vdt == GANIT_F64 || vdt == GANIT_F32, "vector dtype must be GANIT_F64 or GANIT_F32");
+        GH_REQUIRE(vs[vdt].on, "vector dtype not resident in this problem (vdt_mask)");
+        return vs[vdt];
+    }
+    ~Pdhg() {
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        for (void* p : owned) (void)hipFree(p);
+        csr_destroy(A);
+        csr_destroy(AT);
+        csr_destroy(Q);
+    }
+};
+
+void upload_as(Ctx* ctx, int vdt, const double* h, int64_t n, void* d) {
+    if (n == 0) return;
+    if (vdt == GANIT_F64) {
+        GH_CHECK(hipMemcpyAsync(d, h, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+        GH_CHECK(hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> t(h, h + n);
+        GH_CHECK(hipMemcpyAsync(d, t.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+        GH_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+void download_as(Ctx* ctx, int vdt, const void* d, int64_t n, double* h) {
+    if (n == 0 || !h) return;
+    if (vdt == GANIT_F64) {
+        GH_CHECK(hipMemcpyAsync(h, d, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+        GH_CHECK(hipStreamSynchronize(ctx->stream));
+    } else {
+        std::vector<float> t(n);
+        GH_CHECK(hipMemcpyAsync(t.data(), d, sizeof(float) * n, hipMemcpyDeviceToHost, ctx->stream));
+        GH_CHECK(hipStreamSynchronize(ctx->stream));
+        for (int64_t i = 0; i < n; ++i) h[i] = t[i];
+    }
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_zero2(int n, int m, V* __restrict__ a, V* __restrict__ b) {
+    const int64_t total = (int64_t)n + m;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        if (t < n) a[t] = V(0);
+        else b[t - n] = V(0);
+    }
+}
+
+unsigned ew_grid2(int64_t n) { return std::min<unsigned>(grid_for(n), 8192u); }
+
+template <class V>
+void clear_sums(Pdhg* pd, VecSet& s) {
+    hipLaunchKernelGGL(k_zero2<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, pd->ctx->stream, pd->n,
+                       pd->m, (V*)s.sumx, (V*)s.sumy);
+    GH_CHECK(hipGetLastError());
+}
+
+template <class M, class V>
+void launch_iteration(Pdhg* pd, VecSet& s) {
+    hipStream_t st = pd->ctx->stream;
+    const V* x = (const V*)s.X[s.cur];
+    V* xn = (V*)s.X[1 - s.cur];
+    const V* y = (const V*)s.Y[s.cur];
+    V* yn = (V*)s.Y[1 - s.cur];
+    const CsrView<M> at = pd->AT->view<M>();
+    const CsrView<M> a = pd->A->view<M>();
+    if (pd->n > 0) {
+        with_lanes(pd->AT->lanes, [&](auto L) {
+            constexpr int LPR = decltype(L)::value;
+            const unsigned g = grid_for((int64_t)pd->n * LPR);
+            if (pd->hasq)
+                hipLaunchKernelGGL((k_iter_primal<LPR, M, V, true>), dim3(g), dim3(kBlock), 0, st, at,
+                                   pd->Q->view<M>(), y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn,
+                                   (V*)s.xbar, (V*)s.sumx);
+            else
+                hipLaunchKernelGGL((k_iter_primal<LPR, M, V, false>), dim3(g), dim3(kBlock), 0, st, at, at, y, x,
+                                   (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar, (V*)s.sumx);
+        });
+    }
+    if (pd->m > 0) {
+        with_lanes(pd->A->lanes, [&](auto L) {
+            constexpr int LPR = decltype(L)::value;
+            hipLaunchKernelGGL((k_iter_dual<LPR, M, V>), dim3(grid_for((int64_t)pd->m * LPR)), dim3(kBlock), 0, st,
+                               a, (const V*)s.xbar, y, (const V*)s.lo, (const V*)s.hi, pd->prm, yn, (V*)s.sumy);
+        });
+    }
+    s.cur ^= 1;
+}
+
+template <class M, class V>
+void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
+    hipStream_t st = pd->ctx->stream;
+    const V* x = (const V*)s.X[s.cur];
+    const V* y = (const V*)s.Y[s.cur];
+    if (flags & GANIT_CHECK_AVG) {
+        hipLaunchKernelGGL(k_average<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st, pd->n, pd->m,
+                           inv_k, (const V*)s.sumx, (const V*)s.sumy, (V*)s.xa, (V*)s.ya);
+    }
+    ReduceSlot sn{pd->part_n, pd->tickets + 0, pd->out};
+    ReduceSlot sm{pd->part_m, pd->tickets + 1, pd->out + kNOutN};
+    const CsrView<M> at = pd->AT->view<M>();
+    const CsrView<M> a = pd->A->view<M>();
+    with_lanes(pd->AT->lanes, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        const int64_t groups = std::max<int64_t>(pd->n, 1);
+        const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
+        if (pd->hasq)
+            hipLaunchKernelGGL((k_check_n<LPR, M, V, true>), dim3(g), dim3(kBlock), 0, st, at, pd->Q->view<M>(), x, y,
+                               (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
+                               (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
+        else
+            hipLaunchKernelGGL((k_check_n<LPR, M, V, false>), dim3(g), dim3(kBlock), 0, st, at, at, x, y,
+                               (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
+                               (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
+    });
+    with_lanes(pd->A->lanes, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        const int64_t groups = std::max<int64_t>(pd->m, 1);
+        const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
+        hipLaunchKernelGGL((k_check_m<LPR, M, V>), dim3(g), dim3(kBlock), 0, st, a, x, (const V*)s.xa, y,
+                           (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags,
+                           sm);
+    });
+    GH_CHECK(hipGetLastError());
+    GH_CHECK(hipMemcpyAsync(pd->ctx->host_out, pd->out, sizeof(double) * GANIT_CHECK_NOUT, hipMemcpyDeviceToHost, st));
+    GH_CHECK(hipStreamSynchronize(st));
+    std::copy(pd->ctx->host_out, pd->ctx->host_out + GANIT_CHECK_NOUT, host_out);
+}
+
+}  // namespace ganit_hip
+
+using namespace ganit_hip;
+
+extern "C" {
+
+int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigned mdt_mask, unsigned vdt_mask,
+                          ganit_hip_pdhg** out) {
+    return guarded([&] {
+        auto* ctx = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(ctx && lp && out, "null argument");
+        GH_REQUIRE(lp->m >= 0 && lp->n >= 0, "negative problem dimension");
+        GH_REQUIRE((mdt_mask & 7u) && (vdt_mask & 3u), "empty dtype mask");
+        GH_REQUIRE(lp->c && lp->l && lp->u && lp->cres && (lp->m == 0 || (lp->lo && lp->hi && lp->rres)),
+                   "null problem vector");
+        auto* pd = new Pdhg;
+        try {
+            pd->ctx = ctx;
+            pd->m = lp->m;
+            pd->n = lp->n;
+            pd->A = csr_create(ctx, lp->m, lp->n, lp->a_ptr, lp->a_idx, lp->a_val, mdt_mask);
+            pd->AT = csr_create(ctx, lp->n, lp->m, lp->at_ptr, lp->at_idx, lp->at_val, mdt_mask);
+            GH_REQUIRE(pd->A->nnz == pd->AT->nnz, "A and its transpose differ in nnz");
+            pd->hasq = lp->q_ptr && lp->q_ptr[lp->n] > 0;
+            if (pd->hasq) pd->Q = csr_create(ctx, lp->n, lp->n, lp->q_ptr, lp->q_idx, lp->q_val, mdt_mask);
+            if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
+            for (int vdt = 0; vdt < 2; ++vdt) {
+                if (!(vdt_mask & (1u << vdt))) continue;
+                VecSet& s = pd->vs[vdt];
+                s.on = true;
+                s.vdt = vdt;
+                const size_t e = dtype_size(vdt);
+                const int64_t n = lp->n, m = lp->m;
+                s.c = pd->alloc(e * n); s.l = pd->alloc(e * n); s.u = pd->alloc(e * n); s.cres = pd->alloc(e * n);
+                s.lo = pd->alloc(e * m); s.hi = pd->alloc(e * m); s.rres = pd->alloc(e * m);
+                upload_as(ctx, vdt, lp->c, n, s.c); upload_as(ctx, vdt, lp->l, n, s.l);
+                upload_as(ctx, vdt, lp->u, n, s.u); upload_as(ctx, vdt, lp->cres, n, s.cres);
+                upload_as(ctx, vdt, lp->lo, m, s.lo); upload_as(ctx, vdt, lp->hi, m, s.hi);
+                upload_as(ctx, vdt, lp->rres, m, s.rres);
+                for (int k = 0; k < 2; ++k) { s.X[k] = pd->alloc(e * n); s.Y[k] = pd->alloc(e * m); }
+                s.xbar = pd->alloc(e * n); s.sumx = pd->alloc(e * n); s.xa = pd->alloc(e * n); s.xr = pd->alloc(e * n);
+                s.sumy = pd->alloc(e * m); s.ya = pd->alloc(e * m); s.yr = pd->alloc(e * m);
+            }
+            pd->prm = (double*)pd->alloc(sizeof(double) * 2);
+            pd->part_n = (double*)pd->alloc(sizeof(double) * kCheckGridMax * kNOutN);
+            pd->part_m = (double*)pd->alloc(sizeof(double) * kCheckGridMax * kNOutM);
+            pd->tickets = (unsigned*)pd->alloc(sizeof(unsigned) * 2);
+            pd->out = (double*)pd->alloc(sizeof(double) * GANIT_CHECK_NOUT);
+            GH_CHECK(hipMemsetAsync(pd->tickets, 0, sizeof(unsigned) * 2, ctx->stream));
+            GH_CHECK(hipMemsetAsync(pd->out, 0, sizeof(double) * GANIT_CHECK_NOUT, ctx->stream));
+            GH_CHECK(hipStreamSynchronize(ctx->stream));
+            const char* ng = std::getenv("GANIT_HIP_NO_GRAPH");
+            pd->use_graphs = !(ng && ng[0] == '1');
+        } catch (...) {
+            delete pd;
+            throw;
+        }
+        *out = reinterpret_cast<ganit_hip_pdhg*>(pd);
+    });
+}
+
+int ganit_hip_pdhg_destroy(ganit_hip_pdhg* h) {
+    return guarded([&] { delete reinterpret_cast<Pdhg*>(h); });
+}
+
+int ganit_hip_pdhg_plan(const ganit_hip_pdhg* h, int which, int* lanes, int64_t* hist) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<const Pdhg*>(h);
+        GH_REQUIRE(pd && which >= 0 && which <= 2, "bad plan query");
+        const Csr* a = which == 0 ? pd->A : which == 1 ? pd->AT : pd->Q;
+        GH_REQUIRE(a, "problem has no Q");
+        if (lanes) *lanes = a->lanes;
+        if (hist) std::copy(a->hist, a->hist + 9, hist);
+    });
+}
+
+int ganit_hip_pdhg_set_lanes(ganit_hip_pdhg* h, int which, int lanes) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd && which >= 0 && which <= 2, "bad plan override");
+        with_lanes(lanes, [](auto) {});
+        Csr* a = which == 0 ? pd->A : which == 1 ? pd->AT : pd->Q;
+        GH_REQUIRE(a, "problem has no Q");
+        a->lanes = lanes;
+        if (which == 1 && pd->Q) pd->Q->lanes = lanes;  // Q rows share the A^T row group
+        GH_CHECK(hipStreamSynchronize(pd->ctx->stream));
+        for (auto& kv : pd->graphs) (void)hipGraphExecDestroy(kv.second);
+        pd->graphs.clear();
+    });
+}
+
+int ganit_hip_pdhg_load(ganit_hip_pdhg* h, int vdt, const double* x, const double* y, const double* xr,
+                        const double* yr) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd && (pd->n == 0 || x) && (pd->m == 0 || y), "null iterate");
+        VecSet& s = pd->set(vdt);
+        upload_as(pd->ctx, vdt, x, pd->n, s.X[s.cur]);
+        upload_as(pd->ctx, vdt, y, pd->m, s.Y[s.cur]);
+        upload_as(pd->ctx, vdt, xr ? xr : x, pd->n, s.xr);
+        upload_as(pd->ctx, vdt, yr ? yr : y, pd->m, s.yr);
+        with_vdt(vdt, [&](auto T) { clear_sums<typename decltype(T)::type>(pd, s); });
+    });
+}
+
+int ganit_hip_pdhg_handover(ganit_hip_pdhg* h, int from, int to) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd, "null problem");
+        VecSet& a = pd->set(from);
+        VecSet& b = pd->set(to);
+        with_vdt(from, [&](auto S) {
+            with_vdt(to, [&](auto D) {
+                using SV = typename decltype(S)::type;
+                using DV = typename decltype(D)::type;
+                auto conv = [&](const void* src, void* dst, int64_t k) {
+                    if (k == 0 || src == dst) return;
+                    hipLaunchKernelGGL((k_handover<SV, DV>), dim3(ew_grid2(k)), dim3(kBlock), 0, pd->ctx->stream, k,
+                                       (const SV*)src, (DV*)dst);
+                };
+                conv(a.X[a.cur], b.X[b.cur], pd->n);
+                conv(a.Y[a.cur], b.Y[b.cur], pd->m);
+                conv(a.xr, b.xr, pd->n);
+                conv(a.yr, b.yr, pd->m);
+                clear_sums<DV>(pd, b);
+            });
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_pdhg_iterate(ganit_hip_pdhg* h, int mdt, int vdt, int iters, double tau, double sigma) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd && iters >= 0, "bad iterate arguments");
+        VecSet& s = pd->set(vdt);
+        if (iters == 0) return;
+        hipStream_t st = pd->ctx->stream;
+        hipLaunchKernelGGL(k_set_steps, dim3(1), dim3(64), 0, st, pd->prm, tau, sigma);
+        GH_CHECK(hipGetLastError());
+        auto run = [&](int k) {
+            with_mdt(mdt, [&](auto MT) {
+                with_vdt(vdt, [&](auto VT) {
+                    using M = typename decltype(MT)::type;
+                    using V = typename decltype(VT)::type;
+                    for (int t = 0; t < k; ++t) launch_iteration<M, V>(pd, s);
+                });
+            });
+        };
+        if (!pd->use_graphs || iters < 4) {
+            run(iters);
+            return;
+        }
+        const auto key = std::make_tuple(mdt, vdt, iters, s.cur);
+        auto it = pd->graphs.find(key);
+        if (it == pd->graphs.end()) {
+            const int cur0 = s.cur;
+            hipGraph_t g = nullptr;
+            GH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+            try {
+                run(iters);
+            } catch (...) {
+                (void)hipStreamEndCapture(st, &g);
+                if (g) (void)hipGraphDestroy(g);
+                s.cur = cur0;
+                throw;
+            }
+            GH_CHECK(hipStreamEndCapture(st, &g));
+            hipGraphExec_t ex = nullptr;
+            const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            GH_CHECK(e);
+            s.cur = cur0;  // capture recorded the launches without running them
+            it = pd->graphs.emplace(key, ex).first;
+        }
+        GH_CHECK(hipGraphLaunch(it->second, st));
+        if (iters & 1) s.cur ^= 1;
+    });
+}
+
+int ganit_hip_pdhg_check(ganit_hip_pdhg* h, int mdt, int vdt, int flags, double inv_k, double* out) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd && out, "null argument");
+        GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY)) == 0, "unknown check flag");
+        VecSet& s = pd->set(vdt);
+        with_mdt(mdt, [&](auto MT) {
+            with_vdt(vdt, [&](auto VT) {
+                run_check<typename decltype(MT)::type, typename decltype(VT)::type>(pd, s, flags, inv_k, out);
+            });
+        });
+        if (!(flags & GANIT_CHECK_AVG)) for (int k : {4, 5, 6, 7, 10, 11, 14, 15, 17, 18}) out[k] = 0.0;
+        if (!(flags & GANIT_CHECK_RAY)) for (int k : {8, 9, 16}) out[k] = 0.0;
+    });
+}
+
+int ganit_hip_pdhg_restart(ganit_hip_pdhg* h, int vdt, int adopt) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd, "null problem");
+        VecSet& s = pd->set(vdt);
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            hipLaunchKernelGGL(k_restart<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, pd->ctx->stream,
+                               pd->n, pd->m, adopt ? 1 : 0, (V*)s.X[s.cur], (const V*)s.xa, (V*)s.xr, (V*)s.sumx,
+                               (V*)s.Y[s.cur], (const V*)s.ya, (V*)s.yr, (V*)s.sumy, 0);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_pdhg_adopt_average(ganit_hip_pdhg* h, int vdt) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd, "null problem");
+        VecSet& s = pd->set(vdt);
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            hipLaunchKernelGGL(k_restart<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, pd->ctx->stream,
+                               pd->n, pd->m, 1, (V*)s.X[s.cur], (const V*)s.xa, (V*)s.xr, (V*)s.sumx, (V*)s.Y[s.cur],
+                               (const V*)s.ya, (V*)s.yr, (V*)s.sumy, 1);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_pdhg_store(ganit_hip_pdhg* h, int vdt, double* x, double* y, double* xr, double* yr) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd, "null problem");
+        VecSet& s = pd->set(vdt);
+        download_as(pd->ctx, vdt, s.X[s.cur], pd->n, x);
+        download_as(pd->ctx, vdt, s.Y[s.cur], pd->m, y);
+        download_as(pd->ctx, vdt, s.xr, pd->n, xr);
+        download_as(pd->ctx, vdt, s.yr, pd->m, yr);
+    });
+}
+
+}  // extern "C"

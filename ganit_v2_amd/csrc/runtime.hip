@@ -1,0 +1,608 @@
+// libganit_hip.so, part 1: context, device vectors, CSR upload with the row-length plan, and the
+// backend-concept kernels (SpMV, primal / dual step, the FP64 reductions) behind the C ABI.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "pdhg_math.hpp"
+
+namespace ganit_hip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// ================================================================ elementwise kernels
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_fill(int64_t n, double a, V* __restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) d[i] = V(a);
+}
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_copy(int64_t n, const V* __restrict__ s, V* __restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) d[i] = s[i];
+}
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_sub(int64_t n, const V* __restrict__ a, const V* __restrict__ b,
+                                                V* __restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        d[i] = a[i] - b[i];
+}
+// d = V(a * double(s)) -- the running-average formation (src/backend_cuda.cu:135-139)
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_scale(int64_t n, double a, const V* __restrict__ s, V* __restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        d[i] = V(a * double(s[i]));
+}
+template <class S, class D>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const S* __restrict__ s, D* __restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        d[i] = D(s[i]);
+}
+
+unsigned ew_grid(int64_t n) { return std::min<unsigned>(grid_for(n), 8192u); }
+
+// ================================================================ SpMV and the two steps
+template <int LPR, class M, class V>
+__global__ __launch_bounds__(kBlock) void k_spmv(CsrView<M> a, const V* __restrict__ x, V* __restrict__ y) {
+    const int64_t gt = (int64_t)xcd_block_id() * kBlock + threadIdx.x;
+    const int row = (int)(gt / LPR), lane = (int)(gt % LPR);
+    if (row >= a.rows) return;  // a whole row group leaves together (LPR divides the block)
+    const V t = row_dot<LPR>(a, row, lane, x);
+    if (lane == 0) y[row] = t;
+}
+
+template <class V, bool HASQ>
+__global__ __launch_bounds__(kBlock) void k_primal_step(int n, const V* __restrict__ x, const V* __restrict__ aty,
+                                                        const V* __restrict__ qx, const V* __restrict__ c,
+                                                        const V* __restrict__ l, const V* __restrict__ u, V tau,
+                                                        V* __restrict__ xn, V* __restrict__ xbar,
+                                                        V* __restrict__ sumx) {
+    const int j = blockIdx.x * kBlock + threadIdx.x;
+    if (j >= n) return;
+    V s = sumx[j];
+    primal_update<V>(x[j], aty[j], HASQ ? qx[j] : V(0), c[j], l[j], u[j], tau, xn[j], xbar[j], s);
+    sumx[j] = s;
+}
+
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_dual_step(int m, const V* __restrict__ y, const V* __restrict__ axbar,
+                                                      const V* __restrict__ lo, const V* __restrict__ hi, V sigma,
+                                                      V* __restrict__ yn, V* __restrict__ sumy) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= m) return;
+    V s = sumy[i];
+    dual_update<V>(y[i], axbar[i], lo[i], hi[i], sigma, yn[i], s);
+    sumy[i] = s;
+}
+
+// ================================================================ FP64 reductions (two outputs)
+// Per-index contributions follow the reference's functors (src/backend_cuda.cu:199-249).
+template <class F>
+__global__ __launch_bounds__(kBlock) void k_reduce2(int64_t n, F f, ReduceSlot slot) {
+    __shared__ double lds[(kBlock / kWave) * 2];
+    double acc[2] = {0.0, 0.0};
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        double a0, a1;
+        f((int)i, a0, a1);
+        acc[0] += a0;
+        acc[1] += a1;
+    }
+    grid_sum<2>(acc, slot, lds);
+}
+
+template <class V>
+struct PrimalResTerm {
+    const V *ax, *lo, *hi, *rr;
+    __device__ void operator()(int i, double& a0, double& a1) const {
+        const double a = double(ax[i]);
+        const double p = fmin(fmax(a, double(lo[i])), double(hi[i]));
+        const double r = (a - p) / double(rr[i]);
+        a0 = r * r;
+        a1 = 0.0;
+    }
+};
+template <class V>
+struct DualResTerm {
+    const V *aty, *qx, *c, *l, *u, *cr;
+    double cw;
+    __device__ void operator()(int j, double& a0, double& a1) const {
+        const double z = cw * double(c[j]) - double(aty[j]) + (qx ? double(qx[j]) : 0.0);
+        const double lj = double(l[j]), uj = double(u[j]);
+        double r = 0.0, ob = 0.0;
+        if (z > 0) { if (isfinite(lj)) ob = z * lj; else r = z; }
+        else if (z < 0) { if (isfinite(uj)) ob = z * uj; else r = z; }
+        r /= double(cr[j]);
+        a0 = r * r;
+        a1 = ob;
+    }
+};
+template <class V>
+struct RowDualObjTerm {
+    const V *y, *lo, *hi;
+    __device__ void operator()(int i, double& a0, double& a1) const {
+        const double yi = double(y[i]), lw = double(lo[i]), hg = double(hi[i]);
+        a0 = 0.0;
+        if (yi > 0 && isfinite(lw)) a0 = yi * lw;
+        else if (yi < 0 && isfinite(hg)) a0 = yi * hg;
+        a1 = 0.0;
+    }
+};
+template <class V>
+struct DotTerm {
+    const V *a, *b;
+    __device__ void operator()(int i, double& a0, double& a1) const {
+        a0 = double(a[i]) * double(b[i]);
+        a1 = 0.0;
+    }
+};
+template <class V>
+struct Diff2Term {
+    const V *a, *b;
+    __device__ void operator()(int i, double& a0, double& a1) const {
+        const double d = double(a[i]) - double(b[i]);
+        a0 = d * d;
+        a1 = 0.0;
+    }
+};
+
+template <class F>
+void reduce2(Ctx* ctx, int64_t n, F f, double* o0, double* o1) {
+    const unsigned g = std::min<unsigned>(grid_for(std::max<int64_t>(n, 1)), (unsigned)kMaxReduceGrid);
+    ReduceSlot slot{ctx->red_partials, ctx->red_ticket, ctx->red_out};
+    hipLaunchKernelGGL(k_reduce2<F>, dim3(g), dim3(kBlock), 0, ctx->stream, n, f, slot);
+    GH_CHECK(hipGetLastError());
+    GH_CHECK(hipMemcpyAsync(ctx->host_out, ctx->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    GH_CHECK(hipStreamSynchronize(ctx->stream));
+    if (o0) *o0 = ctx->host_out[0];
+    if (o1) *o1 = ctx->host_out[1];
+}
+
+// ================================================================ CSR upload + plan
+uint16_t bf16_bits_from_double(double d) {
+    const float f = (float)d;
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40);  // quiet NaN
+    const uint32_t rounding = 0x7fffu + ((u >> 16) & 1u);                      // nearest, ties to even
+    return (uint16_t)((u + rounding) >> 16);
+}
+
+// Lanes per row from the row-length distribution: the nnz-weighted median row length (half of all
+// nonzeros sit in rows no longer than it), rounded up to a power of two and clamped to one wavefront.
+int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz) {
+    if (nnz == 0) return 1;
+    std::vector<int> sorted(len);
+    std::sort(sorted.begin(), sorted.end());
+    int64_t acc = 0;
+    int med = 1;
+    for (int L : sorted) {
+        acc += L;
+        if (2 * acc >= nnz) { med = L; break; }
+    }
+    int lpr = 1;
+    while (lpr < med && lpr < kWave) lpr <<= 1;
+    return lpr;
+}
+
+Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, const double* val, unsigned mask) {
+    GH_REQUIRE(rows >= 0 && cols >= 0, "negative matrix dimension");
+    GH_REQUIRE(rows == 0 || ptr, "null row pointer");
+    GH_REQUIRE(mask & 7u, "dtype mask selects no value type");
+    auto* a = new Csr;
+    a->ctx = ctx;
+    a->rows = rows;
+    a->cols = cols;
+    a->nnz = rows ? ptr[rows] : 0;
+    try {
+        GH_REQUIRE(a->nnz >= 0 && (rows == 0 || ptr[0] == 0), "malformed CSR row pointer");
+        std::vector<int> len(rows);
+        for (int i = 0; i < rows; ++i) {
+            len[i] = ptr[i + 1] - ptr[i];
+            GH_REQUIRE(len[i] >= 0, "CSR row pointer not monotone");
+            int b = 0;
+            while (b < 8 && len[i] > (1 << b)) ++b;
+            a->hist[b]++;
+        }
+        for (int64_t p = 0; p < a->nnz; ++p) GH_REQUIRE(idx[p] >= 0 && idx[p] < cols, "CSR column index out of range");
+        a->lanes = choose_lanes_from_lengths(len, a->nnz);
+        const size_t nz = (size_t)std::max<int64_t>(a->nnz, 1);
+        GH_CHECK(hipMalloc(&a->ptr, sizeof(int) * (rows + 1)));
+        GH_CHECK(hipMalloc(&a->idx, sizeof(int) * nz));
+        if (rows) GH_CHECK(hipMemcpyAsync(a->ptr, ptr, sizeof(int) * (rows + 1), hipMemcpyHostToDevice, ctx->stream));
+        else GH_CHECK(hipMemsetAsync(a->ptr, 0, sizeof(int), ctx->stream));
+        if (a->nnz) GH_CHECK(hipMemcpyAsync(a->idx, idx, sizeof(int) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
+        if (mask & (1u << GANIT_F64)) {
+            GH_CHECK(hipMalloc(&a->v64, sizeof(double) * nz));
+            if (a->nnz) GH_CHECK(hipMemcpyAsync(a->v64, val, sizeof(double) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
+        }
+        if (mask & (1u << GANIT_F32)) {
+            std::vector<float> h(val, val + a->nnz);
+            GH_CHECK(hipMalloc(&a->v32, sizeof(float) * nz));
+            if (a->nnz) GH_CHECK(hipMemcpyAsync(a->v32, h.data(), sizeof(float) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
+            GH_CHECK(hipStreamSynchronize(ctx->stream));
+        }
+        if (mask & (1u << GANIT_BF16)) {
+            std::vector<uint16_t> h(a->nnz);
+            for (int64_t p = 0; p < a->nnz; ++p) h[p] = bf16_bits_from_double(val[p]);
+            GH_CHECK(hipMalloc(&a->v16, sizeof(uint16_t) * nz));
+            if (a->nnz) GH_CHECK(hipMemcpyAsync(a->v16, h.data(), sizeof(uint16_t) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
+            GH_CHECK(hipStreamSynchronize(ctx->stream));
+        }
+        GH_CHECK(hipStreamSynchronize(ctx->stream));
+    } catch (...) {
+        csr_destroy(a);
+        throw;
+    }
+    return a;
+}
+
+void csr_destroy(Csr* a) {
+    if (!a) return;
+    (void)hipFree(a->ptr);
+    (void)hipFree(a->idx);
+    (void)hipFree(a->v64);
+    (void)hipFree(a->v32);
+    (void)hipFree(a->v16);
+    delete a;
+}
+
+template <class M, class V>
+void launch_spmv(Ctx* ctx, const Csr* a, const V* x, V* y) {
+    if (a->rows == 0) return;
+    const CsrView<M> v = a->view<M>();
+    with_lanes(a->lanes, [&](auto L) {
+        constexpr int LPR = decltype(L)::value;
+        hipLaunchKernelGGL((k_spmv<LPR, M, V>), dim3(grid_for((int64_t)a->rows * LPR)), dim3(kBlock), 0,
+                           ctx->stream, v, x, y);
+    });
+    GH_CHECK(hipGetLastError());
+}
+
+}  // namespace ganit_hip
+
+// ================================================================ C ABI
+using namespace ganit_hip;
+
+extern "C" {
+
+const char* ganit_hip_last_error(void) { return g_last_error.c_str(); }
+int ganit_hip_version(void) { return 100; }
+
+int ganit_hip_ctx_create(int device, ganit_hip_ctx** out) {
+    return guarded([&] {
+        GH_REQUIRE(out, "null output pointer");
+        int count = 0;
+        GH_CHECK(hipGetDeviceCount(&count));
+        GH_REQUIRE(device >= 0 && device < count, "no such HIP device");
+        GH_CHECK(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        GH_CHECK(hipGetDeviceProperties(&prop, device));
+        auto* c = new Ctx;
+        c->device = device;
+        c->cus = prop.multiProcessorCount;
+        c->name = prop.name;
+        if (std::strlen(prop.gcnArchName)) c->name += std::string(" (") + prop.gcnArchName + ")";
+        GH_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        GH_CHECK(hipMalloc(&c->red_partials, sizeof(double) * kMaxReduceGrid * 2));
+        GH_CHECK(hipMalloc(&c->red_ticket, sizeof(unsigned int)));
+        GH_CHECK(hipMalloc(&c->red_out, sizeof(double) * 2));
+        GH_CHECK(hipMemsetAsync(c->red_ticket, 0, sizeof(unsigned int), c->stream));
+        GH_CHECK(hipHostMalloc((void**)&c->host_out, sizeof(double) * 64, hipHostMallocDefault));
+        GH_CHECK(hipStreamSynchronize(c->stream));
+        *out = reinterpret_cast<ganit_hip_ctx*>(c);
+    });
+}
+
+int ganit_hip_ctx_destroy(ganit_hip_ctx* h) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        if (!c) return;
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipFree(c->red_partials);
+        (void)hipFree(c->red_ticket);
+        (void)hipFree(c->red_out);
+        (void)hipHostFree(c->host_out);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
+}
+
+int ganit_hip_ctx_sync(ganit_hip_ctx* h) {
+    return guarded([&] {
+        GH_REQUIRE(h, "null context");
+        GH_CHECK(hipStreamSynchronize(reinterpret_cast<Ctx*>(h)->stream));
+    });
+}
+
+int ganit_hip_ctx_stream(ganit_hip_ctx* h, void** s) {
+    return guarded([&] {
+        GH_REQUIRE(h && s, "null argument");
+        *s = (void*)reinterpret_cast<Ctx*>(h)->stream;
+    });
+}
+
+int ganit_hip_ctx_device_info(ganit_hip_ctx* h, char* name, int name_len, int* cus) {
+    return guarded([&] {
+        GH_REQUIRE(h, "null context");
+        auto* c = reinterpret_cast<Ctx*>(h);
+        if (name && name_len > 0) {
+            std::strncpy(name, c->name.c_str(), (size_t)name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (cus) *cus = c->cus;
+    });
+}
+
+int ganit_hip_alloc(ganit_hip_ctx* h, size_t bytes, void** d) {
+    return guarded([&] {
+        GH_REQUIRE(h && d, "null argument");
+        GH_CHECK(hipMalloc(d, std::max<size_t>(bytes, 1)));
+    });
+}
+
+int ganit_hip_free(ganit_hip_ctx* h, void* d) {
+    return guarded([&] {
+        GH_REQUIRE(h, "null context");
+        if (d) {
+            GH_CHECK(hipStreamSynchronize(reinterpret_cast<Ctx*>(h)->stream));
+            GH_CHECK(hipFree(d));
+        }
+    });
+}
+
+int ganit_hip_upload(ganit_hip_ctx* h, int vdt, const double* host, int64_t n, void* dst) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && (n == 0 || (host && dst)) && n >= 0, "bad upload arguments");
+        if (n == 0) return;
+        if (vdt == GANIT_F64) {
+            GH_CHECK(hipMemcpyAsync(dst, host, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+            GH_CHECK(hipStreamSynchronize(c->stream));
+        } else if (vdt == GANIT_F32) {
+            std::vector<float> t(host, host + n);
+            GH_CHECK(hipMemcpyAsync(dst, t.data(), sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+            GH_CHECK(hipStreamSynchronize(c->stream));
+        } else {
+            throw Error(GANIT_EINVAL, "vector dtype must be GANIT_F64 or GANIT_F32");
+        }
+    });
+}
+
+int ganit_hip_download(ganit_hip_ctx* h, int vdt, const void* src, int64_t n, double* host) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && (n == 0 || (host && src)) && n >= 0, "bad download arguments");
+        if (n == 0) return;
+        if (vdt == GANIT_F64) {
+            GH_CHECK(hipMemcpyAsync(host, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+            GH_CHECK(hipStreamSynchronize(c->stream));
+        } else if (vdt == GANIT_F32) {
+            std::vector<float> t(n);
+            GH_CHECK(hipMemcpyAsync(t.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost, c->stream));
+            GH_CHECK(hipStreamSynchronize(c->stream));
+            for (int64_t i = 0; i < n; ++i) host[i] = t[i];
+        } else {
+            throw Error(GANIT_EINVAL, "vector dtype must be GANIT_F64 or GANIT_F32");
+        }
+    });
+}
+
+int ganit_hip_vec_fill(ganit_hip_ctx* h, int vdt, void* dst, int64_t n, double a) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && n >= 0 && (n == 0 || dst), "bad fill arguments");
+        if (n == 0) return;
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            hipLaunchKernelGGL(k_fill<V>, dim3(ew_grid(n)), dim3(kBlock), 0, c->stream, n, a, (V*)dst);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_vec_copy(ganit_hip_ctx* h, int vdt, const void* src, void* dst, int64_t n) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && n >= 0 && (n == 0 || (src && dst)), "bad copy arguments");
+        if (n == 0) return;
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            hipLaunchKernelGGL(k_copy<V>, dim3(ew_grid(n)), dim3(kBlock), 0, c->stream, n, (const V*)src, (V*)dst);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_vec_sub(ganit_hip_ctx* h, int vdt, const void* a, const void* b, void* d, int64_t n) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && n >= 0 && (n == 0 || (a && b && d)), "bad sub arguments");
+        if (n == 0) return;
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            hipLaunchKernelGGL(k_sub<V>, dim3(ew_grid(n)), dim3(kBlock), 0, c->stream, n, (const V*)a, (const V*)b,
+                               (V*)d);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_vec_scale_into(ganit_hip_ctx* h, int vdt, const void* src, double a, void* dst, int64_t n) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && n >= 0 && (n == 0 || (src && dst)), "bad scale arguments");
+        if (n == 0) return;
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            hipLaunchKernelGGL(k_scale<V>, dim3(ew_grid(n)), dim3(kBlock), 0, c->stream, n, a, (const V*)src, (V*)dst);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_vec_convert(ganit_hip_ctx* h, int sdt, const void* src, int ddt, void* dst, int64_t n) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && n >= 0 && (n == 0 || (src && dst)), "bad convert arguments");
+        if (n == 0) return;
+        with_vdt(sdt, [&](auto S) {
+            with_vdt(ddt, [&](auto D) {
+                using SV = typename decltype(S)::type;
+                using DV = typename decltype(D)::type;
+                hipLaunchKernelGGL((k_convert<SV, DV>), dim3(ew_grid(n)), dim3(kBlock), 0, c->stream, n,
+                                   (const SV*)src, (DV*)dst);
+            });
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_csr_upload(ganit_hip_ctx* h, int rows, int cols, const int* ptr, const int* idx, const double* val,
+                         unsigned mask, ganit_hip_csr** out) {
+    return guarded([&] {
+        GH_REQUIRE(h && out, "null argument");
+        *out = reinterpret_cast<ganit_hip_csr*>(csr_create(reinterpret_cast<Ctx*>(h), rows, cols, ptr, idx, val, mask));
+    });
+}
+
+int ganit_hip_csr_free(ganit_hip_csr* a) {
+    return guarded([&] { csr_destroy(reinterpret_cast<Csr*>(a)); });
+}
+
+int ganit_hip_csr_plan(const ganit_hip_csr* h, int* lanes, int64_t* hist) {
+    return guarded([&] {
+        GH_REQUIRE(h, "null matrix");
+        auto* a = reinterpret_cast<const Csr*>(h);
+        if (lanes) *lanes = a->lanes;
+        if (hist) std::copy(a->hist, a->hist + 9, hist);
+    });
+}
+
+int ganit_hip_csr_set_lanes(ganit_hip_csr* h, int lanes) {
+    return guarded([&] {
+        GH_REQUIRE(h, "null matrix");
+        with_lanes(lanes, [](auto) {});
+        reinterpret_cast<Csr*>(h)->lanes = lanes;
+    });
+}
+
+int ganit_hip_spmv(ganit_hip_ctx* h, const ganit_hip_csr* ah, int mdt, int vdt, const void* x, void* y) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        auto* a = reinterpret_cast<const Csr*>(ah);
+        GH_REQUIRE(c && a && (a->rows == 0 || (x && y)), "bad spmv arguments");
+        with_mdt(mdt, [&](auto MT) {
+            with_vdt(vdt, [&](auto VT) {
+                using M = typename decltype(MT)::type;
+                using V = typename decltype(VT)::type;
+                launch_spmv<M, V>(c, a, (const V*)x, (V*)y);
+            });
+        });
+    });
+}
+
+int ganit_hip_primal_step(ganit_hip_ctx* h, int vdt, int n, const void* x, const void* aty, const void* qx,
+                          const void* cc, const void* l, const void* u, double tau, void* xn, void* xbar,
+                          void* sumx) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && n >= 0, "bad primal_step arguments");
+        if (n == 0) return;
+        GH_REQUIRE(x && aty && cc && l && u && xn && xbar && sumx, "null vector");
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            if (qx)
+                hipLaunchKernelGGL((k_primal_step<V, true>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream, n,
+                                   (const V*)x, (const V*)aty, (const V*)qx, (const V*)cc, (const V*)l, (const V*)u,
+                                   V(tau), (V*)xn, (V*)xbar, (V*)sumx);
+            else
+                hipLaunchKernelGGL((k_primal_step<V, false>), dim3(grid_for(n)), dim3(kBlock), 0, c->stream, n,
+                                   (const V*)x, (const V*)aty, (const V*)nullptr, (const V*)cc, (const V*)l,
+                                   (const V*)u, V(tau), (V*)xn, (V*)xbar, (V*)sumx);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_dual_step(ganit_hip_ctx* h, int vdt, int m, const void* y, const void* axbar, const void* lo,
+                        const void* hi, double sigma, void* yn, void* sumy) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && m >= 0, "bad dual_step arguments");
+        if (m == 0) return;
+        GH_REQUIRE(y && axbar && lo && hi && yn && sumy, "null vector");
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            hipLaunchKernelGGL(k_dual_step<V>, dim3(grid_for(m)), dim3(kBlock), 0, c->stream, m, (const V*)y,
+                               (const V*)axbar, (const V*)lo, (const V*)hi, V(sigma), (V*)yn, (V*)sumy);
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_primal_res2(ganit_hip_ctx* h, int vdt, int m, const void* ax, const void* lo, const void* hi,
+                          const void* rres, double* out) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && out && m >= 0 && (m == 0 || (ax && lo && hi && rres)), "bad primal_res2 arguments");
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            reduce2(c, m, PrimalResTerm<V>{(const V*)ax, (const V*)lo, (const V*)hi, (const V*)rres}, out, nullptr);
+        });
+    });
+}
+
+int ganit_hip_dual_res(ganit_hip_ctx* h, int vdt, int n, const void* aty, const void* qx, const void* cc,
+                       const void* l, const void* u, const void* cres, int with_c, double* rd2, double* vobj) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && rd2 && vobj && n >= 0 && (n == 0 || (aty && cc && l && u && cres)), "bad dual_res arguments");
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            reduce2(c, n,
+                    DualResTerm<V>{(const V*)aty, (const V*)qx, (const V*)cc, (const V*)l, (const V*)u, (const V*)cres,
+                                   with_c ? 1.0 : 0.0},
+                    rd2, vobj);
+        });
+    });
+}
+
+int ganit_hip_row_dual_obj(ganit_hip_ctx* h, int vdt, int m, const void* y, const void* lo, const void* hi,
+                           double* out) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && out && m >= 0 && (m == 0 || (y && lo && hi)), "bad row_dual_obj arguments");
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            reduce2(c, m, RowDualObjTerm<V>{(const V*)y, (const V*)lo, (const V*)hi}, out, nullptr);
+        });
+    });
+}
+
+int ganit_hip_dot(ganit_hip_ctx* h, int vdt, int64_t n, const void* a, const void* b, double* out) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && out && n >= 0 && (n == 0 || (a && b)), "bad dot arguments");
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            reduce2(c, n, DotTerm<V>{(const V*)a, (const V*)b}, out, nullptr);
+        });
+    });
+}
+
+int ganit_hip_diff2(ganit_hip_ctx* h, int vdt, int64_t n, const void* a, const void* b, double* out) {
+    return guarded([&] {
+        auto* c = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(c && out && n >= 0 && (n == 0 || (a && b)), "bad diff2 arguments");
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            reduce2(c, n, Diff2Term<V>{(const V*)a, (const V*)b}, out, nullptr);
+        });
+    });
+}
+
+}  // extern "C"

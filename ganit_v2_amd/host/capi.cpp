@@ -1,0 +1,174 @@
+// C ABI of libganit.so (declared in include/ganit_host.h).
+#include <algorithm>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ganit_host.h"
+#include "generators.hpp"
+#include "pdhg.hpp"
+#include "scaling.hpp"
+
+namespace {
+thread_local std::string g_err;
+struct Handle {
+    ganit::LP lp;
+    std::vector<double> x0, y0;
+};
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return 1;
+    }
+}
+template <class T, class S>
+void put(T* dst, const S& src) {
+    if (dst) std::copy(src.begin(), src.end(), dst);
+}
+}  // namespace
+
+extern "C" {
+
+const char* ganit_last_error(void) { return g_err.c_str(); }
+
+ganit_lp* ganit_lp_read_mps(const char* path) {
+    Handle* h = nullptr;
+    if (guard([&] {
+            h = new Handle;
+            h->lp = ganit::read_mps(path);
+        })) {
+        delete h;
+        return nullptr;
+    }
+    return reinterpret_cast<ganit_lp*>(h);
+}
+
+ganit_lp* ganit_lp_from_triplets(int m, int n, int64_t nnz, const int* rows, const int* cols, const double* vals,
+                                 const double* c, const double* l, const double* u, const double* lo,
+                                 const double* hi, double obj_const, int maximize, int64_t qnnz, const int* qrows,
+                                 const int* qcols, const double* qvals) {
+    Handle* h = nullptr;
+    if (guard([&] {
+            if (m < 0 || n < 0 || nnz < 0 || qnnz < 0) throw std::runtime_error("negative size");
+            h = new Handle;
+            ganit::LP& lp = h->lp;
+            lp.m = m;
+            lp.n = n;
+            lp.A = ganit::csr_from_triplets(m, n, std::vector<int>(rows, rows + nnz), std::vector<int>(cols, cols + nnz),
+                                            std::vector<double>(vals, vals + nnz));
+            lp.Q = qnnz > 0 ? ganit::csr_from_triplets(n, n, std::vector<int>(qrows, qrows + qnnz),
+                                                       std::vector<int>(qcols, qcols + qnnz),
+                                                       std::vector<double>(qvals, qvals + qnnz))
+                            : ganit::empty_csr(n, n);
+            lp.c.assign(c, c + n);
+            lp.l.assign(l, l + n);
+            lp.u.assign(u, u + n);
+            lp.lo.assign(lo, lo + m);
+            lp.hi.assign(hi, hi + m);
+            lp.obj_const = obj_const;
+            lp.maximize = maximize != 0;
+            lp.is_int.assign(n, 0);
+        })) {
+        delete h;
+        return nullptr;
+    }
+    return reinterpret_cast<ganit_lp*>(h);
+}
+
+ganit_lp* ganit_lp_generate(int kind, const double* p, uint64_t seed) {
+    Handle* h = nullptr;
+    if (guard([&] {
+            h = new Handle;
+            if (kind == 0) {
+                ganit::RandomLpSpec sp;
+                sp.m = (int)p[0]; sp.n = (int)p[1]; sp.m_eq = (int)p[2];
+                sp.nnz_min = (int)p[3]; sp.nnz_max = (int)p[4]; sp.tighten_frac = p[5];
+                h->lp = ganit::gen_random_lp(sp, seed, &h->x0, &h->y0);
+            } else if (kind == 1) {
+                h->lp = ganit::gen_blending((int)p[0], (int)p[1], (int)p[2], (int)p[3], seed);
+            } else if (kind == 2) {
+                h->lp = ganit::gen_portfolio((int)p[0], (int)p[1], p[2], seed);
+            } else if (kind == 3) {
+                h->lp = ganit::gen_diff_qp((int)p[0], (int)p[1], (int)p[2], p[3], p[4], seed, &h->x0);
+            } else {
+                throw std::runtime_error("unknown generator kind");
+            }
+        })) {
+        delete h;
+        return nullptr;
+    }
+    return reinterpret_cast<ganit_lp*>(h);
+}
+
+void ganit_lp_free(ganit_lp* h) { delete reinterpret_cast<Handle*>(h); }
+
+void ganit_lp_dims(const ganit_lp* hh, int64_t* d) {
+    const ganit::LP& lp = reinterpret_cast<const Handle*>(hh)->lp;
+    d[0] = lp.m; d[1] = lp.n; d[2] = lp.A.nnz(); d[3] = lp.Q.nnz(); d[4] = lp.maximize ? 1 : 0;
+}
+
+void ganit_lp_export(const ganit_lp* hh, int* aptr, int* aidx, double* aval, int* qptr, int* qidx, double* qval,
+                     double* c, double* l, double* u, double* lo, double* hi, double* obj_const) {
+    const ganit::LP& lp = reinterpret_cast<const Handle*>(hh)->lp;
+    put(aptr, lp.A.ptr); put(aidx, lp.A.idx); put(aval, lp.A.val);
+    put(qptr, lp.Q.ptr); put(qidx, lp.Q.idx); put(qval, lp.Q.val);
+    put(c, lp.c); put(l, lp.l); put(u, lp.u); put(lo, lp.lo); put(hi, lp.hi);
+    if (obj_const) *obj_const = lp.obj_const;
+}
+
+int ganit_lp_planted(const ganit_lp* hh, double* x0, double* y0) {
+    const Handle* h = reinterpret_cast<const Handle*>(hh);
+    if (h->x0.empty()) { g_err = "instance has no planted point"; return 1; }
+    put(x0, h->x0);
+    put(y0, h->y0);
+    return 0;
+}
+
+int ganit_lp_write_mps(const ganit_lp* hh, const char* path) {
+    return guard([&] { ganit::write_mps(reinterpret_cast<const Handle*>(hh)->lp, path); });
+}
+
+int ganit_lp_scaled(const ganit_lp* hh, int ruiz, int pc, double* scal, int* aptr, int* aidx, double* aval, int* tptr,
+                    int* tidx, double* tval, int* qptr, int* qidx, double* qval, double* c, double* l, double* u,
+                    double* lo, double* hi, double* R, double* C, double* Rres, double* Cres) {
+    return guard([&] {
+        const ganit::ScaledLP s = ganit::scale_lp(reinterpret_cast<const Handle*>(hh)->lp, ruiz, pc != 0);
+        if (scal) {
+            const double v[9] = {s.bscale, s.cscale, s.normQ, s.obj_const, s.bnorm, s.cnorm, ganit::estimate_norm(s),
+                                 (double)s.AT.nnz(), s.maximize ? 1.0 : 0.0};
+            std::copy(v, v + 9, scal);
+        }
+        put(aptr, s.A.ptr); put(aidx, s.A.idx); put(aval, s.A.val);
+        put(tptr, s.AT.ptr); put(tidx, s.AT.idx); put(tval, s.AT.val);
+        put(qptr, s.Q.ptr); put(qidx, s.Q.idx); put(qval, s.Q.val);
+        put(c, s.c); put(l, s.l); put(u, s.u); put(lo, s.lo); put(hi, s.hi);
+        put(R, s.R); put(C, s.C); put(Rres, s.Rres); put(Cres, s.Cres);
+    });
+}
+
+int ganit_solve_gpu(const ganit_lp* hh, const double* o, double* out, double* x, double* y) {
+    return guard([&] {
+        ganit::Options opt;
+        opt.tol = o[0]; opt.max_iter = (long)o[1]; opt.time_limit = o[2]; opt.check_every = (int)o[3];
+        opt.ruiz_iters = (int)o[4]; opt.pock_chambolle = o[5] != 0; opt.verbose = (int)o[6];
+        opt.infeas_tol = o[7];
+        const int pr = (int)o[8];
+        opt.precision = pr == 1 ? ganit::Precision::Mixed : pr == 2 ? ganit::Precision::Dynamic : ganit::Precision::Double;
+        opt.stall_checks = (int)o[9]; opt.stall_checks_l2 = (int)o[10];
+        opt.values = (int)o[11] == 1 ? ganit::Values::BF16 : ganit::Values::F32;
+        opt.device = (int)o[12];
+        const ganit::Result r = ganit::solve_pdhg_gpu(reinterpret_cast<const Handle*>(hh)->lp, opt);
+        const double v[18] = {(double)(int)r.status, r.pobj, r.dobj, r.rel_primal, r.rel_dual, r.rel_gap,
+                              (double)r.iterations, (double)r.fp32_iterations, (double)r.l2_iterations,
+                              (double)r.restarts, r.setup_seconds, r.solve_seconds, r.norm_a, r.upload_seconds,
+                              r.iterate_seconds[0], r.iterate_seconds[1], r.iterate_seconds[2], (double)r.checks};
+        std::copy(v, v + 18, out);
+        put(x, r.x);
+        put(y, r.y);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+// Restarted-average PDHG driver over the device-resident C ABI of libganit_hip.so.
+//
+// The control logic is the reference's (src/pdhg_impl.hpp:67-328): the same step sizes, KKT formulas,
+// convergence / limit / infeasibility / stall tests in the same order, the same adaptive-restart rule and
+// primal-weight update, the same precision ladder and the same FP64 certification.  What changes is where
+// the work happens: the scaled problem is uploaded once with every precision it will need; an iteration is
+// two fused kernels replayed from a HIP graph; a check is three kernels and one host synchronisation that
+// return every sum the decisions need; and precision hand-over and FP64 certification convert the iterate
+// on the device instead of round-tripping it through host doubles.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <limits>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ganit_hip.h"
+#include "pdhg.hpp"
+#include "scaling.hpp"
+
+namespace ganit {
+
+const char* to_string(Status s) {
+    switch (s) {
+        case Status::Optimal: return "OPTIMAL";
+        case Status::PrimalInfeasible: return "PRIMAL_INFEASIBLE";
+        case Status::IterationLimit: return "ITERATION_LIMIT";
+        case Status::TimeLimit: return "TIME_LIMIT";
+        default: return "NUMERICAL_ERROR";
+    }
+}
+
+namespace {
+
+void chk(int rc) {
+    if (rc != GANIT_OK) throw std::runtime_error(std::string("HIP backend: ") + ganit_hip_last_error());
+}
+
+struct KKT {  // reference pdhg_impl.hpp:36-42
+    double pobj = 0, dobj = 0, rel_p = 0, rel_d = 0, rel_gap = 0, err = 0;
+    bool finite() const {
+        return std::isfinite(pobj) && std::isfinite(dobj) && std::isfinite(rel_p) && std::isfinite(rel_d);
+    }
+    bool converged(double tol) const { return rel_p <= tol && rel_d <= tol && rel_gap <= tol; }
+};
+
+using Clock = std::chrono::steady_clock;
+
+struct Shared {
+    const ScaledLP& s;
+    const Options& opt;
+    double eta, normA, obj_scale;
+    Clock::time_point t_solve;
+    double last_log = -1e9;
+    double elapsed() const { return std::chrono::duration<double>(Clock::now() - t_solve).count(); }
+};
+
+struct State {
+    double restart_err = -1.0;
+    double w = 1.0;
+    long it = 0;
+    int restarts = 0;
+    long checks = 0;
+    KKT kkt;
+};
+
+enum class Exit { Converged, Stalled, IterationLimit, TimeLimit, Infeasible, Numerical };
+
+struct Level {
+    int mdt, vdt;
+    const char* tag;
+};
+
+// KKT from the raw sums of one point (the eval lambda, reference pdhg_impl.hpp:92-109).
+KKT kkt_from(const Shared& sh, double rd2, double vobj, double cx, double xqx, double rp2, double robj) {
+    const ScaledLP& s = sh.s;
+    KKT k;
+    k.pobj = (cx + 0.5 * xqx) * sh.obj_scale + s.obj_const;
+    k.dobj = (robj + vobj - 0.5 * xqx) * sh.obj_scale + s.obj_const;
+    k.rel_p = std::sqrt(rp2) / (1.0 + s.bnorm);
+    k.rel_d = std::sqrt(rd2) / (1.0 + s.cnorm);
+    k.rel_gap = std::fabs(k.pobj - k.dobj) / (1.0 + std::fabs(k.pobj) + std::fabs(k.dobj));
+    k.err = std::sqrt(k.rel_p * k.rel_p + k.rel_d * k.rel_d + k.rel_gap * k.rel_gap);
+    return k;
+}
+KKT kkt_current(const Shared& sh, const double* o) { return kkt_from(sh, o[0], o[1], o[2], o[3], o[12], o[13]); }
+KKT kkt_average(const Shared& sh, const double* o) { return kkt_from(sh, o[4], o[5], o[6], o[7], o[14], o[15]); }
+
+// KKT of the current iterate of working set `vdt` evaluated with matrix values `mdt`.
+KKT evaluate(Shared& sh, ganit_hip_pdhg* pd, int mdt, int vdt) {
+    double o[GANIT_CHECK_NOUT];
+    chk(ganit_hip_pdhg_check(pd, mdt, vdt, 0, 0.0, o));
+    return kkt_current(sh, o);
+}
+
+// One precision phase (reference pdhg_phase, pdhg_impl.hpp:67-199).  On entry working set lv.vdt holds the
+// iterate and restart point; on exit it holds the phase's best point (average adopted if that was best).
+Exit phase(Shared& sh, ganit_hip_pdhg* pd, const Level& lv, State& st, double tol, bool stall_exit,
+           int stall_checks) {
+    const Options& opt = sh.opt;
+    const ScaledLP& s = sh.s;
+    const bool hasq = s.has_q();
+    const int ce = opt.check_every;
+    double w = st.w;
+    chk(ganit_hip_pdhg_handover(pd, lv.vdt, lv.vdt));  // fresh running sums
+
+    KKT k_restart = evaluate(sh, pd, lv.mdt, lv.vdt);
+    if (st.restart_err > 0) k_restart.err = std::max(k_restart.err, st.restart_err);
+    double k_prev_cand = std::numeric_limits<double>::infinity();
+    long it_since = 0;
+    double best_err = k_restart.err;
+    int no_progress = 0;
+    Exit why = Exit::IterationLimit;
+    KKT best;
+    bool best_is_avg = false;
+    double o[GANIT_CHECK_NOUT];
+
+    while (true) {
+        double sigma = sh.eta * w, tau = sh.eta / w;  // step sizes (pdhg_impl.hpp:87-91)
+        if (hasq) tau = std::min(tau, 0.998 / (0.5 * s.normQ + sigma * sh.normA * sh.normA));
+        // Iterations are only inspected at multiples of check_every (pdhg_impl.hpp:142), so a block of
+        // check_every iterations runs without host involvement.
+        const long to_check = ce - (st.it % ce);
+        chk(ganit_hip_pdhg_iterate(pd, lv.mdt, lv.vdt, (int)to_check, tau, sigma));
+        st.it += to_check;
+        it_since += to_check;
+
+        const int flags = GANIT_CHECK_AVG | (it_since >= 4L * ce ? GANIT_CHECK_RAY : 0);
+        chk(ganit_hip_pdhg_check(pd, lv.mdt, lv.vdt, flags, 1.0 / static_cast<double>(it_since), o));
+        ++st.checks;
+        const KKT kc = kkt_current(sh, o);
+        const KKT ka = kkt_average(sh, o);
+        const double t = sh.elapsed();
+
+        if (!kc.finite() && !ka.finite()) { why = Exit::Numerical; best = kc; break; }
+        const bool avg_better = ka.finite() && (!kc.finite() || ka.err < kc.err);
+        const KKT kcand = avg_better ? ka : kc;
+        if (opt.verbose && (t - sh.last_log >= opt.log_interval)) {
+            std::printf("%9ld %8.2fs %4s %15.8e %15.8e %9.2e %9.2e %9.2e %9.2e\n", st.it, t, lv.tag, kcand.pobj,
+                        kcand.dobj, kcand.rel_p, kcand.rel_d, kcand.rel_gap, w);
+            sh.last_log = t;
+        }
+        if (kc.converged(tol)) { why = Exit::Converged; best = kc; best_is_avg = false; break; }
+        if (ka.converged(tol)) { why = Exit::Converged; best = ka; best_is_avg = true; break; }
+        if (st.it >= opt.max_iter) { why = Exit::IterationLimit; best = kcand; best_is_avg = avg_better; break; }
+        if (t >= opt.time_limit) { why = Exit::TimeLimit; best = kcand; best_is_avg = avg_better; break; }
+        if (flags & GANIT_CHECK_RAY) {  // Farkas-type certificate from y - yr (pdhg_impl.hpp:110-118)
+            const double dray = (o[16] + o[9]) * sh.obj_scale;
+            if (dray > 0 && std::sqrt(o[8]) * (1.0 + s.bnorm) / dray <= opt.infeas_tol) {
+                why = Exit::Infeasible; best = kcand; best_is_avg = avg_better; break;
+            }
+        }
+        if (kcand.err < 0.8 * best_err) { best_err = kcand.err; no_progress = 0; }
+        else if (++no_progress >= stall_checks && stall_exit) {
+            why = Exit::Stalled; best = kcand; best_is_avg = avg_better; break;
+        }
+        // adaptive restart (pdhg_impl.hpp:172-189)
+        const bool sufficient = kcand.err <= 0.2 * k_restart.err;
+        const bool necessary = kcand.err <= 0.8 * k_restart.err && kcand.err > k_prev_cand;
+        const bool artificial = it_since >= 0.36 * static_cast<double>(st.it);
+        k_prev_cand = kcand.err;
+        if (sufficient || necessary || artificial) {
+            const double dx = std::sqrt(avg_better ? o[11] : o[10]);
+            const double dy = std::sqrt(avg_better ? o[18] : o[17]);
+            if (dx > 1e-10 && dy > 1e-10) w = std::exp(0.5 * std::log(dy / dx) + 0.5 * std::log(w));
+            chk(ganit_hip_pdhg_restart(pd, lv.vdt, avg_better ? 1 : 0));
+            it_since = 0;
+            k_restart = kcand;
+            k_prev_cand = std::numeric_limits<double>::infinity();
+            ++st.restarts;
+        }
+    }
+    if (best_is_avg) chk(ganit_hip_pdhg_adopt_average(pd, lv.vdt));
+    st.restart_err = k_restart.err;
+    st.w = w;
+    st.kkt = best;
+    return why;
+}
+
+Status status_from(Exit e) {
+    switch (e) {
+        case Exit::Converged: return Status::Optimal;
+        case Exit::Infeasible: return Status::PrimalInfeasible;
+        case Exit::TimeLimit: return Status::TimeLimit;
+        case Exit::Numerical: return Status::NumericalError;
+        default: return Status::IterationLimit;
+    }
+}
+
+struct CtxGuard {
+    ganit_hip_ctx* ctx = nullptr;
+    ganit_hip_pdhg* pd = nullptr;
+    ~CtxGuard() {
+        if (pd) ganit_hip_pdhg_destroy(pd);
+        if (ctx) ganit_hip_ctx_destroy(ctx);
+    }
+};
+
+}  // namespace
+
+Result solve_pdhg_gpu(const LP& lp, const Options& opt) {
+    CtxGuard g;
+    chk(ganit_hip_ctx_create(opt.device, &g.ctx));
+    char name[256] = {0};
+    chk(ganit_hip_ctx_device_info(g.ctx, name, sizeof(name), nullptr));
+    if (opt.check_every < 1) throw std::runtime_error("check_every must be positive");
+
+    const auto t_start = Clock::now();
+    Result res;
+    res.device = std::string("gpu:") + name;
+    ScaledLP s = scale_lp(lp, opt.ruiz_iters, opt.pock_chambolle);
+    const double normA = estimate_norm(s);
+    res.norm_a = normA;
+    Shared sh{s, opt, 0.998 / normA, normA, s.cscale * s.bscale, Clock::now()};
+
+    std::vector<double> x0(s.n), y0(s.m, 0.0);
+    for (int j = 0; j < s.n; ++j) x0[j] = std::min(std::max(0.0, s.l[j]), s.u[j]);
+    State st;
+    {
+        const double cn = norm2(s.c), bn = finite_bound_norm(s.lo, s.hi);
+        st.w = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+    }
+    const bool mixed = opt.precision != Precision::Double;
+    const int low_mdt = opt.values == Values::BF16 ? GANIT_BF16 : GANIT_F32;
+    const char* pname = opt.precision == Precision::Dynamic ? "dynamic L1->L2->L3"
+                      : opt.precision == Precision::Mixed ? "mixed L1->L3" : "FP64";
+    if (opt.verbose) {
+        std::printf("GANIT PDHG | device %s | %s | precision %s%s | rows %d cols %d nnz %lld | ||A||~%.3e | tol %.1e\n",
+                    res.device.c_str(), s.has_q() ? "QP" : "LP", pname,
+                    mixed && opt.values == Values::BF16 ? " (BF16 values)" : "", s.m, s.n, (long long)s.A.nnz(),
+                    normA, opt.tol);
+        if (s.has_q()) std::printf("Hessian nnz %lld | ||Q||~%.3e\n", (long long)s.Q.nnz(), s.normQ);
+        std::printf("%9s %9s %4s %15s %15s %9s %9s %9s %9s\n", "iter", "time", "prec", "primal obj", "dual obj",
+                    "rel_p", "rel_d", "rel_gap", "weight");
+    }
+    res.setup_seconds = std::chrono::duration<double>(Clock::now() - t_start).count();
+    sh.t_solve = Clock::now();
+
+    // ---- device upload: once, with every precision the ladder will use
+    ganit_hip_lp_desc d{};
+    d.m = s.m; d.n = s.n;
+    d.a_ptr = s.A.ptr.data(); d.a_idx = s.A.idx.data(); d.a_val = s.A.val.data();
+    d.at_ptr = s.AT.ptr.data(); d.at_idx = s.AT.idx.data(); d.at_val = s.AT.val.data();
+    if (s.has_q()) { d.q_ptr = s.Q.ptr.data(); d.q_idx = s.Q.idx.data(); d.q_val = s.Q.val.data(); }
+    d.c = s.c.data(); d.l = s.l.data(); d.u = s.u.data(); d.lo = s.lo.data(); d.hi = s.hi.data();
+    d.rres = s.Rres.data(); d.cres = s.Cres.data();
+    const unsigned mmask = (1u << GANIT_F64) | (mixed ? (1u << low_mdt) : 0u);
+    const unsigned vmask = (1u << GANIT_F64) | (mixed ? (1u << GANIT_F32) : 0u);
+    chk(ganit_hip_pdhg_create(g.ctx, &d, mmask, vmask, &g.pd));
+    ganit_hip_pdhg* pd = g.pd;
+    const int first_vdt = mixed ? GANIT_F32 : GANIT_F64;
+    chk(ganit_hip_pdhg_load(pd, first_vdt, x0.data(), y0.data(), nullptr, nullptr));
+    res.upload_seconds = sh.elapsed();
+
+    const Level L1{low_mdt, GANIT_F32, "L1"}, L2{low_mdt, GANIT_F64, "L2"}, L3{GANIT_F64, GANIT_F64, "L3"};
+    int last_vdt = first_vdt;
+    // FP64 KKT of the phase's result (pdhg_evaluate<Bd>): hand the iterate to the FP64 set if needed.
+    auto fp64_eval = [&]() {
+        if (last_vdt != GANIT_F64) {
+            chk(ganit_hip_pdhg_handover(pd, last_vdt, GANIT_F64));
+            last_vdt = GANIT_F64;
+        }
+        return evaluate(sh, pd, GANIT_F64, GANIT_F64);
+    };
+    auto finished = [&](Exit e, const char* level, const char* next) {
+        const KKT k64 = fp64_eval();
+        const bool done = (e == Exit::Converged && k64.converged(opt.tol)) || e == Exit::TimeLimit ||
+                          e == Exit::IterationLimit || e == Exit::Infeasible;
+        if (opt.verbose)
+            std::printf("  %s phase: %ld iters total, %.3fs, FP64-checked error %.2e -> %s\n", level, st.it,
+                        sh.elapsed(), k64.err, done ? "done" : next);
+        return done;
+    };
+    auto timed_phase = [&](int slot, const Level& lv, bool stall_exit, int stall_checks) {
+        const double t0 = sh.elapsed();
+        const Exit e = phase(sh, pd, lv, st, opt.tol, stall_exit, stall_checks);
+        res.iterate_seconds[slot] += sh.elapsed() - t0;
+        last_vdt = lv.vdt;
+        return e;
+    };
+
+    Exit ex;
+    long fp32_iters = 0;
+    if (mixed) {
+        ex = timed_phase(0, L1, true, opt.stall_checks);
+        fp32_iters = st.it;
+        bool done = finished(ex, "L1 (FP32)",
+                             opt.precision == Precision::Dynamic ? "switching to L2" : "switching to L3");
+        if (!done && opt.precision == Precision::Dynamic) {
+            ex = timed_phase(1, L2, true, opt.stall_checks_l2);
+            res.l2_iterations = st.it - fp32_iters;
+            done = finished(ex, "L2 (FP32 matrix, FP64 vectors)", "switching to L3");
+        }
+        if (!done) ex = timed_phase(2, L3, false, 0);
+    } else {
+        ex = timed_phase(2, L3, false, 0);
+    }
+    const KKT fin = fp64_eval();  // final FP64 certification (pdhg_impl.hpp:300-303)
+    Status status = status_from(ex);
+    if (status == Status::Optimal && !fin.converged(opt.tol * 1.0000001)) status = Status::NumericalError;
+
+    std::vector<double> xs(s.n), ys(s.m);
+    chk(ganit_hip_pdhg_store(pd, GANIT_F64, xs.data(), ys.data(), nullptr, nullptr));
+    res.solve_seconds = sh.elapsed();
+    res.status = status;
+    res.iterations = st.it;
+    res.fp32_iterations = fp32_iters;
+    res.restarts = st.restarts;
+    res.checks = st.checks;
+    const double sign = s.maximize ? -1.0 : 1.0;
+    res.x.resize(s.n);
+    res.y.resize(s.m);
+    for (int j = 0; j < s.n; ++j) res.x[j] = xs[j] * s.C[j] * s.bscale;  // pdhg_impl.hpp:313-314
+    for (int i = 0; i < s.m; ++i) res.y[i] = sign * ys[i] * s.R[i] * s.cscale;
+    res.pobj = sign * fin.pobj;
+    res.dobj = sign * fin.dobj;
+    res.rel_primal = fin.rel_p;
+    res.rel_dual = fin.rel_d;
+    res.rel_gap = fin.rel_gap;
+    if (opt.verbose) {
+        std::printf("Status %s | iters %ld (L1 %ld, L2 %ld) | restarts %d | solve %.3fs (setup %.3fs)\n",
+                    to_string(status), st.it, fp32_iters, res.l2_iterations, res.restarts, res.solve_seconds,
+                    res.setup_seconds);
+        std::printf("Primal obj %.10e | Dual obj %.10e | FP64-certified rel errors p %.1e d %.1e gap %.1e\n",
+                    res.pobj, res.dobj, fin.rel_p, fin.rel_d, fin.rel_gap);
+    }
+    return res;
+}
+
+}  // namespace ganit

@@ -1,0 +1,393 @@
+// CSR construction, transpose and the MPS/QPS reader and writer of the ganit host library.
+#include "lp.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <stdexcept>
+#include <unordered_map>
+#include <unordered_set>
+
+namespace ganit {
+
+Csr empty_csr(int rows, int cols) {
+    Csr a;
+    a.rows = rows;
+    a.cols = cols;
+    a.ptr.assign(rows + 1, 0);
+    return a;
+}
+
+// Counting sort into row buckets, then per row: order by column, merge equal columns in their input order,
+// drop exact zeros.  (Behaviour of the reference's lp.cpp:9-53.)
+Csr csr_from_triplets(int rows, int cols, const std::vector<int>& r, const std::vector<int>& c,
+                      const std::vector<double>& v) {
+    if (r.size() != v.size() || c.size() != v.size()) throw std::runtime_error("csr_from_triplets: size mismatch");
+    std::vector<int> head(rows + 1, 0);
+    for (size_t k = 0; k < v.size(); ++k) {
+        if (r[k] < 0 || r[k] >= rows || c[k] < 0 || c[k] >= cols)
+            throw std::runtime_error("csr_from_triplets: index out of range");
+        ++head[r[k] + 1];
+    }
+    for (int i = 0; i < rows; ++i) head[i + 1] += head[i];
+    std::vector<std::pair<int, double>> bucket(v.size());
+    {
+        std::vector<int> at(head.begin(), head.end() - 1);
+        for (size_t k = 0; k < v.size(); ++k) bucket[at[r[k]]++] = {c[k], v[k]};
+    }
+    Csr a = empty_csr(rows, cols);
+    a.idx.reserve(v.size());
+    a.val.reserve(v.size());
+    for (int i = 0; i < rows; ++i) {
+        auto b = bucket.begin() + head[i], e = bucket.begin() + head[i + 1];
+        std::sort(b, e, [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (auto p = b; p != e;) {
+            const int col = p->first;
+            double s = 0.0;
+            for (; p != e && p->first == col; ++p) s += p->second;
+            if (s != 0.0) {
+                a.idx.push_back(col);
+                a.val.push_back(s);
+            }
+        }
+        a.ptr[i + 1] = static_cast<int>(a.idx.size());
+    }
+    return a;
+}
+
+Csr transpose(const Csr& a) {
+    Csr t = empty_csr(a.cols, a.rows);
+    for (int j : a.idx) ++t.ptr[j + 1];
+    for (int j = 0; j < a.cols; ++j) t.ptr[j + 1] += t.ptr[j];
+    t.idx.resize(a.idx.size());
+    t.val.resize(a.val.size());
+    std::vector<int> at(t.ptr.begin(), t.ptr.end() - 1);
+    for (int i = 0; i < a.rows; ++i)
+        for (int p = a.ptr[i]; p < a.ptr[i + 1]; ++p) {
+            const int q = at[a.idx[p]]++;
+            t.idx[q] = i;
+            t.val[q] = a.val[p];
+        }
+    return t;
+}
+
+int LP::num_int() const { return static_cast<int>(std::count(is_int.begin(), is_int.end(), 1)); }
+
+// ---------------------------------------------------------------------------------------------- reader
+namespace {
+
+std::string load_file(const std::string& path) {
+    const bool gz = path.size() > 3 && path.compare(path.size() - 3, 3, ".gz") == 0;
+    std::string out;
+    if (gz) {
+        const std::string cmd = "gzip -dc '" + path + "'";
+        FILE* f = popen(cmd.c_str(), "r");
+        if (!f) throw std::runtime_error("cannot run gzip on " + path);
+        char buf[1 << 16];
+        size_t k;
+        while ((k = fread(buf, 1, sizeof(buf), f)) > 0) out.append(buf, k);
+        pclose(f);
+        if (out.empty()) throw std::runtime_error("empty or unreadable file " + path);
+        return out;
+    }
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) throw std::runtime_error("cannot open " + path);
+    std::fseek(f, 0, SEEK_END);
+    const long len = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    out.resize(len > 0 ? (size_t)len : 0);
+    if (len > 0 && std::fread(&out[0], 1, (size_t)len, f) != (size_t)len) {
+        std::fclose(f);
+        throw std::runtime_error("short read on " + path);
+    }
+    std::fclose(f);
+    return out;
+}
+
+// Whitespace tokens of one line as (pointer, length) views into the file buffer.
+struct Tok {
+    const char* p;
+    size_t n;
+    bool is(const char* s) const { return std::strlen(s) == n && std::memcmp(p, s, n) == 0; }
+    std::string str() const { return std::string(p, n); }
+};
+
+double to_num(const Tok& t, int line) {
+    char tmp[128];
+    const size_t k = std::min(t.n, sizeof(tmp) - 1);
+    std::memcpy(tmp, t.p, k);
+    tmp[k] = 0;
+    char* end = nullptr;
+    const double v = std::strtod(tmp, &end);
+    if (end == tmp) throw std::runtime_error("MPS line " + std::to_string(line) + ": bad number '" + t.str() + "'");
+    return v;
+}
+
+}  // namespace
+
+LP read_mps(const std::string& path) {
+    const std::string text = load_file(path);
+    enum Sec { None, Name, ObjSense, Rows, Columns, Rhs, Ranges, Bounds, QuadObj, QMatrix };
+    LP lp;
+    std::unordered_map<std::string, int> row_of, col_of;
+    std::unordered_set<std::string> extra_n;
+    std::string obj_name, last_col;
+    std::vector<char> rtype, lower_set;
+    std::vector<double> rhs, range;
+    std::vector<int> tr, tc, qr, qc;
+    std::vector<double> tv, qv;
+    bool int_mode = false;
+    int neg_up = 0, cur = -1, line_no = 0;
+    Sec sec = None;
+    std::vector<Tok> tok;
+    auto col_index = [&](const Tok& t) {
+        auto it = col_of.find(t.str());
+        if (it == col_of.end())
+            throw std::runtime_error("MPS: unknown column '" + t.str() + "' at line " + std::to_string(line_no));
+        return it->second;
+    };
+    auto row_index = [&](const Tok& t) -> int {  // -1 for a row of an ignored extra N row
+        const std::string s = t.str();
+        auto it = row_of.find(s);
+        if (it != row_of.end()) return it->second;
+        if (extra_n.count(s)) return -1;
+        throw std::runtime_error("MPS: unknown row '" + s + "' at line " + std::to_string(line_no));
+    };
+
+    const char* p = text.data();
+    const char* const end = p + text.size();
+    while (p < end) {
+        const char* eol = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
+        if (!eol) eol = end;
+        const char* line = p;
+        size_t len = (size_t)(eol - p);
+        p = eol + (eol < end ? 1 : 0);
+        ++line_no;
+        if (len && line[len - 1] == '\r') --len;
+        if (len == 0 || line[0] == '*') continue;
+        tok.clear();
+        for (size_t i = 0; i < len;) {
+            while (i < len && std::isspace((unsigned char)line[i])) ++i;
+            if (i >= len) break;
+            size_t j = i;
+            while (j < len && !std::isspace((unsigned char)line[j])) ++j;
+            tok.push_back({line + i, j - i});
+            i = j;
+        }
+        if (tok.empty()) continue;
+        if (!std::isspace((unsigned char)line[0])) {  // section header
+            const Tok& h = tok[0];
+            if (h.is("NAME")) { sec = Name; if (tok.size() > 1) lp.name = tok[1].str(); }
+            else if (h.is("OBJSENSE")) {
+                sec = ObjSense;
+                if (tok.size() > 1) lp.maximize = tok[1].is("MAX") || tok[1].is("MAXIMIZE");
+            }
+            else if (h.is("ROWS")) sec = Rows;
+            else if (h.is("COLUMNS")) sec = Columns;
+            else if (h.is("RHS")) sec = Rhs;
+            else if (h.is("RANGES")) sec = Ranges;
+            else if (h.is("BOUNDS")) sec = Bounds;
+            else if (h.is("QUADOBJ")) sec = QuadObj;
+            else if (h.is("QMATRIX") || h.is("QSECTION")) sec = QMatrix;
+            else if (h.is("ENDATA")) break;
+            else if (h.is("OBJSENCE")) sec = ObjSense;
+            else throw std::runtime_error("MPS: unknown section '" + h.str() + "' at line " + std::to_string(line_no));
+            continue;
+        }
+        switch (sec) {
+            case ObjSense:
+                lp.maximize = tok[0].is("MAX") || tok[0].is("MAXIMIZE");
+                break;
+            case Rows: {
+                if (tok.size() < 2) throw std::runtime_error("MPS: bad ROWS line " + std::to_string(line_no));
+                const char t = (char)std::toupper((unsigned char)tok[0].p[0]);
+                if (t == 'N') {
+                    if (obj_name.empty()) obj_name = tok[1].str(); else extra_n.insert(tok[1].str());
+                } else {
+                    row_of[tok[1].str()] = lp.m++;
+                    lp.row_names.push_back(tok[1].str());
+                    rtype.push_back(t);
+                    rhs.push_back(0.0);
+                }
+                break;
+            }
+            case Columns: {
+                if (tok.size() >= 3 && tok[1].is("'MARKER'")) {
+                    if (tok[2].is("'INTORG'")) int_mode = true;
+                    else if (tok[2].is("'INTEND'")) int_mode = false;
+                    break;
+                }
+                if (!tok[0].is(last_col.c_str()) || cur < 0) {
+                    last_col = tok[0].str();
+                    cur = lp.n++;
+                    col_of[last_col] = cur;
+                    lp.col_names.push_back(last_col);
+                    lp.c.push_back(0.0);
+                    lp.l.push_back(0.0);
+                    lp.u.push_back(kInf);
+                    lp.is_int.push_back(int_mode ? 1 : 0);
+                    lower_set.push_back(0);
+                }
+                for (size_t k = 1; k + 1 < tok.size(); k += 2) {
+                    const double v = to_num(tok[k + 1], line_no);
+                    if (tok[k].is(obj_name.c_str())) { lp.c[cur] += v; continue; }
+                    const int r = row_index(tok[k]);
+                    if (r < 0) continue;
+                    tr.push_back(r);
+                    tc.push_back(cur);
+                    tv.push_back(v);
+                }
+                break;
+            }
+            case Rhs:
+            case Ranges: {
+                const size_t first = (tok.size() % 2 == 1) ? 1 : 0;  // optional set name
+                for (size_t k = first; k + 1 < tok.size(); k += 2) {
+                    const double v = to_num(tok[k + 1], line_no);
+                    if (tok[k].is(obj_name.c_str())) {
+                        if (sec == Rhs) lp.obj_const = -v;  // objective RHS is minus the constant
+                        continue;
+                    }
+                    const int r = row_index(tok[k]);
+                    if (r < 0) continue;
+                    if (sec == Rhs) rhs[r] = v;
+                    else {
+                        if (range.empty()) range.assign(lp.m, std::nan(""));
+                        range[r] = v;
+                    }
+                }
+                break;
+            }
+            case Bounds: {
+                const Tok& t = tok[0];
+                const bool valued = t.is("UP") || t.is("LO") || t.is("FX") || t.is("LI") || t.is("UI");
+                const Tok* cname;
+                double v = 0.0;
+                if (valued) {
+                    if (tok.size() >= 4) { cname = &tok[2]; v = to_num(tok[3], line_no); }
+                    else if (tok.size() == 3) { cname = &tok[1]; v = to_num(tok[2], line_no); }
+                    else throw std::runtime_error("MPS: bad BOUNDS line " + std::to_string(line_no));
+                } else {
+                    cname = tok.size() >= 3 ? &tok[2] : &tok[1];
+                }
+                const int j = col_index(*cname);
+                if (t.is("UP") || t.is("UI")) {
+                    lp.u[j] = v;
+                    if (v < 0 && lp.l[j] == 0.0 && !lower_set[j]) { lp.l[j] = -kInf; ++neg_up; }
+                    if (t.is("UI")) lp.is_int[j] = 1;
+                } else if (t.is("LO") || t.is("LI")) {
+                    lp.l[j] = v;
+                    lower_set[j] = 1;
+                    if (t.is("LI")) lp.is_int[j] = 1;
+                } else if (t.is("FX")) { lp.l[j] = v; lp.u[j] = v; lower_set[j] = 1; }
+                else if (t.is("FR")) { lp.l[j] = -kInf; lp.u[j] = kInf; lower_set[j] = 1; }
+                else if (t.is("MI")) { lp.l[j] = -kInf; lower_set[j] = 1; }
+                else if (t.is("PL")) { lp.u[j] = kInf; }
+                else if (t.is("BV")) { lp.l[j] = 0; lp.u[j] = 1; lp.is_int[j] = 1; lower_set[j] = 1; }
+                else throw std::runtime_error("MPS: unknown bound type '" + t.str() + "'");
+                break;
+            }
+            case QuadObj:
+            case QMatrix: {
+                if (tok.size() < 3) throw std::runtime_error("MPS: bad quadratic line " + std::to_string(line_no));
+                const int a = col_index(tok[0]), b = col_index(tok[1]);
+                const double v = to_num(tok[2], line_no);
+                qr.push_back(a); qc.push_back(b); qv.push_back(v);
+                if (sec == QuadObj && a != b) { qr.push_back(b); qc.push_back(a); qv.push_back(v); }
+                break;
+            }
+            default:
+                break;
+        }
+    }
+    lp.lo.assign(lp.m, 0.0);
+    lp.hi.assign(lp.m, 0.0);
+    for (int i = 0; i < lp.m; ++i) {
+        const double b = rhs[i];
+        switch (rtype[i]) {
+            case 'E': lp.lo[i] = b; lp.hi[i] = b; break;
+            case 'L': lp.lo[i] = -kInf; lp.hi[i] = b; break;
+            case 'G': lp.lo[i] = b; lp.hi[i] = kInf; break;
+            default: throw std::runtime_error("MPS: unknown row type for " + lp.row_names[i]);
+        }
+        if (!range.empty() && !std::isnan(range[i])) {
+            const double r = range[i];
+            if (rtype[i] == 'E') { if (r > 0) lp.hi[i] = b + r; else lp.lo[i] = b + r; }
+            else if (rtype[i] == 'L') lp.lo[i] = b - std::fabs(r);
+            else lp.hi[i] = b + std::fabs(r);
+        }
+    }
+    lp.A = csr_from_triplets(lp.m, lp.n, tr, tc, tv);
+    lp.Q = qv.empty() ? empty_csr(lp.n, lp.n) : csr_from_triplets(lp.n, lp.n, qr, qc, qv);
+    if (neg_up)
+        std::cerr << "[mps] warning: " << neg_up << " negative UP bounds with zero lower bound; lower set to -inf\n";
+    return lp;
+}
+
+// ---------------------------------------------------------------------------------------------- writer
+void write_mps(const LP& lp, const std::string& path) {
+    FILE* f = std::fopen(path.c_str(), "w");
+    if (!f) throw std::runtime_error("cannot write " + path);
+    auto rname = [&](int i) { return i < (int)lp.row_names.size() ? lp.row_names[i] : "R" + std::to_string(i); };
+    auto cname = [&](int j) { return j < (int)lp.col_names.size() ? lp.col_names[j] : "C" + std::to_string(j); };
+    std::fprintf(f, "NAME %s\n", lp.name.empty() ? "GANIT" : lp.name.c_str());
+    if (lp.maximize) std::fprintf(f, "OBJSENSE\n    MAX\n");
+    std::fprintf(f, "ROWS\n N OBJ\n");
+    std::vector<char> type(lp.m);
+    std::vector<double> rhs(lp.m, 0.0), rng(lp.m, 0.0);
+    std::vector<char> has_rng(lp.m, 0);
+    for (int i = 0; i < lp.m; ++i) {
+        const double lo = lp.lo[i], hi = lp.hi[i];
+        if (lo == hi) { type[i] = 'E'; rhs[i] = lo; }
+        else if (std::isinf(lo) && lo < 0) { type[i] = 'L'; rhs[i] = hi; }      // hi may be +inf: a free row
+        else if (std::isinf(hi) && hi > 0) { type[i] = 'G'; rhs[i] = lo; }
+        else { type[i] = 'G'; rhs[i] = lo; rng[i] = hi - lo; has_rng[i] = 1; }  // G row: hi = lo + |range|
+        std::fprintf(f, " %c %s\n", type[i], rname(i).c_str());
+    }
+    std::fprintf(f, "COLUMNS\n");
+    const Csr at = transpose(lp.A);
+    for (int j = 0; j < lp.n; ++j) {
+        const std::string cn = cname(j);
+        // one objective entry per column, even if zero, so that every column is declared
+        std::fprintf(f, "    %s OBJ %.17g\n", cn.c_str(), lp.c[j]);
+        for (int p = at.ptr[j]; p < at.ptr[j + 1]; ++p)
+            std::fprintf(f, "    %s %s %.17g\n", cn.c_str(), rname(at.idx[p]).c_str(), at.val[p]);
+    }
+    std::fprintf(f, "RHS\n");
+    if (lp.obj_const != 0.0) std::fprintf(f, "    RHS OBJ %.17g\n", -lp.obj_const);
+    for (int i = 0; i < lp.m; ++i)
+        if (rhs[i] != 0.0) std::fprintf(f, "    RHS %s %.17g\n", rname(i).c_str(), rhs[i]);
+    bool any_rng = false;
+    for (int i = 0; i < lp.m; ++i) any_rng |= has_rng[i] != 0;
+    if (any_rng) {
+        std::fprintf(f, "RANGES\n");
+        for (int i = 0; i < lp.m; ++i)
+            if (has_rng[i]) std::fprintf(f, "    RNG %s %.17g\n", rname(i).c_str(), rng[i]);
+    }
+    std::fprintf(f, "BOUNDS\n");
+    for (int j = 0; j < lp.n; ++j) {
+        const std::string cn = cname(j);
+        const double l = lp.l[j], u = lp.u[j];
+        if (l == u) { std::fprintf(f, " FX BND %s %.17g\n", cn.c_str(), l); continue; }
+        if (std::isinf(l) && std::isinf(u)) { std::fprintf(f, " FR BND %s\n", cn.c_str()); continue; }
+        if (std::isinf(l)) std::fprintf(f, " MI BND %s\n", cn.c_str());
+        else if (l != 0.0 || u < 0.0) std::fprintf(f, " LO BND %s %.17g\n", cn.c_str(), l);
+        if (!std::isinf(u)) std::fprintf(f, " UP BND %s %.17g\n", cn.c_str(), u);
+    }
+    if (lp.has_q()) {
+        std::fprintf(f, "QUADOBJ\n");
+        for (int i = 0; i < lp.n; ++i)
+            for (int p = lp.Q.ptr[i]; p < lp.Q.ptr[i + 1]; ++p)
+                if (lp.Q.idx[p] <= i)
+                    std::fprintf(f, "    %s %s %.17g\n", cname(i).c_str(), cname(lp.Q.idx[p]).c_str(), lp.Q.val[p]);
+    }
+    std::fprintf(f, "ENDATA\n");
+    std::fclose(f);
+}
+
+}  // namespace ganit

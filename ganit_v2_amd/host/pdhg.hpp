@@ -1,0 +1,60 @@
+// ganit restarted-PDHG LP/QP solver on MI355X -- public C++ API.
+// Mirrors the reference's include/ganit/pdhg.hpp:10-47 (Precision, Options, Status, Result,
+// solve_pdhg_gpu); additions are marked "extension".
+#pragma once
+#include <string>
+#include <vector>
+
+#include "lp.hpp"
+
+namespace ganit {
+
+// Double = L3 only; Mixed = L1 -> L3; Dynamic = L1 -> L2 -> L3 (reference pdhg.hpp:10, pdhg_impl.hpp:13-22).
+enum class Precision { Double, Mixed, Dynamic };
+// extension: element type of the matrix values in the reduced-precision levels L1 and L2.
+enum class Values { F32, BF16 };
+
+struct Options {
+    double tol = 1e-4;           // relative KKT tolerance (primal, dual, gap)
+    long max_iter = 10000000;
+    double time_limit = 3600.0;  // seconds
+    int check_every = 64;        // evaluate KKT every N iterations
+    int ruiz_iters = 10;
+    bool pock_chambolle = true;
+    int verbose = 1;
+    double log_interval = 1.0;   // seconds between progress lines
+    double infeas_tol = 1e-8;    // relative tolerance for infeasibility certificates
+    Precision precision = Precision::Double;
+    int stall_checks = 12;       // checks without 20% progress before leaving L1
+    int stall_checks_l2 = 24;    // same for L2
+    // extensions
+    Values values = Values::F32;  // matrix values in L1/L2
+    int device = 0;               // HIP device ordinal
+};
+
+enum class Status { Optimal, PrimalInfeasible, IterationLimit, TimeLimit, NumericalError };
+const char* to_string(Status s);
+
+struct Result {
+    Status status = Status::NumericalError;
+    std::vector<double> x;  // primal solution (original space)
+    std::vector<double> y;  // row duals (original space, reduced cost = c - A^T y)
+    double pobj = 0, dobj = 0;
+    double rel_primal = 0, rel_dual = 0, rel_gap = 0;
+    long iterations = 0;
+    long fp32_iterations = 0;
+    long l2_iterations = 0;
+    int restarts = 0;
+    double setup_seconds = 0, solve_seconds = 0;
+    std::string device;
+    // extensions
+    double norm_a = 0;
+    double upload_seconds = 0;        // device upload of the scaled problem (inside solve_seconds)
+    double iterate_seconds[3] = {0};  // wall time in L1 / L2 / L3 iteration blocks + checks
+    long checks = 0;
+};
+
+// Runs restarted PDHG on the MI355X through libganit_hip.so; throws std::runtime_error on HIP errors.
+Result solve_pdhg_gpu(const LP& lp, const Options& opt);
+
+}  // namespace ganit

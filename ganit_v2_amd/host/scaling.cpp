@@ -1,0 +1,183 @@
+// Ruiz + Pock-Chambolle scaling, bound/objective rescaling and norm estimates (reference src/scaling.cpp).
+// Every floating-point operation happens in the reference's order; parallel loops only split work whose
+// per-element arithmetic is sequential (a row's or a column's own sum in ascending index order, or a max).
+#include "scaling.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+namespace ganit {
+
+double norm2(const std::vector<double>& v) {
+    double s = 0.0;
+    for (double x : v) s += x * x;
+    return std::sqrt(s);
+}
+
+double finite_bound_norm(const std::vector<double>& lo, const std::vector<double>& hi) {
+    double s = 0.0;
+    for (size_t i = 0; i < lo.size(); ++i) {
+        const double a = std::isfinite(lo[i]) ? std::fabs(lo[i]) : 0.0;
+        const double b = std::isfinite(hi[i]) ? std::fabs(hi[i]) : 0.0;
+        const double v = std::max(a, b);
+        s += v * v;
+    }
+    return std::sqrt(s);
+}
+
+namespace {
+
+void inv_sqrt_or_one(std::vector<double>& v) {
+#pragma omp parallel for schedule(static)
+    for (long k = 0; k < (long)v.size(); ++k) v[k] = v[k] > 0 ? 1.0 / std::sqrt(v[k]) : 1.0;
+}
+
+// A <- diag(rs) A diag(cs); R *= rs; C *= cs   (reference scaling.cpp:27-31)
+void apply(Csr& a, const std::vector<double>& rs, const std::vector<double>& cs, std::vector<double>& R,
+           std::vector<double>& C) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int i = 0; i < a.rows; ++i)
+        for (int p = a.ptr[i]; p < a.ptr[i + 1]; ++p) a.val[p] *= rs[i] * cs[a.idx[p]];
+    for (int i = 0; i < a.rows; ++i) R[i] *= rs[i];
+    for (int j = 0; j < a.cols; ++j) C[j] *= cs[j];
+}
+
+// CSR SpMV with each row summed sequentially (the reference's host loops).
+void spmv_seq_rows(const Csr& a, const std::vector<double>& x, std::vector<double>& y) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int i = 0; i < a.rows; ++i) {
+        double t = 0.0;
+        for (int p = a.ptr[i]; p < a.ptr[i + 1]; ++p) t += a.val[p] * x[a.idx[p]];
+        y[i] = t;
+    }
+}
+
+}  // namespace
+
+ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_obj_rescale) {
+    ScaledLP s;
+    s.m = lp.m;
+    s.n = lp.n;
+    s.A = lp.A;
+    s.R.assign(lp.m, 1.0);
+    s.C.assign(lp.n, 1.0);
+    s.maximize = lp.maximize;
+    const double sign = lp.maximize ? -1.0 : 1.0;
+
+    // Column view of A's pattern: entry q of column j is A entry perm[q]; rows ascending within a column.
+    std::vector<int> cptr(lp.n + 1, 0), perm(s.A.val.size());
+    for (int j : s.A.idx) ++cptr[j + 1];
+    for (int j = 0; j < lp.n; ++j) cptr[j + 1] += cptr[j];
+    {
+        std::vector<int> at(cptr.begin(), cptr.end() - 1);
+        for (int i = 0; i < lp.m; ++i)
+            for (int p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) perm[at[s.A.idx[p]]++] = p;
+    }
+
+    std::vector<double> rs(lp.m), cs(lp.n);
+    for (int it = 0; it < ruiz_iters; ++it) {  // inf-norm Ruiz (reference scaling.cpp:45-57)
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int i = 0; i < lp.m; ++i) {
+            double mx = 0.0;
+            for (int p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) mx = std::max(mx, std::fabs(s.A.val[p]));
+            rs[i] = mx;
+        }
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int j = 0; j < lp.n; ++j) {
+            double mx = 0.0;
+            for (int q = cptr[j]; q < cptr[j + 1]; ++q) mx = std::max(mx, std::fabs(s.A.val[perm[q]]));
+            cs[j] = mx;
+        }
+        inv_sqrt_or_one(rs);
+        inv_sqrt_or_one(cs);
+        apply(s.A, rs, cs, s.R, s.C);
+    }
+    if (pock_chambolle) {  // alpha = 1: row and column 1-norms (reference scaling.cpp:58-70)
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int i = 0; i < lp.m; ++i) {
+            double t = 0.0;
+            for (int p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) t += std::fabs(s.A.val[p]);
+            rs[i] = t;
+        }
+#pragma omp parallel for schedule(dynamic, 1024)
+        for (int j = 0; j < lp.n; ++j) {  // ascending row order, as the reference's row-major accumulation
+            double t = 0.0;
+            for (int q = cptr[j]; q < cptr[j + 1]; ++q) t += std::fabs(s.A.val[perm[q]]);
+            cs[j] = t;
+        }
+        inv_sqrt_or_one(rs);
+        inv_sqrt_or_one(cs);
+        apply(s.A, rs, cs, s.R, s.C);
+    }
+    s.AT = transpose(s.A);
+
+    s.c.resize(lp.n);
+    s.l.resize(lp.n);
+    s.u.resize(lp.n);
+    for (int j = 0; j < lp.n; ++j) {
+        s.c[j] = sign * lp.c[j] * s.C[j];
+        s.l[j] = lp.l[j] / s.C[j];
+        s.u[j] = lp.u[j] / s.C[j];
+    }
+    s.lo.resize(lp.m);
+    s.hi.resize(lp.m);
+    for (int i = 0; i < lp.m; ++i) {
+        s.lo[i] = lp.lo[i] * s.R[i];
+        s.hi[i] = lp.hi[i] * s.R[i];
+    }
+    if (bound_obj_rescale) {
+        s.bscale = finite_bound_norm(s.lo, s.hi) + 1.0;
+        s.cscale = norm2(s.c) + 1.0;
+        for (auto& v : s.c) v /= s.cscale;
+        for (auto& v : s.l) v /= s.bscale;
+        for (auto& v : s.u) v /= s.bscale;
+        for (auto& v : s.lo) v /= s.bscale;
+        for (auto& v : s.hi) v /= s.bscale;
+    }
+    s.Rres.resize(lp.m);
+    s.Cres.resize(lp.n);
+    for (int i = 0; i < lp.m; ++i) s.Rres[i] = s.R[i] / s.bscale;
+    for (int j = 0; j < lp.n; ++j) s.Cres[j] = s.C[j] / s.cscale;
+    s.obj_const = sign * lp.obj_const;
+    s.Q = empty_csr(lp.n, lp.n);
+    if (lp.has_q()) {
+        s.Q = lp.Q;
+        for (int i = 0; i < lp.n; ++i)
+            for (int p = s.Q.ptr[i]; p < s.Q.ptr[i + 1]; ++p)
+                s.Q.val[p] *= sign * s.C[i] * s.C[s.Q.idx[p]] * s.bscale / s.cscale;
+        std::vector<double> v(lp.n, 1.0), w(lp.n);
+        const double nv = norm2(v);
+        for (auto& t : v) t /= nv;
+        double lam = 0.0;
+        for (int it = 0; it < 60; ++it) {
+            spmv_seq_rows(s.Q, v, w);
+            const double nw = norm2(w);
+            if (nw == 0.0) break;
+            lam = nw;
+            for (int i = 0; i < lp.n; ++i) v[i] = w[i] / nw;
+        }
+        s.normQ = lam * 1.01;
+    }
+    s.bnorm = finite_bound_norm(lp.lo, lp.hi);
+    s.cnorm = norm2(lp.c);
+    return s;
+}
+
+double estimate_norm(const ScaledLP& s, int iters) {
+    if (s.A.nnz() == 0) return 1.0;
+    std::vector<double> v(s.n, 1.0), w(s.m), z(s.n);
+    const double nv = norm2(v);
+    for (auto& x : v) x /= nv;
+    double lambda = 0.0;
+    for (int it = 0; it < iters; ++it) {
+        spmv_seq_rows(s.A, v, w);
+        spmv_seq_rows(s.AT, w, z);
+        const double nz = norm2(z);
+        if (nz == 0.0) break;
+        lambda = nz;  // ||v|| = 1, so ||A'^T A' v|| -> sigma_max^2
+        for (int j = 0; j < s.n; ++j) v[j] = z[j] / nz;
+    }
+    return std::sqrt(lambda) * 1.01;
+}
+
+}  // namespace ganit

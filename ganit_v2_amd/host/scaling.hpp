@@ -1,0 +1,33 @@
+// Problem scaling for PDHG (mirror of the reference's src/scaling.hpp:8-23 and scaling.cpp:8-155).
+//
+// Scaled problem: A' = diag(R) A diag(C), x = C x' * bscale, y = R y' * cscale; always a minimisation.
+// The arithmetic is the reference's, operation for operation, so the scaled problem is bit-identical to
+// the reference's; the loops that can run in parallel without changing any rounding (row / column
+// maxima, per-row and per-column sums in their sequential order, row-wise SpMV) use OpenMP.
+#pragma once
+#include "lp.hpp"
+
+namespace ganit {
+
+struct ScaledLP {
+    int m = 0, n = 0;
+    Csr A, AT;
+    std::vector<double> c, l, u, lo, hi;  // scaled data
+    std::vector<double> R, C;             // row / column scaling factors
+    double bscale = 1.0, cscale = 1.0;    // uniform bound / objective rescaling
+    std::vector<double> Rres, Cres;       // residual unscaling: R / bscale, C / cscale
+    Csr Q;                                // sign * C Q C * bscale / cscale
+    double normQ = 0.0;                   // ||Q'||_2 estimate (60 power steps, x1.01)
+    double obj_const = 0.0;
+    double bnorm = 0.0, cnorm = 0.0;      // norms of the ORIGINAL rhs and objective
+    bool maximize = false;
+    bool has_q() const { return Q.nnz() > 0; }
+};
+
+ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_obj_rescale = true);
+// ||A'||_2 by power iteration on A'^T A' (50 steps, x1.01).
+double estimate_norm(const ScaledLP& s, int iters = 50);
+double norm2(const std::vector<double>& v);
+double finite_bound_norm(const std::vector<double>& lo, const std::vector<double>& hi);
+
+}  // namespace ganit

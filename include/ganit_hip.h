@@ -1,0 +1,169 @@
+/* ganit_hip.h -- C ABI of libganit_hip.so, the MI355X (gfx950) restarted-PDHG engine.
+ *
+ * This is the drop-in boundary for the reference's GPU backend.  The reference drives a compile-time
+ * backend concept from its driver (src/pdhg_impl.hpp:67-68, 201-202, 243-244); its CUDA implementation is
+ * src/backend_cuda.cu:286-378.  Every entry point below either replaces one member of that concept (the
+ * citation says which) or is a coarse, fused operation used by this library's own driver.
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a nonzero ganit_status otherwise; no exception crosses the
+ *     ABI.  ganit_hip_last_error() returns the calling thread's last message (the reference throws
+ *     std::runtime_error from GANIT_CUDA_CHECK instead, src/backend_cuda.cu:22-29).
+ *   - Device vectors are plain device pointers; their element type is given by a ganit_dtype argument.
+ *     Host arrays are caller-owned; the library never frees caller memory.
+ *   - All work of a context is ordered on the context's own HIP stream.  Functions that return host
+ *     scalars synchronise that stream; the others are asynchronous.
+ *   - Reductions are deterministic: the same inputs give the same bits on every call.
+ */
+#ifndef GANIT_HIP_H
+#define GANIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Element types.  Matrix values may be F64, F32 or BF16; vectors F64 or F32. */
+enum ganit_dtype { GANIT_F64 = 0, GANIT_F32 = 1, GANIT_BF16 = 2 };
+
+enum ganit_status {
+    GANIT_OK = 0,
+    GANIT_EINVAL = 1,    /* bad argument (null pointer, size mismatch, unsupported dtype) */
+    GANIT_EHIP = 2,      /* a HIP runtime call failed */
+    GANIT_ENOMEM = 3,    /* device allocation failed */
+    GANIT_EINTERNAL = 4  /* anything else */
+};
+
+typedef struct ganit_hip_ctx ganit_hip_ctx;   /* device + stream + reduction scratch */
+typedef struct ganit_hip_csr ganit_hip_csr;   /* a CSR matrix resident on the device */
+typedef struct ganit_hip_pdhg ganit_hip_pdhg; /* a scaled LP/QP resident on the device + PDHG state */
+
+/* ------------------------------------------------------------------ errors / version */
+const char* ganit_hip_last_error(void);
+int ganit_hip_version(void);
+
+/* ------------------------------------------------------------------ context
+ * Replaces the reference's implicit CUDA device/default stream (src/backend_cuda.cu:299-302 reads the
+ * SM count for its reduction grid; here the context owns the stream and scratch instead). */
+int ganit_hip_ctx_create(int device, ganit_hip_ctx** out);
+int ganit_hip_ctx_destroy(ganit_hip_ctx* ctx);
+int ganit_hip_ctx_sync(ganit_hip_ctx* ctx);
+int ganit_hip_ctx_stream(ganit_hip_ctx* ctx, void** hip_stream);
+/* Device name and compute-unit count (the reference's solve_pdhg_gpu reads cudaDeviceProp,
+ * src/backend_cuda.cu:383-388). */
+int ganit_hip_ctx_device_info(ganit_hip_ctx* ctx, char* name, int name_len, int* compute_units);
+
+/* ------------------------------------------------------------------ device vectors
+ * Replace Vec = DevBuf<V> and its helpers (src/backend_cuda.cu:44-74, 305-322, 345-350). */
+int ganit_hip_alloc(ganit_hip_ctx* ctx, size_t bytes, void** dptr);
+int ganit_hip_free(ganit_hip_ctx* ctx, void* dptr);
+/* from_host: narrow a host double array to dtype vdt on the device (src/backend_cuda.cu:306 upload_as). */
+int ganit_hip_upload(ganit_hip_ctx* ctx, int vdt, const double* host, int64_t n, void* dst);
+/* to_host: widen to double and copy back; synchronises (src/backend_cuda.cu:307-315). */
+int ganit_hip_download(ganit_hip_ctx* ctx, int vdt, const void* src, int64_t n, double* host);
+/* fill (src/backend_cuda.cu:320-322), copy (:316-319), sub (:345-347), scale_into (:348-350). */
+int ganit_hip_vec_fill(ganit_hip_ctx* ctx, int vdt, void* dst, int64_t n, double a);
+int ganit_hip_vec_copy(ganit_hip_ctx* ctx, int vdt, const void* src, void* dst, int64_t n);
+int ganit_hip_vec_sub(ganit_hip_ctx* ctx, int vdt, const void* a, const void* b, void* d, int64_t n);
+int ganit_hip_vec_scale_into(ganit_hip_ctx* ctx, int vdt, const void* src, double a, void* dst, int64_t n);
+/* Element-type conversion on the device (src/backend_cuda.cu:154-163 k_convert / k_to_double). */
+int ganit_hip_vec_convert(ganit_hip_ctx* ctx, int src_dt, const void* src, int dst_dt, void* dst, int64_t n);
+
+/* ------------------------------------------------------------------ CSR matrices
+ * Replaces DevCsr / to_device (src/backend_cuda.cu:252-284).  Indices are uploaded once; values are kept
+ * resident in every dtype whose bit (1 << dtype) is set in dtype_mask, so precision levels share one
+ * upload.  The row-length histogram picks the lanes-per-row variant (the reference's rule is a single
+ * avg-row-length > 12 switch between thread- and 32-lane-warp-per-row, :281-282). */
+int ganit_hip_csr_upload(ganit_hip_ctx* ctx, int rows, int cols, const int* ptr, const int* idx,
+                         const double* val, unsigned dtype_mask, ganit_hip_csr** out);
+int ganit_hip_csr_free(ganit_hip_csr* a);
+/* lanes_per_row: the variant in use (1..64); hist[k] (k = 0..7) counts rows with length in
+ * [0,1], (1,2], (2,4], ... , (64,128], and hist[8] rows longer than 128. */
+int ganit_hip_csr_plan(const ganit_hip_csr* a, int* lanes_per_row, int64_t* hist9);
+/* Override the variant (power of two in 1..64); used by the autotuner and by tests. */
+int ganit_hip_csr_set_lanes(ganit_hip_csr* a, int lanes_per_row);
+
+/* ------------------------------------------------------------------ backend-concept kernels
+ * spmv_A / spmv_AT / spmv_Q (src/backend_cuda.cu:324-335): y = a * x, accumulated in vdt. */
+int ganit_hip_spmv(ganit_hip_ctx* ctx, const ganit_hip_csr* a, int mdt, int vdt, const void* x, void* y);
+/* primal_step (src/backend_cuda.cu:337-340, kernel :102-117); qx may be NULL (LP). */
+int ganit_hip_primal_step(ganit_hip_ctx* ctx, int vdt, int n, const void* x, const void* aty, const void* qx,
+                          const void* c, const void* l, const void* u, double tau, void* xn, void* xbar,
+                          void* sumx);
+/* dual_step (src/backend_cuda.cu:341-344, kernel :119-133). */
+int ganit_hip_dual_step(ganit_hip_ctx* ctx, int vdt, int m, const void* y, const void* axbar, const void* lo,
+                        const void* hi, double sigma, void* yn, void* sumy);
+/* FP64-accumulated reductions (src/backend_cuda.cu:363-377 with functors :199-249); synchronise. */
+int ganit_hip_primal_res2(ganit_hip_ctx* ctx, int vdt, int m, const void* ax, const void* lo, const void* hi,
+                          const void* rres, double* out);
+int ganit_hip_dual_res(ganit_hip_ctx* ctx, int vdt, int n, const void* aty, const void* qx, const void* c,
+                       const void* l, const void* u, const void* cres, int with_c, double* rd2, double* vobj);
+int ganit_hip_row_dual_obj(ganit_hip_ctx* ctx, int vdt, int m, const void* y, const void* lo, const void* hi,
+                           double* out);
+int ganit_hip_dot(ganit_hip_ctx* ctx, int vdt, int64_t n, const void* a, const void* b, double* out);
+int ganit_hip_diff2(ganit_hip_ctx* ctx, int vdt, int64_t n, const void* a, const void* b, double* out);
+
+/* ------------------------------------------------------------------ device-resident PDHG
+ * The fused path behind this library's own driver.  One ganit_hip_pdhg holds the scaled problem
+ * (ScaledLP, src/scaling.hpp:8-23) uploaded ONCE, with matrix values in every dtype of mdt_mask and
+ * problem vectors in every dtype of vdt_mask, plus the iterate working set per vector dtype.  It replaces
+ * the per-phase backend rebuild and per-evaluation re-upload of the reference (src/pdhg_impl.hpp:73, 207). */
+typedef struct {
+    int m, n;
+    const int* a_ptr; const int* a_idx; const double* a_val;     /* scaled A, CSR m x n      */
+    const int* at_ptr; const int* at_idx; const double* at_val;  /* its stored transpose     */
+    const int* q_ptr; const int* q_idx; const double* q_val;     /* scaled Q (n x n) or NULL */
+    const double *c, *l, *u, *lo, *hi, *rres, *cres;            /* scaled vectors           */
+} ganit_hip_lp_desc;
+
+int ganit_hip_pdhg_create(ganit_hip_ctx* ctx, const ganit_hip_lp_desc* lp, unsigned mdt_mask, unsigned vdt_mask,
+                          ganit_hip_pdhg** out);
+int ganit_hip_pdhg_destroy(ganit_hip_pdhg* pd);
+/* The matrices' variant plans (A, AT, Q); lanes may be overridden per matrix (0 = A, 1 = AT, 2 = Q). */
+int ganit_hip_pdhg_plan(const ganit_hip_pdhg* pd, int which, int* lanes_per_row, int64_t* hist9);
+int ganit_hip_pdhg_set_lanes(ganit_hip_pdhg* pd, int which, int lanes_per_row);
+
+/* Load an iterate (x, y) and restart point (xr, yr) from the host into the vdt working set and clear the
+ * running sums (the vector set-up of pdhg_phase, src/pdhg_impl.hpp:78-84).  xr / yr may be NULL (= x, y). */
+int ganit_hip_pdhg_load(ganit_hip_pdhg* pd, int vdt, const double* x, const double* y, const double* xr,
+                        const double* yr);
+/* Device-side hand-over between precision levels: copy (x, y, xr, yr) of working set from_vdt into
+ * working set to_vdt with conversion, and clear to_vdt's running sums (replaces to_host / from_host of
+ * PdhgState across phases, src/pdhg_impl.hpp:191-197, 78-82). */
+int ganit_hip_pdhg_handover(ganit_hip_pdhg* pd, int from_vdt, int to_vdt);
+/* Run `iters` fused PDHG iterations (src/pdhg_impl.hpp:130-141) in matrix dtype mdt and vector dtype vdt
+ * with step sizes tau, sigma.  Each iteration is two kernels: [AT y (+ Q x) -> primal prox] and
+ * [A xbar -> dual prox].  Blocks of the same length are replayed from a captured HIP graph. */
+int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, double tau, double sigma);
+
+/* Check flags and the layout of the sums ganit_hip_pdhg_check returns (all FP64-accumulated). */
+#define GANIT_CHECK_AVG 1  /* also form (xa, ya) = (sumx, sumy) * inv_k and evaluate them          */
+#define GANIT_CHECK_RAY 2  /* also the infeasibility ray y - yr (src/pdhg_impl.hpp:110-118)        */
+#define GANIT_CHECK_NOUT 19
+/* out[ 0.. 3]  current point : rd2, vobj, c'x, x'Qx      (dual_res, dot_c, dot with Qx)
+ * out[ 4.. 7]  average point : rd2, vobj, c'x, x'Qx
+ * out[ 8.. 9]  ray           : rd2, vobj with cw = 0 and no Qx
+ * out[10..11]  ||x - xr||^2, ||xa - xr||^2
+ * out[12..13]  current point : primal_res2, row_dual_obj
+ * out[14..15]  average point : primal_res2, row_dual_obj
+ * out[16]      ray           : row_dual_obj(y - yr)
+ * out[17..18]  ||y - yr||^2, ||ya - yr||^2
+ * Entries whose flag is off are returned as 0.  Replaces the 8-12 separate reductions and host syncs of
+ * one check (src/pdhg_impl.hpp:92-118, 144-147, 179) with three kernels and one synchronisation. */
+int ganit_hip_pdhg_check(ganit_hip_pdhg* pd, int mdt, int vdt, int flags, double inv_k, double* out);
+/* Restart (src/pdhg_impl.hpp:177-188): optionally adopt the average, then (xr, yr) = (x, y) and clear the
+ * running sums. */
+int ganit_hip_pdhg_restart(ganit_hip_pdhg* pd, int vdt, int adopt_average);
+/* Make the average the current iterate without restarting (the best_is_avg hand-back at the end of a
+ * phase, src/pdhg_impl.hpp:191-192). */
+int ganit_hip_pdhg_adopt_average(ganit_hip_pdhg* pd, int vdt);
+/* Copy the current iterate (x, y) and restart point (xr, yr) of working set vdt to the host as doubles;
+ * any pointer may be NULL.  Synchronises. */
+int ganit_hip_pdhg_store(ganit_hip_pdhg* pd, int vdt, double* x, double* y, double* xr, double* yr);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GANIT_HIP_H */

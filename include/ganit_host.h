@@ -1,0 +1,67 @@
+/* ganit_host.h -- C ABI of libganit.so, the host side of the MI355X PDHG engine: problem I/O, scaling,
+ * workload generators and the GPU solve.  This is what a ctypes / cgo / N-API binding calls; the C++
+ * interface it wraps mirrors the reference's public API (include/ganit/pdhg.hpp:10-47 and
+ * include/ganit/lp.hpp:19-50).
+ *
+ * Errors: functions returning int give 0 on success; functions returning a handle give NULL on failure.
+ * ganit_last_error() then holds the calling thread's message.
+ */
+#ifndef GANIT_HOST_H
+#define GANIT_HOST_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ganit_lp ganit_lp; /* an LP/QP in original (unscaled) space: the reference's ganit::LP */
+
+const char* ganit_last_error(void);
+
+/* read_mps (reference src/mps.cpp:69-272): free MPS / QPS, .gz through gzip. */
+ganit_lp* ganit_lp_read_mps(const char* path);
+/* csr_from_triplets (reference src/lp.cpp:9-53) + the LP fields (reference include/ganit/lp.hpp:34-47).
+ * Q triplets may be absent (qnnz = 0). */
+ganit_lp* ganit_lp_from_triplets(int m, int n, int64_t nnz, const int* rows, const int* cols, const double* vals,
+                                 const double* c, const double* l, const double* u, const double* lo,
+                                 const double* hi, double obj_const, int maximize, int64_t qnnz, const int* qrows,
+                                 const int* qcols, const double* qvals);
+/* Seeded workloads (no reference counterpart; the reference ships Python generators instead):
+ *   kind 0 random LP        p = [m, n, m_eq, nnz_min, nnz_max, tighten_frac]
+ *   kind 1 blending LP      p = [crudes, products, specs, periods]
+ *   kind 2 portfolio QP     p = [assets, factors, gamma]
+ *   kind 3 differentiable QP p = [n, m_eq, m_le, density, qdensity]                                    */
+ganit_lp* ganit_lp_generate(int kind, const double* p, uint64_t seed);
+void ganit_lp_free(ganit_lp* lp);
+/* dims[0..4] = m, n, nnz(A), nnz(Q), maximize */
+void ganit_lp_dims(const ganit_lp* lp, int64_t* dims);
+/* Copy out the original data; any pointer may be NULL.  Sizes from ganit_lp_dims. */
+void ganit_lp_export(const ganit_lp* lp, int* aptr, int* aidx, double* aval, int* qptr, int* qidx, double* qval,
+                     double* c, double* l, double* u, double* lo, double* hi, double* obj_const);
+/* The planted point of a generated instance (random LP: x0, y0; differentiable QP: x0).  Returns nonzero
+ * when the instance has none. */
+int ganit_lp_planted(const ganit_lp* lp, double* x0, double* y0);
+int ganit_lp_write_mps(const ganit_lp* lp, const char* path);
+
+/* scale_lp + estimate_norm (reference src/scaling.cpp:36-155).
+ * scal[0..8] = bscale, cscale, normQ, obj_const, bnorm, cnorm, normA, nnz(AT), maximize; any array pointer
+ * may be NULL (CSR of A, AT, Q; then c, l, u, lo, hi, R, C, Rres, Cres). */
+int ganit_lp_scaled(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, double* scal, int* aptr, int* aidx,
+                    double* aval, int* tptr, int* tidx, double* tval, int* qptr, int* qidx, double* qval, double* c,
+                    double* l, double* u, double* lo, double* hi, double* R, double* C, double* Rres, double* Cres);
+
+/* solve_pdhg_gpu (reference include/ganit/pdhg.hpp:45-47, src/backend_cuda.cu:382-393).
+ * opts[0..12] = tol, max_iter, time_limit, check_every, ruiz_iters, pock_chambolle, verbose, infeas_tol,
+ *               precision (0 double, 1 mixed, 2 dynamic), stall_checks, stall_checks_l2,
+ *               values (0 FP32, 1 BF16 matrix values in L1/L2), device
+ * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical),
+ *               pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,
+ *               setup_s, solve_s, normA, upload_s, L1_s, L2_s, L3_s, checks
+ * x (n) and y (m) receive the original-space solution; either may be NULL. */
+int ganit_solve_gpu(const ganit_lp* lp, const double* opts, double* out, double* x, double* y);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GANIT_HOST_H */

@@ -1,0 +1,197 @@
+"""Host library (libganit.so): reader, writer, scaling and generators, checked against the oracle's
+restatement of the reference and against closed forms.  CPU only."""
+import os
+
+import numpy as np
+import pytest
+
+import ganit_v2_amd as g
+from conftest import instance_path
+
+ALL = ["afiro", "adlittle", "bgetam", "etamacro", "stair", "shell", "25fv47", "greenbea", "port_tiny",
+       "port_small", "qblend_small"]
+
+
+def _same_lp(p: g.Problem, o) -> None:
+    a = p.export()
+    b = o.export()
+    assert np.array_equal(a["A"].ptr, b["aptr"]) and np.array_equal(a["A"].idx, b["aidx"])
+    assert np.array_equal(a["A"].val, b["aval"])
+    if a["Q"].nnz:
+        assert np.array_equal(a["Q"].ptr, b["qptr"]) and np.array_equal(a["Q"].idx, b["qidx"])
+        assert np.array_equal(a["Q"].val, b["qval"])
+    else:
+        assert len(b["qval"]) == 0
+    for k in ("c", "l", "u", "lo", "hi"):
+        assert np.array_equal(a[k], b[k]), k
+    assert a["obj_const"] == b["obj_const"]
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_reader_matches_oracle(oracle, name):
+    _same_lp(g.Problem.read_mps(instance_path(name)), oracle.OracleLP.read_mps(instance_path(name)))
+
+
+@pytest.mark.parametrize("name", ALL)
+def test_scaling_bitwise_equal_to_oracle(oracle, name):
+    s = g.Problem.read_mps(instance_path(name)).scaled()
+    o = oracle.OracleLP.read_mps(instance_path(name)).scaled()
+    for a, b in [("A", "a"), ("AT", "t"), ("Q", "q")]:
+        assert np.array_equal(s[a].ptr, o[b + "ptr"]) and np.array_equal(s[a].idx, o[b + "idx"])
+        assert np.array_equal(s[a].val, o[b + "val"]), a
+    for k in ("c", "l", "u", "lo", "hi", "R", "C", "Rres", "Cres"):
+        assert np.array_equal(s[k], o[k]), k
+    for k in ("bscale", "cscale", "normQ", "obj_const", "bnorm", "cnorm", "normA"):
+        assert s[k] == o[k], k
+
+
+@pytest.mark.parametrize("name", ["afiro", "bgetam", "port_tiny", "qblend_small", "etamacro"])
+def test_mps_write_read_round_trip(tmp_path, name):
+    p = g.Problem.read_mps(instance_path(name))
+    f = str(tmp_path / "rt.mps")
+    p.write_mps(f)
+    q = g.Problem.read_mps(f)
+    a, b = p.export(), q.export()
+    for k in ("c", "l", "u", "lo", "hi"):
+        assert np.array_equal(a[k], b[k]), k
+    for M in ("A", "Q"):
+        assert np.array_equal(a[M].ptr, b[M].ptr) and np.array_equal(a[M].idx, b[M].idx)
+        assert np.array_equal(a[M].val, b[M].val)
+    assert a["obj_const"] == b["obj_const"] and a["maximize"] == b["maximize"]
+
+
+def test_reader_edge_cases(tmp_path, oracle):
+    """RANGES on E/L/G rows, negative UP on a default lower bound, objective constant, extra N rows,
+    MARKER lines, OBJSENSE MAX, duplicate entries and explicit zeros -- reader vs oracle restatement."""
+    text = """NAME EDGE
+OBJSENSE
+    MAX
+ROWS
+ N COST
+ N OTHER
+ E R1
+ L R2
+ G R3
+ E R4
+COLUMNS
+    X1 COST 1 R1 1
+    X1 R2 2 OTHER 5
+    MARKER 'MARKER' 'INTORG'
+    X2 COST -1 R1 0
+    X2 R3 1 R2 1
+    X2 R2 1
+    MARKER 'MARKER' 'INTEND'
+    X3 R4 1 R3 -1
+RHS
+    RHS COST 3.5 R1 1
+    RHS R2 4 R3 -2
+    RHS R4 2
+RANGES
+    RNG R1 2 R2 3
+    RNG R3 -1 R4 -4
+BOUNDS
+ UP BND X1 -3
+ LO BND X2 -1
+ UP BND X2 -0.5
+ FR BND X3
+ENDATA
+"""
+    f = tmp_path / "edge.mps"
+    f.write_text(text)
+    p = g.Problem.read_mps(str(f))
+    _same_lp(p, oracle.OracleLP.read_mps(str(f)))
+    e = p.export()
+    assert e["maximize"] and e["obj_const"] == -3.5
+    assert e["l"][0] == -np.inf and e["u"][0] == -3          # negative UP frees the default lower bound
+    assert e["l"][1] == -1 and e["u"][1] == -0.5            # explicit LO is kept
+    assert e["l"][2] == -np.inf and e["u"][2] == np.inf
+    assert list(e["lo"]) == [1, 1, -2, -2] and list(e["hi"]) == [3, 4, -1, 2]
+    A = e["A"].to_dense()
+    assert A[1, 1] == 2.0                                   # duplicates summed
+    assert A[0, 1] == 0.0 and e["A"].nnz == 6               # explicit zero dropped
+
+
+def test_random_lp_generator_planted_point():
+    p = g.Problem.generate("random_lp", seed=3)
+    e = p.export()
+    x0, y0 = p.planted()
+    A = e["A"]
+    assert (A.rows, A.cols, A.nnz) == (2000, 5000, 50000)
+    cols = np.bincount(A.idx, minlength=5000)
+    assert np.all(cols == 10)                               # 10 distinct rows per column
+    ax = A.matvec(x0)
+    assert np.allclose(ax[:1000], e["lo"][:1000], rtol=0, atol=1e-9)
+    assert np.array_equal(e["lo"][:1000], e["hi"][:1000])
+    slack = e["hi"][1000:] - ax[1000:]
+    assert np.all(slack >= -1e-9) and np.all(slack <= 1 + 1e-9)
+    assert np.all(np.isneginf(e["lo"][1000:]))
+    assert np.all((x0 >= 0) & (x0 <= 10)) and abs(np.mean(x0 > 0) - 0.7) < 0.03
+    assert np.all(y0[1000:] <= 0)
+    # c = A^T y0 + s0 with s0 >= 0 only where x0 == 0
+    s0 = e["c"] - np.bincount(A.idx, weights=A.val * y0[np.repeat(np.arange(2000), np.diff(A.ptr))], minlength=5000)
+    assert np.all(s0 > -1e-9) and np.all(np.abs(s0[x0 > 0]) < 1e-9)
+    # weak duality bracket: dual objective of (y0, s0) <= c'x0
+    dual = y0[:1000] @ e["lo"][:1000] + y0[1000:] @ e["hi"][1000:] + s0 @ e["l"]
+    assert dual <= e["c"] @ x0 + 1e-6
+
+
+def test_generators_are_deterministic():
+    a = g.Problem.generate("random_lp", seed=7, m=300, n=700, m_eq=150).export()
+    b = g.Problem.generate("random_lp", seed=7, m=300, n=700, m_eq=150).export()
+    c = g.Problem.generate("random_lp", seed=8, m=300, n=700, m_eq=150).export()
+    assert np.array_equal(a["A"].val, b["A"].val) and np.array_equal(a["c"], b["c"])
+    assert not np.array_equal(a["A"].val, c["A"].val)
+
+
+def test_random_lp_tightening_and_ragged_columns():
+    p = g.Problem.generate("random_lp", seed=11, m=400, n=1500, m_eq=200, nnz_min=5, nnz_max=15,
+                           tighten_frac=0.05)
+    e = p.export()
+    x0, _ = p.planted()
+    k = np.bincount(e["A"].idx, minlength=1500)
+    assert k.min() >= 5 and k.max() <= 15
+    tight = (e["u"] < 10) | (e["l"] > 0)
+    assert 0.02 < tight.mean() < 0.09
+    assert np.all(e["u"][e["u"] < 10] == np.floor(x0[e["u"] < 10]))
+    assert np.all(e["l"][e["l"] > 0] == np.ceil(x0[e["l"] > 0]))
+
+
+def test_blending_generator_structure():
+    p = g.Problem.generate("blending", crudes=6, products=3, specs=5, periods=4, seed=1)
+    d = p.dims()
+    C, P, K, T = 6, 3, 5, 4
+    assert d["n"] == T * (C * P + 2 * C + P)
+    assert d["m"] == T * (P + 2 * C + P * K + 1)
+    # per x variable: demand, carry-over, capacity and K specs, minus the one exact-zero spec coefficient
+    # per (spec, product, period) that the CSR build drops
+    # (r: availability + carry-over; ci and pi: two balance rows except in the last period)
+    full = T * (C * P * (3 + K) + 2 * C + 2 * C + 2 * P) - C - P
+    assert d["nnz"] == full - K * P * T
+
+
+def test_blending_full_size():
+    d = g.Problem.generate("blending", seed=0).dims()
+    assert d["m"] == 365 * 217 and d["n"] == 365 * 1096
+    assert 2_800_000 <= d["nnz"] <= 2_950_000
+
+
+def test_portfolio_generator_structure():
+    p = g.Problem.generate("portfolio", assets=2000, factors=10, seed=0)
+    e = p.export()
+    d = p.dims()
+    assert (d["m"], d["n"], d["nnz"], d["qnnz"]) == (11, 2010, 2000 * 11 + 10, 2010)
+    assert np.all(e["u"][:2000] == 0.01) and np.all(np.isinf(e["l"][2000:]))
+    assert np.all(e["Q"].val > 0)
+
+
+def test_diff_qp_generator_convex():
+    p = g.Problem.generate("diff_qp", n=300, m_eq=40, m_le=30, seed=5)
+    e = p.export()
+    Q = e["Q"].to_dense()
+    assert np.array_equal(Q, Q.T)
+    off = np.abs(Q).sum(1) - np.abs(np.diag(Q))
+    assert np.all(np.diag(Q) > off)                       # strictly diagonally dominant -> positive definite
+    x0, _ = p.planted()
+    ax = e["A"].matvec(x0)
+    assert np.allclose(ax[:40], e["lo"][:40], atol=1e-9)
+    assert np.all(e["hi"][40:] - ax[40:] >= -1e-9)
