@@ -1,0 +1,184 @@
+"""Benchmark: time to 1e-4 relative KKT (FP64-certified) of restarted PDHG on the ~2.9M-nnz blending LP,
+mixed precision (L1 FP32 -> L3 FP64) vs plain FP64, on one MI355X; plus the fused SpMV kernels' HBM GB/s.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cpu-sample-s S]
+
+A step is one full solve from the initial point with the scaled problem already resident in HBM (upload,
+scaling and norm estimation happen before the timed region).  Prints ONE JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+WORKLOAD = dict(crudes=60, products=16, specs=5, periods=365)
+
+
+def kernel_bytes(m, n, nnz, s_m, s_v):
+    """Algorithmic bytes per launch of the two fused iteration kernels.
+    primal: A^T (row pointers + nnz index/value) + gathered y + x, c, l, u, sumx read + xn, xbar, sumx written
+    dual:   A   (row pointers + nnz index/value) + gathered xbar + y, lo, hi, sumy read + yn, sumy written"""
+    primal = 4 * (n + 1) + nnz * (4 + s_m) + m * s_v + 8 * n * s_v
+    dual = 4 * (m + 1) + nnz * (4 + s_m) + n * s_v + 6 * m * s_v
+    return primal, dual
+
+
+def reference_sequence_bytes(m, n, nnz, s_m, s_v):
+    """B_iter of the reference's un-fused kernel sequence (2 SpMV + primal + dual)."""
+    return 2 * nnz * (4 + s_m) + 4 * (m + n + 2) + s_v * (11 * n + 9 * m)
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(path):
+        with open(path) as f:
+            return json.load(f)
+    return None
+
+
+def cpu_baseline(problem, gpu_run, sample_s):
+    """The oracle (CPU restatement of the reference path, C++/OpenMP) on this host: a bounded sample of the
+    same solve, giving seconds per iteration at each level; the estimate is the GPU run's iteration split
+    times those rates."""
+    from oracle import pyoracle
+    threads = os.cpu_count() or 1
+    pyoracle.set_threads(threads)
+    o = pyoracle.OracleLP.from_export(problem.export())
+    rates = {}
+    # L1 rate from a time-limited mixed run (its first phase is L1), L3 rate from a time-limited FP64 run
+    r1 = o.solve(precision="mixed", time_limit=sample_s / 2, stall_checks=10 ** 9)
+    rates["l1"] = r1["solve_s"] / max(r1["iters"], 1)
+    r3 = o.solve(precision="double", time_limit=sample_s / 2)
+    rates["l3"] = r3["solve_s"] / max(r3["iters"], 1)
+    l1 = gpu_run["fp32_iters"]
+    l3 = gpu_run["iters"] - l1
+    est = l1 * rates["l1"] + l3 * rates["l3"]
+    return {
+        "value": est, "unit": "s", "cores": threads, "kind": "port",
+        "sample": (f"oracle mixed run capped at {sample_s / 2:.0f}s ({r1['iters']} L1 iterations) and FP64 run "
+                   f"capped at {sample_s / 2:.0f}s ({r3['iters']} iterations); estimate = GPU iteration split "
+                   f"(L1 {l1}, L3 {l3}) x measured CPU s/iteration (L1 {rates['l1'] * 1e3:.2f} ms, "
+                   f"L3 {rates['l3'] * 1e3:.2f} ms)"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample-s", type=float, default=20.0)
+    ap.add_argument("--reps", type=int, default=50, help="launches per kernel for the HIP-event timing")
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "dynamic", "double"])
+    ap.add_argument("--values", default="f32", choices=["f32", "bf16"])
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (requested --gpus {a.gpus}); a single LP does not shard "
+              "profitably across GPUs and multi-GPU runs are not part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import ganit_v2_amd as g
+
+    t0 = time.time()
+    prob = g.Problem.generate("blending", seed=0, **WORKLOAD)
+    d = prob.dims()
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    print(f"[bench] blending LP m={m} n={n} nnz={nnz} generated in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+
+    def staged(precision):
+        s = prob.solver(precision=precision, values=a.values, tol=1e-4, verbose=0)
+        s.upload()
+        return s
+
+    mixed = staged(a.precision)
+    for _ in range(a.warmup):
+        mixed.run()
+    t = time.perf_counter()
+    runs = [mixed.run(return_solution=(i == a.steps - 1)) for i in range(a.steps)]
+    wall = time.perf_counter() - t
+    last = runs[-1]
+    print(f"[bench] {a.precision}: {[round(r['solve_s'], 4) for r in runs]} s, status {last['status']}, "
+          f"iters {last['iters']} (L1 {last['fp32_iters']})", file=sys.stderr, flush=True)
+
+    fp64 = staged("double")
+    fp64.run()
+    f_runs = [fp64.run(return_solution=(i == a.steps - 1)) for i in range(a.steps)]
+    f_last = f_runs[-1]
+    print(f"[bench] fp64: {[round(r['solve_s'], 4) for r in f_runs]} s, status {f_last['status']}, "
+          f"iters {f_last['iters']}", file=sys.stderr, flush=True)
+    t_mixed = sum(r["solve_s"] for r in runs) / len(runs)
+    t_fp64 = sum(r["solve_s"] for r in f_runs) / len(f_runs)
+    obj_err = abs(last["pobj"] - f_last["pobj"]) / (1 + abs(f_last["pobj"]))
+
+    # per-kernel device times (HIP events on the solver's stream) at L1 and L3
+    s_m_low = 2 if a.values == "bf16" else 4
+    kern = {}
+    for level, (sm, sv, solver) in {"L1": (s_m_low, 4, mixed), "L3": (8, 8, fp64)}.items():
+        if level == "L1" and a.precision == "double":
+            continue
+        ms_p, ms_d = solver.kernel_times(0 if level == "L1" else 2, a.reps)
+        bp, bd = kernel_bytes(m, n, nnz, sm, sv)
+        kern[level] = dict(primal_ms=ms_p, dual_ms=ms_d, primal_gbs=bp / (ms_p * 1e6), dual_gbs=bd / (ms_d * 1e6),
+                           primal_bytes=bp, dual_bytes=bd,
+                           iteration_us=(ms_p + ms_d) * 1e3,
+                           ref_sequence_gbs=reference_sequence_bytes(m, n, nnz, sm, sv) / ((ms_p + ms_d) * 1e6))
+    print(f"[bench] kernels {json.dumps(kern)}", file=sys.stderr, flush=True)
+    lead = kern.get("L1", kern["L3"])
+    dom = "primal" if lead["primal_ms"] >= lead["dual_ms"] else "dual"
+    level_dom = "L1" if "L1" in kern else "L3"
+    traffic = load_traffic()
+    roof = {
+        "bound": "hbm",
+        "kernel": f"k_iter_{dom} ({level_dom})",
+        "achieved": round(lead[f"{dom}_gbs"], 1),
+        "peak": PEAK_HBM_GBS,
+        "unit": "GB/s",
+        "frac": round(lead[f"{dom}_gbs"] / PEAK_HBM_GBS, 4),
+        "traffic": (traffic or {}).get(f"k_iter_{dom}_{level_dom}"),
+    }
+
+    cpu = None
+    if not a.no_cpu:
+        cpu = cpu_baseline(prob, last, a.cpu_sample_s)
+        print(f"[bench] cpu baseline {json.dumps(cpu)}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "time-to-1e-4 rel KKT, mixed vs FP64 PDHG on 2.9M-nnz LP; SpMV HBM GB/s",
+        "value": round(t_mixed, 5),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(wall / a.steps * 1e3, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if a.precision != "double" else "f64",
+        "data": "synthetic (seeded blending LP generator)",
+        "config": {"workload": "blending LP 60 crudes x 16 products x 5 specs x 365 periods, seed 0",
+                   "m": m, "n": n, "nnz": nnz, "precision": a.precision, "values": a.values, "tol": 1e-4,
+                   "parallelism": "single GPU"},
+        "status": last["status"],
+        "iters": last["iters"], "l1_iters": last["fp32_iters"], "restarts": last["restarts"],
+        "fp64_s": round(t_fp64, 5), "fp64_iters": f_last["iters"], "fp64_status": f_last["status"],
+        "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+        "obj": last["pobj"], "obj_fp64": f_last["pobj"], "obj_rel_err_vs_fp64": obj_err,
+        "certified_rel": [last["rel_p"], last["rel_d"], last["rel_gap"]],
+        "kernels": kern,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,12 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_lp_write_mps.argtypes = [_vp, ctypes.c_char_p]
     lib.ganit_lp_scaled.argtypes = [_vp, ctypes.c_int, ctypes.c_int] + [_vp] * 19
     lib.ganit_solve_gpu.argtypes = [_vp, _f64p, _f64p, _vp, _vp]
+    lib.ganit_solver_create.restype = _vp
+    lib.ganit_solver_create.argtypes = [_vp, _f64p]
+    lib.ganit_solver_upload.argtypes = [_vp]
+    lib.ganit_solver_run.argtypes = [_vp, _f64p, _vp, _vp]
+    lib.ganit_solver_kernel_times.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p]
+    lib.ganit_solver_free.argtypes = [_vp]
     _host = lib
     return lib
 
@@ -210,26 +216,71 @@ class Problem:
         return out
 
     # ---- solve
-    def solve_gpu(self, tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10,
-                  pock_chambolle=True, verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12,
-                  stall_checks_l2=24, values="f32", device=0, return_solution=True) -> dict:
+    def solve_gpu(self, return_solution=True, **opts) -> dict:
+        """One solve_pdhg_gpu call (scaling, upload and solve; timing as the reference defines it)."""
         d = self.dims()
-        opts = np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
-                         infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device],
-                        dtype=np.float64)
         out = np.zeros(18)
         x = np.zeros(d["n"]) if return_solution else None
         y = np.zeros(d["m"]) if return_solution else None
-        _check_host(host_lib().ganit_solve_gpu(self._h, opts, out, _ptr(x), _ptr(y)), "solve_gpu")
-        keys = ["status", "pobj", "dobj", "rel_p", "rel_d", "rel_gap", "iters", "fp32_iters", "l2_iters", "restarts",
-                "setup_s", "solve_s", "norm_a", "upload_s", "l1_s", "l2_s", "l3_s", "checks"]
-        r = dict(zip(keys, out.tolist()))
-        r["status"] = STATUS[int(out[0])]
-        for k in ("iters", "fp32_iters", "l2_iters", "restarts", "checks"):
-            r[k] = int(r[k])
-        if return_solution:
-            r["x"], r["y"] = x, y
-        return r
+        _check_host(host_lib().ganit_solve_gpu(self._h, _options(**opts), out, _ptr(x), _ptr(y)), "solve_gpu")
+        return _result(out, x, y)
+
+    def solver(self, **opts) -> "Solver":
+        """A staged solver: set-up now, upload() and run() repeatedly with the problem resident in HBM."""
+        return Solver(self, **opts)
 
 
-__all__ = ["Problem", "CSR", "host_lib", "NativeLibraryMissing", "STATUS", "LIB_DIR", "BIN_DIR"]
+_RESULT_KEYS = ["status", "pobj", "dobj", "rel_p", "rel_d", "rel_gap", "iters", "fp32_iters", "l2_iters",
+                "restarts", "setup_s", "solve_s", "norm_a", "upload_s", "l1_s", "l2_s", "l3_s", "checks"]
+
+
+def _options(tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10, pock_chambolle=True,
+             verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12, stall_checks_l2=24, values="f32",
+             device=0) -> np.ndarray:
+    return np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
+                     infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device],
+                    dtype=np.float64)
+
+
+def _result(out, x, y) -> dict:
+    r = dict(zip(_RESULT_KEYS, out.tolist()))
+    r["status"] = STATUS[int(out[0])]
+    for k in ("iters", "fp32_iters", "l2_iters", "restarts", "checks"):
+        r[k] = int(r[k])
+    if x is not None:
+        r["x"], r["y"] = x, y
+    return r
+
+
+class Solver:
+    def __init__(self, problem: Problem, **opts):
+        self._problem = problem
+        self._dims = problem.dims()
+        h = host_lib().ganit_solver_create(problem._h, _options(**opts))
+        if not h:
+            raise RuntimeError(host_lib().ganit_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _host is not None:
+            _host.ganit_solver_free(h)
+            self._h = None
+
+    def upload(self) -> None:
+        _check_host(host_lib().ganit_solver_upload(self._h), "solver_upload")
+
+    def run(self, return_solution=False) -> dict:
+        out = np.zeros(18)
+        x = np.zeros(self._dims["n"]) if return_solution else None
+        y = np.zeros(self._dims["m"]) if return_solution else None
+        _check_host(host_lib().ganit_solver_run(self._h, out, _ptr(x), _ptr(y)), "solver_run")
+        return _result(out, x, y)
+
+    def kernel_times(self, level: int, reps: int = 50):
+        ms = np.zeros(2)
+        _check_host(host_lib().ganit_solver_kernel_times(self._h, int(level), int(reps), ms), "kernel_times")
+        return float(ms[0]), float(ms[1])
+
+
+__all__ = ["Problem", "Solver", "CSR", "host_lib", "NativeLibraryMissing", "STATUS", "LIB_DIR", "BIN_DIR"]

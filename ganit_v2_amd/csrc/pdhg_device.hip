@@ -339,16 +339,17 @@ void clear_sums(Pdhg* pd, VecSet& s) {
     GH_CHECK(hipGetLastError());
 }
 
+// Launch one of the two fused kernels of an iteration (0: primal over the columns, 1: dual over the rows)
+// on the current iterate, without advancing the iterate parity.
 template <class M, class V>
-void launch_iteration(Pdhg* pd, VecSet& s) {
+void launch_one(Pdhg* pd, VecSet& s, int which) {
     hipStream_t st = pd->ctx->stream;
     const V* x = (const V*)s.X[s.cur];
     V* xn = (V*)s.X[1 - s.cur];
     const V* y = (const V*)s.Y[s.cur];
     V* yn = (V*)s.Y[1 - s.cur];
-    const CsrView<M> at = pd->AT->view<M>();
-    const CsrView<M> a = pd->A->view<M>();
-    if (pd->n > 0) {
+    if (which == 0 && pd->n > 0) {
+        const CsrView<M> at = pd->AT->view<M>();
         with_lanes(pd->AT->lanes, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
             const unsigned g = grid_for((int64_t)pd->n * LPR);
@@ -361,13 +362,21 @@ void launch_iteration(Pdhg* pd, VecSet& s) {
                                    (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar, (V*)s.sumx);
         });
     }
-    if (pd->m > 0) {
+    if (which == 1 && pd->m > 0) {
+        const CsrView<M> a = pd->A->view<M>();
         with_lanes(pd->A->lanes, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
             hipLaunchKernelGGL((k_iter_dual<LPR, M, V>), dim3(grid_for((int64_t)pd->m * LPR)), dim3(kBlock), 0, st,
                                a, (const V*)s.xbar, y, (const V*)s.lo, (const V*)s.hi, pd->prm, yn, (V*)s.sumy);
         });
     }
+    GH_CHECK(hipGetLastError());
+}
+
+template <class M, class V>
+void launch_iteration(Pdhg* pd, VecSet& s) {
+    launch_one<M, V>(pd, s, 0);
+    launch_one<M, V>(pd, s, 1);
     s.cur ^= 1;
 }
 
@@ -649,4 +658,42 @@ int ganit_hip_pdhg_store(ganit_hip_pdhg* h, int vdt, double* x, double* y, doubl
     });
 }
 
+int ganit_hip_pdhg_time_kernels(ganit_hip_pdhg* h, int mdt, int vdt, int reps, double tau, double sigma,
+                                double* ms) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<Pdhg*>(h);
+        GH_REQUIRE(pd && ms && reps > 0, "bad timing arguments");
+        VecSet& s = pd->set(vdt);
+        hipStream_t st = pd->ctx->stream;
+        hipEvent_t ev[3];
+        for (auto& e : ev) GH_CHECK(hipEventCreate(&e));
+        try {
+            hipLaunchKernelGGL(k_set_steps, dim3(1), dim3(64), 0, st, pd->prm, tau, sigma);
+            with_mdt(mdt, [&](auto MT) {
+                with_vdt(vdt, [&](auto VT) {
+                    using M = typename decltype(MT)::type;
+                    using V = typename decltype(VT)::type;
+                    launch_iteration<M, V>(pd, s);  // warm-up, and leaves xbar defined
+                    GH_CHECK(hipEventRecord(ev[0], st));
+                    for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 0);
+                    GH_CHECK(hipEventRecord(ev[1], st));
+                    for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 1);
+                    GH_CHECK(hipEventRecord(ev[2], st));
+                });
+            });
+            GH_CHECK(hipEventSynchronize(ev[2]));
+            float a = 0, b = 0;
+            GH_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
+            GH_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
+            ms[0] = a / reps;
+            ms[1] = b / reps;
+        } catch (...) {
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto& e : ev) GH_CHECK(hipEventDestroy(e));
+    });
+}
+
 }  // extern "C"
+

@@ -66,6 +66,7 @@ def lib() -> ctypes.CDLL:
         "ganit_hip_pdhg_restart": [_vp, _i, _i],
         "ganit_hip_pdhg_adopt_average": [_vp, _i],
         "ganit_hip_pdhg_store": [_vp, _i, _vp, _vp, _vp, _vp],
+        "ganit_hip_pdhg_time_kernels": [_vp, _i, _i, _i, _d, _d, _vp],
     }
     for name, args in sig.items():
         f = getattr(L, name)

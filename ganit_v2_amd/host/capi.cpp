@@ -149,26 +149,58 @@ int ganit_lp_scaled(const ganit_lp* hh, int ruiz, int pc, double* scal, int* apt
     });
 }
 
+namespace {
+ganit::Options options_from(const double* o) {
+    ganit::Options opt;
+    opt.tol = o[0]; opt.max_iter = (long)o[1]; opt.time_limit = o[2]; opt.check_every = (int)o[3];
+    opt.ruiz_iters = (int)o[4]; opt.pock_chambolle = o[5] != 0; opt.verbose = (int)o[6];
+    opt.infeas_tol = o[7];
+    const int pr = (int)o[8];
+    opt.precision = pr == 1 ? ganit::Precision::Mixed : pr == 2 ? ganit::Precision::Dynamic : ganit::Precision::Double;
+    opt.stall_checks = (int)o[9]; opt.stall_checks_l2 = (int)o[10];
+    opt.values = (int)o[11] == 1 ? ganit::Values::BF16 : ganit::Values::F32;
+    opt.device = (int)o[12];
+    return opt;
+}
+void result_out(const ganit::Result& r, double* out, double* x, double* y) {
+    const double v[18] = {(double)(int)r.status, r.pobj, r.dobj, r.rel_primal, r.rel_dual, r.rel_gap,
+                          (double)r.iterations, (double)r.fp32_iterations, (double)r.l2_iterations,
+                          (double)r.restarts, r.setup_seconds, r.solve_seconds, r.norm_a, r.upload_seconds,
+                          r.iterate_seconds[0], r.iterate_seconds[1], r.iterate_seconds[2], (double)r.checks};
+    if (out) std::copy(v, v + 18, out);
+    put(x, r.x);
+    put(y, r.y);
+}
+}  // namespace
+
 int ganit_solve_gpu(const ganit_lp* hh, const double* o, double* out, double* x, double* y) {
     return guard([&] {
-        ganit::Options opt;
-        opt.tol = o[0]; opt.max_iter = (long)o[1]; opt.time_limit = o[2]; opt.check_every = (int)o[3];
-        opt.ruiz_iters = (int)o[4]; opt.pock_chambolle = o[5] != 0; opt.verbose = (int)o[6];
-        opt.infeas_tol = o[7];
-        const int pr = (int)o[8];
-        opt.precision = pr == 1 ? ganit::Precision::Mixed : pr == 2 ? ganit::Precision::Dynamic : ganit::Precision::Double;
-        opt.stall_checks = (int)o[9]; opt.stall_checks_l2 = (int)o[10];
-        opt.values = (int)o[11] == 1 ? ganit::Values::BF16 : ganit::Values::F32;
-        opt.device = (int)o[12];
-        const ganit::Result r = ganit::solve_pdhg_gpu(reinterpret_cast<const Handle*>(hh)->lp, opt);
-        const double v[18] = {(double)(int)r.status, r.pobj, r.dobj, r.rel_primal, r.rel_dual, r.rel_gap,
-                              (double)r.iterations, (double)r.fp32_iterations, (double)r.l2_iterations,
-                              (double)r.restarts, r.setup_seconds, r.solve_seconds, r.norm_a, r.upload_seconds,
-                              r.iterate_seconds[0], r.iterate_seconds[1], r.iterate_seconds[2], (double)r.checks};
-        std::copy(v, v + 18, out);
-        put(x, r.x);
-        put(y, r.y);
+        const ganit::Result r = ganit::solve_pdhg_gpu(reinterpret_cast<const Handle*>(hh)->lp, options_from(o));
+        result_out(r, out, x, y);
     });
 }
+
+ganit_solver* ganit_solver_create(const ganit_lp* hh, const double* o) {
+    ganit::GpuSolver* s = nullptr;
+    if (guard([&] { s = new ganit::GpuSolver(reinterpret_cast<const Handle*>(hh)->lp, options_from(o)); })) {
+        delete s;
+        return nullptr;
+    }
+    return reinterpret_cast<ganit_solver*>(s);
+}
+
+int ganit_solver_upload(ganit_solver* h) {
+    return guard([&] { reinterpret_cast<ganit::GpuSolver*>(h)->upload(); });
+}
+
+int ganit_solver_run(ganit_solver* h, double* out, double* x, double* y) {
+    return guard([&] { result_out(reinterpret_cast<ganit::GpuSolver*>(h)->run(), out, x, y); });
+}
+
+int ganit_solver_kernel_times(ganit_solver* h, int level, int reps, double* ms) {
+    return guard([&] { reinterpret_cast<ganit::GpuSolver*>(h)->kernel_times(level, reps, ms); });
+}
+
+void ganit_solver_free(ganit_solver* h) { delete reinterpret_cast<ganit::GpuSolver*>(h); }
 
 }  // extern "C"

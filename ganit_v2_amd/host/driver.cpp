@@ -188,56 +188,56 @@ Status status_from(Exit e) {
     }
 }
 
-struct CtxGuard {
+}  // namespace
+
+struct GpuSolver::Impl {
+    const Options opt;
     ganit_hip_ctx* ctx = nullptr;
     ganit_hip_pdhg* pd = nullptr;
-    ~CtxGuard() {
+    std::string device;
+    ScaledLP s;
+    double normA = 0, setup_seconds = 0;
+    bool mixed = false;
+    int low_mdt = GANIT_F32;
+    std::vector<double> x0, y0;
+    double w0 = 1.0;
+
+    explicit Impl(const Options& o) : opt(o) {}
+    ~Impl() {
         if (pd) ganit_hip_pdhg_destroy(pd);
         if (ctx) ganit_hip_ctx_destroy(ctx);
     }
 };
 
-}  // namespace
-
-Result solve_pdhg_gpu(const LP& lp, const Options& opt) {
-    CtxGuard g;
-    chk(ganit_hip_ctx_create(opt.device, &g.ctx));
-    char name[256] = {0};
-    chk(ganit_hip_ctx_device_info(g.ctx, name, sizeof(name), nullptr));
+GpuSolver::GpuSolver(const LP& lp, const Options& opt) : p_(new Impl(opt)) {
+    Impl& I = *p_;
     if (opt.check_every < 1) throw std::runtime_error("check_every must be positive");
+    chk(ganit_hip_ctx_create(opt.device, &I.ctx));
+    char name[256] = {0};
+    chk(ganit_hip_ctx_device_info(I.ctx, name, sizeof(name), nullptr));
+    I.device = std::string("gpu:") + name;
+    const auto t0 = Clock::now();
+    I.s = scale_lp(lp, opt.ruiz_iters, opt.pock_chambolle);
+    I.normA = estimate_norm(I.s);
+    I.x0.resize(I.s.n);
+    for (int j = 0; j < I.s.n; ++j) I.x0[j] = std::min(std::max(0.0, I.s.l[j]), I.s.u[j]);
+    I.y0.assign(I.s.m, 0.0);
+    const double cn = norm2(I.s.c), bn = finite_bound_norm(I.s.lo, I.s.hi);
+    I.w0 = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+    I.mixed = opt.precision != Precision::Double;
+    I.low_mdt = opt.values == Values::BF16 ? GANIT_BF16 : GANIT_F32;
+    I.setup_seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+}
 
-    const auto t_start = Clock::now();
-    Result res;
-    res.device = std::string("gpu:") + name;
-    ScaledLP s = scale_lp(lp, opt.ruiz_iters, opt.pock_chambolle);
-    const double normA = estimate_norm(s);
-    res.norm_a = normA;
-    Shared sh{s, opt, 0.998 / normA, normA, s.cscale * s.bscale, Clock::now()};
+GpuSolver::~GpuSolver() = default;
 
-    std::vector<double> x0(s.n), y0(s.m, 0.0);
-    for (int j = 0; j < s.n; ++j) x0[j] = std::min(std::max(0.0, s.l[j]), s.u[j]);
-    State st;
-    {
-        const double cn = norm2(s.c), bn = finite_bound_norm(s.lo, s.hi);
-        st.w = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
-    }
-    const bool mixed = opt.precision != Precision::Double;
-    const int low_mdt = opt.values == Values::BF16 ? GANIT_BF16 : GANIT_F32;
-    const char* pname = opt.precision == Precision::Dynamic ? "dynamic L1->L2->L3"
-                      : opt.precision == Precision::Mixed ? "mixed L1->L3" : "FP64";
-    if (opt.verbose) {
-        std::printf("GANIT PDHG | device %s | %s | precision %s%s | rows %d cols %d nnz %lld | ||A||~%.3e | tol %.1e\n",
-                    res.device.c_str(), s.has_q() ? "QP" : "LP", pname,
-                    mixed && opt.values == Values::BF16 ? " (BF16 values)" : "", s.m, s.n, (long long)s.A.nnz(),
-                    normA, opt.tol);
-        if (s.has_q()) std::printf("Hessian nnz %lld | ||Q||~%.3e\n", (long long)s.Q.nnz(), s.normQ);
-        std::printf("%9s %9s %4s %15s %15s %9s %9s %9s %9s\n", "iter", "time", "prec", "primal obj", "dual obj",
-                    "rel_p", "rel_d", "rel_gap", "weight");
-    }
-    res.setup_seconds = std::chrono::duration<double>(Clock::now() - t_start).count();
-    sh.t_solve = Clock::now();
+const ScaledLP& GpuSolver::scaled() const { return p_->s; }
+double GpuSolver::norm_a() const { return p_->normA; }
 
-    // ---- device upload: once, with every precision the ladder will use
+void GpuSolver::upload() {
+    Impl& I = *p_;
+    if (I.pd) return;
+    const ScaledLP& s = I.s;
     ganit_hip_lp_desc d{};
     d.m = s.m; d.n = s.n;
     d.a_ptr = s.A.ptr.data(); d.a_idx = s.A.idx.data(); d.a_val = s.A.val.data();
@@ -245,12 +245,51 @@ Result solve_pdhg_gpu(const LP& lp, const Options& opt) {
     if (s.has_q()) { d.q_ptr = s.Q.ptr.data(); d.q_idx = s.Q.idx.data(); d.q_val = s.Q.val.data(); }
     d.c = s.c.data(); d.l = s.l.data(); d.u = s.u.data(); d.lo = s.lo.data(); d.hi = s.hi.data();
     d.rres = s.Rres.data(); d.cres = s.Cres.data();
-    const unsigned mmask = (1u << GANIT_F64) | (mixed ? (1u << low_mdt) : 0u);
-    const unsigned vmask = (1u << GANIT_F64) | (mixed ? (1u << GANIT_F32) : 0u);
-    chk(ganit_hip_pdhg_create(g.ctx, &d, mmask, vmask, &g.pd));
-    ganit_hip_pdhg* pd = g.pd;
+    // every precision the ladder will use is resident at once, plus FP64 for certification
+    const unsigned mmask = (1u << GANIT_F64) | (I.mixed ? (1u << I.low_mdt) : 0u);
+    const unsigned vmask = (1u << GANIT_F64) | (I.mixed ? (1u << GANIT_F32) : 0u);
+    chk(ganit_hip_pdhg_create(I.ctx, &d, mmask, vmask, &I.pd));
+}
+
+void GpuSolver::kernel_times(int level, int reps, double* ms) {
+    Impl& I = *p_;
+    upload();
+    const int mdt = level == 2 ? GANIT_F64 : I.low_mdt, vdt = level == 0 ? GANIT_F32 : GANIT_F64;
+    if (level != 2 && !I.mixed) throw std::runtime_error("reduced-precision level not resident (precision double)");
+    const double sigma = (0.998 / I.normA) * I.w0, tau = (0.998 / I.normA) / I.w0;
+    chk(ganit_hip_pdhg_load(I.pd, vdt, I.x0.data(), I.y0.data(), nullptr, nullptr));
+    chk(ganit_hip_pdhg_time_kernels(I.pd, mdt, vdt, reps, tau, sigma, ms));
+}
+
+Result GpuSolver::run() {
+    Impl& I = *p_;
+    const Options& opt = I.opt;
+    const ScaledLP& s = I.s;
+    Result res;
+    res.device = I.device;
+    res.norm_a = I.normA;
+    res.setup_seconds = I.setup_seconds;
+    Shared sh{s, opt, 0.998 / I.normA, I.normA, s.cscale * s.bscale, Clock::now()};
+    State st;
+    st.w = I.w0;
+    const bool mixed = I.mixed;
+    const int low_mdt = I.low_mdt;
+    const char* pname = opt.precision == Precision::Dynamic ? "dynamic L1->L2->L3"
+                      : opt.precision == Precision::Mixed ? "mixed L1->L3" : "FP64";
+    if (opt.verbose) {
+        std::printf("GANIT PDHG | device %s | %s | precision %s%s | rows %d cols %d nnz %lld | ||A||~%.3e | tol %.1e\n",
+                    res.device.c_str(), s.has_q() ? "QP" : "LP", pname,
+                    mixed && opt.values == Values::BF16 ? " (BF16 values)" : "", s.m, s.n, (long long)s.A.nnz(),
+                    I.normA, opt.tol);
+        if (s.has_q()) std::printf("Hessian nnz %lld | ||Q||~%.3e\n", (long long)s.Q.nnz(), s.normQ);
+        std::printf("%9s %9s %4s %15s %15s %9s %9s %9s %9s\n", "iter", "time", "prec", "primal obj", "dual obj",
+                    "rel_p", "rel_d", "rel_gap", "weight");
+    }
+    sh.t_solve = Clock::now();
+    upload();  // no-op when the caller uploaded beforehand
+    ganit_hip_pdhg* pd = I.pd;
     const int first_vdt = mixed ? GANIT_F32 : GANIT_F64;
-    chk(ganit_hip_pdhg_load(pd, first_vdt, x0.data(), y0.data(), nullptr, nullptr));
+    chk(ganit_hip_pdhg_load(pd, first_vdt, I.x0.data(), I.y0.data(), nullptr, nullptr));
     res.upload_seconds = sh.elapsed();
 
     const Level L1{low_mdt, GANIT_F32, "L1"}, L2{low_mdt, GANIT_F64, "L2"}, L3{GANIT_F64, GANIT_F64, "L3"};
@@ -326,6 +365,11 @@ Result solve_pdhg_gpu(const LP& lp, const Options& opt) {
                     res.pobj, res.dobj, fin.rel_p, fin.rel_d, fin.rel_gap);
     }
     return res;
+}
+
+Result solve_pdhg_gpu(const LP& lp, const Options& opt) {
+    GpuSolver solver(lp, opt);
+    return solver.run();
 }
 
 }  // namespace ganit

@@ -2,6 +2,7 @@
 // Mirrors the reference's include/ganit/pdhg.hpp:10-47 (Precision, Options, Status, Result,
 // solve_pdhg_gpu); additions are marked "extension".
 #pragma once
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -55,6 +56,30 @@ struct Result {
 };
 
 // Runs restarted PDHG on the MI355X through libganit_hip.so; throws std::runtime_error on HIP errors.
+// Timing follows the reference: setup_seconds = scaling + norm estimate; solve_seconds includes the device
+// upload and the FP64 certification (pdhg_impl.hpp:273-274, 305).
 Result solve_pdhg_gpu(const LP& lp, const Options& opt);
+
+struct ScaledLP;
+// extension: the same solve split into set-up (scaling, norm), device upload and repeated runs from the
+// initial point, so a caller can time solves with the problem already resident in HBM.
+class GpuSolver {
+   public:
+    GpuSolver(const LP& lp, const Options& opt);  // HIP context + scaling + norm estimate
+    ~GpuSolver();
+    GpuSolver(const GpuSolver&) = delete;
+    GpuSolver& operator=(const GpuSolver&) = delete;
+    void upload();   // device-resident problem in every precision the ladder needs; idempotent
+    Result run();    // one full solve from the initial point (uploads first if needed)
+    // Average device milliseconds of the two fused iteration kernels at a precision level
+    // (0 = L1, 1 = L2, 2 = L3): ms[0] primal kernel, ms[1] dual kernel.
+    void kernel_times(int level, int reps, double* ms);
+    const ScaledLP& scaled() const;
+    double norm_a() const;
+
+   private:
+    struct Impl;
+    std::unique_ptr<Impl> p_;
+};
 
 }  // namespace ganit

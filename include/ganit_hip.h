@@ -163,6 +163,13 @@ int ganit_hip_pdhg_adopt_average(ganit_hip_pdhg* pd, int vdt);
  * any pointer may be NULL.  Synchronises. */
 int ganit_hip_pdhg_store(ganit_hip_pdhg* pd, int vdt, double* x, double* y, double* xr, double* yr);
 
+/* Average device time in milliseconds of each fused iteration kernel on the current iterate, measured with
+ * HIP events on the context stream over `reps` back-to-back launches: ms[0] = primal kernel
+ * (A^T y [+ Q x] + primal prox), ms[1] = dual kernel (A xbar + dual prox).  Advances the iterate; no
+ * reference counterpart (the reference has no per-kernel timers). */
+int ganit_hip_pdhg_time_kernels(ganit_hip_pdhg* pd, int mdt, int vdt, int reps, double tau, double sigma,
+                                double* ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -61,6 +61,18 @@ int ganit_lp_scaled(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, doub
  * x (n) and y (m) receive the original-space solution; either may be NULL. */
 int ganit_solve_gpu(const ganit_lp* lp, const double* opts, double* out, double* x, double* y);
 
+/* The same solve split into stages (no reference counterpart: the reference always re-uploads per solve).
+ * create = HIP context + scaling + norm estimate (opts as above); upload = device-resident problem in every
+ * precision the ladder needs; run = one full solve from the initial point with the problem already in HBM
+ * (out / x / y as ganit_solve_gpu); kernel_times = average device ms of the two fused iteration kernels
+ * at level 0 (L1), 1 (L2) or 2 (L3): ms[0] primal, ms[1] dual. */
+typedef struct ganit_solver ganit_solver;
+ganit_solver* ganit_solver_create(const ganit_lp* lp, const double* opts);
+int ganit_solver_upload(ganit_solver* s);
+int ganit_solver_run(ganit_solver* s, double* out, double* x, double* y);
+int ganit_solver_kernel_times(ganit_solver* s, int level, int reps, double* ms);
+void ganit_solver_free(ganit_solver* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -212,7 +212,7 @@ def test_oracle_scaling_invariants_and_norm(oracle, name):
 def test_oracle_objective_matches_reference_simplex(oracle, name):
     """Objective oracle: FP64 PDHG at 1e-8 agrees with the reference's dual simplex to 1e-6 relative."""
     simplex = next(r for r in golden_runs() if r["instance"] == name and r["args"][:2] == ["--method", "simplex"])
-    oracle.set_threads(8)
+    oracle.set_threads(1)
     r = oracle.OracleLP.read_mps(instance_path(name)).solve(tol=1e-8, time_limit=60)
     assert r["status"] == "OPTIMAL"
     assert abs(r["pobj"] - simplex["pobj"]) <= 1e-6 * (1 + abs(simplex["pobj"]))
