@@ -18,6 +18,10 @@ namespace ganit_hip {
 constexpr int kWave = 64;
 constexpr int kBlock = 256;
 constexpr int kNumXcd = 8;
+// LDS-staged row blocks ("stream" variant): a block covers at most kBlock rows and kTile nonzeros; a row
+// longer than kLong is a block of its own and is reduced by the whole block instead.
+constexpr int kTile = 2048;
+constexpr int kLong = 256;
 
 // ---------------------------------------------------------------- BF16 storage
 struct bf16_t {
@@ -69,6 +73,80 @@ __device__ __forceinline__ V row_dot(const CsrView<M>& a, int row, int lane, con
     V t = V(0);
     for (int p = beg + lane; p < end; p += LPR) t += widen<V>(a.val[p]) * x[a.idx[p]];
     return group_sum<LPR>(t);
+}
+
+// ---------------------------------------------------------------- block sums in the vector type
+// Sum NX values of type V over the block; valid in thread 0.  Fixed order (wave tree, then waves in order).
+template <class V, int NX>
+__device__ __forceinline__ void block_sum_v(V (&v)[NX], V* lds /* (kBlock / kWave) * NX */) {
+    const int lane = threadIdx.x & (kWave - 1), wave = threadIdx.x / kWave;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, kWave);
+    }
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) lds[wave * NX + k] = v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            V s = lds[k];
+            for (int w = 1; w < kBlock / kWave; ++w) s += lds[w * NX + k];
+            v[k] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- LDS-staged row blocks
+// One block of the stream variant: rows [r0, r1) of `a` whose nonzeros fit one LDS tile.  Every product
+// val * x_k[idx] (k < NX gathered vectors, `gather(k, i)`) is formed by a coalesced sweep over the block's
+// nonzeros and stored in LDS (rounded to V there); then thread t adds up row r0 + t sequentially in row
+// order -- the same operation sequence as a scalar CPU row loop, so these row sums are bit-identical to it.
+// A single row longer than kLong is instead swept by all threads and reduced with a fixed tree.
+// `row_fn(r, sums)` runs once per row, in the thread that owns it (thread 0 for a long row).
+template <class M, class V, int NX, class Gather, class RowFn>
+__device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __restrict__ blocks, int b, Gather gather,
+                                             RowFn row_fn, V* lds /* NX * kTile */) {
+    const int r0 = blocks[b], r1 = blocks[b + 1];
+    const int base = a.ptr[r0];
+    const int cnt = a.ptr[r1] - base;
+    if (r1 - r0 == 1 && cnt > kLong) {
+        V s[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) s[k] = V(0);
+        for (int p = base + threadIdx.x; p < base + cnt; p += kBlock) {
+            const V av = widen<V>(a.val[p]);
+            const int i = a.idx[p];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+        }
+        block_sum_v<V, NX>(s, lds);
+        if (threadIdx.x == 0) row_fn(r0, s);
+        return;
+    }
+    for (int q = threadIdx.x; q < cnt; q += kBlock) {
+        const V av = widen<V>(a.val[base + q]);
+        const int i = a.idx[base + q];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) lds[k * kTile + q] = av * gather(k, i);
+    }
+    __syncthreads();
+    const int r = r0 + (int)threadIdx.x;
+    if (r < r1) {
+        const int beg = a.ptr[r] - base, end = a.ptr[r + 1] - base;
+        V s[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) s[k] = V(0);
+        for (int q = beg; q < end; ++q) {
+#pragma unroll
+            for (int k = 0; k < NX; ++k) s[k] += lds[k * kTile + q];
+        }
+        row_fn(r, s);
+    }
 }
 
 // ---------------------------------------------------------------- deterministic grid reduction

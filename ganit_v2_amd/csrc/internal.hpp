@@ -78,8 +78,10 @@ struct Csr {
     double* v64 = nullptr;
     float* v32 = nullptr;
     bf16_t* v16 = nullptr;
-    int lanes = 8;
+    int lanes = 8;           // row-group variant: lanes per row (1..64); 0 selects the LDS-staged row blocks
     int64_t hist[9] = {0};
+    int* blocks = nullptr;   // row-block starts of the LDS-staged variant (nblocks + 1 entries, device)
+    int nblocks = 0;
     template <class M>
     CsrView<M> view() const;
 };
@@ -103,6 +105,9 @@ inline CsrView<bf16_t> Csr::view<bf16_t>() const {
 Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, const double* val, unsigned mask);
 void csr_destroy(Csr* a);
 int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz);
+bool valid_variant(int lanes);
+// Time the candidate variants of `a` with the stand-alone SpMV in (mdt, vdt) and keep the fastest.
+void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps);
 uint16_t bf16_bits_from_double(double d);
 
 // ---------------------------------------------------------------- dispatch
@@ -128,6 +133,7 @@ void with_mdt(int mdt, F&& f) {
     else throw Error(GANIT_EINVAL, "matrix dtype must be GANIT_F64, GANIT_F32 or GANIT_BF16");
 }
 
+// Dispatch on the row-group width; 0 (the LDS-staged variant) must be handled by the caller first.
 template <class F>
 void with_lanes(int lanes, F&& f) {
     switch (lanes) {
@@ -138,7 +144,7 @@ void with_lanes(int lanes, F&& f) {
         case 16: f(std::integral_constant<int, 16>{}); break;
         case 32: f(std::integral_constant<int, 32>{}); break;
         case 64: f(std::integral_constant<int, 64>{}); break;
-        default: throw Error(GANIT_EINVAL, "lanes per row must be a power of two in 1..64");
+        default: throw Error(GANIT_EINVAL, "lanes per row must be 0 (row blocks) or a power of two in 1..64");
     }
 }
 

@@ -57,6 +57,50 @@ __global__ __launch_bounds__(kBlock) void k_iter_dual(CsrView<M> A, const V* __r
     }
 }
 
+// The same two kernels on LDS-staged row blocks (one block per row block; see stream_block).
+template <class M, class V, bool HASQ>
+__global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const int* __restrict__ blocks,
+                                                           CsrView<M> Q, const V* __restrict__ y,
+                                                           const V* __restrict__ x, const V* __restrict__ c,
+                                                           const V* __restrict__ l, const V* __restrict__ u,
+                                                           const double* __restrict__ prm, V* __restrict__ xn,
+                                                           V* __restrict__ xbar, V* __restrict__ sumx) {
+    __shared__ V lds[kTile];
+    stream_block<M, V, 1>(
+        AT, blocks, xcd_block_id(), [&](int, int i) { return y[i]; },
+        [&](int j, V (&aty)[1]) {
+#pragma clang fp contract(off)
+            V qx = V(0);
+            if (HASQ) {  // Q row summed sequentially by the owning thread
+                for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
+                    const V prod = widen<V>(Q.val[p]) * x[Q.idx[p]];
+                    qx = qx + prod;
+                }
+            }
+            V s = sumx[j];
+            primal_update<V>(x[j], aty[0], qx, c[j], l[j], u[j], V(prm[0]), xn[j], xbar[j], s);
+            sumx[j] = s;
+        },
+        lds);
+}
+
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_iter_dual_rb(CsrView<M> A, const int* __restrict__ blocks,
+                                                         const V* __restrict__ xbar, const V* __restrict__ y,
+                                                         const V* __restrict__ lo, const V* __restrict__ hi,
+                                                         const double* __restrict__ prm, V* __restrict__ yn,
+                                                         V* __restrict__ sumy) {
+    __shared__ V lds[kTile];
+    stream_block<M, V, 1>(
+        A, blocks, xcd_block_id(), [&](int, int j) { return xbar[j]; },
+        [&](int i, V (&axb)[1]) {
+            V s = sumy[i];
+            dual_update<V>(y[i], axb[0], lo[i], hi[i], V(prm[1]), yn[i], s);
+            sumy[i] = s;
+        },
+        lds);
+}
+
 __global__ void k_set_steps(double* prm, double tau, double sigma) {
     if (threadIdx.x == 0) {
         prm[0] = tau;
@@ -215,6 +259,109 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
     grid_sum<kNOutM>(acc, slot, lds);
 }
 
+// The two check passes on LDS-staged row blocks (grid = number of row blocks).
+template <class M, class V, bool HASQ>
+__global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int* __restrict__ blocks, CsrView<M> Q,
+                                                       const V* __restrict__ x, const V* __restrict__ y,
+                                                       const V* __restrict__ xa, const V* __restrict__ ya,
+                                                       const V* __restrict__ xr, const V* __restrict__ yr,
+                                                       const V* __restrict__ c, const V* __restrict__ l,
+                                                       const V* __restrict__ u, const V* __restrict__ cres, int flags,
+                                                       ReduceSlot slot) {
+    __shared__ V lds[3 * kTile];
+    __shared__ double red[(kBlock / kWave) * kNOutN];
+    double acc[kNOutN];
+#pragma unroll
+    for (int k = 0; k < kNOutN; ++k) acc[k] = 0.0;
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
+    stream_block<M, V, 3>(
+        AT, blocks, xcd_block_id(),
+        [&](int k, int i) {
+            return k == 0 ? y[i] : k == 1 ? (avg ? ya[i] : V(0)) : (ray ? V(y[i] - yr[i]) : V(0));
+        },
+        [&](int j, V (&t)[3]) {
+#pragma clang fp contract(off)
+            V qc = V(0), qa = V(0);
+            if (HASQ) {
+                for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
+                    const V a = widen<V>(Q.val[p]);
+                    const V pc = a * x[Q.idx[p]];
+                    qc = qc + pc;
+                    if (avg) {
+                        const V pa = a * xa[Q.idx[p]];
+                        qa = qa + pa;
+                    }
+                }
+            }
+            const double cj = double(c[j]), lj = double(l[j]), uj = double(u[j]), cr = double(cres[j]);
+            const double xj = double(x[j]);
+            double r2, ob;
+            dual_terms(cj - double(t[0]) + (HASQ ? double(qc) : 0.0), lj, uj, cr, r2, ob);
+            acc[0] += r2;
+            acc[1] += ob;
+            acc[2] += cj * xj;
+            if (HASQ) acc[3] += xj * double(qc);
+            if (avg) {
+                const double xaj = double(xa[j]), xrj = double(xr[j]);
+                dual_terms(cj - double(t[1]) + (HASQ ? double(qa) : 0.0), lj, uj, cr, r2, ob);
+                acc[4] += r2;
+                acc[5] += ob;
+                acc[6] += cj * xaj;
+                if (HASQ) acc[7] += xaj * double(qa);
+                const double d0 = xj - xrj, d1 = xaj - xrj;
+                acc[10] += d0 * d0;
+                acc[11] += d1 * d1;
+            }
+            if (ray) {
+                dual_terms(0.0 * cj - double(t[2]) + 0.0, lj, uj, cr, r2, ob);
+                acc[8] += r2;
+                acc[9] += ob;
+            }
+        },
+        lds);
+    __syncthreads();
+    grid_sum<kNOutN>(acc, slot, red);
+}
+
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* __restrict__ blocks,
+                                                       const V* __restrict__ x, const V* __restrict__ xa,
+                                                       const V* __restrict__ y, const V* __restrict__ ya,
+                                                       const V* __restrict__ yr, const V* __restrict__ lo,
+                                                       const V* __restrict__ hi, const V* __restrict__ rres, int flags,
+                                                       ReduceSlot slot) {
+    __shared__ V lds[2 * kTile];
+    __shared__ double red[(kBlock / kWave) * kNOutM];
+    double acc[kNOutM];
+#pragma unroll
+    for (int k = 0; k < kNOutM; ++k) acc[k] = 0.0;
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
+    stream_block<M, V, 2>(
+        A, blocks, xcd_block_id(), [&](int k, int j) { return k == 0 ? x[j] : (avg ? xa[j] : V(0)); },
+        [&](int i, V (&t)[2]) {
+            const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
+            auto pres = [&](double a) {
+                const double p = fmin(fmax(a, lw), hg);
+                const double r = (a - p) / rr;
+                return r * r;
+            };
+            acc[0] += pres(double(t[0]));
+            acc[1] += row_obj_term(yi, lw, hg);
+            if (avg) {
+                const double yai = double(ya[i]), yri = double(yr[i]);
+                acc[2] += pres(double(t[1]));
+                acc[3] += row_obj_term(yai, lw, hg);
+                const double d0 = yi - yri, d1 = yai - yri;
+                acc[5] += d0 * d0;
+                acc[6] += d1 * d1;
+            }
+            if (ray) acc[4] += row_obj_term(double(V(y[i] - yr[i])), lw, hg);
+        },
+        lds);
+    __syncthreads();
+    grid_sum<kNOutM>(acc, slot, red);
+}
+
 // ================================================================ restart / hand-over kernels
 template <class V>
 __global__ __launch_bounds__(kBlock) void k_restart(int n, int m, int adopt, V* __restrict__ x,
@@ -348,7 +495,17 @@ void launch_one(Pdhg* pd, VecSet& s, int which) {
     V* xn = (V*)s.X[1 - s.cur];
     const V* y = (const V*)s.Y[s.cur];
     V* yn = (V*)s.Y[1 - s.cur];
-    if (which == 0 && pd->n > 0) {
+    if (which == 0 && pd->n > 0 && pd->AT->lanes == 0) {
+        const CsrView<M> at = pd->AT->view<M>();
+        const dim3 g(pd->AT->nblocks);
+        if (pd->hasq)
+            hipLaunchKernelGGL((k_iter_primal_rb<M, V, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
+                               pd->Q->view<M>(), y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn,
+                               (V*)s.xbar, (V*)s.sumx);
+        else
+            hipLaunchKernelGGL((k_iter_primal_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, at, y, x,
+                               (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar, (V*)s.sumx);
+    } else if (which == 0 && pd->n > 0) {
         const CsrView<M> at = pd->AT->view<M>();
         with_lanes(pd->AT->lanes, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
@@ -362,7 +519,11 @@ void launch_one(Pdhg* pd, VecSet& s, int which) {
                                    (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar, (V*)s.sumx);
         });
     }
-    if (which == 1 && pd->m > 0) {
+    if (which == 1 && pd->m > 0 && pd->A->lanes == 0) {
+        hipLaunchKernelGGL((k_iter_dual_rb<M, V>), dim3(pd->A->nblocks), dim3(kBlock), 0, st, pd->A->view<M>(),
+                           pd->A->blocks, (const V*)s.xbar, y, (const V*)s.lo, (const V*)s.hi, pd->prm, yn,
+                           (V*)s.sumy);
+    } else if (which == 1 && pd->m > 0) {
         const CsrView<M> a = pd->A->view<M>();
         with_lanes(pd->A->lanes, [&](auto L) {
             constexpr int LPR = decltype(L)::value;
@@ -393,7 +554,18 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
     ReduceSlot sm{pd->part_m, pd->tickets + 1, pd->out + kNOutN};
     const CsrView<M> at = pd->AT->view<M>();
     const CsrView<M> a = pd->A->view<M>();
-    with_lanes(pd->AT->lanes, [&](auto L) {
+    if (pd->AT->lanes == 0 && pd->n > 0) {
+        const dim3 g(pd->AT->nblocks);
+        if (pd->hasq)
+            hipLaunchKernelGGL((k_check_n_rb<M, V, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
+                               pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
+                               (const V*)s.yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags,
+                               sn);
+        else
+            hipLaunchKernelGGL((k_check_n_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, at, x, y,
+                               (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
+                               (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
+    } else with_lanes(pd->AT->lanes == 0 ? 1 : pd->AT->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         const int64_t groups = std::max<int64_t>(pd->n, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
@@ -406,7 +578,11 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
                                (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     });
-    with_lanes(pd->A->lanes, [&](auto L) {
+    if (pd->A->lanes == 0 && pd->m > 0) {
+        hipLaunchKernelGGL((k_check_m_rb<M, V>), dim3(pd->A->nblocks), dim3(kBlock), 0, st, a, pd->A->blocks, x,
+                           (const V*)s.xa, y, (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi,
+                           (const V*)s.rres, flags, sm);
+    } else with_lanes(pd->A->lanes == 0 ? 1 : pd->A->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         const int64_t groups = std::max<int64_t>(pd->m, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
@@ -445,6 +621,15 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
             GH_REQUIRE(pd->A->nnz == pd->AT->nnz, "A and its transpose differ in nnz");
             pd->hasq = lp->q_ptr && lp->q_ptr[lp->n] > 0;
             if (pd->hasq) pd->Q = csr_create(ctx, lp->n, lp->n, lp->q_ptr, lp->q_idx, lp->q_val, mdt_mask);
+            {  // per-matrix variant: time the candidates at the lowest resident precision (the hot level)
+                const char* at = std::getenv("GANIT_HIP_AUTOTUNE");
+                if (!(at && at[0] == '0')) {
+                    const int mdt = (mdt_mask & 4u) ? GANIT_BF16 : (mdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
+                    const int vdt = (vdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
+                    autotune_csr(ctx, pd->A, mdt, vdt, 5);
+                    autotune_csr(ctx, pd->AT, mdt, vdt, 5);
+                }
+            }
             if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
             for (int vdt = 0; vdt < 2; ++vdt) {
                 if (!(vdt_mask & (1u << vdt))) continue;
@@ -464,8 +649,9 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 s.sumy = pd->alloc(e * m); s.ya = pd->alloc(e * m); s.yr = pd->alloc(e * m);
             }
             pd->prm = (double*)pd->alloc(sizeof(double) * 2);
-            pd->part_n = (double*)pd->alloc(sizeof(double) * kCheckGridMax * kNOutN);
-            pd->part_m = (double*)pd->alloc(sizeof(double) * kCheckGridMax * kNOutM);
+            const size_t gmax = (size_t)std::max({kCheckGridMax, pd->A->nblocks, pd->AT->nblocks});
+            pd->part_n = (double*)pd->alloc(sizeof(double) * gmax * kNOutN);
+            pd->part_m = (double*)pd->alloc(sizeof(double) * gmax * kNOutM);
             pd->tickets = (unsigned*)pd->alloc(sizeof(unsigned) * 2);
             pd->out = (double*)pd->alloc(sizeof(double) * GANIT_CHECK_NOUT);
             GH_CHECK(hipMemsetAsync(pd->tickets, 0, sizeof(unsigned) * 2, ctx->stream));
@@ -500,7 +686,7 @@ int ganit_hip_pdhg_set_lanes(ganit_hip_pdhg* h, int which, int lanes) {
     return guarded([&] {
         auto* pd = reinterpret_cast<Pdhg*>(h);
         GH_REQUIRE(pd && which >= 0 && which <= 2, "bad plan override");
-        with_lanes(lanes, [](auto) {});
+        GH_REQUIRE(valid_variant(lanes), "lanes per row must be 0 (row blocks) or a power of two in 1..64");
         Csr* a = which == 0 ? pd->A : which == 1 ? pd->AT : pd->Q;
         GH_REQUIRE(a, "problem has no Q");
         a->lanes = lanes;

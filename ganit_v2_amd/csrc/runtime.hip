@@ -55,6 +55,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<M> a, const V* __restri
     if (lane == 0) y[row] = t;
 }
 
+// Stand-alone SpMV on LDS-staged row blocks (one block per row block).
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_spmv_rb(CsrView<M> a, const int* __restrict__ blocks,
+                                                    const V* __restrict__ x, V* __restrict__ y) {
+    __shared__ V lds[kTile];
+    stream_block<M, V, 1>(
+        a, blocks, xcd_block_id(), [&](int, int j) { return x[j]; }, [&](int r, V (&t)[1]) { y[r] = t[0]; }, lds);
+}
+
 template <class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_primal_step(int n, const V* __restrict__ x, const V* __restrict__ aty,
                                                         const V* __restrict__ qx, const V* __restrict__ c,
@@ -188,6 +197,32 @@ int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz) {
     return lpr;
 }
 
+bool valid_variant(int lanes) { return lanes == 0 || (lanes >= 1 && lanes <= kWave && (lanes & (lanes - 1)) == 0); }
+
+// Row blocks of the LDS-staged variant: consecutive rows while the block holds at most kBlock rows and kTile
+// nonzeros and every row has at most kLong entries; a longer row forms a block of its own.
+std::vector<int> row_blocks(const int* ptr, int rows) {
+    std::vector<int> b{0};
+    int r = 0;
+    while (r < rows) {
+        const int len = ptr[r + 1] - ptr[r];
+        if (len > kLong) {
+            ++r;
+        } else {
+            int cnt = 0, nz = 0;
+            while (r < rows && cnt < kBlock) {
+                const int l = ptr[r + 1] - ptr[r];
+                if (l > kLong || nz + l > kTile) break;
+                nz += l;
+                ++cnt;
+                ++r;
+            }
+        }
+        b.push_back(r);
+    }
+    return b;
+}
+
 Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, const double* val, unsigned mask) {
     GH_REQUIRE(rows >= 0 && cols >= 0, "negative matrix dimension");
     GH_REQUIRE(rows == 0 || ptr, "null row pointer");
@@ -209,6 +244,12 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
         }
         for (int64_t p = 0; p < a->nnz; ++p) GH_REQUIRE(idx[p] >= 0 && idx[p] < cols, "CSR column index out of range");
         a->lanes = choose_lanes_from_lengths(len, a->nnz);
+        const std::vector<int> rb = row_blocks(ptr, rows);
+        a->nblocks = (int)rb.size() - 1;
+        GH_CHECK(hipMalloc(&a->blocks, sizeof(int) * rb.size()));
+        GH_CHECK(hipMemcpyAsync(a->blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, ctx->stream));
+        // short rows (nnz-weighted median under 16 entries) start on the row-block variant
+        if (a->lanes < 16 && a->nblocks > 0) a->lanes = 0;
         const size_t nz = (size_t)std::max<int64_t>(a->nnz, 1);
         GH_CHECK(hipMalloc(&a->ptr, sizeof(int) * (rows + 1)));
         GH_CHECK(hipMalloc(&a->idx, sizeof(int) * nz));
@@ -247,6 +288,7 @@ void csr_destroy(Csr* a) {
     (void)hipFree(a->v64);
     (void)hipFree(a->v32);
     (void)hipFree(a->v16);
+    (void)hipFree(a->blocks);
     delete a;
 }
 
@@ -254,12 +296,65 @@ template <class M, class V>
 void launch_spmv(Ctx* ctx, const Csr* a, const V* x, V* y) {
     if (a->rows == 0) return;
     const CsrView<M> v = a->view<M>();
+    if (a->lanes == 0) {
+        hipLaunchKernelGGL((k_spmv_rb<M, V>), dim3(a->nblocks), dim3(kBlock), 0, ctx->stream, v, a->blocks, x, y);
+        GH_CHECK(hipGetLastError());
+        return;
+    }
     with_lanes(a->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         hipLaunchKernelGGL((k_spmv<LPR, M, V>), dim3(grid_for((int64_t)a->rows * LPR)), dim3(kBlock), 0,
                            ctx->stream, v, x, y);
     });
     GH_CHECK(hipGetLastError());
+}
+
+// Time the stand-alone SpMV of each candidate variant (the row-block variant and the row-group widths
+// around the histogram choice) with HIP events and keep the fastest.  The fused kernels share the row
+// traversal of the stand-alone one, so its ranking carries over.
+void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps) {
+    if (a->rows == 0 || a->nnz == 0) return;
+    std::vector<int> cand{0};
+    for (int l : {1, 2, 4, 8, 16, 32, 64}) cand.push_back(l);
+    const size_t vs = dtype_size(vdt);
+    void *x = nullptr, *y = nullptr;
+    GH_CHECK(hipMalloc(&x, vs * std::max(a->cols, 1)));
+    GH_CHECK(hipMalloc(&y, vs * std::max(a->rows, 1)));
+    hipEvent_t e0, e1;
+    GH_CHECK(hipEventCreate(&e0));
+    GH_CHECK(hipEventCreate(&e1));
+    int best = a->lanes;
+    float best_ms = 1e30f;
+    try {
+        GH_CHECK(hipMemsetAsync(x, 0, vs * std::max(a->cols, 1), ctx->stream));
+        const int keep = a->lanes;
+        for (int l : cand) {
+            a->lanes = l;
+            with_mdt(mdt, [&](auto MT) {
+                with_vdt(vdt, [&](auto VT) {
+                    using M = typename decltype(MT)::type;
+                    using V = typename decltype(VT)::type;
+                    launch_spmv<M, V>(ctx, a, (const V*)x, (V*)y);  // warm-up
+                    GH_CHECK(hipEventRecord(e0, ctx->stream));
+                    for (int r = 0; r < reps; ++r) launch_spmv<M, V>(ctx, a, (const V*)x, (V*)y);
+                    GH_CHECK(hipEventRecord(e1, ctx->stream));
+                });
+            });
+            GH_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best_ms) { best_ms = ms; best = l; }
+        }
+        a->lanes = keep;
+    } catch (...) {
+        (void)hipFree(x); (void)hipFree(y); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        throw;
+    }
+    a->lanes = best;
+    (void)hipFree(x);
+    (void)hipFree(y);
+    GH_CHECK(hipEventDestroy(e0));
+    GH_CHECK(hipEventDestroy(e1));
 }
 
 }  // namespace ganit_hip
@@ -485,8 +580,15 @@ int ganit_hip_csr_plan(const ganit_hip_csr* h, int* lanes, int64_t* hist) {
 int ganit_hip_csr_set_lanes(ganit_hip_csr* h, int lanes) {
     return guarded([&] {
         GH_REQUIRE(h, "null matrix");
-        with_lanes(lanes, [](auto) {});
+        GH_REQUIRE(valid_variant(lanes), "lanes per row must be 0 (row blocks) or a power of two in 1..64");
         reinterpret_cast<Csr*>(h)->lanes = lanes;
+    });
+}
+
+int ganit_hip_csr_autotune(ganit_hip_ctx* h, ganit_hip_csr* ah, int mdt, int vdt, int reps) {
+    return guarded([&] {
+        GH_REQUIRE(h && ah && reps > 0, "bad autotune arguments");
+        autotune_csr(reinterpret_cast<Ctx*>(h), reinterpret_cast<Csr*>(ah), mdt, vdt, reps);
     });
 }
 

@@ -47,6 +47,7 @@ def lib() -> ctypes.CDLL:
         "ganit_hip_csr_free": [_vp],
         "ganit_hip_csr_plan": [_vp, ctypes.POINTER(_i), _vp],
         "ganit_hip_csr_set_lanes": [_vp, _i],
+        "ganit_hip_csr_autotune": [_vp, _vp, _i, _i, _i],
         "ganit_hip_spmv": [_vp, _vp, _i, _i, _vp, _vp],
         "ganit_hip_primal_step": [_vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _d, _vp, _vp, _vp],
         "ganit_hip_dual_step": [_vp, _i, _i, _vp, _vp, _vp, _vp, _d, _vp, _vp],
@@ -188,6 +189,10 @@ class DeviceCsr:
 
     def set_lanes(self, lanes: int):
         check(lib().ganit_hip_csr_set_lanes(self.h, lanes), "csr_set_lanes")
+
+    def autotune(self, mdt: int = F64, vdt: int = F64, reps: int = 5) -> int:
+        check(lib().ganit_hip_csr_autotune(self.ctx.h, self.h, mdt, vdt, reps), "csr_autotune")
+        return self.plan()[0]
 
     def spmv(self, x: DeviceVector, y: DeviceVector, mdt: int = F64):
         check(lib().ganit_hip_spmv(self.ctx.h, self.h, mdt, x.vdt, x.ptr, y.ptr), "spmv")

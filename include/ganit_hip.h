@@ -79,11 +79,15 @@ int ganit_hip_vec_convert(ganit_hip_ctx* ctx, int src_dt, const void* src, int d
 int ganit_hip_csr_upload(ganit_hip_ctx* ctx, int rows, int cols, const int* ptr, const int* idx,
                          const double* val, unsigned dtype_mask, ganit_hip_csr** out);
 int ganit_hip_csr_free(ganit_hip_csr* a);
-/* lanes_per_row: the variant in use (1..64); hist[k] (k = 0..7) counts rows with length in
+/* lanes_per_row: the variant in use (0 = row blocks, else 1..64); hist[k] (k = 0..7) counts rows with length in
  * [0,1], (1,2], (2,4], ... , (64,128], and hist[8] rows longer than 128. */
 int ganit_hip_csr_plan(const ganit_hip_csr* a, int* lanes_per_row, int64_t* hist9);
-/* Override the variant (power of two in 1..64); used by the autotuner and by tests. */
+/* Override the variant: a power of two in 1..64 selects row groups of that many lanes per row; 0 selects
+ * LDS-staged row blocks (up to 256 rows / 2048 nonzeros per workgroup, products staged in LDS and each row
+ * summed sequentially in row order, so those sums are bit-identical to a scalar CPU row loop). */
 int ganit_hip_csr_set_lanes(ganit_hip_csr* a, int lanes_per_row);
+/* Time every variant with `reps` stand-alone SpMVs in (mdt, vdt) and keep the fastest. */
+int ganit_hip_csr_autotune(ganit_hip_ctx* ctx, ganit_hip_csr* a, int mdt, int vdt, int reps);
 
 /* ------------------------------------------------------------------ backend-concept kernels
  * spmv_A / spmv_AT / spmv_Q (src/backend_cuda.cu:324-335): y = a * x, accumulated in vdt. */
@@ -121,7 +125,9 @@ typedef struct {
 int ganit_hip_pdhg_create(ganit_hip_ctx* ctx, const ganit_hip_lp_desc* lp, unsigned mdt_mask, unsigned vdt_mask,
                           ganit_hip_pdhg** out);
 int ganit_hip_pdhg_destroy(ganit_hip_pdhg* pd);
-/* The matrices' variant plans (A, AT, Q); lanes may be overridden per matrix (0 = A, 1 = AT, 2 = Q). */
+/* The matrices' variant plans (A, AT, Q); the variant may be overridden per matrix (which: 0 = A, 1 = AT,
+ * 2 = Q), with the lanes_per_row values of ganit_hip_csr_set_lanes.  ganit_hip_pdhg_create picks each
+ * matrix's variant by timing them (ganit_hip_csr_autotune) unless GANIT_HIP_AUTOTUNE=0. */
 int ganit_hip_pdhg_plan(const ganit_hip_pdhg* pd, int which, int* lanes_per_row, int64_t* hist9);
 int ganit_hip_pdhg_set_lanes(ganit_hip_pdhg* pd, int which, int lanes_per_row);
 

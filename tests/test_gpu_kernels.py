@@ -15,7 +15,7 @@ pytestmark = pytest.mark.gpu
 
 F64, F32, BF16 = 0, 1, 2
 PAIRS = [(F64, F64), (F32, F64), (F32, F32), (BF16, F32), (BF16, F64)]
-LANES = [1, 2, 4, 8, 16, 32, 64]
+LANES = [0, 1, 2, 4, 8, 16, 32, 64]  # 0 = LDS-staged row blocks
 
 
 def ragged_csr(rng, m, n, integer=False):
@@ -89,6 +89,39 @@ def test_spmv_integer_data_is_exact(hipctx, lanes):
         assert np.array_equal(yd.numpy(), want)
 
 
+@pytest.mark.parametrize("mdt,vdt", PAIRS)
+def test_rowblock_spmv_bitwise_vs_oracle(hipctx, oracle, mdt, vdt):
+    """Row blocks stage rounded products in LDS and add each row up in row order: the row sums are the CPU
+    loop's bits exactly (rows up to 256 entries; longer rows take the block-reduction path)."""
+    from ganit_v2_amd import hip
+    rng = np.random.default_rng(20 + 3 * mdt + vdt)
+    m, n = 3000, 5000
+    lens = rng.integers(0, 257, m)
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens]).astype(np.int32)
+    val = rng.standard_normal(ptr[-1])
+    x = rng.standard_normal(n)
+    A = hip.DeviceCsr(hipctx, m, n, ptr, idx, val)
+    A.set_lanes(0)
+    xd, yd = hipctx.vector(x, vdt), hipctx.empty(m, vdt)
+    A.spmv(xd, yd, mdt)
+    assert np.array_equal(yd.numpy(), oracle.spmv(mdt, vdt, m, n, ptr, idx, val, x))
+
+
+def test_autotune_picks_a_valid_variant(hipctx):
+    from ganit_v2_amd import hip
+    import ganit_v2_amd as g
+    s = g.Problem.generate("blending", crudes=20, products=6, specs=5, periods=30, seed=0).scaled()
+    for M in (s["A"], s["AT"]):
+        d = hip.DeviceCsr(hipctx, M.rows, M.cols, M.ptr, M.idx, M.val)
+        lanes = d.autotune(F32, F32, 3)
+        assert lanes in LANES
+        x = np.random.default_rng(0).standard_normal(M.cols)
+        xd, yd = hipctx.vector(x), hipctx.empty(M.rows)
+        d.spmv(xd, yd, F64)
+        assert np.allclose(yd.numpy(), M.matvec(x), rtol=1e-12, atol=1e-12)
+
+
 def test_spmv_plan_from_row_histogram(hipctx):
     from ganit_v2_amd import hip
     import ganit_v2_amd as g
@@ -100,6 +133,9 @@ def test_spmv_plan_from_row_histogram(hipctx):
     assert ha.sum() == s["m"] and ht.sum() == s["n"]
     assert lt == 16 and ht[4] == s["n"]          # every A^T row has exactly 10 entries -> bin (8,16]
     assert la == 32                               # rows of A average 25 entries
+    blend = g.Problem.generate("blending", crudes=20, products=6, specs=5, periods=30, seed=0).scaled()
+    BT = hip.DeviceCsr(hipctx, blend["n"], blend["m"], blend["AT"].ptr, blend["AT"].idx, blend["AT"].val)
+    assert BT.plan()[0] == 0                      # short columns (<16 entries) start on the row blocks
 
 
 @pytest.mark.parametrize("vdt", [F64, F32])
@@ -292,6 +328,34 @@ def test_pdhg_iteration_matches_oracle_steps(hipctx, oracle):
     scale_x = tau * (np.abs(AT.val) @ np.ones(1) if False else 1.0)
     assert np.max(np.abs(x1 - xn)) <= 1e-12 * (1 + np.max(np.abs(xn)))
     assert np.max(np.abs(y1 - yn)) <= 1e-11 * (1 + np.max(np.abs(yn)))
+
+
+def test_rowblock_iteration_is_bit_exact(hipctx, oracle):
+    """With A and A^T on row blocks, one fused iteration reproduces the oracle's spmv_AT -> primal_step ->
+    spmv_A -> dual_step bit for bit (FP64 and FP32)."""
+    import ganit_v2_amd as g
+    from ganit_v2_amd import hip
+    s = g.Problem.read_mps(instance_path("25fv47")).scaled()
+    assert max(np.diff(s["A"].ptr).max(), np.diff(s["AT"].ptr).max()) <= 256
+    m, n = s["m"], s["n"]
+    pd = hip.DevicePdhg(hipctx, s, mdt_mask=3, vdt_mask=3)
+    pd.set_lanes(0, 0)
+    pd.set_lanes(1, 0)
+    rng = np.random.default_rng(6)
+    for vdt, mdt in ((F64, F64), (F32, F32)):
+        cast = (lambda a: a) if vdt == F64 else (lambda a: a.astype(np.float32).astype(np.float64))
+        x0 = cast(np.clip(rng.standard_normal(n), s["l"], s["u"]))
+        y0 = cast(rng.standard_normal(m))
+        pd.load(vdt, x0, y0)
+        tau, sigma = 0.3, 0.4
+        pd.iterate(mdt, vdt, 1, tau, sigma)
+        x1, y1, _, _ = pd.store(vdt)
+        AT, A = s["AT"], s["A"]
+        aty = oracle.spmv(mdt, vdt, n, m, AT.ptr, AT.idx, AT.val, y0)
+        xn, xbar, _ = oracle.primal_step(vdt, x0, aty, None, s["c"], s["l"], s["u"], tau, np.zeros(n))
+        axb = oracle.spmv(mdt, vdt, m, n, A.ptr, A.idx, A.val, xbar)
+        yn, _ = oracle.dual_step(vdt, y0, axb, s["lo"], s["hi"], sigma, np.zeros(m))
+        assert np.array_equal(x1, xn) and np.array_equal(y1, yn)
 
 
 def test_graph_replay_matches_eager(hipctx, monkeypatch):
