@@ -158,13 +158,6 @@ struct ReduceSlot {
     double* out;  // N doubles
 };
 
-__device__ __forceinline__ void store_coherent(double* p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_coherent(const double* p) {
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // Block-wide sum of N values per thread; the result is valid in thread 0.  `lds` must hold
 // (blockDim.x / 64) * N doubles.
 template <int N>
@@ -192,13 +185,16 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* lds) {
 }
 
 // Grid-wide sum: every block calls this with its per-thread values; the last block writes slot.out.
+// Memory ordering: a block's partials are published by the agent-scope release fence before its ticket
+// increment; the last block's agent-scope acquire fence (after it reads the final ticket) makes every
+// partial visible to its ordinary loads, across the XCDs' separate L2s.
 template <int N>
 __device__ void grid_sum(double (&v)[N], const ReduceSlot& slot, double* lds) {
     block_sum<N>(v, lds);
     __shared__ bool last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) store_coherent(&slot.partials[(size_t)blockIdx.x * N + k], v[k]);
+        for (int k = 0; k < N; ++k) slot.partials[(size_t)blockIdx.x * N + k] = v[k];
         __threadfence();
         const unsigned t = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         last = (t == gridDim.x - 1);
@@ -211,8 +207,9 @@ __device__ void grid_sum(double (&v)[N], const ReduceSlot& slot, double* lds) {
 #pragma unroll
     for (int k = 0; k < N; ++k) s[k] = 0.0;
     for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
+        const double* p = slot.partials + (size_t)b * N;
 #pragma unroll
-        for (int k = 0; k < N; ++k) s[k] += load_coherent(&slot.partials[(size_t)b * N + k]);
+        for (int k = 0; k < N; ++k) s[k] += p[k];
     }
     block_sum<N>(s, lds);
     if (threadIdx.x == 0) {

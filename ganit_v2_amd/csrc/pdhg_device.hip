@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <map>
 #include <tuple>
+#include <type_traits>
 
 #include "internal.hpp"
 #include "pdhg_math.hpp"
@@ -139,8 +140,10 @@ __device__ __forceinline__ double row_obj_term(double yi, double lw, double hg) 
     return 0.0;
 }
 
-// One pass over A^T (and Q): out[0..11] as documented for ganit_hip_pdhg_check.
-template <int LPR, class M, class V, bool HASQ>
+// One pass over A^T (and Q): out[0..11] as documented for ganit_hip_pdhg_check.  Vectors are stored in V;
+// the row sums are formed in C (C = V reproduces the level's own evaluation, C = double with FP64 matrix
+// values is the FP64 evaluation of a reduced-precision iterate).
+template <int LPR, class M, class V, class C, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q, const V* __restrict__ x,
                                                     const V* __restrict__ y, const V* __restrict__ xa,
                                                     const V* __restrict__ ya, const V* __restrict__ xr,
@@ -157,20 +160,20 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
     for (int64_t g = (int64_t)xcd_block_id() * kGroups + threadIdx.x / LPR; g < AT.rows;
          g += (int64_t)gridDim.x * kGroups) {
         const int j = (int)g;
-        V sc = V(0), sa = V(0), sr = V(0), qc = V(0), qa = V(0);
+        C sc = C(0), sa = C(0), sr = C(0), qc = C(0), qa = C(0);
         for (int p = AT.ptr[j] + lane; p < AT.ptr[j + 1]; p += LPR) {
-            const V a = widen<V>(AT.val[p]);
+            const C a = widen<C>(AT.val[p]);
             const int i = AT.idx[p];
-            sc += a * y[i];
-            if (avg) sa += a * ya[i];
-            if (ray) sr += a * V(y[i] - yr[i]);
+            sc += a * C(y[i]);
+            if (avg) sa += a * C(ya[i]);
+            if (ray) sr += a * C(V(y[i] - yr[i]));
         }
         if (HASQ) {
             for (int p = Q.ptr[j] + lane; p < Q.ptr[j + 1]; p += LPR) {
-                const V a = widen<V>(Q.val[p]);
+                const C a = widen<C>(Q.val[p]);
                 const int i = Q.idx[p];
-                qc += a * x[i];
-                if (avg) qa += a * xa[i];
+                qc += a * C(x[i]);
+                if (avg) qa += a * C(xa[i]);
             }
         }
         sc = group_sum<LPR>(sc);
@@ -210,8 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
     grid_sum<kNOutN>(acc, slot, lds);
 }
 
-// One pass over A: out[12..18] as documented for ganit_hip_pdhg_check.
-template <int LPR, class M, class V>
+// One pass over A: out[12..18] as documented for ganit_hip_pdhg_check (sums in C as for k_check_n).
+template <int LPR, class M, class V, class C>
 __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __restrict__ x, const V* __restrict__ xa,
                                                     const V* __restrict__ y, const V* __restrict__ ya,
                                                     const V* __restrict__ yr, const V* __restrict__ lo,
@@ -227,12 +230,12 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
     for (int64_t g = (int64_t)xcd_block_id() * kGroups + threadIdx.x / LPR; g < A.rows;
          g += (int64_t)gridDim.x * kGroups) {
         const int i = (int)g;
-        V sc = V(0), sa = V(0);
+        C sc = C(0), sa = C(0);
         for (int p = A.ptr[i] + lane; p < A.ptr[i + 1]; p += LPR) {
-            const V a = widen<V>(A.val[p]);
+            const C a = widen<C>(A.val[p]);
             const int j = A.idx[p];
-            sc += a * x[j];
-            if (avg) sa += a * xa[j];
+            sc += a * C(x[j]);
+            if (avg) sa += a * C(xa[j]);
         }
         sc = group_sum<LPR>(sc);
         if (avg) sa = group_sum<LPR>(sa);
@@ -259,8 +262,8 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
     grid_sum<kNOutM>(acc, slot, lds);
 }
 
-// The two check passes on LDS-staged row blocks (grid = number of row blocks).
-template <class M, class V, bool HASQ>
+// The two check passes on LDS-staged row blocks (grid = number of row blocks); sums in C as above.
+template <class M, class V, class C, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int* __restrict__ blocks, CsrView<M> Q,
                                                        const V* __restrict__ x, const V* __restrict__ y,
                                                        const V* __restrict__ xa, const V* __restrict__ ya,
@@ -268,27 +271,27 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
                                                        const V* __restrict__ c, const V* __restrict__ l,
                                                        const V* __restrict__ u, const V* __restrict__ cres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ V lds[3 * kTile];
+    __shared__ C lds[3 * kTile];
     __shared__ double red[(kBlock / kWave) * kNOutN];
     double acc[kNOutN];
 #pragma unroll
     for (int k = 0; k < kNOutN; ++k) acc[k] = 0.0;
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
-    stream_block<M, V, 3>(
+    stream_block<M, C, 3>(
         AT, blocks, xcd_block_id(),
         [&](int k, int i) {
-            return k == 0 ? y[i] : k == 1 ? (avg ? ya[i] : V(0)) : (ray ? V(y[i] - yr[i]) : V(0));
+            return k == 0 ? C(y[i]) : k == 1 ? (avg ? C(ya[i]) : C(0)) : (ray ? C(V(y[i] - yr[i])) : C(0));
         },
-        [&](int j, V (&t)[3]) {
+        [&](int j, C (&t)[3]) {
 #pragma clang fp contract(off)
-            V qc = V(0), qa = V(0);
+            C qc = C(0), qa = C(0);
             if (HASQ) {
                 for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
-                    const V a = widen<V>(Q.val[p]);
-                    const V pc = a * x[Q.idx[p]];
+                    const C a = widen<C>(Q.val[p]);
+                    const C pc = a * C(x[Q.idx[p]]);
                     qc = qc + pc;
                     if (avg) {
-                        const V pa = a * xa[Q.idx[p]];
+                        const C pa = a * C(xa[Q.idx[p]]);
                         qa = qa + pa;
                     }
                 }
@@ -323,22 +326,22 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
     grid_sum<kNOutN>(acc, slot, red);
 }
 
-template <class M, class V>
+template <class M, class V, class C>
 __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* __restrict__ blocks,
                                                        const V* __restrict__ x, const V* __restrict__ xa,
                                                        const V* __restrict__ y, const V* __restrict__ ya,
                                                        const V* __restrict__ yr, const V* __restrict__ lo,
                                                        const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ V lds[2 * kTile];
+    __shared__ C lds[2 * kTile];
     __shared__ double red[(kBlock / kWave) * kNOutM];
     double acc[kNOutM];
 #pragma unroll
     for (int k = 0; k < kNOutM; ++k) acc[k] = 0.0;
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
-    stream_block<M, V, 2>(
-        A, blocks, xcd_block_id(), [&](int k, int j) { return k == 0 ? x[j] : (avg ? xa[j] : V(0)); },
-        [&](int i, V (&t)[2]) {
+    stream_block<M, C, 2>(
+        A, blocks, xcd_block_id(), [&](int k, int j) { return k == 0 ? C(x[j]) : (avg ? C(xa[j]) : C(0)); },
+        [&](int i, C (&t)[2]) {
             const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
             auto pres = [&](double a) {
                 const double p = fmin(fmax(a, lw), hg);
@@ -541,7 +544,7 @@ void launch_iteration(Pdhg* pd, VecSet& s) {
     s.cur ^= 1;
 }
 
-template <class M, class V>
+template <class M, class V, class C>
 void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
     hipStream_t st = pd->ctx->stream;
     const V* x = (const V*)s.X[s.cur];
@@ -557,12 +560,12 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
     if (pd->AT->lanes == 0 && pd->n > 0) {
         const dim3 g(pd->AT->nblocks);
         if (pd->hasq)
-            hipLaunchKernelGGL((k_check_n_rb<M, V, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
+            hipLaunchKernelGGL((k_check_n_rb<M, V, C, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
                                pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
                                (const V*)s.yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags,
                                sn);
         else
-            hipLaunchKernelGGL((k_check_n_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, at, x, y,
+            hipLaunchKernelGGL((k_check_n_rb<M, V, C, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, at, x, y,
                                (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     } else with_lanes(pd->AT->lanes == 0 ? 1 : pd->AT->lanes, [&](auto L) {
@@ -570,23 +573,23 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
         const int64_t groups = std::max<int64_t>(pd->n, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
         if (pd->hasq)
-            hipLaunchKernelGGL((k_check_n<LPR, M, V, true>), dim3(g), dim3(kBlock), 0, st, at, pd->Q->view<M>(), x, y,
+            hipLaunchKernelGGL((k_check_n<LPR, M, V, C, true>), dim3(g), dim3(kBlock), 0, st, at, pd->Q->view<M>(), x, y,
                                (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
         else
-            hipLaunchKernelGGL((k_check_n<LPR, M, V, false>), dim3(g), dim3(kBlock), 0, st, at, at, x, y,
+            hipLaunchKernelGGL((k_check_n<LPR, M, V, C, false>), dim3(g), dim3(kBlock), 0, st, at, at, x, y,
                                (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     });
     if (pd->A->lanes == 0 && pd->m > 0) {
-        hipLaunchKernelGGL((k_check_m_rb<M, V>), dim3(pd->A->nblocks), dim3(kBlock), 0, st, a, pd->A->blocks, x,
+        hipLaunchKernelGGL((k_check_m_rb<M, V, C>), dim3(pd->A->nblocks), dim3(kBlock), 0, st, a, pd->A->blocks, x,
                            (const V*)s.xa, y, (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi,
                            (const V*)s.rres, flags, sm);
     } else with_lanes(pd->A->lanes == 0 ? 1 : pd->A->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         const int64_t groups = std::max<int64_t>(pd->m, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
-        hipLaunchKernelGGL((k_check_m<LPR, M, V>), dim3(g), dim3(kBlock), 0, st, a, x, (const V*)s.xa, y,
+        hipLaunchKernelGGL((k_check_m<LPR, M, V, C>), dim3(g), dim3(kBlock), 0, st, a, x, (const V*)s.xa, y,
                            (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags,
                            sm);
     });
@@ -790,11 +793,16 @@ int ganit_hip_pdhg_check(ganit_hip_pdhg* h, int mdt, int vdt, int flags, double 
     return guarded([&] {
         auto* pd = reinterpret_cast<Pdhg*>(h);
         GH_REQUIRE(pd && out, "null argument");
-        GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY)) == 0, "unknown check flag");
+        GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY | GANIT_CHECK_FP64)) == 0, "unknown check flag");
         VecSet& s = pd->set(vdt);
         with_mdt(mdt, [&](auto MT) {
             with_vdt(vdt, [&](auto VT) {
-                run_check<typename decltype(MT)::type, typename decltype(VT)::type>(pd, s, flags, inv_k, out);
+                using M = typename decltype(MT)::type;
+                using V = typename decltype(VT)::type;
+                if ((flags & GANIT_CHECK_FP64) && !std::is_same<V, double>::value)
+                    run_check<M, V, double>(pd, s, flags, inv_k, out);
+                else
+                    run_check<M, V, V>(pd, s, flags, inv_k, out);
             });
         });
         if (!(flags & GANIT_CHECK_AVG)) for (int k : {4, 5, 6, 7, 10, 11, 14, 15, 17, 18}) out[k] = 0.0;

@@ -314,8 +314,7 @@ void launch_spmv(Ctx* ctx, const Csr* a, const V* x, V* y) {
 // traversal of the stand-alone one, so its ranking carries over.
 void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps) {
     if (a->rows == 0 || a->nnz == 0) return;
-    std::vector<int> cand{0};
-    for (int l : {1, 2, 4, 8, 16, 32, 64}) cand.push_back(l);
+    const std::vector<int> cand{0, 1, 2, 4, 8, 16, 32, 64};
     const size_t vs = dtype_size(vdt);
     void *x = nullptr, *y = nullptr;
     GH_CHECK(hipMalloc(&x, vs * std::max(a->cols, 1)));
@@ -323,34 +322,44 @@ void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps) {
     hipEvent_t e0, e1;
     GH_CHECK(hipEventCreate(&e0));
     GH_CHECK(hipEventCreate(&e1));
-    int best = a->lanes;
-    float best_ms = 1e30f;
+    std::vector<float> best_ms(cand.size(), 1e30f);
+    const int keep = a->lanes;
     try {
         GH_CHECK(hipMemsetAsync(x, 0, vs * std::max(a->cols, 1), ctx->stream));
-        const int keep = a->lanes;
-        for (int l : cand) {
-            a->lanes = l;
+        auto launch = [&](int k) {
             with_mdt(mdt, [&](auto MT) {
                 with_vdt(vdt, [&](auto VT) {
                     using M = typename decltype(MT)::type;
                     using V = typename decltype(VT)::type;
-                    launch_spmv<M, V>(ctx, a, (const V*)x, (V*)y);  // warm-up
-                    GH_CHECK(hipEventRecord(e0, ctx->stream));
-                    for (int r = 0; r < reps; ++r) launch_spmv<M, V>(ctx, a, (const V*)x, (V*)y);
-                    GH_CHECK(hipEventRecord(e1, ctx->stream));
+                    for (int r = 0; r < k; ++r) launch_spmv<M, V>(ctx, a, (const V*)x, (V*)y);
                 });
             });
-            GH_CHECK(hipEventSynchronize(e1));
-            float ms = 0;
-            GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
-            if (ms < best_ms) { best_ms = ms; best = l; }
-        }
+        };
+        // Warm the clocks and caches first, then time every candidate in two interleaved rounds and keep
+        // each one's best time, so a candidate timed early is not penalised by a cold GPU.
         a->lanes = keep;
+        launch(20);
+        for (int round = 0; round < 2; ++round) {
+            for (size_t c = 0; c < cand.size(); ++c) {
+                a->lanes = cand[c];
+                launch(2);
+                GH_CHECK(hipEventRecord(e0, ctx->stream));
+                launch(reps);
+                GH_CHECK(hipEventRecord(e1, ctx->stream));
+                GH_CHECK(hipEventSynchronize(e1));
+                float ms = 0;
+                GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+                best_ms[c] = std::min(best_ms[c], ms);
+            }
+        }
     } catch (...) {
+        a->lanes = keep;
         (void)hipFree(x); (void)hipFree(y); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
         throw;
     }
-    a->lanes = best;
+    size_t bi = 0;
+    for (size_t c = 1; c < cand.size(); ++c) if (best_ms[c] < best_ms[bi]) bi = c;
+    a->lanes = cand[bi];
     (void)hipFree(x);
     (void)hipFree(y);
     GH_CHECK(hipEventDestroy(e0));

@@ -87,10 +87,11 @@ KKT kkt_from(const Shared& sh, double rd2, double vobj, double cx, double xqx, d
 KKT kkt_current(const Shared& sh, const double* o) { return kkt_from(sh, o[0], o[1], o[2], o[3], o[12], o[13]); }
 KKT kkt_average(const Shared& sh, const double* o) { return kkt_from(sh, o[4], o[5], o[6], o[7], o[14], o[15]); }
 
-// KKT of the current iterate of working set `vdt` evaluated with matrix values `mdt`.
-KKT evaluate(Shared& sh, ganit_hip_pdhg* pd, int mdt, int vdt) {
+// KKT of the current iterate of working set `vdt` evaluated with matrix values `mdt` (extra check flags,
+// e.g. GANIT_CHECK_FP64, in `flags`).
+KKT evaluate(Shared& sh, ganit_hip_pdhg* pd, int mdt, int vdt, int flags = 0) {
     double o[GANIT_CHECK_NOUT];
-    chk(ganit_hip_pdhg_check(pd, mdt, vdt, 0, 0.0, o));
+    chk(ganit_hip_pdhg_check(pd, mdt, vdt, flags, 0.0, o));
     return kkt_current(sh, o);
 }
 
@@ -104,8 +105,11 @@ Exit phase(Shared& sh, ganit_hip_pdhg* pd, const Level& lv, State& st, double to
     const int ce = opt.check_every;
     double w = st.w;
     chk(ganit_hip_pdhg_handover(pd, lv.vdt, lv.vdt));  // fresh running sums
+    // checks of a reduced-precision level: at the level's precision (the reference's rule) or in FP64
+    const bool e64 = opt.eval_fp64 && (lv.mdt != GANIT_F64 || lv.vdt != GANIT_F64);
+    const int ev_mdt = e64 ? GANIT_F64 : lv.mdt, ev_flags = e64 ? GANIT_CHECK_FP64 : 0;
 
-    KKT k_restart = evaluate(sh, pd, lv.mdt, lv.vdt);
+    KKT k_restart = evaluate(sh, pd, ev_mdt, lv.vdt, ev_flags);
     if (st.restart_err > 0) k_restart.err = std::max(k_restart.err, st.restart_err);
     double k_prev_cand = std::numeric_limits<double>::infinity();
     long it_since = 0;
@@ -127,7 +131,7 @@ Exit phase(Shared& sh, ganit_hip_pdhg* pd, const Level& lv, State& st, double to
         it_since += to_check;
 
         const int flags = GANIT_CHECK_AVG | (it_since >= 4L * ce ? GANIT_CHECK_RAY : 0);
-        chk(ganit_hip_pdhg_check(pd, lv.mdt, lv.vdt, flags, 1.0 / static_cast<double>(it_since), o));
+        chk(ganit_hip_pdhg_check(pd, ev_mdt, lv.vdt, flags | ev_flags, 1.0 / static_cast<double>(it_since), o));
         ++st.checks;
         const KKT kc = kkt_current(sh, o);
         const KKT ka = kkt_average(sh, o);
@@ -279,7 +283,10 @@ Result GpuSolver::run() {
     if (opt.verbose) {
         std::printf("GANIT PDHG | device %s | %s | precision %s%s | rows %d cols %d nnz %lld | ||A||~%.3e | tol %.1e\n",
                     res.device.c_str(), s.has_q() ? "QP" : "LP", pname,
-                    mixed && opt.values == Values::BF16 ? " (BF16 values)" : "", s.m, s.n, (long long)s.A.nnz(),
+                    mixed ? (opt.values == Values::BF16 ? (opt.eval_fp64 ? " (BF16 values, FP64 checks)" : " (BF16 values)")
+                                                        : (opt.eval_fp64 ? " (FP64 checks)" : ""))
+                          : "",
+                    s.m, s.n, (long long)s.A.nnz(),
                     I.normA, opt.tol);
         if (s.has_q()) std::printf("Hessian nnz %lld | ||Q||~%.3e\n", (long long)s.Q.nnz(), s.normQ);
         std::printf("%9s %9s %4s %15s %15s %9s %9s %9s %9s\n", "iter", "time", "prec", "primal obj", "dual obj",

@@ -24,6 +24,8 @@ static void usage() {
         "  --precision double|mixed|dynamic  PDHG arithmetic; mixed: FP32->FP64,\n"
         "                     dynamic: FP32 -> FP32 matrix/FP64 vectors -> FP64 (always FP64-certified)\n"
         "  --values f32|bf16  matrix values of the reduced-precision levels (default f32)\n"
+        "  --eval level|fp64  KKT checks of the reduced-precision levels at their own precision (default)\n"
+        "                     or in FP64 with the FP64 matrix\n"
         "  --time S           time limit in seconds (default 3600)\n"
         "  --iters N          iteration limit\n"
         "  --stall-checks K   checks without progress before leaving L1 (default 12)\n"
@@ -63,6 +65,12 @@ int main(int argc, char** argv) {
             if (v == "f32") opt.values = Values::F32;
             else if (v == "bf16") opt.values = Values::BF16;
             else { std::fprintf(stderr, "--values must be f32 or bf16\n"); return 1; }
+        }
+        else if (s == "--eval") {
+            const std::string v = next();
+            if (v == "level") opt.eval_fp64 = false;
+            else if (v == "fp64") opt.eval_fp64 = true;
+            else { std::fprintf(stderr, "--eval must be level or fp64\n"); return 1; }
         }
         else if (s == "--time") opt.time_limit = std::stod(next());
         else if (s == "--iters") opt.max_iter = std::stol(next());

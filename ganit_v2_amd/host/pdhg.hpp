@@ -30,6 +30,8 @@ struct Options {
     int stall_checks_l2 = 24;    // same for L2
     // extensions
     Values values = Values::F32;  // matrix values in L1/L2
+    bool eval_fp64 = false;       // L1/L2 checks (restart candidates, stall and convergence tests) evaluated
+                                  // in FP64 with the FP64 matrix instead of at the level's own precision
     int device = 0;               // HIP device ordinal
 };
 

@@ -147,6 +147,8 @@ int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, doub
 /* Check flags and the layout of the sums ganit_hip_pdhg_check returns (all FP64-accumulated). */
 #define GANIT_CHECK_AVG 1  /* also form (xa, ya) = (sumx, sumy) * inv_k and evaluate them          */
 #define GANIT_CHECK_RAY 2  /* also the infeasibility ray y - yr (src/pdhg_impl.hpp:110-118)        */
+#define GANIT_CHECK_FP64 4 /* form the row sums in FP64 even for an FP32 working set (with mdt =
+                              GANIT_F64: the FP64 evaluation of a reduced-precision iterate)        */
 #define GANIT_CHECK_NOUT 19
 /* out[ 0.. 3]  current point : rd2, vobj, c'x, x'Qx      (dual_res, dot_c, dot with Qx)
  * out[ 4.. 7]  average point : rd2, vobj, c'x, x'Qx

@@ -335,7 +335,7 @@ def test_rowblock_iteration_is_bit_exact(hipctx, oracle):
     spmv_A -> dual_step bit for bit (FP64 and FP32)."""
     import ganit_v2_amd as g
     from ganit_v2_amd import hip
-    s = g.Problem.read_mps(instance_path("25fv47")).scaled()
+    s = g.Problem.generate("random_lp", seed=4).scaled()
     assert max(np.diff(s["A"].ptr).max(), np.diff(s["AT"].ptr).max()) <= 256
     m, n = s["m"], s["n"]
     pd = hip.DevicePdhg(hipctx, s, mdt_mask=3, vdt_mask=3)
@@ -370,7 +370,9 @@ def test_graph_replay_matches_eager(hipctx, monkeypatch):
     pg = hip.DevicePdhg(hipctx, s, 1, 1)
     monkeypatch.setenv("GANIT_HIP_NO_GRAPH", "1")
     pe = hip.DevicePdhg(hipctx, s, 1, 1)
-    for pd in (pg, pe):
+    for pd in (pg, pe):  # same variants in both (the autotuner may pick differently per object)
+        pd.set_lanes(0, 0)
+        pd.set_lanes(1, 8)
         pd.load(F64, x0, y0)
     for rep in range(4):
         for pd in (pg, pe):
@@ -382,6 +384,26 @@ def test_graph_replay_matches_eager(hipctx, monkeypatch):
         oa = pg.check(F64, F64, hip.CHECK_AVG, 1.0 / (64 * (rep + 1)))
         ob = pe.check(F64, F64, hip.CHECK_AVG, 1.0 / (64 * (rep + 1)))
         assert np.array_equal(oa, ob)
+
+
+def test_fp64_check_of_fp32_iterate(hipctx):
+    """GANIT_CHECK_FP64 on an FP32 working set equals the FP64 check of the same (widened) iterate."""
+    import ganit_v2_amd as g
+    from ganit_v2_amd import hip
+    s = g.Problem.read_mps(instance_path("25fv47")).scaled()
+    pd = hip.DevicePdhg(hipctx, s, mdt_mask=3, vdt_mask=3)
+    for which, lanes in ((0, 0), (1, 0)):
+        pd.set_lanes(which, lanes)
+    rng = np.random.default_rng(11)
+    x = np.clip(rng.standard_normal(s["n"]), s["l"], s["u"]).astype(np.float32).astype(np.float64)
+    y = rng.standard_normal(s["m"]).astype(np.float32).astype(np.float64)
+    pd.load(F32, x, y)
+    pd.load(F64, x, y)
+    a = pd.check(F64, F32, hip.CHECK_RAY | hip.CHECK_FP64)
+    b = pd.check(F64, F64, hip.CHECK_RAY)
+    assert np.array_equal(a, b)
+    c = pd.check(F32, F32, hip.CHECK_RAY)   # the level's own evaluation differs (FP32 sums)
+    assert not np.array_equal(a, c)
 
 
 def test_handover_and_restart_semantics(hipctx):

@@ -84,6 +84,16 @@ def test_ladder_is_fp64_certified(name, precision, values):
     assert abs(r["pobj"] - d64["pobj"]) <= 5e-4 * (1 + abs(d64["pobj"]))
 
 
+@pytest.mark.parametrize("name", ["adlittle", "25fv47", "random_lp", "port_small"])
+def test_ladder_with_fp64_checks(name):
+    p = g.Problem.generate("random_lp", seed=1) if name == "random_lp" else g.Problem.read_mps(instance_path(name))
+    d64 = p.solve_gpu(time_limit=120)
+    for values in ("f32", "bf16"):
+        r = p.solve_gpu(precision="mixed", values=values, eval_fp64=True, time_limit=120)
+        assert r["status"] == "OPTIMAL" and _certified(r, 1e-4)
+        assert abs(r["pobj"] - d64["pobj"]) <= 5e-4 * (1 + abs(d64["pobj"]))
+
+
 def test_ladder_escalates_below_the_fp32_floor():
     """A tolerance under the FP32 accuracy floor forces the FP64 level (L1 -> L3)."""
     p = g.Problem.read_mps(instance_path("adlittle"))
