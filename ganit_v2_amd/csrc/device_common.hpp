@@ -22,6 +22,10 @@ constexpr int kNumXcd = 8;
 // longer than kLong is a block of its own and is reduced by the whole block instead.
 constexpr int kTile = 2048;
 constexpr int kLong = 256;
+// Rows longer than kSplit are cut into chunks of kChunk nonzeros, one workgroup each; the partial sums are
+// combined in chunk order by the last chunk to finish (so very long rows spread over many CUs).
+constexpr int kSplit = 8192;
+constexpr int kChunk = 4096;
 
 // ---------------------------------------------------------------- BF16 storage
 struct bf16_t {
@@ -101,6 +105,18 @@ __device__ __forceinline__ void block_sum_v(V (&v)[NX], V* lds /* (kBlock / kWav
     }
 }
 
+// ---------------------------------------------------------------- split rows
+// chunk[4c .. 4c+3] = (row, first nonzero, end nonzero, split-row index s); first[s] .. first[s+1] are the
+// chunks of split row s; ticket[s] counts its finished chunks (zero between launches); part holds NX
+// partial sums per chunk (stored as double, formed in the vector type).
+struct SplitPlan {
+    int nblocks;  // workgroups [0, nblocks) are row blocks, [nblocks, nblocks + nchunks) are chunks
+    const int* __restrict__ chunk;
+    const int* __restrict__ first;
+    unsigned int* ticket;
+    double* part;
+};
+
 // ---------------------------------------------------------------- LDS-staged row blocks
 // One block of the stream variant: rows [r0, r1) of `a` whose nonzeros fit one LDS tile.  Every product
 // val * x_k[idx] (k < NX gathered vectors, `gather(k, i)`) is formed by a coalesced sweep over the block's
@@ -110,10 +126,48 @@ __device__ __forceinline__ void block_sum_v(V (&v)[NX], V* lds /* (kBlock / kWav
 // `row_fn(r, sums)` runs once per row, in the thread that owns it (thread 0 for a long row).
 template <class M, class V, int NX, class Gather, class RowFn>
 __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __restrict__ blocks, int b, Gather gather,
-                                             RowFn row_fn, V* lds /* NX * kTile */) {
+                                             RowFn row_fn, V* lds /* NX * kTile */, const SplitPlan& sp) {
+    if (b >= sp.nblocks) {  // one chunk of a split row
+        const int c = b - sp.nblocks;
+        const int row = sp.chunk[4 * c], p0 = sp.chunk[4 * c + 1], p1 = sp.chunk[4 * c + 2], sr = sp.chunk[4 * c + 3];
+        V s[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) s[k] = V(0);
+        for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+            const V av = widen<V>(a.val[p]);
+            const int i = a.idx[p];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+        }
+        block_sum_v<V, NX>(s, lds);
+        __shared__ bool last_chunk;
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int k = 0; k < NX; ++k) sp.part[(size_t)c * NX + k] = (double)s[k];
+            __threadfence();
+            const int nch = sp.first[sr + 1] - sp.first[sr];
+            const unsigned t = __hip_atomic_fetch_add(&sp.ticket[sr], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            last_chunk = (t == (unsigned)nch - 1);
+        }
+        __syncthreads();
+        if (last_chunk && threadIdx.x == 0) {
+            __threadfence();
+            V tot[NX];
+#pragma unroll
+            for (int k = 0; k < NX; ++k) tot[k] = V(0);
+            for (int q = sp.first[sr]; q < sp.first[sr + 1]; ++q) {
+#pragma unroll
+                for (int k = 0; k < NX; ++k) tot[k] += (V)sp.part[(size_t)q * NX + k];
+            }
+            __hip_atomic_store(&sp.ticket[sr], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            row_fn(row, tot);
+        }
+        return;
+    }
     const int r0 = blocks[b], r1 = blocks[b + 1];
     const int base = a.ptr[r0];
     const int cnt = a.ptr[r1] - base;
+    if (r1 - r0 == 1 && cnt > kSplit) return;  // a split row: its chunks do the work
     if (r1 - r0 == 1 && cnt > kLong) {
         V s[NX];
 #pragma unroll

@@ -82,6 +82,14 @@ struct Csr {
     int64_t hist[9] = {0};
     int* blocks = nullptr;   // row-block starts of the LDS-staged variant (nblocks + 1 entries, device)
     int nblocks = 0;
+    // rows longer than kSplit nonzeros, cut into kChunk chunks (device arrays; see SplitPlan)
+    int nsplit = 0, nchunks = 0;
+    int* chunk = nullptr;
+    int* split_first = nullptr;
+    unsigned int* split_ticket = nullptr;
+    double* split_part = nullptr;  // 3 doubles per chunk
+    SplitPlan split() const { return {nblocks, chunk, split_first, split_ticket, split_part}; }
+    unsigned rb_grid() const { return (unsigned)(nblocks + nchunks); }
     template <class M>
     CsrView<M> view() const;
 };

@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_dual(CsrView<M> A, const V* __r
 // The same two kernels on LDS-staged row blocks (one block per row block; see stream_block).
 template <class M, class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const int* __restrict__ blocks,
-                                                           CsrView<M> Q, const V* __restrict__ y,
+                                                           SplitPlan sp, CsrView<M> Q, const V* __restrict__ y,
                                                            const V* __restrict__ x, const V* __restrict__ c,
                                                            const V* __restrict__ l, const V* __restrict__ u,
                                                            const double* __restrict__ prm, V* __restrict__ xn,
@@ -82,12 +82,12 @@ __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const 
             primal_update<V>(x[j], aty[0], qx, c[j], l[j], u[j], V(prm[0]), xn[j], xbar[j], s);
             sumx[j] = s;
         },
-        lds);
+        lds, sp);
 }
 
 template <class M, class V>
 __global__ __launch_bounds__(kBlock) void k_iter_dual_rb(CsrView<M> A, const int* __restrict__ blocks,
-                                                         const V* __restrict__ xbar, const V* __restrict__ y,
+                                                         SplitPlan sp, const V* __restrict__ xbar, const V* __restrict__ y,
                                                          const V* __restrict__ lo, const V* __restrict__ hi,
                                                          const double* __restrict__ prm, V* __restrict__ yn,
                                                          V* __restrict__ sumy) {
@@ -99,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_dual_rb(CsrView<M> A, const int
             dual_update<V>(y[i], axb[0], lo[i], hi[i], V(prm[1]), yn[i], s);
             sumy[i] = s;
         },
-        lds);
+        lds, sp);
 }
 
 __global__ void k_set_steps(double* prm, double tau, double sigma) {
@@ -264,7 +264,8 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
 
 // The two check passes on LDS-staged row blocks (grid = number of row blocks); sums in C as above.
 template <class M, class V, class C, bool HASQ>
-__global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int* __restrict__ blocks, CsrView<M> Q,
+__global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int* __restrict__ blocks, SplitPlan sp,
+                                                       CsrView<M> Q,
                                                        const V* __restrict__ x, const V* __restrict__ y,
                                                        const V* __restrict__ xa, const V* __restrict__ ya,
                                                        const V* __restrict__ xr, const V* __restrict__ yr,
@@ -321,13 +322,13 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
                 acc[9] += ob;
             }
         },
-        lds);
+        lds, sp);
     __syncthreads();
     grid_sum<kNOutN>(acc, slot, red);
 }
 
 template <class M, class V, class C>
-__global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* __restrict__ blocks,
+__global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* __restrict__ blocks, SplitPlan sp,
                                                        const V* __restrict__ x, const V* __restrict__ xa,
                                                        const V* __restrict__ y, const V* __restrict__ ya,
                                                        const V* __restrict__ yr, const V* __restrict__ lo,
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
             }
             if (ray) acc[4] += row_obj_term(double(V(y[i] - yr[i])), lw, hg);
         },
-        lds);
+        lds, sp);
     __syncthreads();
     grid_sum<kNOutM>(acc, slot, red);
 }
@@ -500,13 +501,14 @@ void launch_one(Pdhg* pd, VecSet& s, int which) {
     V* yn = (V*)s.Y[1 - s.cur];
     if (which == 0 && pd->n > 0 && pd->AT->lanes == 0) {
         const CsrView<M> at = pd->AT->view<M>();
-        const dim3 g(pd->AT->nblocks);
+        const dim3 g(pd->AT->rb_grid());
         if (pd->hasq)
             hipLaunchKernelGGL((k_iter_primal_rb<M, V, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
-                               pd->Q->view<M>(), y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn,
+                               pd->AT->split(), pd->Q->view<M>(), y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn,
                                (V*)s.xbar, (V*)s.sumx);
         else
-            hipLaunchKernelGGL((k_iter_primal_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, at, y, x,
+            hipLaunchKernelGGL((k_iter_primal_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
+                               pd->AT->split(), at, y, x,
                                (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar, (V*)s.sumx);
     } else if (which == 0 && pd->n > 0) {
         const CsrView<M> at = pd->AT->view<M>();
@@ -523,8 +525,8 @@ void launch_one(Pdhg* pd, VecSet& s, int which) {
         });
     }
     if (which == 1 && pd->m > 0 && pd->A->lanes == 0) {
-        hipLaunchKernelGGL((k_iter_dual_rb<M, V>), dim3(pd->A->nblocks), dim3(kBlock), 0, st, pd->A->view<M>(),
-                           pd->A->blocks, (const V*)s.xbar, y, (const V*)s.lo, (const V*)s.hi, pd->prm, yn,
+        hipLaunchKernelGGL((k_iter_dual_rb<M, V>), dim3(pd->A->rb_grid()), dim3(kBlock), 0, st, pd->A->view<M>(),
+                           pd->A->blocks, pd->A->split(), (const V*)s.xbar, y, (const V*)s.lo, (const V*)s.hi, pd->prm, yn,
                            (V*)s.sumy);
     } else if (which == 1 && pd->m > 0) {
         const CsrView<M> a = pd->A->view<M>();
@@ -545,11 +547,11 @@ void launch_iteration(Pdhg* pd, VecSet& s) {
 }
 
 template <class M, class V, class C>
-void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
+void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, bool form_avg = true) {
     hipStream_t st = pd->ctx->stream;
     const V* x = (const V*)s.X[s.cur];
     const V* y = (const V*)s.Y[s.cur];
-    if (flags & GANIT_CHECK_AVG) {
+    if ((flags & GANIT_CHECK_AVG) && form_avg) {
         hipLaunchKernelGGL(k_average<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st, pd->n, pd->m,
                            inv_k, (const V*)s.sumx, (const V*)s.sumy, (V*)s.xa, (V*)s.ya);
     }
@@ -558,14 +560,15 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
     const CsrView<M> at = pd->AT->view<M>();
     const CsrView<M> a = pd->A->view<M>();
     if (pd->AT->lanes == 0 && pd->n > 0) {
-        const dim3 g(pd->AT->nblocks);
+        const dim3 g(pd->AT->rb_grid());
         if (pd->hasq)
             hipLaunchKernelGGL((k_check_n_rb<M, V, C, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
-                               pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
+                               pd->AT->split(), pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
                                (const V*)s.yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags,
                                sn);
         else
-            hipLaunchKernelGGL((k_check_n_rb<M, V, C, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, at, x, y,
+            hipLaunchKernelGGL((k_check_n_rb<M, V, C, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
+                               pd->AT->split(), at, x, y,
                                (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     } else with_lanes(pd->AT->lanes == 0 ? 1 : pd->AT->lanes, [&](auto L) {
@@ -582,7 +585,8 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out) {
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     });
     if (pd->A->lanes == 0 && pd->m > 0) {
-        hipLaunchKernelGGL((k_check_m_rb<M, V, C>), dim3(pd->A->nblocks), dim3(kBlock), 0, st, a, pd->A->blocks, x,
+        hipLaunchKernelGGL((k_check_m_rb<M, V, C>), dim3(pd->A->rb_grid()), dim3(kBlock), 0, st, a, pd->A->blocks,
+                           pd->A->split(), x,
                            (const V*)s.xa, y, (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi,
                            (const V*)s.rres, flags, sm);
     } else with_lanes(pd->A->lanes == 0 ? 1 : pd->A->lanes, [&](auto L) {
@@ -652,7 +656,7 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 s.sumy = pd->alloc(e * m); s.ya = pd->alloc(e * m); s.yr = pd->alloc(e * m);
             }
             pd->prm = (double*)pd->alloc(sizeof(double) * 2);
-            const size_t gmax = (size_t)std::max({kCheckGridMax, pd->A->nblocks, pd->AT->nblocks});
+            const size_t gmax = (size_t)std::max({(unsigned)kCheckGridMax, pd->A->rb_grid(), pd->AT->rb_grid()});
             pd->part_n = (double*)pd->alloc(sizeof(double) * gmax * kNOutN);
             pd->part_m = (double*)pd->alloc(sizeof(double) * gmax * kNOutM);
             pd->tickets = (unsigned*)pd->alloc(sizeof(unsigned) * 2);
@@ -795,16 +799,42 @@ int ganit_hip_pdhg_check(ganit_hip_pdhg* h, int mdt, int vdt, int flags, double 
         GH_REQUIRE(pd && out, "null argument");
         GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY | GANIT_CHECK_FP64)) == 0, "unknown check flag");
         VecSet& s = pd->set(vdt);
-        with_mdt(mdt, [&](auto MT) {
-            with_vdt(vdt, [&](auto VT) {
-                using M = typename decltype(MT)::type;
-                using V = typename decltype(VT)::type;
-                if ((flags & GANIT_CHECK_FP64) && !std::is_same<V, double>::value)
-                    run_check<M, V, double>(pd, s, flags, inv_k, out);
-                else
-                    run_check<M, V, V>(pd, s, flags, inv_k, out);
+        if ((flags & GANIT_CHECK_FP64) && vdt == GANIT_F32) {
+            // FP64 evaluation of the FP32 iterate: form the average at FP32 (as the iteration would adopt
+            // it), widen iterate, average and restart point into the FP64 working set, and evaluate there
+            // against the FP64 problem data.
+            VecSet& d = pd->set(GANIT_F64);
+            hipStream_t st = pd->ctx->stream;
+            if (flags & GANIT_CHECK_AVG)
+                hipLaunchKernelGGL(k_average<float>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st,
+                                   pd->n, pd->m, inv_k, (const float*)s.sumx, (const float*)s.sumy, (float*)s.xa,
+                                   (float*)s.ya);
+            auto widen_vec = [&](const void* src, void* dst, int64_t k) {
+                if (k > 0)
+                    hipLaunchKernelGGL((k_handover<float, double>), dim3(ew_grid2(k)), dim3(kBlock), 0, st, k,
+                                       (const float*)src, (double*)dst);
+            };
+            widen_vec(s.X[s.cur], d.X[d.cur], pd->n);
+            widen_vec(s.Y[s.cur], d.Y[d.cur], pd->m);
+            widen_vec(s.xr, d.xr, pd->n);
+            widen_vec(s.yr, d.yr, pd->m);
+            if (flags & GANIT_CHECK_AVG) {
+                widen_vec(s.xa, d.xa, pd->n);
+                widen_vec(s.ya, d.ya, pd->m);
+            }
+            GH_CHECK(hipGetLastError());
+            with_mdt(mdt, [&](auto MT) {
+                run_check<typename decltype(MT)::type, double, double>(pd, d, flags, inv_k, out, false);
             });
-        });
+        } else {
+            with_mdt(mdt, [&](auto MT) {
+                with_vdt(vdt, [&](auto VT) {
+                    using M = typename decltype(MT)::type;
+                    using V = typename decltype(VT)::type;
+                    run_check<M, V, V>(pd, s, flags, inv_k, out);
+                });
+            });
+        }
         if (!(flags & GANIT_CHECK_AVG)) for (int k : {4, 5, 6, 7, 10, 11, 14, 15, 17, 18}) out[k] = 0.0;
         if (!(flags & GANIT_CHECK_RAY)) for (int k : {8, 9, 16}) out[k] = 0.0;
     });

@@ -57,11 +57,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<M> a, const V* __restri
 
 // Stand-alone SpMV on LDS-staged row blocks (one block per row block).
 template <class M, class V>
-__global__ __launch_bounds__(kBlock) void k_spmv_rb(CsrView<M> a, const int* __restrict__ blocks,
+__global__ __launch_bounds__(kBlock) void k_spmv_rb(CsrView<M> a, const int* __restrict__ blocks, SplitPlan sp,
                                                     const V* __restrict__ x, V* __restrict__ y) {
     __shared__ V lds[kTile];
     stream_block<M, V, 1>(
-        a, blocks, xcd_block_id(), [&](int, int j) { return x[j]; }, [&](int r, V (&t)[1]) { y[r] = t[0]; }, lds);
+        a, blocks, xcd_block_id(), [&](int, int j) { return x[j]; }, [&](int r, V (&t)[1]) { y[r] = t[0]; }, lds,
+        sp);
 }
 
 template <class V, bool HASQ>
@@ -248,6 +249,29 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
         a->nblocks = (int)rb.size() - 1;
         GH_CHECK(hipMalloc(&a->blocks, sizeof(int) * rb.size()));
         GH_CHECK(hipMemcpyAsync(a->blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, ctx->stream));
+        {  // chunks of the rows longer than kSplit
+            std::vector<int> ch, first{0};
+            for (int i = 0; i < rows; ++i) {
+                if (len[i] <= kSplit) continue;
+                const int s_idx = (int)first.size() - 1;
+                for (int p0 = ptr[i]; p0 < ptr[i + 1]; p0 += kChunk) {
+                    ch.insert(ch.end(), {i, p0, std::min(p0 + kChunk, ptr[i + 1]), s_idx});
+                }
+                first.push_back((int)ch.size() / 4);
+            }
+            a->nsplit = (int)first.size() - 1;
+            a->nchunks = (int)ch.size() / 4;
+            if (a->nsplit > 0) {
+                GH_CHECK(hipMalloc(&a->chunk, sizeof(int) * ch.size()));
+                GH_CHECK(hipMalloc(&a->split_first, sizeof(int) * first.size()));
+                GH_CHECK(hipMalloc(&a->split_ticket, sizeof(unsigned) * a->nsplit));
+                GH_CHECK(hipMalloc(&a->split_part, sizeof(double) * 3 * a->nchunks));
+                GH_CHECK(hipMemcpyAsync(a->chunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice, ctx->stream));
+                GH_CHECK(hipMemcpyAsync(a->split_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice,
+                                        ctx->stream));
+                GH_CHECK(hipMemsetAsync(a->split_ticket, 0, sizeof(unsigned) * a->nsplit, ctx->stream));
+            }
+        }
         // short rows (nnz-weighted median under 16 entries) start on the row-block variant
         if (a->lanes < 16 && a->nblocks > 0) a->lanes = 0;
         const size_t nz = (size_t)std::max<int64_t>(a->nnz, 1);
@@ -289,6 +313,10 @@ void csr_destroy(Csr* a) {
     (void)hipFree(a->v32);
     (void)hipFree(a->v16);
     (void)hipFree(a->blocks);
+    (void)hipFree(a->chunk);
+    (void)hipFree(a->split_first);
+    (void)hipFree(a->split_ticket);
+    (void)hipFree(a->split_part);
     delete a;
 }
 
@@ -297,7 +325,8 @@ void launch_spmv(Ctx* ctx, const Csr* a, const V* x, V* y) {
     if (a->rows == 0) return;
     const CsrView<M> v = a->view<M>();
     if (a->lanes == 0) {
-        hipLaunchKernelGGL((k_spmv_rb<M, V>), dim3(a->nblocks), dim3(kBlock), 0, ctx->stream, v, a->blocks, x, y);
+        hipLaunchKernelGGL((k_spmv_rb<M, V>), dim3(a->rb_grid()), dim3(kBlock), 0, ctx->stream, v, a->blocks,
+                           a->split(), x, y);
         GH_CHECK(hipGetLastError());
         return;
     }

@@ -5,6 +5,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -95,14 +96,23 @@ def _np_ptr(a: np.ndarray):
 
 
 class Context:
+    """A device context.  Matrices and PDHG objects made from it are released before the context itself
+    when it closes, so no handle outlives the stream it was created on."""
+
     def __init__(self, device: int = 0):
         h = _vp()
         check(lib().ganit_hip_ctx_create(device, ctypes.byref(h)), "ctx_create")
         self.h = h
         self._bufs = []
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj) -> None:
+        self._children.add(obj)
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.release()
             for b in self._bufs:
                 lib().ganit_hip_free(self.h, b)
             self._bufs = []
@@ -175,11 +185,15 @@ class DeviceCsr:
         check(lib().ganit_hip_csr_upload(ctx.h, rows, cols, _np_ptr(self._ptr), _np_ptr(self._idx),
                                          _np_ptr(self._val), mask, ctypes.byref(h)), "csr_upload")
         self.h = h
+        ctx._adopt(self)
 
-    def __del__(self):
+    def release(self):
         if getattr(self, "h", None):
             lib().ganit_hip_csr_free(self.h)
             self.h = None
+
+    def __del__(self):
+        self.release()
 
     def plan(self):
         lanes = _i()
@@ -231,11 +245,15 @@ class DevicePdhg:
         h = _vp()
         check(lib().ganit_hip_pdhg_create(ctx.h, ctypes.byref(d), mdt_mask, vdt_mask, ctypes.byref(h)), "pdhg_create")
         self.h = h
+        ctx._adopt(self)
 
-    def __del__(self):
+    def release(self):
         if getattr(self, "h", None):
             lib().ganit_hip_pdhg_destroy(self.h)
             self.h = None
+
+    def __del__(self):
+        self.release()
 
     def plan(self, which: int):
         lanes = _i()

@@ -147,8 +147,10 @@ int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, doub
 /* Check flags and the layout of the sums ganit_hip_pdhg_check returns (all FP64-accumulated). */
 #define GANIT_CHECK_AVG 1  /* also form (xa, ya) = (sumx, sumy) * inv_k and evaluate them          */
 #define GANIT_CHECK_RAY 2  /* also the infeasibility ray y - yr (src/pdhg_impl.hpp:110-118)        */
-#define GANIT_CHECK_FP64 4 /* form the row sums in FP64 even for an FP32 working set (with mdt =
-                              GANIT_F64: the FP64 evaluation of a reduced-precision iterate)        */
+#define GANIT_CHECK_FP64 4 /* with vdt = GANIT_F32: evaluate the FP32 iterate (and its FP32 average)
+                              in FP64 -- widened into the FP64 working set and checked there against
+                              the FP64 problem data with matrix values mdt (GANIT_F64 for a full
+                              FP64 evaluation).  Requires the FP64 working set (vdt_mask bit 0). */
 #define GANIT_CHECK_NOUT 19
 /* out[ 0.. 3]  current point : rd2, vobj, c'x, x'Qx      (dual_res, dot_c, dot with Qx)
  * out[ 4.. 7]  average point : rd2, vobj, c'x, x'Qx

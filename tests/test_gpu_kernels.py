@@ -108,6 +108,31 @@ def test_rowblock_spmv_bitwise_vs_oracle(hipctx, oracle, mdt, vdt):
     assert np.array_equal(yd.numpy(), oracle.spmv(mdt, vdt, m, n, ptr, idx, val, x))
 
 
+@pytest.mark.parametrize("mdt,vdt", [(F64, F64), (F32, F32), (BF16, F32)])
+def test_split_rows_spmv(hipctx, oracle, mdt, vdt):
+    """Rows longer than 8192 entries are cut into 4096-entry chunks on separate workgroups and combined in
+    chunk order by the last one: within the SpMV error bound, and deterministic."""
+    from ganit_v2_amd import hip
+    rng = np.random.default_rng(30 + mdt)
+    n = 120000
+    lens = np.array([100001, 3, 8193, 0, 50000, 8192, 17, 40000, 1])
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens]).astype(np.int32)
+    val = rng.standard_normal(ptr[-1])
+    x = rng.standard_normal(n)
+    A = hip.DeviceCsr(hipctx, len(lens), n, ptr, idx, val)
+    A.set_lanes(0)
+    xd, yd = hipctx.vector(x, vdt), hipctx.empty(len(lens), vdt)
+    A.spmv(xd, yd, mdt)
+    y = yd.numpy()
+    xr = x if vdt == F64 else x.astype(np.float32).astype(np.float64)
+    exact, bound = _exact_and_bound(ptr, idx, rounded(val, mdt, oracle), xr, 2.0 ** (-53 if vdt == F64 else -24))
+    assert np.all(np.abs(y - exact) <= bound + 1e-300)
+    for _ in range(3):
+        A.spmv(xd, yd, mdt)
+        assert np.array_equal(yd.numpy(), y)
+
+
 def test_autotune_picks_a_valid_variant(hipctx):
     from ganit_v2_amd import hip
     import ganit_v2_amd as g

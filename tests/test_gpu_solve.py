@@ -102,6 +102,20 @@ def test_ladder_escalates_below_the_fp32_floor():
     assert 0 < r["fp32_iters"] < r["iters"]
 
 
+@pytest.mark.parametrize("assets,factors", [(2000, 10), (20000, 50)])
+def test_portfolio_qp_vs_oracle(oracle, assets, factors):
+    """Factor-model portfolio QP (dense factor rows: the long-row split path) at 1e-6, GPU vs the oracle."""
+    p = g.Problem.generate("portfolio", assets=assets, factors=factors, seed=0)
+    r = p.solve_gpu(tol=1e-6, time_limit=120)
+    oracle.set_threads(8)
+    o = oracle.OracleLP.from_export(p.export()).solve(tol=1e-6, time_limit=300)
+    assert r["status"] == o["status"] == "OPTIMAL"
+    assert _certified(r, 1e-6)
+    assert abs(r["pobj"] - o["pobj"]) <= 1e-5 * (1 + abs(o["pobj"]))
+    x = r["x"][:assets]
+    assert abs(x.sum() - 1.0) < 1e-4 and x.min() > -1e-6 and x.max() < 0.01 + 1e-6
+
+
 def test_limits_and_odd_check_interval():
     p = g.Problem.read_mps(instance_path("25fv47"))
     r = p.solve_gpu(max_iter=640)
