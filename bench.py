@@ -79,6 +79,10 @@ def main():
     ap.add_argument("--reps", type=int, default=50, help="launches per kernel for the HIP-event timing")
     ap.add_argument("--precision", default="mixed", choices=["mixed", "dynamic", "double"])
     ap.add_argument("--values", default="f32", choices=["f32", "bf16"])
+    ap.add_argument("--stall-checks", type=int, default=60,
+                    help="checks without 20%% progress before leaving L1 (the reference's default is 12)")
+    ap.add_argument("--eval", default="fp64", choices=["fp64", "level"],
+                    help="evaluate L1 restart candidates in FP64 (default) or at L1 precision")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {a.gpus}); a single LP does not shard "
@@ -94,12 +98,12 @@ def main():
     print(f"[bench] blending LP m={m} n={n} nnz={nnz} generated in {time.time() - t0:.1f}s", file=sys.stderr,
           flush=True)
 
-    def staged(precision):
-        s = prob.solver(precision=precision, values=a.values, tol=1e-4, verbose=0)
+    def staged(precision, **kw):
+        s = prob.solver(precision=precision, tol=1e-4, verbose=0, **kw)
         s.upload()
         return s
 
-    mixed = staged(a.precision)
+    mixed = staged(a.precision, values=a.values, stall_checks=a.stall_checks, eval_fp64=(a.eval == "fp64"))
     for _ in range(a.warmup):
         mixed.run()
     t = time.perf_counter()
@@ -115,6 +119,17 @@ def main():
     f_last = f_runs[-1]
     print(f"[bench] fp64: {[round(r['solve_s'], 4) for r in f_runs]} s, status {f_last['status']}, "
           f"iters {f_last['iters']}", file=sys.stderr, flush=True)
+    # other ladder configurations, one timed solve each after a warm-up (reported, not the headline)
+    ladder = []
+    for cfg in (dict(precision="mixed"), dict(precision="dynamic"),
+                dict(precision="mixed", values="bf16", stall_checks=a.stall_checks, eval_fp64=True)):
+        sv = staged(**cfg)
+        sv.run()
+        r = sv.run()
+        ladder.append(dict(cfg=cfg, solve_s=round(r["solve_s"], 5), status=r["status"], iters=r["iters"],
+                           l1_iters=r["fp32_iters"], l2_iters=r["l2_iters"], obj=r["pobj"]))
+        print(f"[bench] ladder {ladder[-1]}", file=sys.stderr, flush=True)
+        del sv
     t_mixed = sum(r["solve_s"] for r in runs) / len(runs)
     t_fp64 = sum(r["solve_s"] for r in f_runs) / len(f_runs)
     obj_err = abs(last["pobj"] - f_last["pobj"]) / (1 + abs(f_last["pobj"]))
@@ -166,13 +181,14 @@ def main():
         "data": "synthetic (seeded blending LP generator)",
         "config": {"workload": "blending LP 60 crudes x 16 products x 5 specs x 365 periods, seed 0",
                    "m": m, "n": n, "nnz": nnz, "precision": a.precision, "values": a.values, "tol": 1e-4,
-                   "parallelism": "single GPU"},
+                   "stall_checks": a.stall_checks, "eval": a.eval, "parallelism": "single GPU"},
         "status": last["status"],
         "iters": last["iters"], "l1_iters": last["fp32_iters"], "restarts": last["restarts"],
         "fp64_s": round(t_fp64, 5), "fp64_iters": f_last["iters"], "fp64_status": f_last["status"],
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
         "obj": last["pobj"], "obj_fp64": f_last["pobj"], "obj_rel_err_vs_fp64": obj_err,
         "certified_rel": [last["rel_p"], last["rel_d"], last["rel_gap"]],
+        "ladder": ladder,
         "kernels": kern,
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -2,9 +2,9 @@
 //
 //   * value types: FP64 / FP32 / BF16 matrix values, FP64 / FP32 vectors;
 //   * row groups: LPR lanes of one wavefront cooperate on one CSR row (LPR in {1..64});
-//   * deterministic reductions: per-block tree sums in a fixed order, partials in HBM, and the last
-//     block to arrive (agent-scope ticket) folds the partials in block order -- no FP atomics, so a
-//     reduction returns the same bits on every run;
+//   * deterministic reductions: per-block tree sums in a fixed order, partials in HBM folded in block
+//     order by a one-block follow-up launch -- no FP atomics, so a reduction returns the same bits on
+//     every run;
 //   * XCD-aware block numbering: the 8 XCDs of an MI355X receive workgroups round-robin, so consecutive
 //     hardware block ids are remapped to give each XCD one contiguous slab of rows, keeping the slab's
 //     gathers of x inside that XCD's own L2.
@@ -182,6 +182,7 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
         if (threadIdx.x == 0) row_fn(r0, s);
         return;
     }
+#pragma unroll 4
     for (int q = threadIdx.x; q < cnt; q += kBlock) {
         const V av = widen<V>(a.val[base + q]);
         const int i = a.idx[base + q];
@@ -204,12 +205,12 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
 }
 
 // ---------------------------------------------------------------- deterministic grid reduction
-// Scratch for one reduction call site: `partials` holds gridDim.x * N doubles, `ticket` one counter that
-// is zero between launches (the last block resets it, so no memset is ever needed -- graph-safe).
+// Two launches, no atomics: every block of the producing kernel writes its N block sums to partials[b][.]
+// (block_partial); one block of a follow-up kernel then adds them up in block order (sum_partials).  A
+// single shared ticket counter would serialise ~2000 blocks on one address; this never does.
 struct ReduceSlot {
-    double* partials;
-    unsigned int* ticket;
-    double* out;  // N doubles
+    double* partials;  // gridDim.x * N doubles
+    double* out;       // N doubles
 };
 
 // Block-wide sum of N values per thread; the result is valid in thread 0.  `lds` must hold
@@ -238,38 +239,31 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double* lds) {
     __syncthreads();
 }
 
-// Grid-wide sum: every block calls this with its per-thread values; the last block writes slot.out.
-// Memory ordering: a block's partials are published by the agent-scope release fence before its ticket
-// increment; the last block's agent-scope acquire fence (after it reads the final ticket) makes every
-// partial visible to its ordinary loads, across the XCDs' separate L2s.
 template <int N>
-__device__ void grid_sum(double (&v)[N], const ReduceSlot& slot, double* lds) {
+__device__ __forceinline__ void block_partial(double (&v)[N], const ReduceSlot& slot, double* lds) {
     block_sum<N>(v, lds);
-    __shared__ bool last;
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int k = 0; k < N; ++k) slot.partials[(size_t)blockIdx.x * N + k] = v[k];
-        __threadfence();
-        const unsigned t = __hip_atomic_fetch_add(slot.ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        last = (t == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // Fixed assignment of partials to threads, then the fixed block tree: deterministic.
+}
+
+// Run by every thread of one block: out[k] = sum over b < nparts of partials[b][k], fixed order.
+template <int N>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ partials, int nparts, double* out,
+                                             double* lds) {
     double s[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) s[k] = 0.0;
-    for (int b = threadIdx.x; b < (int)gridDim.x; b += blockDim.x) {
-        const double* p = slot.partials + (size_t)b * N;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+        const double* p = partials + (size_t)b * N;
 #pragma unroll
         for (int k = 0; k < N; ++k) s[k] += p[k];
     }
     block_sum<N>(s, lds);
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int k = 0; k < N; ++k) slot.out[k] = s[k];
-        __hip_atomic_store(slot.ticket, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        for (int k = 0; k < N; ++k) out[k] = s[k];
     }
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -62,9 +63,8 @@ struct Ctx {
     int cus = 0;
     std::string name;
     hipStream_t stream = nullptr;
-    // scratch of the fine-grained reductions: partials [kMaxReduceGrid][2], ticket, out[2]
+    // scratch of the fine-grained reductions: partials [kMaxReduceGrid][2], out[2]
     double* red_partials = nullptr;
-    unsigned int* red_ticket = nullptr;
     double* red_out = nullptr;
     double* host_out = nullptr;  // pinned, 64 doubles
 };
@@ -82,6 +82,7 @@ struct Csr {
     int64_t hist[9] = {0};
     int* blocks = nullptr;   // row-block starts of the LDS-staged variant (nblocks + 1 entries, device)
     int nblocks = 0;
+    int rb_rows = kBlock, rb_nnz = kTile;  // row-block caps the partition was built with
     // rows longer than kSplit nonzeros, cut into kChunk chunks (device arrays; see SplitPlan)
     int nsplit = 0, nchunks = 0;
     int* chunk = nullptr;
@@ -114,7 +115,18 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
 void csr_destroy(Csr* a);
 int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz);
 bool valid_variant(int lanes);
-// Time the candidate variants of `a` with the stand-alone SpMV in (mdt, vdt) and keep the fastest.
+// Rebuild the row-block partition with at most `rows` rows and `nnz` nonzeros per block.
+void csr_set_partition(Ctx* ctx, Csr* a, int rows, int nnz);
+// Candidate variants: (lanes, rb_rows, rb_nnz); lanes 0 = row blocks with that partition.
+struct Variant {
+    int lanes, rows, nnz;
+};
+const std::vector<Variant>& variant_candidates();
+void apply_variant(Ctx* ctx, Csr* a, const Variant& v);
+// Time every candidate variant of `a` with `launch(k)` (k back-to-back launches of the kernel that will
+// use it) and keep the fastest.
+void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& launch);
+// The same with the stand-alone SpMV in (mdt, vdt).
 void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps);
 uint16_t bf16_bits_from_double(double d);
 

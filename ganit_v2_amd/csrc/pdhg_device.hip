@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
             }
         }
     }
-    grid_sum<kNOutN>(acc, slot, lds);
+    block_partial<kNOutN>(acc, slot, lds);
 }
 
 // One pass over A: out[12..18] as documented for ganit_hip_pdhg_check (sums in C as for k_check_n).
@@ -259,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
             if (ray) acc[4] += row_obj_term(double(V(y[i] - yr[i])), lw, hg);
         }
     }
-    grid_sum<kNOutM>(acc, slot, lds);
+    block_partial<kNOutM>(acc, slot, lds);
 }
 
 // The two check passes on LDS-staged row blocks (grid = number of row blocks); sums in C as above.
@@ -324,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
         },
         lds, sp);
     __syncthreads();
-    grid_sum<kNOutN>(acc, slot, red);
+    block_partial<kNOutN>(acc, slot, red);
 }
 
 template <class M, class V, class C>
@@ -363,7 +363,16 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
         },
         lds, sp);
     __syncthreads();
-    grid_sum<kNOutM>(acc, slot, red);
+    block_partial<kNOutM>(acc, slot, red);
+}
+
+// Folds the per-block partials of both check passes: block 0 the A^T pass, block 1 the A pass.
+__global__ __launch_bounds__(kBlock) void k_check_finish(const double* __restrict__ part_n, int nb_n,
+                                                         const double* __restrict__ part_m, int nb_m,
+                                                         double* __restrict__ out) {
+    __shared__ double lds[(kBlock / kWave) * kNOutN];
+    if (blockIdx.x == 0) sum_partials<kNOutN>(part_n, nb_n, out, lds);
+    else sum_partials<kNOutM>(part_m, nb_m, out + kNOutN, lds);
 }
 
 // ================================================================ restart / hand-over kernels
@@ -420,7 +429,7 @@ struct Pdhg {
     double* prm = nullptr;
     double* part_n = nullptr;
     double* part_m = nullptr;
-    unsigned* tickets = nullptr;  // [2]
+    size_t part_cap = 0;          // blocks the partial buffers hold
     double* out = nullptr;        // [GANIT_CHECK_NOUT]
     std::vector<void*> owned;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
@@ -555,12 +564,14 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
         hipLaunchKernelGGL(k_average<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st, pd->n, pd->m,
                            inv_k, (const V*)s.sumx, (const V*)s.sumy, (V*)s.xa, (V*)s.ya);
     }
-    ReduceSlot sn{pd->part_n, pd->tickets + 0, pd->out};
-    ReduceSlot sm{pd->part_m, pd->tickets + 1, pd->out + kNOutN};
+    ReduceSlot sn{pd->part_n, pd->out};
+    ReduceSlot sm{pd->part_m, pd->out + kNOutN};
+    unsigned nb_n = 0, nb_m = 0;  // grids of the two passes (number of partials)
     const CsrView<M> at = pd->AT->view<M>();
     const CsrView<M> a = pd->A->view<M>();
     if (pd->AT->lanes == 0 && pd->n > 0) {
         const dim3 g(pd->AT->rb_grid());
+        nb_n = g.x;
         if (pd->hasq)
             hipLaunchKernelGGL((k_check_n_rb<M, V, C, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
                                pd->AT->split(), pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
@@ -575,6 +586,7 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
         constexpr int LPR = decltype(L)::value;
         const int64_t groups = std::max<int64_t>(pd->n, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
+        nb_n = g;
         if (pd->hasq)
             hipLaunchKernelGGL((k_check_n<LPR, M, V, C, true>), dim3(g), dim3(kBlock), 0, st, at, pd->Q->view<M>(), x, y,
                                (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
@@ -585,6 +597,7 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     });
     if (pd->A->lanes == 0 && pd->m > 0) {
+        nb_m = pd->A->rb_grid();
         hipLaunchKernelGGL((k_check_m_rb<M, V, C>), dim3(pd->A->rb_grid()), dim3(kBlock), 0, st, a, pd->A->blocks,
                            pd->A->split(), x,
                            (const V*)s.xa, y, (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi,
@@ -593,10 +606,14 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
         constexpr int LPR = decltype(L)::value;
         const int64_t groups = std::max<int64_t>(pd->m, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
+        nb_m = g;
         hipLaunchKernelGGL((k_check_m<LPR, M, V, C>), dim3(g), dim3(kBlock), 0, st, a, x, (const V*)s.xa, y,
                            (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags,
                            sm);
     });
+    GH_REQUIRE(nb_n <= pd->part_cap && nb_m <= pd->part_cap, "check grid exceeds the partial buffers");
+    hipLaunchKernelGGL(k_check_finish, dim3(2), dim3(kBlock), 0, st, pd->part_n, (int)nb_n, pd->part_m, (int)nb_m,
+                       pd->out);
     GH_CHECK(hipGetLastError());
     GH_CHECK(hipMemcpyAsync(pd->ctx->host_out, pd->out, sizeof(double) * GANIT_CHECK_NOUT, hipMemcpyDeviceToHost, st));
     GH_CHECK(hipStreamSynchronize(st));
@@ -628,15 +645,6 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
             GH_REQUIRE(pd->A->nnz == pd->AT->nnz, "A and its transpose differ in nnz");
             pd->hasq = lp->q_ptr && lp->q_ptr[lp->n] > 0;
             if (pd->hasq) pd->Q = csr_create(ctx, lp->n, lp->n, lp->q_ptr, lp->q_idx, lp->q_val, mdt_mask);
-            {  // per-matrix variant: time the candidates at the lowest resident precision (the hot level)
-                const char* at = std::getenv("GANIT_HIP_AUTOTUNE");
-                if (!(at && at[0] == '0')) {
-                    const int mdt = (mdt_mask & 4u) ? GANIT_BF16 : (mdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
-                    const int vdt = (vdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
-                    autotune_csr(ctx, pd->A, mdt, vdt, 5);
-                    autotune_csr(ctx, pd->AT, mdt, vdt, 5);
-                }
-            }
             if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
             for (int vdt = 0; vdt < 2; ++vdt) {
                 if (!(vdt_mask & (1u << vdt))) continue;
@@ -656,16 +664,52 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 s.sumy = pd->alloc(e * m); s.ya = pd->alloc(e * m); s.yr = pd->alloc(e * m);
             }
             pd->prm = (double*)pd->alloc(sizeof(double) * 2);
-            const size_t gmax = (size_t)std::max({(unsigned)kCheckGridMax, pd->A->rb_grid(), pd->AT->rb_grid()});
-            pd->part_n = (double*)pd->alloc(sizeof(double) * gmax * kNOutN);
-            pd->part_m = (double*)pd->alloc(sizeof(double) * gmax * kNOutM);
-            pd->tickets = (unsigned*)pd->alloc(sizeof(unsigned) * 2);
             pd->out = (double*)pd->alloc(sizeof(double) * GANIT_CHECK_NOUT);
-            GH_CHECK(hipMemsetAsync(pd->tickets, 0, sizeof(unsigned) * 2, ctx->stream));
             GH_CHECK(hipMemsetAsync(pd->out, 0, sizeof(double) * GANIT_CHECK_NOUT, ctx->stream));
             GH_CHECK(hipStreamSynchronize(ctx->stream));
             const char* ng = std::getenv("GANIT_HIP_NO_GRAPH");
             pd->use_graphs = !(ng && ng[0] == '1');
+            const char* at = std::getenv("GANIT_HIP_AUTOTUNE");
+            if (!(at && at[0] == '0')) {
+                // Per-matrix variant, timed on the fused iteration kernels that will run, at the lowest resident
+                // precision (the hot level) on a zeroed working set.
+                const int mdt = (mdt_mask & 4u) ? GANIT_BF16 : (mdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
+                const int vdt = (vdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
+                VecSet& s = pd->set(vdt);
+                with_vdt(vdt, [&](auto VT) {
+                    using V = typename decltype(VT)::type;
+                    hipLaunchKernelGGL(k_zero2<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, ctx->stream,
+                                       pd->n, pd->m, (V*)s.X[0], (V*)s.Y[0]);
+                    hipLaunchKernelGGL(k_zero2<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, ctx->stream,
+                                       pd->n, pd->m, (V*)s.X[1], (V*)s.Y[1]);
+                    hipLaunchKernelGGL(k_zero2<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, ctx->stream,
+                                       pd->n, pd->m, (V*)s.xbar, (V*)s.sumy);
+                    hipLaunchKernelGGL(k_zero2<V>, dim3(ew_grid2((int64_t)pd->n)), dim3(kBlock), 0, ctx->stream, pd->n,
+                                       0, (V*)s.sumx, (V*)nullptr);
+                });
+                hipLaunchKernelGGL(k_set_steps, dim3(1), dim3(64), 0, ctx->stream, pd->prm, 0.0, 0.0);
+                GH_CHECK(hipGetLastError());
+                for (int which = 0; which < 2; ++which) {
+                    Csr* a = which == 0 ? pd->AT : pd->A;
+                    if ((which == 0 && pd->n == 0) || (which == 1 && pd->m == 0)) continue;
+                    autotune_with(ctx, a, 5, [&](int k) {
+                        if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
+                        with_mdt(mdt, [&](auto MT) {
+                            with_vdt(vdt, [&](auto VT) {
+                                using M = typename decltype(MT)::type;
+                                using V = typename decltype(VT)::type;
+                                for (int r = 0; r < k; ++r) launch_one<M, V>(pd, s, which);
+                            });
+                        });
+                    });
+                }
+                if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
+            }
+            // partials for the widest check grid of the chosen variants
+            const size_t gmax = (size_t)std::max({(unsigned)kCheckGridMax, pd->A->rb_grid(), pd->AT->rb_grid()});
+            pd->part_cap = gmax;
+            pd->part_n = (double*)pd->alloc(sizeof(double) * gmax * kNOutN);
+            pd->part_m = (double*)pd->alloc(sizeof(double) * gmax * kNOutM);
         } catch (...) {
             delete pd;
             throw;

@@ -101,7 +101,13 @@ __global__ __launch_bounds__(kBlock) void k_reduce2(int64_t n, F f, ReduceSlot s
         acc[0] += a0;
         acc[1] += a1;
     }
-    grid_sum<2>(acc, slot, lds);
+    block_partial<2>(acc, slot, lds);
+}
+
+__global__ __launch_bounds__(kBlock) void k_finish2(const double* __restrict__ partials, int nparts,
+                                                    double* __restrict__ out) {
+    __shared__ double lds[(kBlock / kWave) * 2];
+    sum_partials<2>(partials, nparts, out, lds);
 }
 
 template <class V>
@@ -162,8 +168,9 @@ struct Diff2Term {
 template <class F>
 void reduce2(Ctx* ctx, int64_t n, F f, double* o0, double* o1) {
     const unsigned g = std::min<unsigned>(grid_for(std::max<int64_t>(n, 1)), (unsigned)kMaxReduceGrid);
-    ReduceSlot slot{ctx->red_partials, ctx->red_ticket, ctx->red_out};
+    ReduceSlot slot{ctx->red_partials, ctx->red_out};
     hipLaunchKernelGGL(k_reduce2<F>, dim3(g), dim3(kBlock), 0, ctx->stream, n, f, slot);
+    hipLaunchKernelGGL(k_finish2, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->red_partials, (int)g, ctx->red_out);
     GH_CHECK(hipGetLastError());
     GH_CHECK(hipMemcpyAsync(ctx->host_out, ctx->red_out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     GH_CHECK(hipStreamSynchronize(ctx->stream));
@@ -200,9 +207,10 @@ int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz) {
 
 bool valid_variant(int lanes) { return lanes == 0 || (lanes >= 1 && lanes <= kWave && (lanes & (lanes - 1)) == 0); }
 
-// Row blocks of the LDS-staged variant: consecutive rows while the block holds at most kBlock rows and kTile
-// nonzeros and every row has at most kLong entries; a longer row forms a block of its own.
-std::vector<int> row_blocks(const int* ptr, int rows) {
+// Row blocks of the LDS-staged variant: consecutive rows while the block holds at most max_rows (<= kBlock)
+// rows and max_nnz (<= kTile) nonzeros and every row has at most kLong entries; a longer row forms a block
+// of its own.
+std::vector<int> row_blocks(const int* ptr, int rows, int max_rows = kBlock, int max_nnz = kTile) {
     std::vector<int> b{0};
     int r = 0;
     while (r < rows) {
@@ -211,9 +219,9 @@ std::vector<int> row_blocks(const int* ptr, int rows) {
             ++r;
         } else {
             int cnt = 0, nz = 0;
-            while (r < rows && cnt < kBlock) {
+            while (r < rows && cnt < max_rows) {
                 const int l = ptr[r + 1] - ptr[r];
-                if (l > kLong || nz + l > kTile) break;
+                if (l > kLong || nz + l > max_nnz) break;
                 nz += l;
                 ++cnt;
                 ++r;
@@ -341,36 +349,50 @@ void launch_spmv(Ctx* ctx, const Csr* a, const V* x, V* y) {
 // Time the stand-alone SpMV of each candidate variant (the row-block variant and the row-group widths
 // around the histogram choice) with HIP events and keep the fastest.  The fused kernels share the row
 // traversal of the stand-alone one, so its ranking carries over.
-void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps) {
+void csr_set_partition(Ctx* ctx, Csr* a, int rows, int nnz) {
+    GH_REQUIRE(rows >= 1 && rows <= kBlock && nnz >= kLong && nnz <= kTile, "bad row-block partition caps");
+    if (a->rb_rows == rows && a->rb_nnz == nnz) return;
+    std::vector<int> ptr(a->rows + 1);
+    GH_CHECK(hipMemcpyAsync(ptr.data(), a->ptr, sizeof(int) * ptr.size(), hipMemcpyDeviceToHost, ctx->stream));
+    GH_CHECK(hipStreamSynchronize(ctx->stream));
+    const std::vector<int> rb = row_blocks(ptr.data(), a->rows, rows, nnz);
+    int* blocks = nullptr;
+    GH_CHECK(hipMalloc(&blocks, sizeof(int) * rb.size()));
+    GH_CHECK(hipMemcpyAsync(blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, ctx->stream));
+    GH_CHECK(hipStreamSynchronize(ctx->stream));
+    GH_CHECK(hipFree(a->blocks));
+    a->blocks = blocks;
+    a->nblocks = (int)rb.size() - 1;
+    a->rb_rows = rows;
+    a->rb_nnz = nnz;
+}
+
+const std::vector<Variant>& variant_candidates() {
+    static const std::vector<Variant> v{{0, 256, 2048}, {0, 256, 1024}, {0, 128, 1024}, {0, 64, 512},
+                                        {1, 0, 0},      {2, 0, 0},      {4, 0, 0},      {8, 0, 0},
+                                        {16, 0, 0},     {32, 0, 0},     {64, 0, 0}};
+    return v;
+}
+
+void apply_variant(Ctx* ctx, Csr* a, const Variant& v) {
+    a->lanes = v.lanes;
+    if (v.lanes == 0) csr_set_partition(ctx, a, v.rows, v.nnz);
+}
+
+void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& launch) {
     if (a->rows == 0 || a->nnz == 0) return;
-    const std::vector<int> cand{0, 1, 2, 4, 8, 16, 32, 64};
-    const size_t vs = dtype_size(vdt);
-    void *x = nullptr, *y = nullptr;
-    GH_CHECK(hipMalloc(&x, vs * std::max(a->cols, 1)));
-    GH_CHECK(hipMalloc(&y, vs * std::max(a->rows, 1)));
+    const auto& cand = variant_candidates();
     hipEvent_t e0, e1;
     GH_CHECK(hipEventCreate(&e0));
     GH_CHECK(hipEventCreate(&e1));
-    std::vector<float> best_ms(cand.size(), 1e30f);
-    const int keep = a->lanes;
+    std::vector<float> best(cand.size(), 1e30f);
     try {
-        GH_CHECK(hipMemsetAsync(x, 0, vs * std::max(a->cols, 1), ctx->stream));
-        auto launch = [&](int k) {
-            with_mdt(mdt, [&](auto MT) {
-                with_vdt(vdt, [&](auto VT) {
-                    using M = typename decltype(MT)::type;
-                    using V = typename decltype(VT)::type;
-                    for (int r = 0; r < k; ++r) launch_spmv<M, V>(ctx, a, (const V*)x, (V*)y);
-                });
-            });
-        };
         // Warm the clocks and caches first, then time every candidate in two interleaved rounds and keep
         // each one's best time, so a candidate timed early is not penalised by a cold GPU.
-        a->lanes = keep;
         launch(20);
         for (int round = 0; round < 2; ++round) {
             for (size_t c = 0; c < cand.size(); ++c) {
-                a->lanes = cand[c];
+                apply_variant(ctx, a, cand[c]);
                 launch(2);
                 GH_CHECK(hipEventRecord(e0, ctx->stream));
                 launch(reps);
@@ -378,21 +400,46 @@ void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps) {
                 GH_CHECK(hipEventSynchronize(e1));
                 float ms = 0;
                 GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                best_ms[c] = std::min(best_ms[c], ms);
+                best[c] = std::min(best[c], ms);
             }
         }
     } catch (...) {
-        a->lanes = keep;
-        (void)hipFree(x); (void)hipFree(y); (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         throw;
     }
     size_t bi = 0;
-    for (size_t c = 1; c < cand.size(); ++c) if (best_ms[c] < best_ms[bi]) bi = c;
-    a->lanes = cand[bi];
-    (void)hipFree(x);
-    (void)hipFree(y);
+    for (size_t c = 1; c < cand.size(); ++c)
+        if (best[c] < best[bi]) bi = c;
+    apply_variant(ctx, a, cand[bi]);
     GH_CHECK(hipEventDestroy(e0));
     GH_CHECK(hipEventDestroy(e1));
+}
+
+void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps) {
+    if (a->rows == 0 || a->nnz == 0) return;
+    const size_t vs = dtype_size(vdt);
+    void *x = nullptr, *y = nullptr;
+    GH_CHECK(hipMalloc(&x, vs * std::max(a->cols, 1)));
+    GH_CHECK(hipMalloc(&y, vs * std::max(a->rows, 1)));
+    try {
+        GH_CHECK(hipMemsetAsync(x, 0, vs * std::max(a->cols, 1), ctx->stream));
+        autotune_with(ctx, a, reps, [&](int k) {
+            with_mdt(mdt, [&](auto MT) {
+                with_vdt(vdt, [&](auto VT) {
+                    using M = typename decltype(MT)::type;
+                    using V = typename decltype(VT)::type;
+                    for (int r = 0; r < k; ++r) launch_spmv<M, V>(ctx, a, (const V*)x, (V*)y);
+                });
+            });
+        });
+    } catch (...) {
+        (void)hipFree(x);
+        (void)hipFree(y);
+        throw;
+    }
+    (void)hipFree(x);
+    (void)hipFree(y);
 }
 
 }  // namespace ganit_hip
@@ -421,9 +468,7 @@ int ganit_hip_ctx_create(int device, ganit_hip_ctx** out) {
         if (std::strlen(prop.gcnArchName)) c->name += std::string(" (") + prop.gcnArchName + ")";
         GH_CHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         GH_CHECK(hipMalloc(&c->red_partials, sizeof(double) * kMaxReduceGrid * 2));
-        GH_CHECK(hipMalloc(&c->red_ticket, sizeof(unsigned int)));
         GH_CHECK(hipMalloc(&c->red_out, sizeof(double) * 2));
-        GH_CHECK(hipMemsetAsync(c->red_ticket, 0, sizeof(unsigned int), c->stream));
         GH_CHECK(hipHostMalloc((void**)&c->host_out, sizeof(double) * 64, hipHostMallocDefault));
         GH_CHECK(hipStreamSynchronize(c->stream));
         *out = reinterpret_cast<ganit_hip_ctx*>(c);
@@ -436,7 +481,6 @@ int ganit_hip_ctx_destroy(ganit_hip_ctx* h) {
         if (!c) return;
         (void)hipStreamSynchronize(c->stream);
         (void)hipFree(c->red_partials);
-        (void)hipFree(c->red_ticket);
         (void)hipFree(c->red_out);
         (void)hipHostFree(c->host_out);
         (void)hipStreamDestroy(c->stream);
