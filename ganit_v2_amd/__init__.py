@@ -71,6 +71,12 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_solver_run.argtypes = [_vp, _f64p, _vp, _vp]
     lib.ganit_solver_kernel_times.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p]
     lib.ganit_solver_free.argtypes = [_vp]
+    lib.ganit_batch_create.restype = _vp
+    lib.ganit_batch_create.argtypes = [_vp, ctypes.c_int, _f64p]
+    lib.ganit_batch_upload.argtypes = [_vp]
+    lib.ganit_batch_run.argtypes = [_vp, _f64p, _vp, _vp]
+    lib.ganit_batch_stats.argtypes = [_vp, _f64p]
+    lib.ganit_batch_free.argtypes = [_vp]
     _host = lib
     return lib
 
@@ -136,6 +142,9 @@ class Problem:
             "portfolio": (2, ["assets", "factors", "gamma"], dict(assets=100000, factors=50, gamma=1.0)),
             "diff_qp": (3, ["n", "m_eq", "m_le", "density", "qdensity"],
                         dict(n=2000, m_eq=300, m_le=200, density=0.01, qdensity=0.001)),
+            "batch_lp": (4, ["m_lo", "m_hi", "n_lo", "n_hi", "nnz_min", "nnz_max", "tighten_frac"],
+                         dict(m_lo=200, m_hi=4000, n_lo=500, n_hi=10000, nnz_min=5, nnz_max=15,
+                              tighten_frac=0.05)),
         }
         k, names, defaults = kinds[kind]
         vals = dict(defaults)
@@ -283,4 +292,65 @@ class Solver:
         return float(ms[0]), float(ms[1])
 
 
-__all__ = ["Problem", "Solver", "CSR", "host_lib", "NativeLibraryMissing", "STATUS", "LIB_DIR", "BIN_DIR"]
+_BATCH_STATS_KEYS = ["rows", "cols", "nnz", "blocks_a", "blocks_at", "upload_s", "solve_s", "certify_s", "l1_s",
+                     "l2_s", "l3_s", "l1_instances", "l2_instances", "l3_instances", "max_iters", "total_iters",
+                     "autotune_pick", "autotune_ms0", "autotune_ms1", "autotune_ms2", "autotune_ms3", "setup_s"]
+
+
+class Batch:
+    """Many LPs solved together by the block-diagonal batched kernels (one launch sequence for all).
+
+    Options as Problem.solve_gpu (check_every must be even).  run() returns one result dict per instance
+    (solve_s is the whole batch's wall time) and stats() the batch-level figures of that run."""
+
+    def __init__(self, problems, **opts):
+        self._problems = list(problems)
+        if not self._problems:
+            raise ValueError("empty batch")
+        self._dims = [p.dims() for p in self._problems]
+        arr = (ctypes.c_void_p * len(self._problems))(*[p._h for p in self._problems])
+        h = host_lib().ganit_batch_create(ctypes.cast(arr, ctypes.c_void_p), len(self._problems), _options(**opts))
+        if not h:
+            raise RuntimeError(host_lib().ganit_last_error().decode())
+        self._h = ctypes.c_void_p(h)
+
+    def __len__(self):
+        return len(self._problems)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _host is not None:
+            _host.ganit_batch_free(h)
+            self._h = None
+
+    def upload(self) -> None:
+        _check_host(host_lib().ganit_batch_upload(self._h), "batch_upload")
+
+    def run(self, return_solution=False) -> list:
+        K = len(self._problems)
+        out = np.zeros(18 * K)
+        N = sum(d["n"] for d in self._dims)
+        M = sum(d["m"] for d in self._dims)
+        x = np.zeros(N) if return_solution else None
+        y = np.zeros(M) if return_solution else None
+        _check_host(host_lib().ganit_batch_run(self._h, out, _ptr(x), _ptr(y)), "batch_run")
+        res, xo, yo = [], 0, 0
+        for k, d in enumerate(self._dims):
+            xs = x[xo:xo + d["n"]] if x is not None else None
+            ys = y[yo:yo + d["m"]] if y is not None else None
+            res.append(_result(out[18 * k:18 * (k + 1)], xs, ys))
+            xo += d["n"]
+            yo += d["m"]
+        return res
+
+    def stats(self) -> dict:
+        st = np.zeros(24)
+        _check_host(host_lib().ganit_batch_stats(self._h, st), "batch_stats")
+        r = dict(zip(_BATCH_STATS_KEYS, st.tolist()))
+        for k in ("rows", "cols", "nnz", "blocks_a", "blocks_at", "l1_instances", "l2_instances", "l3_instances",
+                  "max_iters", "total_iters", "autotune_pick"):
+            r[k] = int(r[k])
+        return r
+
+
+__all__ = ["Problem", "Solver", "Batch", "CSR", "host_lib", "NativeLibraryMissing", "STATUS", "LIB_DIR", "BIN_DIR"]

@@ -25,8 +25,8 @@ LIB = os.path.join(PKG, "lib")
 BIN = os.path.join(PKG, "bin")
 OBJ = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["runtime.hip", "pdhg_device.hip"]
-HOST_SOURCES = ["lp.cpp", "scaling.cpp", "generators.cpp", "driver.cpp", "capi.cpp"]
+HIP_SOURCES = ["runtime.hip", "pdhg_device.hip", "batch.hip"]
+HOST_SOURCES = ["lp.cpp", "scaling.cpp", "generators.cpp", "driver.cpp", "batch.cpp", "capi.cpp"]
 
 ARCH = os.environ.get("GANIT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

@@ -1,0 +1,1127 @@
+// libganit_hip.so, part 3: many uneven LPs solved together (branch-and-bound node relaxations, strong-
+// branching probes).
+//
+// Layout: the instances are packed into one block-diagonal problem -- A_all and its transpose are the
+// instances' matrices placed on the diagonal (global column indices), every vector is the concatenation of
+// the instances' vectors -- and the LDS-staged row-block partition is built per instance, so a workgroup
+// never mixes instances and carries its instance id.  Every kernel therefore runs over the whole batch
+// with full-GPU parallelism, whatever the instance sizes, instead of one instance per workgroup where the
+// largest instance would set the pace.
+//
+// State: the restart / step-size / convergence state of every instance lives on the device
+// (BatchState); after each block of check_every iterations a check pass produces 19 sums per instance
+// (per-block partials folded per instance in block order: deterministic) and k_bt_decide applies the
+// reference's check logic (src/pdhg_impl.hpp:144-189) to each instance in one thread.  Workgroups of
+// instances that have stopped return at once.  A block of iterations plus its check is one captured HIP
+// graph, replayed until no instance is running; the host reads a single counter per replay.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <cstdlib>
+#include <limits>
+#include <map>
+#include <tuple>
+#include <type_traits>
+#include <vector>
+
+#include "internal.hpp"
+#include "pdhg_math.hpp"
+
+namespace ganit_hip {
+
+constexpr int kBN = 12, kBM = 7, kBOut = GANIT_CHECK_NOUT;
+
+enum BatchStatus { kRunning = 0, kConverged = 1, kInfeasible = 2, kIterLimit = 3, kNumerical = 4, kStalled = 5,
+                   kTimeLimit = 6 };
+
+struct BatchInst {
+    int m, n, row0, col0;
+    double eta, norm_a, obj_scale, obj_const, bnorm, cnorm, w0;
+};
+
+struct BatchState {
+    double w, tau, sigma, inv_k;
+    double restart_err, prev_cand, best_err;
+    double best[6];  // pobj, dobj, rel_p, rel_d, rel_gap, err of the phase's result
+    double kkt64[6];
+    double cand[6];  // candidate of the latest check (the result if the phase is stopped by the clock)
+    int it, it_since, restarts, no_progress, status, best_avg, adopt, active, cand_avg;
+};
+
+struct BatchParams {
+    double tol, infeas_tol;
+    int max_iter, check_every, stall_checks, stall_exit;
+};
+
+// ------------------------------------------------------------------ iteration kernels
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* __restrict__ blocks,
+                                                      const int* __restrict__ list,
+                                                      const int* __restrict__ blk_inst,
+                                                      const BatchState* __restrict__ st, const V* __restrict__ y,
+                                                      const V* __restrict__ x, const V* __restrict__ c,
+                                                      const V* __restrict__ l, const V* __restrict__ u,
+                                                      V* __restrict__ xn, V* __restrict__ xbar,
+                                                      V* __restrict__ sumx) {
+    __shared__ V lds[kTile];
+    const int b = list[xcd_block_id()];  // a workgroup of a running instance
+    const BatchState& s = st[blk_inst[b]];
+    if (!s.active) return;
+    const V tau = V(s.tau);
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    stream_block<M, V, 1>(
+        AT, blocks, b, [&](int, int i) { return y[i]; },
+        [&](int j, V (&aty)[1]) {
+            V sm = sumx[j];
+            primal_update<V>(x[j], aty[0], V(0), c[j], l[j], u[j], tau, xn[j], xbar[j], sm);
+            sumx[j] = sm;
+        },
+        lds, none);
+}
+
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __restrict__ blocks,
+                                                    const int* __restrict__ list,
+                                                    const int* __restrict__ blk_inst, const BatchState* __restrict__ st,
+                                                    const V* __restrict__ xbar, const V* __restrict__ y,
+                                                    const V* __restrict__ lo, const V* __restrict__ hi,
+                                                    V* __restrict__ yn, V* __restrict__ sumy) {
+    __shared__ V lds[kTile];
+    const int b = list[xcd_block_id()];  // a workgroup of a running instance
+    const BatchState& s = st[blk_inst[b]];
+    if (!s.active) return;
+    const V sigma = V(s.sigma);
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    stream_block<M, V, 1>(
+        A, blocks, b, [&](int, int j) { return xbar[j]; },
+        [&](int i, V (&axb)[1]) {
+            V sm = sumy[i];
+            dual_update<V>(y[i], axb[0], lo[i], hi[i], sigma, yn[i], sm);
+            sumy[i] = sm;
+        },
+        lds, none);
+}
+
+// ------------------------------------------------------------------ check kernels
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_bt_average(int N, int M_, const int* __restrict__ col_inst,
+                                                       const int* __restrict__ row_inst,
+                                                       const BatchState* __restrict__ st, const V* __restrict__ sumx,
+                                                       const V* __restrict__ sumy, V* __restrict__ xa,
+                                                       V* __restrict__ ya) {
+    const int64_t total = (int64_t)N + M_;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        if (t < N) {
+            const BatchState& s = st[col_inst[t]];
+            if (s.active) xa[t] = V(s.inv_k * double(sumx[t]));
+        } else {
+            const int64_t i = t - N;
+            const BatchState& s = st[row_inst[i]];
+            if (s.active) ya[i] = V(s.inv_k * double(sumy[i]));
+        }
+    }
+}
+
+__device__ __forceinline__ void bt_dual_terms(double z, double lj, double uj, double cr, double& r2, double& ob) {
+    double r = 0.0, o = 0.0;
+    if (z > 0) { if (isfinite(lj)) o = z * lj; else r = z; }
+    else if (z < 0) { if (isfinite(uj)) o = z * uj; else r = z; }
+    r /= cr;
+    r2 = r * r;
+    ob = o;
+}
+__device__ __forceinline__ double bt_row_obj(double yi, double lw, double hg) {
+    if (yi > 0 && isfinite(lw)) return yi * lw;
+    if (yi < 0 && isfinite(hg)) return yi * hg;
+    return 0.0;
+}
+
+// Same sums as k_check_n_rb of the single-instance path, one partial row per workgroup.
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int* __restrict__ blocks,
+                                                       const int* __restrict__ list,
+                                                       const int* __restrict__ blk_inst,
+                                                       const BatchState* __restrict__ st, const V* __restrict__ x,
+                                                       const V* __restrict__ y, const V* __restrict__ xa,
+                                                       const V* __restrict__ ya, const V* __restrict__ xr,
+                                                       const V* __restrict__ yr, const V* __restrict__ c,
+                                                       const V* __restrict__ l, const V* __restrict__ u,
+                                                       const V* __restrict__ cres, int avg,
+                                                       double* __restrict__ partials) {
+    __shared__ V lds[kTile];
+    __shared__ double red[(kBlock / kWave) * kBN];
+    const int b = list[xcd_block_id()];  // a workgroup of a running instance
+    if (!st[blk_inst[b]].active) return;
+    double acc[kBN];
+#pragma unroll
+    for (int k = 0; k < kBN; ++k) acc[k] = 0.0;
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    stream_block_seq<M, V, 3>(
+        AT, blocks, b,
+        [&](int k, int i) { return k == 0 ? y[i] : k == 1 ? (avg ? ya[i] : V(0)) : V(y[i] - yr[i]); },
+        [&](int j, V (&t)[3]) {
+            const double cj = double(c[j]), lj = double(l[j]), uj = double(u[j]), cr = double(cres[j]);
+            const double xj = double(x[j]);
+            double r2, ob;
+            bt_dual_terms(cj - double(t[0]) + 0.0, lj, uj, cr, r2, ob);
+            acc[0] += r2;
+            acc[1] += ob;
+            acc[2] += cj * xj;
+            if (avg) {
+                const double xaj = double(xa[j]), xrj = double(xr[j]);
+                bt_dual_terms(cj - double(t[1]) + 0.0, lj, uj, cr, r2, ob);
+                acc[4] += r2;
+                acc[5] += ob;
+                acc[6] += cj * xaj;
+                const double d0 = xj - xrj, d1 = xaj - xrj;
+                acc[10] += d0 * d0;
+                acc[11] += d1 * d1;
+            }
+            bt_dual_terms(0.0 * cj - double(t[2]) + 0.0, lj, uj, cr, r2, ob);
+            acc[8] += r2;
+            acc[9] += ob;
+        },
+        lds, none);
+    __syncthreads();
+    block_sum<kBN>(acc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kBN; ++k) partials[(size_t)b * kBN + k] = acc[k];
+    }
+}
+
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* __restrict__ blocks,
+                                                       const int* __restrict__ list,
+                                                       const int* __restrict__ blk_inst,
+                                                       const BatchState* __restrict__ st, const V* __restrict__ x,
+                                                       const V* __restrict__ xa, const V* __restrict__ y,
+                                                       const V* __restrict__ ya, const V* __restrict__ yr,
+                                                       const V* __restrict__ lo, const V* __restrict__ hi,
+                                                       const V* __restrict__ rres, int avg,
+                                                       double* __restrict__ partials) {
+    __shared__ V lds[kTile];
+    __shared__ double red[(kBlock / kWave) * kBM];
+    const int b = list[xcd_block_id()];  // a workgroup of a running instance
+    if (!st[blk_inst[b]].active) return;
+    double acc[kBM];
+#pragma unroll
+    for (int k = 0; k < kBM; ++k) acc[k] = 0.0;
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    stream_block_seq<M, V, 2>(
+        A, blocks, b, [&](int k, int j) { return k == 0 ? x[j] : (avg ? xa[j] : V(0)); },
+        [&](int i, V (&t)[2]) {
+            const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
+            auto pres = [&](double a) {
+                const double p = fmin(fmax(a, lw), hg);
+                const double r = (a - p) / rr;
+                return r * r;
+            };
+            acc[0] += pres(double(t[0]));
+            acc[1] += bt_row_obj(yi, lw, hg);
+            if (avg) {
+                const double yai = double(ya[i]), yri = double(yr[i]);
+                acc[2] += pres(double(t[1]));
+                acc[3] += bt_row_obj(yai, lw, hg);
+                const double d0 = yi - yri, d1 = yai - yri;
+                acc[5] += d0 * d0;
+                acc[6] += d1 * d1;
+            }
+            acc[4] += bt_row_obj(double(V(y[i] - yr[i])), lw, hg);
+        },
+        lds, none);
+    __syncthreads();
+    block_sum<kBM>(acc, red);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kBM; ++k) partials[(size_t)b * kBM + k] = acc[k];
+    }
+}
+
+// One wave per instance: fold its workgroups' partials in block order into sums[inst][19].
+__global__ __launch_bounds__(kWave) void k_bt_reduce(int ninst, const int* __restrict__ first_n,
+                                                     const int* __restrict__ first_m,
+                                                     const BatchState* __restrict__ st,
+                                                     const double* __restrict__ part_n,
+                                                     const double* __restrict__ part_m, double* __restrict__ sums) {
+    const int i = blockIdx.x;
+    if (i >= ninst || !st[i].active) return;
+    double a[kBN], bm[kBM];
+#pragma unroll
+    for (int k = 0; k < kBN; ++k) a[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < kBM; ++k) bm[k] = 0.0;
+    for (int q = first_n[i] + (int)threadIdx.x; q < first_n[i + 1]; q += kWave) {
+#pragma unroll
+        for (int k = 0; k < kBN; ++k) a[k] += part_n[(size_t)q * kBN + k];
+    }
+    for (int q = first_m[i] + (int)threadIdx.x; q < first_m[i + 1]; q += kWave) {
+#pragma unroll
+        for (int k = 0; k < kBM; ++k) bm[k] += part_m[(size_t)q * kBM + k];
+    }
+#pragma unroll
+    for (int k = 0; k < kBN; ++k) {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) a[k] += __shfl_down(a[k], off, kWave);
+    }
+#pragma unroll
+    for (int k = 0; k < kBM; ++k) {
+#pragma unroll
+        for (int off = kWave / 2; off > 0; off >>= 1) bm[k] += __shfl_down(bm[k], off, kWave);
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kBN; ++k) sums[(size_t)i * kBOut + k] = a[k];
+#pragma unroll
+        for (int k = 0; k < kBM; ++k) sums[(size_t)i * kBOut + kBN + k] = bm[k];
+    }
+}
+
+// ------------------------------------------------------------------ per-instance decisions
+struct Kkt {
+    double pobj, dobj, rel_p, rel_d, rel_gap, err;
+    __device__ bool finite() const { return isfinite(pobj) && isfinite(dobj) && isfinite(rel_p) && isfinite(rel_d); }
+    __device__ bool converged(double tol) const { return rel_p <= tol && rel_d <= tol && rel_gap <= tol; }
+};
+
+__device__ Kkt bt_kkt(const BatchInst& in, double rd2, double vobj, double cx, double xqx, double rp2, double robj) {
+    Kkt k;  // src/pdhg_impl.hpp:102-107
+    k.pobj = (cx + 0.5 * xqx) * in.obj_scale + in.obj_const;
+    k.dobj = (robj + vobj - 0.5 * xqx) * in.obj_scale + in.obj_const;
+    k.rel_p = sqrt(rp2) / (1.0 + in.bnorm);
+    k.rel_d = sqrt(rd2) / (1.0 + in.cnorm);
+    k.rel_gap = fabs(k.pobj - k.dobj) / (1.0 + fabs(k.pobj) + fabs(k.dobj));
+    k.err = sqrt(k.rel_p * k.rel_p + k.rel_d * k.rel_d + k.rel_gap * k.rel_gap);
+    return k;
+}
+
+__device__ void bt_store(double* dst, const Kkt& k) {
+    dst[0] = k.pobj; dst[1] = k.dobj; dst[2] = k.rel_p; dst[3] = k.rel_d; dst[4] = k.rel_gap; dst[5] = k.err;
+}
+
+enum DecideMode { kInit = 0, kCheck = 1, kFinal = 2 };
+
+// One thread per instance; block 0 also counts the instances still running into *running.
+__global__ __launch_bounds__(1024) void k_bt_decide(int ninst, const BatchInst* __restrict__ inst,
+                                                    BatchState* __restrict__ st, const double* __restrict__ sums,
+                                                    BatchParams P, int mode, int* __restrict__ running) {
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int i = threadIdx.x; i < ninst; i += blockDim.x) {
+        BatchState s = st[i];
+        const BatchInst in = inst[i];
+        const double* o = sums + (size_t)i * kBOut;
+        if (mode == kFinal) {
+            if (s.active) bt_store(s.kkt64, bt_kkt(in, o[0], o[1], o[2], o[3], o[12], o[13]));
+            st[i] = s;
+            continue;
+        }
+        if (!s.active) continue;
+        const int ce = P.check_every;
+        if (mode == kInit) {  // phase start (src/pdhg_impl.hpp:120-125)
+            double e = bt_kkt(in, o[0], o[1], o[2], o[3], o[12], o[13]).err;
+            if (s.restart_err > 0) e = fmax(e, s.restart_err);
+            s.restart_err = e;
+            s.best_err = e;
+            s.prev_cand = INFINITY;
+            s.it_since = 0;
+            s.no_progress = 0;
+            s.adopt = -1;
+            s.sigma = in.eta * s.w;
+            s.tau = in.eta / s.w;
+            s.inv_k = 1.0 / ce;
+            ++mine;
+            st[i] = s;
+            continue;
+        }
+        // a check after ce more iterations (src/pdhg_impl.hpp:144-189)
+        s.it += ce;
+        s.it_since += ce;
+        s.adopt = -1;
+        const Kkt kc = bt_kkt(in, o[0], o[1], o[2], o[3], o[12], o[13]);
+        const Kkt ka = bt_kkt(in, o[4], o[5], o[6], o[7], o[14], o[15]);
+        int why = kRunning;
+        Kkt best = kc;
+        int best_avg = 0;
+        const bool avg_better = ka.finite() && (!kc.finite() || ka.err < kc.err);
+        const Kkt kcand = avg_better ? ka : kc;
+        if (!kc.finite() && !ka.finite()) { why = kNumerical; best = kc; }
+        else if (kc.converged(P.tol)) { why = kConverged; best = kc; best_avg = 0; }
+        else if (ka.converged(P.tol)) { why = kConverged; best = ka; best_avg = 1; }
+        else if (s.it >= P.max_iter) { why = kIterLimit; best = kcand; best_avg = avg_better; }
+        else {
+            if (s.it_since >= 4 * ce) {  // src/pdhg_impl.hpp:110-118
+                const double dray = (o[16] + o[9]) * in.obj_scale;
+                if (dray > 0 && sqrt(o[8]) * (1.0 + in.bnorm) / dray <= P.infeas_tol) {
+                    why = kInfeasible; best = kcand; best_avg = avg_better;
+                }
+            }
+            if (why == kRunning) {
+                if (kcand.err < 0.8 * s.best_err) { s.best_err = kcand.err; s.no_progress = 0; }
+                else if (++s.no_progress >= P.stall_checks && P.stall_exit) {
+                    why = kStalled; best = kcand; best_avg = avg_better;
+                }
+            }
+            if (why == kRunning) {
+                const bool sufficient = kcand.err <= 0.2 * s.restart_err;
+                const bool necessary = kcand.err <= 0.8 * s.restart_err && kcand.err > s.prev_cand;
+                const bool artificial = s.it_since >= 0.36 * (double)s.it;
+                s.prev_cand = kcand.err;
+                if (sufficient || necessary || artificial) {
+                    const double dx = sqrt(avg_better ? o[11] : o[10]), dy = sqrt(avg_better ? o[18] : o[17]);
+                    if (dx > 1e-10 && dy > 1e-10) s.w = exp(0.5 * log(dy / dx) + 0.5 * log(s.w));
+                    s.adopt = avg_better ? 1 : 0;
+                    s.it_since = 0;
+                    s.restart_err = kcand.err;
+                    s.prev_cand = INFINITY;
+                    ++s.restarts;
+                    s.sigma = in.eta * s.w;
+                    s.tau = in.eta / s.w;
+                }
+            }
+        }
+        bt_store(s.cand, kcand);
+        s.cand_avg = avg_better ? 1 : 0;
+        if (why != kRunning) {
+            s.status = why;
+            s.best_avg = best_avg;
+            bt_store(s.best, best);
+            s.adopt = best_avg ? 2 : -1;  // hand the average back as the phase's result
+        } else {
+            ++mine;
+        }
+        s.inv_k = 1.0 / (double)(s.it_since + ce);
+        st[i] = s;
+    }
+    atomicAdd(&cnt, mine);
+    __syncthreads();
+    if (threadIdx.x == 0) *running = cnt;
+}
+
+// Apply each instance's decision: adopt 0/1 = restart (1: from the average), 2 = adopt the average as the
+// phase result; then instances whose phase ended stop taking part.
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_bt_restart(int N, int M_, const int* __restrict__ col_inst,
+                                                       const int* __restrict__ row_inst,
+                                                       const BatchState* __restrict__ st, V* __restrict__ x,
+                                                       const V* __restrict__ xa, V* __restrict__ xr,
+                                                       V* __restrict__ sumx, V* __restrict__ y,
+                                                       const V* __restrict__ ya, V* __restrict__ yr,
+                                                       V* __restrict__ sumy) {
+    const int64_t total = (int64_t)N + M_;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        const bool col = t < N;
+        const int64_t e = col ? t : t - N;
+        const int a = st[col ? col_inst[e] : row_inst[e]].adopt;
+        if (a < 0) continue;
+        V* v = col ? x : y;
+        const V* va = col ? xa : ya;
+        const V nv = (a >= 1) ? va[e] : v[e];
+        v[e] = nv;
+        if (a <= 1) {
+            (col ? xr : yr)[e] = nv;
+            (col ? sumx : sumy)[e] = V(0);
+        }
+    }
+}
+
+// The clock ran out: every running instance ends its phase with the candidate of its latest check
+// (src/pdhg_impl.hpp:162).
+__global__ void k_bt_stop(int ninst, BatchState* __restrict__ st) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) {
+        BatchState& s = st[i];
+        if (!s.active || s.status != kRunning) continue;
+        s.status = kTimeLimit;
+        s.best_avg = s.cand_avg;
+        for (int k = 0; k < 6; ++k) s.best[k] = s.cand[k];
+        s.adopt = s.cand_avg ? 2 : -1;
+    }
+}
+
+// Next phase: instances flagged in `sel` run again (status running), the others stay stopped.
+__global__ void k_bt_select(int ninst, BatchState* __restrict__ st, const int* __restrict__ sel) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) {
+        BatchState& s = st[i];
+        s.active = sel[i] ? 1 : 0;
+        if (sel[i]) s.status = kRunning;
+        s.adopt = -1;
+    }
+}
+
+template <class S, class D>
+__global__ __launch_bounds__(kBlock) void k_bt_convert(int64_t n, const S* __restrict__ s, D* __restrict__ d) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        d[i] = D(s[i]);
+}
+
+__global__ void k_bt_deactivate(int ninst, BatchState* __restrict__ st) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ninst; i += gridDim.x * blockDim.x) {
+        if (st[i].status != kRunning) st[i].active = 0;
+        st[i].adopt = -1;
+    }
+}
+
+// ------------------------------------------------------------------ host object
+struct BatchVecs {
+    void *c = nullptr, *l = nullptr, *u = nullptr, *lo = nullptr, *hi = nullptr, *rres = nullptr, *cres = nullptr;
+    void *X[2] = {nullptr, nullptr}, *Y[2] = {nullptr, nullptr};
+    void *xbar = nullptr, *sumx = nullptr, *sumy = nullptr, *xa = nullptr, *ya = nullptr, *xr = nullptr, *yr = nullptr;
+};
+
+struct Batch {
+    Ctx* ctx = nullptr;
+    int ninst = 0, N = 0, Mr = 0;
+    Csr *A = nullptr, *AT = nullptr;
+    int *blk_inst_a = nullptr, *blk_inst_at = nullptr, *first_a = nullptr, *first_at = nullptr;
+    int *col_inst = nullptr, *row_inst = nullptr;
+    BatchInst* inst = nullptr;
+    BatchState* st = nullptr;
+    double *part_n = nullptr, *part_m = nullptr, *sums = nullptr;
+    int* running = nullptr;
+    int* host_running = nullptr;  // pinned
+    // host copies for re-partitioning (autotuner)
+    std::vector<int> aptr_h, atptr_h;
+    std::vector<BatchInst> inst_h;
+    int part_rows = kBlock, part_nnz = kTile;
+    size_t part_n_cap = 0, part_m_cap = 0;
+    // launch lists: the row blocks of the instances taking part (kernels map blockIdx through them)
+    std::vector<int> first_a_h, first_at_h;
+    int *list_a = nullptr, *list_at = nullptr;
+    int nact_a = 0, nact_at = 0;
+    std::vector<char> listed;  // instances in the current lists
+    BatchVecs vs[2];
+    bool on[2] = {false, false};
+    int cur = 0;
+    BatchParams P{};
+    std::vector<void*> owned;
+    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+    void* alloc(size_t bytes) {
+        void* p = nullptr;
+        GH_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
+        owned.push_back(p);
+        return p;
+    }
+    void drop_graphs() {
+        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
+    }
+    ~Batch() {
+        if (ctx) (void)hipStreamSynchronize(ctx->stream);
+        drop_graphs();
+        for (void* p : owned) (void)hipFree(p);
+        for (void* p : {(void*)blk_inst_a, (void*)blk_inst_at, (void*)first_a, (void*)first_at, (void*)part_n,
+                        (void*)part_m, (void*)list_a, (void*)list_at})
+            if (p) (void)hipFree(p);
+        if (host_running) (void)hipHostFree(host_running);
+        csr_destroy(A);
+        csr_destroy(AT);
+    }
+};
+
+unsigned bt_grid(int64_t n) { return std::min<unsigned>(grid_for(n), 8192u); }
+
+// Launch lists for the instances flagged in `on`: the workgroups of instances that have stopped are left out
+// of the grid instead of being launched to return at once, which matters when a few slow instances remain.
+// Changing the lists changes grid sizes, so captured graphs are dropped.
+void bt_list(Batch* bt, const std::vector<char>& on) {
+    if (on == bt->listed) return;
+    auto fill = [&](const std::vector<int>& first, int* dev, int& count) {
+        std::vector<int> l;
+        for (int k = 0; k < bt->ninst; ++k)
+            if (on[k])
+                for (int b = first[k]; b < first[k + 1]; ++b) l.push_back(b);
+        count = (int)l.size();
+        if (count > 0)
+            GH_CHECK(hipMemcpyAsync(dev, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice, bt->ctx->stream));
+        GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
+    };
+    fill(bt->first_a_h, bt->list_a, bt->nact_a);
+    fill(bt->first_at_h, bt->list_at, bt->nact_at);
+    bt->listed = on;
+    bt->drop_graphs();
+}
+
+// The instances whose state says active.
+std::vector<char> bt_active(Batch* bt) {
+    std::vector<BatchState> s(bt->ninst);
+    GH_CHECK(hipMemcpyAsync(s.data(), bt->st, sizeof(BatchState) * bt->ninst, hipMemcpyDeviceToHost, bt->ctx->stream));
+    GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
+    std::vector<char> on(bt->ninst);
+    for (int k = 0; k < bt->ninst; ++k) on[k] = s[k].active ? 1 : 0;
+    return on;
+}
+
+// Row-block partition of A and A^T, built per instance (at most `rows` rows and `nnz` nonzeros per block;
+// rows longer than kLong get a block of their own), so a workgroup never mixes instances.  Replaces the
+// previous partition and drops captured graphs, whose kernel arguments name the old arrays.
+void bt_partition(Batch* bt, int rows, int nnz) {
+    Ctx* ctx = bt->ctx;
+    const int ninst = bt->ninst;
+    auto build = [&](Csr* a, const std::vector<int>& ptr, bool rows_of_a, int** blk_inst, int** first,
+                     std::vector<int>& first_h, int** list) {
+        std::vector<int> blocks{0}, bi, fi{0};
+        std::vector<int> lp;
+        for (int k = 0; k < ninst; ++k) {
+            const BatchInst& in = bt->inst_h[k];
+            const int r0 = rows_of_a ? in.row0 : in.col0;
+            const int nr = rows_of_a ? in.m : in.n;
+            lp.resize(nr + 1);
+            for (int r = 0; r <= nr; ++r) lp[r] = ptr[r0 + r] - ptr[r0];
+            const std::vector<int> rb = row_blocks(lp.data(), nr, rows, nnz);
+            for (size_t q = 1; q < rb.size(); ++q) {
+                blocks.push_back(r0 + rb[q]);
+                bi.push_back(k);
+            }
+            fi.push_back((int)bi.size());
+        }
+        if (a->blocks) GH_CHECK(hipFree(a->blocks));
+        a->blocks = nullptr;
+        GH_CHECK(hipMalloc(&a->blocks, sizeof(int) * blocks.size()));
+        GH_CHECK(hipMemcpyAsync(a->blocks, blocks.data(), sizeof(int) * blocks.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+        a->nblocks = (int)blocks.size() - 1;
+        a->lanes = 0;
+        a->rb_rows = rows;
+        a->rb_nnz = nnz;
+        if (*blk_inst) GH_CHECK(hipFree(*blk_inst));
+        if (*first) GH_CHECK(hipFree(*first));
+        *blk_inst = nullptr;
+        *first = nullptr;
+        GH_CHECK(hipMalloc(blk_inst, sizeof(int) * std::max<size_t>(bi.size(), 1)));
+        GH_CHECK(hipMalloc(first, sizeof(int) * fi.size()));
+        if (!bi.empty())
+            GH_CHECK(hipMemcpyAsync(*blk_inst, bi.data(), sizeof(int) * bi.size(), hipMemcpyHostToDevice,
+                                    ctx->stream));
+        GH_CHECK(hipMemcpyAsync(*first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice, ctx->stream));
+        GH_CHECK(hipStreamSynchronize(ctx->stream));
+        first_h = fi;
+        if (*list) GH_CHECK(hipFree(*list));
+        *list = nullptr;
+        GH_CHECK(hipMalloc(list, sizeof(int) * std::max<size_t>(bi.size(), 1)));
+    };
+    GH_CHECK(hipStreamSynchronize(ctx->stream));
+    bt->drop_graphs();
+    build(bt->A, bt->aptr_h, true, &bt->blk_inst_a, &bt->first_a, bt->first_a_h, &bt->list_a);
+    build(bt->AT, bt->atptr_h, false, &bt->blk_inst_at, &bt->first_at, bt->first_at_h, &bt->list_at);
+    bt->listed.assign(ninst, 0);
+    bt_list(bt, std::vector<char>(ninst, 1));
+    const size_t need_n = (size_t)kBN * std::max(bt->AT->nblocks, 1), need_m = (size_t)kBM * std::max(bt->A->nblocks, 1);
+    if (need_n > bt->part_n_cap) {
+        if (bt->part_n) GH_CHECK(hipFree(bt->part_n));
+        bt->part_n = nullptr;
+        GH_CHECK(hipMalloc(&bt->part_n, sizeof(double) * need_n));
+        bt->part_n_cap = need_n;
+    }
+    // the partial buffer of A also stages the instance selection (one int per instance)
+    const size_t need_m2 = std::max(need_m, (size_t)(ninst + 1) / 2 + 1);
+    if (need_m2 > bt->part_m_cap) {
+        if (bt->part_m) GH_CHECK(hipFree(bt->part_m));
+        bt->part_m = nullptr;
+        GH_CHECK(hipMalloc(&bt->part_m, sizeof(double) * need_m2));
+        bt->part_m_cap = need_m2;
+    }
+    bt->part_rows = rows;
+    bt->part_nnz = nnz;
+}
+
+template <class M, class V>
+void bt_iteration(Batch* bt, int mdt_unused) {
+    (void)mdt_unused;
+    hipStream_t s = bt->ctx->stream;
+    BatchVecs& v = bt->vs[std::is_same<V, double>::value ? 0 : 1];
+    const int c = bt->cur;
+    if (bt->nact_at > 0)
+        hipLaunchKernelGGL((k_bt_primal<M, V>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
+                           bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
+                           (const V*)v.c, (const V*)v.l, (const V*)v.u, (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
+    if (bt->nact_a > 0)
+        hipLaunchKernelGGL((k_bt_dual<M, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<M>(), bt->A->blocks,
+                           bt->list_a, bt->blk_inst_a, bt->st, (const V*)v.xbar, (const V*)v.Y[c], (const V*)v.lo,
+                           (const V*)v.hi, (V*)v.Y[1 - c], (V*)v.sumy);
+    bt->cur ^= 1;
+}
+
+template <class M, class V>
+void bt_check(Batch* bt, int avg, int mode) {
+    hipStream_t s = bt->ctx->stream;
+    BatchVecs& v = bt->vs[std::is_same<V, double>::value ? 0 : 1];
+    const int c = bt->cur;
+    if (avg)
+        hipLaunchKernelGGL(k_bt_average<V>, dim3(bt_grid((int64_t)bt->N + bt->Mr)), dim3(kBlock), 0, s, bt->N, bt->Mr,
+                           bt->col_inst, bt->row_inst, bt->st, (const V*)v.sumx, (const V*)v.sumy, (V*)v.xa,
+                           (V*)v.ya);
+    if (bt->nact_at > 0)
+    hipLaunchKernelGGL((k_bt_check_n<M, V>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
+                       bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->st, (const V*)v.X[c], (const V*)v.Y[c], (const V*)v.xa,
+                       (const V*)v.ya, (const V*)v.xr, (const V*)v.yr, (const V*)v.c, (const V*)v.l, (const V*)v.u,
+                       (const V*)v.cres, avg, bt->part_n);
+    if (bt->nact_a > 0)
+    hipLaunchKernelGGL((k_bt_check_m<M, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<M>(),
+                       bt->A->blocks, bt->list_a, bt->blk_inst_a, bt->st, (const V*)v.X[c], (const V*)v.xa, (const V*)v.Y[c],
+                       (const V*)v.ya, (const V*)v.yr, (const V*)v.lo, (const V*)v.hi, (const V*)v.rres, avg,
+                       bt->part_m);
+    hipLaunchKernelGGL(k_bt_reduce, dim3(bt->ninst), dim3(kWave), 0, s, bt->ninst, bt->first_at, bt->first_a, bt->st,
+                       bt->part_n, bt->part_m, bt->sums);
+    hipLaunchKernelGGL(k_bt_decide, dim3(1), dim3(1024), 0, s, bt->ninst, bt->inst, bt->st, bt->sums, bt->P, mode,
+                       bt->running);
+    if (mode == kCheck)
+        hipLaunchKernelGGL(k_bt_restart<V>, dim3(bt_grid((int64_t)bt->N + bt->Mr)), dim3(kBlock), 0, s, bt->N, bt->Mr,
+                           bt->col_inst, bt->row_inst, bt->st, (V*)v.X[c], (const V*)v.xa, (V*)v.xr, (V*)v.sumx,
+                           (V*)v.Y[c], (const V*)v.ya, (V*)v.yr, (V*)v.sumy);
+    GH_CHECK(hipGetLastError());
+}
+
+template <class F>
+void bt_types(int mdt, int vdt, F&& f) {
+    with_mdt(mdt, [&](auto MT) {
+        with_vdt(vdt, [&](auto VT) { f(MT, VT); });
+    });
+}
+
+}  // namespace ganit_hip
+
+using namespace ganit_hip;
+
+extern "C" {
+
+int ganit_hip_batch_create(ganit_hip_ctx* h, int ninst, const ganit_hip_lp_desc* const* lps,
+                           const double* const* inst_scalars, unsigned mdt_mask, unsigned vdt_mask,
+                           ganit_hip_batch** out) {
+    return guarded([&] {
+        auto* ctx = reinterpret_cast<Ctx*>(h);
+        GH_REQUIRE(ctx && out && ninst > 0 && lps && inst_scalars, "bad batch arguments");
+        GH_REQUIRE((vdt_mask & 3u) && (mdt_mask & 7u), "empty dtype mask");
+        auto* bt = new Batch;
+        try {
+            bt->ctx = ctx;
+            bt->ninst = ninst;
+            // ---- pack the block-diagonal A and A^T with global indices
+            std::vector<int> aptr{0}, aidx, atptr{0}, atidx;
+            std::vector<double> aval, atval;
+            std::vector<BatchInst> inst(ninst);
+            std::vector<int> col_inst, row_inst;
+            for (int k = 0; k < ninst; ++k) {
+                const ganit_hip_lp_desc& d = *lps[k];
+                GH_REQUIRE(d.m >= 0 && d.n >= 0 && !d.q_ptr, "batched instances must be LPs");
+                BatchInst& in = inst[k];
+                in.m = d.m; in.n = d.n; in.row0 = bt->Mr; in.col0 = bt->N;
+                const double* sc = inst_scalars[k];  // eta, norm_a, obj_scale, obj_const, bnorm, cnorm, w0
+                in.eta = sc[0]; in.norm_a = sc[1]; in.obj_scale = sc[2]; in.obj_const = sc[3];
+                in.bnorm = sc[4]; in.cnorm = sc[5]; in.w0 = sc[6];
+                for (int i = 0; i < d.m; ++i) {
+                    GH_REQUIRE(d.a_ptr[i + 1] - d.a_ptr[i] <= kSplit, "batched rows are limited to 8192 entries");
+                    for (int p = d.a_ptr[i]; p < d.a_ptr[i + 1]; ++p) {
+                        aidx.push_back(in.col0 + d.a_idx[p]);
+                        aval.push_back(d.a_val[p]);
+                    }
+                    aptr.push_back((int)aidx.size());
+                    row_inst.push_back(k);
+                }
+                for (int j = 0; j < d.n; ++j) {
+                    GH_REQUIRE(d.at_ptr[j + 1] - d.at_ptr[j] <= kSplit, "batched columns are limited to 8192 entries");
+                    for (int p = d.at_ptr[j]; p < d.at_ptr[j + 1]; ++p) {
+                        atidx.push_back(in.row0 + d.at_idx[p]);
+                        atval.push_back(d.at_val[p]);
+                    }
+                    atptr.push_back((int)atidx.size());
+                    col_inst.push_back(k);
+                }
+                bt->Mr += d.m;
+                bt->N += d.n;
+                GH_REQUIRE(aidx.size() < (size_t)std::numeric_limits<int>::max(),
+                           "batched nonzeros exceed 32-bit indexing");
+            }
+            bt->A = csr_create(ctx, bt->Mr, bt->N, aptr.data(), aidx.data(), aval.data(), mdt_mask);
+            bt->AT = csr_create(ctx, bt->N, bt->Mr, atptr.data(), atidx.data(), atval.data(), mdt_mask);
+            bt->aptr_h = std::move(aptr);
+            bt->atptr_h = std::move(atptr);
+            bt->inst_h = inst;
+            bt_partition(bt, kBlock, kTile);
+            bt->col_inst = (int*)bt->alloc(sizeof(int) * std::max<size_t>(col_inst.size(), 1));
+            bt->row_inst = (int*)bt->alloc(sizeof(int) * std::max<size_t>(row_inst.size(), 1));
+            if (!col_inst.empty())
+                GH_CHECK(hipMemcpyAsync(bt->col_inst, col_inst.data(), sizeof(int) * col_inst.size(), hipMemcpyHostToDevice, ctx->stream));
+            if (!row_inst.empty())
+                GH_CHECK(hipMemcpyAsync(bt->row_inst, row_inst.data(), sizeof(int) * row_inst.size(), hipMemcpyHostToDevice, ctx->stream));
+            bt->inst = (BatchInst*)bt->alloc(sizeof(BatchInst) * ninst);
+            GH_CHECK(hipMemcpyAsync(bt->inst, inst.data(), sizeof(BatchInst) * ninst, hipMemcpyHostToDevice, ctx->stream));
+            bt->st = (BatchState*)bt->alloc(sizeof(BatchState) * ninst);
+            bt->sums = (double*)bt->alloc(sizeof(double) * kBOut * ninst);
+            bt->running = (int*)bt->alloc(sizeof(int));
+            GH_CHECK(hipHostMalloc((void**)&bt->host_running, sizeof(int), hipHostMallocDefault));
+            // ---- vectors (concatenated)
+            std::vector<double> cat_n(bt->N), cat_m(bt->Mr);
+            for (int vdt = 0; vdt < 2; ++vdt) {
+                if (!(vdt_mask & (1u << vdt))) continue;
+                const int slot = vdt == GANIT_F64 ? 0 : 1;
+                bt->on[slot] = true;
+                BatchVecs& v = bt->vs[slot];
+                const size_t e = dtype_size(vdt);
+                auto upload_cat = [&](void** dst, bool is_col, const double* ganit_hip_lp_desc::*field) {
+                    std::vector<double>& cat = is_col ? cat_n : cat_m;
+                    for (int k = 0; k < ninst; ++k) {
+                        const int len = is_col ? inst[k].n : inst[k].m, off = is_col ? inst[k].col0 : inst[k].row0;
+                        const double* src = lps[k]->*field;
+                        for (int q = 0; q < len; ++q) cat[off + q] = src[q];
+                    }
+                    *dst = bt->alloc(e * cat.size());
+                    if (vdt == GANIT_F64) {
+                        GH_CHECK(hipMemcpyAsync(*dst, cat.data(), 8 * cat.size(), hipMemcpyHostToDevice, ctx->stream));
+                        GH_CHECK(hipStreamSynchronize(ctx->stream));
+                    } else {
+                        std::vector<float> f(cat.begin(), cat.end());
+                        GH_CHECK(hipMemcpyAsync(*dst, f.data(), 4 * f.size(), hipMemcpyHostToDevice, ctx->stream));
+                        GH_CHECK(hipStreamSynchronize(ctx->stream));
+                    }
+                };
+                upload_cat(&v.c, true, &ganit_hip_lp_desc::c);
+                upload_cat(&v.l, true, &ganit_hip_lp_desc::l);
+                upload_cat(&v.u, true, &ganit_hip_lp_desc::u);
+                upload_cat(&v.cres, true, &ganit_hip_lp_desc::cres);
+                upload_cat(&v.lo, false, &ganit_hip_lp_desc::lo);
+                upload_cat(&v.hi, false, &ganit_hip_lp_desc::hi);
+                upload_cat(&v.rres, false, &ganit_hip_lp_desc::rres);
+                for (int q = 0; q < 2; ++q) { v.X[q] = bt->alloc(e * bt->N); v.Y[q] = bt->alloc(e * bt->Mr); }
+                v.xbar = bt->alloc(e * bt->N); v.sumx = bt->alloc(e * bt->N); v.xa = bt->alloc(e * bt->N);
+                v.xr = bt->alloc(e * bt->N);
+                v.sumy = bt->alloc(e * bt->Mr); v.ya = bt->alloc(e * bt->Mr); v.yr = bt->alloc(e * bt->Mr);
+            }
+            GH_CHECK(hipStreamSynchronize(ctx->stream));
+        } catch (...) {
+            delete bt;
+            throw;
+        }
+        *out = reinterpret_cast<ganit_hip_batch*>(bt);
+    });
+}
+
+int ganit_hip_batch_destroy(ganit_hip_batch* h) {
+    return guarded([&] { delete reinterpret_cast<Batch*>(h); });
+}
+
+int ganit_hip_batch_dims(const ganit_hip_batch* h, int64_t* dims) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<const Batch*>(h);
+        GH_REQUIRE(bt && dims, "null argument");
+        dims[0] = bt->ninst; dims[1] = bt->Mr; dims[2] = bt->N; dims[3] = bt->A->nnz;
+        dims[4] = bt->A->nblocks; dims[5] = bt->AT->nblocks;
+    });
+}
+
+int ganit_hip_batch_load(ganit_hip_batch* h, int vdt, const double* x, const double* y, const double* w) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && x && y && w, "null argument");
+        const int slot = vdt == GANIT_F64 ? 0 : 1;
+        GH_REQUIRE(bt->on[slot], "vector dtype not resident");
+        BatchVecs& v = bt->vs[slot];
+        bt->cur = 0;
+        hipStream_t st = bt->ctx->stream;
+        auto up = [&](const double* src, void* dst, int64_t k) {
+            if (k == 0) return;
+            if (vdt == GANIT_F64) {
+                GH_CHECK(hipMemcpyAsync(dst, src, 8 * k, hipMemcpyHostToDevice, st));
+            } else {
+                std::vector<float> f(src, src + k);
+                GH_CHECK(hipMemcpyAsync(dst, f.data(), 4 * k, hipMemcpyHostToDevice, st));
+            }
+            GH_CHECK(hipStreamSynchronize(st));
+        };
+        std::vector<double> zn(bt->N, 0.0), zm(bt->Mr, 0.0);
+        up(x, v.X[0], bt->N); up(y, v.Y[0], bt->Mr); up(x, v.xr, bt->N); up(y, v.yr, bt->Mr);
+        up(zn.data(), v.sumx, bt->N); up(zm.data(), v.sumy, bt->Mr);
+        std::vector<BatchState> s(bt->ninst);
+        for (int k = 0; k < bt->ninst; ++k) {
+            std::memset(&s[k], 0, sizeof(BatchState));
+            s[k].w = w[k];
+            s[k].restart_err = -1.0;
+            s[k].status = kRunning;
+            s[k].active = 1;
+            s[k].adopt = -1;
+        }
+        GH_CHECK(hipMemcpyAsync(bt->st, s.data(), sizeof(BatchState) * bt->ninst, hipMemcpyHostToDevice, st));
+        GH_CHECK(hipStreamSynchronize(st));
+        bt_list(bt, std::vector<char>(bt->ninst, 1));
+    });
+}
+
+int ganit_hip_batch_handover(ganit_hip_batch* h, int from, int to) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt, "null batch");
+        const int sa = from == GANIT_F64 ? 0 : 1, sb = to == GANIT_F64 ? 0 : 1;
+        GH_REQUIRE(bt->on[sa] && bt->on[sb], "vector dtype not resident");
+        BatchVecs &a = bt->vs[sa], &b = bt->vs[sb];
+        hipStream_t st = bt->ctx->stream;
+        with_vdt(from, [&](auto S) {
+            with_vdt(to, [&](auto D) {
+                using SV = typename decltype(S)::type;
+                using DV = typename decltype(D)::type;
+                auto conv = [&](void* src, void* dst, int64_t k) {
+                    if (k > 0 && src != dst)
+                        hipLaunchKernelGGL((k_bt_convert<SV, DV>), dim3(bt_grid(k)), dim3(kBlock), 0, st, k,
+                                           (const SV*)src, (DV*)dst);
+                };
+                conv(a.X[bt->cur], b.X[bt->cur], bt->N);
+                conv(a.Y[bt->cur], b.Y[bt->cur], bt->Mr);
+                conv(a.xr, b.xr, bt->N);
+                conv(a.yr, b.yr, bt->Mr);
+            });
+        });
+        // fresh running sums for the next phase
+        with_vdt(to, [&](auto D) {
+            using DV = typename decltype(D)::type;
+            std::vector<DV> zz(std::max(bt->N, bt->Mr), DV(0));
+            GH_CHECK(hipMemcpyAsync(b.sumx, zz.data(), sizeof(DV) * bt->N, hipMemcpyHostToDevice, st));
+            GH_CHECK(hipMemcpyAsync(b.sumy, zz.data(), sizeof(DV) * bt->Mr, hipMemcpyHostToDevice, st));
+            GH_CHECK(hipStreamSynchronize(st));
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+int ganit_hip_batch_select(ganit_hip_batch* h, const int* sel) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && sel, "null argument");
+        int* d = (int*)bt->part_m;  // the partial buffer is free between phases
+        GH_REQUIRE((size_t)bt->ninst * sizeof(int) <= sizeof(double) * bt->part_m_cap,
+                   "selection does not fit the staging buffer");
+        GH_CHECK(hipMemcpyAsync(d, sel, sizeof(int) * bt->ninst, hipMemcpyHostToDevice, bt->ctx->stream));
+        hipLaunchKernelGGL(k_bt_select, dim3(std::min(1024, (bt->ninst + 255) / 256)), dim3(256), 0, bt->ctx->stream,
+                           bt->ninst, bt->st, d);
+        GH_CHECK(hipGetLastError());
+        GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
+    });
+}
+
+// params: tol, infeas_tol, max_iter, check_every (even), stall_checks, stall_exit, time_limit_s
+int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* params, int* running_left) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && params, "null argument");
+        const int slot = vdt == GANIT_F64 ? 0 : 1;
+        GH_REQUIRE(bt->on[slot], "vector dtype not resident");
+        bt->P.tol = params[0];
+        bt->P.infeas_tol = params[1];
+        bt->P.max_iter = (int)std::min(params[2], 2.0e9);
+        bt->P.check_every = (int)params[3];
+        bt->P.stall_checks = (int)params[4];
+        bt->P.stall_exit = params[5] != 0 ? 1 : 0;
+        const double time_limit = params[6];
+        GH_REQUIRE(bt->P.check_every >= 2 && bt->P.check_every % 2 == 0, "batched check interval must be even");
+        hipStream_t st = bt->ctx->stream;
+        const auto t0 = std::chrono::steady_clock::now();
+        bt_types(mdt, vdt, [&](auto MT, auto VT) {
+            using M = typename decltype(MT)::type;
+            using V = typename decltype(VT)::type;
+            bt_list(bt, bt_active(bt));  // launch only the selected instances' workgroups
+            bt_check<M, V>(bt, 0, kInit);  // k_restart of every running instance
+            const auto key = std::make_tuple(mdt, vdt, bt->P.check_every);
+            auto block = [&]() {
+                for (int k = 0; k < bt->P.check_every; ++k) bt_iteration<M, V>(bt, mdt);
+                bt_check<M, V>(bt, 1, kCheck);
+                hipLaunchKernelGGL(k_bt_deactivate, dim3(std::min(1024, (bt->ninst + 255) / 256)), dim3(256), 0, st,
+                                   bt->ninst, bt->st);
+            };
+            const char* ng = std::getenv("GANIT_HIP_NO_GRAPH");
+            const bool graphs = !(ng && ng[0] == '1');
+            // the graph of one block (check_every iterations + check) for the current launch lists
+            auto graph = [&]() -> hipGraphExec_t {
+                auto it = bt->graphs.find(key);
+                if (it != bt->graphs.end()) return it->second;
+                hipGraph_t g = nullptr;
+                const int cur0 = bt->cur;
+                GH_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                try {
+                    block();
+                } catch (...) {
+                    (void)hipStreamEndCapture(st, &g);
+                    if (g) (void)hipGraphDestroy(g);
+                    bt->cur = cur0;
+                    throw;
+                }
+                GH_CHECK(hipStreamEndCapture(st, &g));
+                hipGraphExec_t ex = nullptr;
+                const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                GH_CHECK(e);
+                bt->cur = cur0;
+                bt->graphs.emplace(key, ex);
+                return ex;
+            };
+            GH_CHECK(hipMemcpyAsync(bt->host_running, bt->running, sizeof(int), hipMemcpyDeviceToHost, st));
+            GH_CHECK(hipStreamSynchronize(st));
+            int running = *bt->host_running;
+            int listed = running;  // instances in the launch lists
+            while (running > 0) {
+                if (graphs) GH_CHECK(hipGraphLaunch(graph(), st));
+                else block();
+                GH_CHECK(hipMemcpyAsync(bt->host_running, bt->running, sizeof(int), hipMemcpyDeviceToHost, st));
+                GH_CHECK(hipStreamSynchronize(st));
+                running = *bt->host_running;
+                const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+                if (running > 0 && el >= time_limit) {
+                    hipLaunchKernelGGL(k_bt_stop, dim3(std::min(1024, (bt->ninst + 255) / 256)), dim3(256), 0, st,
+                                       bt->ninst, bt->st);
+                    BatchVecs& v = bt->vs[slot];
+                    hipLaunchKernelGGL(k_bt_restart<V>, dim3(bt_grid((int64_t)bt->N + bt->Mr)), dim3(kBlock), 0, st,
+                                       bt->N, bt->Mr, bt->col_inst, bt->row_inst, bt->st, (V*)v.X[bt->cur],
+                                       (const V*)v.xa, (V*)v.xr, (V*)v.sumx, (V*)v.Y[bt->cur], (const V*)v.ya,
+                                       (V*)v.yr, (V*)v.sumy);
+                    hipLaunchKernelGGL(k_bt_deactivate, dim3(std::min(1024, (bt->ninst + 255) / 256)), dim3(256), 0,
+                                       st, bt->ninst, bt->st);
+                    GH_CHECK(hipStreamSynchronize(st));
+                    break;
+                }
+                // shrink the grids once half the listed instances have stopped (a few recaptures per phase)
+                if (running > 0 && 2 * running <= listed) {
+                    bt_list(bt, bt_active(bt));
+                    listed = running;
+                }
+            }
+            if (running_left) *running_left = running;
+        });
+        GH_CHECK(hipGetLastError());
+    });
+}
+
+// Autotuner: time `reps` batched iterations (primal + dual kernel) at (mdt, vdt) for every row-block
+// partition candidate, twice, and keep the fastest.  Runs on whatever iterate is loaded and advances it.
+int ganit_hip_batch_autotune(ganit_hip_batch* h, int mdt, int vdt, int reps, double* ms, int* chosen) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && reps > 0, "bad autotune arguments");
+        const int slot = vdt == GANIT_F64 ? 0 : 1;
+        GH_REQUIRE(bt->on[slot], "vector dtype not resident");
+        std::vector<std::pair<int, int>> cand;
+        for (const Variant& v : variant_candidates())
+            if (v.lanes == 0) cand.emplace_back(v.rows, v.nnz);
+        GH_REQUIRE(!cand.empty(), "no row-block candidates");
+        hipStream_t st = bt->ctx->stream;
+        hipEvent_t e0, e1;
+        GH_CHECK(hipEventCreate(&e0));
+        GH_CHECK(hipEventCreate(&e1));
+        std::vector<double> best(cand.size(), std::numeric_limits<double>::infinity());
+        try {
+            bt_types(mdt, vdt, [&](auto MT, auto VT) {
+                using M = typename decltype(MT)::type;
+                using V = typename decltype(VT)::type;
+                // every instance takes part while timing
+                std::vector<BatchState> keep(bt->ninst);
+                GH_CHECK(hipMemcpyAsync(keep.data(), bt->st, sizeof(BatchState) * bt->ninst, hipMemcpyDeviceToHost, st));
+                GH_CHECK(hipStreamSynchronize(st));
+                std::vector<BatchState> tmp(keep);
+                for (int k = 0; k < bt->ninst; ++k) {
+                    tmp[k].active = 1;
+                    tmp[k].sigma = bt->inst_h[k].eta * tmp[k].w;
+                    tmp[k].tau = bt->inst_h[k].eta / tmp[k].w;
+                }
+                GH_CHECK(hipMemcpyAsync(bt->st, tmp.data(), sizeof(BatchState) * bt->ninst, hipMemcpyHostToDevice, st));
+                for (int round = 0; round < 2; ++round) {
+                    for (size_t c = 0; c < cand.size(); ++c) {
+                        bt_partition(bt, cand[c].first, cand[c].second);
+                        for (int k = 0; k < 4; ++k) bt_iteration<M, V>(bt, mdt);
+                        GH_CHECK(hipEventRecord(e0, st));
+                        for (int k = 0; k < reps; ++k) bt_iteration<M, V>(bt, mdt);
+                        GH_CHECK(hipEventRecord(e1, st));
+                        GH_CHECK(hipEventSynchronize(e1));
+                        float t = 0.f;
+                        GH_CHECK(hipEventElapsedTime(&t, e0, e1));
+                        best[c] = std::min(best[c], (double)t / reps);
+                    }
+                }
+                GH_CHECK(hipMemcpyAsync(bt->st, keep.data(), sizeof(BatchState) * bt->ninst, hipMemcpyHostToDevice, st));
+                GH_CHECK(hipStreamSynchronize(st));
+            });
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const size_t pick = (size_t)(std::min_element(best.begin(), best.end()) - best.begin());
+        bt_partition(bt, cand[pick].first, cand[pick].second);
+        if (ms)
+            for (size_t c = 0; c < cand.size(); ++c) ms[c] = best[c];
+        if (chosen) *chosen = (int)pick;
+    });
+}
+
+// FP64 KKT of the FP64 working set's iterate of every instance (the certification, src/pdhg_impl.hpp:301).
+int ganit_hip_batch_certify(ganit_hip_batch* h, int mdt) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && bt->on[0], "FP64 working set not resident");
+        hipStream_t st = bt->ctx->stream;
+        // evaluate every instance: mark all active, run the check in FINAL mode, restore the flags
+        std::vector<BatchState> keep(bt->ninst);
+        GH_CHECK(hipMemcpyAsync(keep.data(), bt->st, sizeof(BatchState) * bt->ninst, hipMemcpyDeviceToHost, st));
+        GH_CHECK(hipStreamSynchronize(st));
+        std::vector<BatchState> tmp(keep);
+        for (auto& s : tmp) s.active = 1;
+        GH_CHECK(hipMemcpyAsync(bt->st, tmp.data(), sizeof(BatchState) * bt->ninst, hipMemcpyHostToDevice, st));
+        bt_list(bt, std::vector<char>(bt->ninst, 1));
+        with_mdt(mdt, [&](auto MT) { bt_check<typename decltype(MT)::type, double>(bt, 0, kFinal); });
+        GH_CHECK(hipMemcpyAsync(tmp.data(), bt->st, sizeof(BatchState) * bt->ninst, hipMemcpyDeviceToHost, st));
+        GH_CHECK(hipStreamSynchronize(st));
+        for (int k = 0; k < bt->ninst; ++k) {
+            std::memcpy(keep[k].kkt64, tmp[k].kkt64, sizeof(keep[k].kkt64));
+        }
+        GH_CHECK(hipMemcpyAsync(bt->st, keep.data(), sizeof(BatchState) * bt->ninst, hipMemcpyHostToDevice, st));
+        GH_CHECK(hipStreamSynchronize(st));
+    });
+}
+
+// ints[5k..5k+4] = status, iterations, restarts, best_is_avg, active; dbl[13k..13k+12] = phase result KKT
+// (pobj, dobj, rel_p, rel_d, rel_gap, err), FP64 KKT of the same six, primal weight.
+int ganit_hip_batch_results(ganit_hip_batch* h, int* ints, double* dbl) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && ints && dbl, "null argument");
+        std::vector<BatchState> s(bt->ninst);
+        GH_CHECK(hipMemcpyAsync(s.data(), bt->st, sizeof(BatchState) * bt->ninst, hipMemcpyDeviceToHost, bt->ctx->stream));
+        GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
+        for (int k = 0; k < bt->ninst; ++k) {
+            ints[5 * k + 0] = s[k].status;
+            ints[5 * k + 1] = s[k].it;
+            ints[5 * k + 2] = s[k].restarts;
+            ints[5 * k + 3] = s[k].best_avg;
+            ints[5 * k + 4] = s[k].active;
+            for (int q = 0; q < 6; ++q) dbl[13 * k + q] = s[k].best[q];
+            for (int q = 0; q < 6; ++q) dbl[13 * k + 6 + q] = s[k].kkt64[q];
+            dbl[13 * k + 12] = s[k].w;
+        }
+    });
+}
+
+int ganit_hip_batch_store(ganit_hip_batch* h, int vdt, double* x, double* y) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt, "null batch");
+        const int slot = vdt == GANIT_F64 ? 0 : 1;
+        GH_REQUIRE(bt->on[slot], "vector dtype not resident");
+        BatchVecs& v = bt->vs[slot];
+        hipStream_t st = bt->ctx->stream;
+        auto down = [&](const void* src, double* dst, int64_t k) {
+            if (!dst || k == 0) return;
+            if (vdt == GANIT_F64) {
+                GH_CHECK(hipMemcpyAsync(dst, src, 8 * k, hipMemcpyDeviceToHost, st));
+                GH_CHECK(hipStreamSynchronize(st));
+            } else {
+                std::vector<float> f(k);
+                GH_CHECK(hipMemcpyAsync(f.data(), src, 4 * k, hipMemcpyDeviceToHost, st));
+                GH_CHECK(hipStreamSynchronize(st));
+                for (int64_t i = 0; i < k; ++i) dst[i] = f[i];
+            }
+        };
+        down(v.X[bt->cur], x, bt->N);
+        down(v.Y[bt->cur], y, bt->Mr);
+    });
+}
+
+}  // extern "C"

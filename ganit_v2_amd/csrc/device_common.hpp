@@ -204,6 +204,41 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
     }
 }
 
+// stream_block for NX gathered vectors with ONE tile of LDS: on a staged block the NX products are staged and
+// summed one vector after another (the block's indices and values come from cache on the later passes), so a
+// check pass needs kTile * sizeof(V) bytes of LDS instead of NX times that.  Same row sums, same bits.  Long
+// rows and chunks of split rows need only (kBlock / kWave) * NX elements of LDS and take the NX-wide path
+// (a chunk's per-row ticket must be taken once per chunk).
+template <class M, class V, int NX, class Gather, class RowFn>
+__device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int* __restrict__ blocks, int b,
+                                                 Gather gather, RowFn row_fn, V* lds /* kTile */,
+                                                 const SplitPlan& sp) {
+    static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
+    if (b >= sp.nblocks || (blocks[b + 1] - blocks[b] == 1 && a.ptr[blocks[b + 1]] - a.ptr[blocks[b]] > kLong)) {
+        stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
+        return;
+    }
+    V acc[NX];
+    bool have = false;
+    int row = -1;
+#pragma unroll
+    for (int k = 0; k < NX; ++k) {
+        V got = V(0);
+        int r_got = -1;
+        stream_block<M, V, 1>(
+            a, blocks, b, [&](int, int i) { return gather(k, i); },
+            [&](int r, V (&t)[1]) {
+                got = t[0];
+                r_got = r;
+            },
+            lds, sp);
+        __syncthreads();  // the next vector reuses the tile
+        acc[k] = got;
+        if (r_got >= 0) { have = true; row = r_got; }
+    }
+    if (have) row_fn(row, acc);
+}
+
 // ---------------------------------------------------------------- deterministic grid reduction
 // Two launches, no atomics: every block of the producing kernel writes its N block sums to partials[b][.]
 // (block_partial); one block of a follow-up kernel then adds them up in block order (sum_partials).  A
@@ -246,6 +281,16 @@ __device__ __forceinline__ void block_partial(double (&v)[N], const ReduceSlot& 
 #pragma unroll
         for (int k = 0; k < N; ++k) slot.partials[(size_t)blockIdx.x * N + k] = v[k];
     }
+}
+
+// Run by every thread of one block: out[k] = sum over b < nparts of partials[b][k] for the single output k,
+// fixed order (one block per output keeps the fold parallel).
+__device__ __forceinline__ void sum_partials_one(const double* __restrict__ partials, int N, int k, int nparts,
+                                                 double* out, double* lds) {
+    double s[1] = {0.0};
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s[0] += partials[(size_t)b * N + k];
+    block_sum<1>(s, lds);
+    if (threadIdx.x == 0) *out = s[0];
 }
 
 // Run by every thread of one block: out[k] = sum over b < nparts of partials[b][k], fixed order.

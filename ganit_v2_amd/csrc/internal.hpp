@@ -115,6 +115,7 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
 void csr_destroy(Csr* a);
 int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz);
 bool valid_variant(int lanes);
+std::vector<int> row_blocks(const int* ptr, int rows, int max_rows = kBlock, int max_nnz = kTile);
 // Rebuild the row-block partition with at most `rows` rows and `nnz` nonzeros per block.
 void csr_set_partition(Ctx* ctx, Csr* a, int rows, int nnz);
 // Candidate variants: (lanes, rb_rows, rb_nnz); lanes 0 = row blocks with that partition.

@@ -272,13 +272,13 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
                                                        const V* __restrict__ c, const V* __restrict__ l,
                                                        const V* __restrict__ u, const V* __restrict__ cres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ C lds[3 * kTile];
+    __shared__ C lds[kTile];
     __shared__ double red[(kBlock / kWave) * kNOutN];
     double acc[kNOutN];
 #pragma unroll
     for (int k = 0; k < kNOutN; ++k) acc[k] = 0.0;
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
-    stream_block<M, C, 3>(
+    stream_block_seq<M, C, 3>(
         AT, blocks, xcd_block_id(),
         [&](int k, int i) {
             return k == 0 ? C(y[i]) : k == 1 ? (avg ? C(ya[i]) : C(0)) : (ray ? C(V(y[i] - yr[i])) : C(0));
@@ -334,13 +334,13 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
                                                        const V* __restrict__ yr, const V* __restrict__ lo,
                                                        const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ C lds[2 * kTile];
+    __shared__ C lds[kTile];
     __shared__ double red[(kBlock / kWave) * kNOutM];
     double acc[kNOutM];
 #pragma unroll
     for (int k = 0; k < kNOutM; ++k) acc[k] = 0.0;
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
-    stream_block<M, C, 2>(
+    stream_block_seq<M, C, 2>(
         A, blocks, xcd_block_id(), [&](int k, int j) { return k == 0 ? C(x[j]) : (avg ? C(xa[j]) : C(0)); },
         [&](int i, C (&t)[2]) {
             const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
@@ -370,9 +370,10 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
 __global__ __launch_bounds__(kBlock) void k_check_finish(const double* __restrict__ part_n, int nb_n,
                                                          const double* __restrict__ part_m, int nb_m,
                                                          double* __restrict__ out) {
-    __shared__ double lds[(kBlock / kWave) * kNOutN];
-    if (blockIdx.x == 0) sum_partials<kNOutN>(part_n, nb_n, out, lds);
-    else sum_partials<kNOutM>(part_m, nb_m, out + kNOutN, lds);
+    __shared__ double lds[kBlock / kWave];
+    const int k = blockIdx.x;  // one block per output
+    if (k < kNOutN) sum_partials_one(part_n, kNOutN, k, nb_n, out + k, lds);
+    else sum_partials_one(part_m, kNOutM, k - kNOutN, nb_m, out + k, lds);
 }
 
 // ================================================================ restart / hand-over kernels
@@ -612,7 +613,8 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
                            sm);
     });
     GH_REQUIRE(nb_n <= pd->part_cap && nb_m <= pd->part_cap, "check grid exceeds the partial buffers");
-    hipLaunchKernelGGL(k_check_finish, dim3(2), dim3(kBlock), 0, st, pd->part_n, (int)nb_n, pd->part_m, (int)nb_m,
+    hipLaunchKernelGGL(k_check_finish, dim3(kNOutN + kNOutM), dim3(kBlock), 0, st, pd->part_n, (int)nb_n, pd->part_m,
+                       (int)nb_m,
                        pd->out);
     GH_CHECK(hipGetLastError());
     GH_CHECK(hipMemcpyAsync(pd->ctx->host_out, pd->out, sizeof(double) * GANIT_CHECK_NOUT, hipMemcpyDeviceToHost, st));

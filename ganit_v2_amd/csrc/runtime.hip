@@ -210,7 +210,7 @@ bool valid_variant(int lanes) { return lanes == 0 || (lanes >= 1 && lanes <= kWa
 // Row blocks of the LDS-staged variant: consecutive rows while the block holds at most max_rows (<= kBlock)
 // rows and max_nnz (<= kTile) nonzeros and every row has at most kLong entries; a longer row forms a block
 // of its own.
-std::vector<int> row_blocks(const int* ptr, int rows, int max_rows = kBlock, int max_nnz = kTile) {
+std::vector<int> row_blocks(const int* ptr, int rows, int max_rows, int max_nnz) {
     std::vector<int> b{0};
     int r = 0;
     while (r < rows) {

@@ -1,0 +1,284 @@
+// Batched restarted PDHG: many independent LPs (branch-and-bound node relaxations, strong-branching probes)
+// solved together by the block-diagonal kernels of libganit_hip.so (ganit_hip_batch_*).
+//
+// Every instance follows the single-instance driver's rules (reference src/pdhg_impl.hpp:67-328): its own
+// scaling, norm estimate, initial point and primal weight; the same step sizes, checks, adaptive restarts,
+// stall exit and precision ladder.  The difference is the schedule: a precision level runs for all selected
+// instances at once (the device decides per instance after every block of check_every iterations), then
+// every instance's result is certified in FP64 in one pass, and only the instances that the ladder's rule
+// sends on (pdhg_impl.hpp:276-299) take part in the next level.
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ganit_hip.h"
+#include "pdhg.hpp"
+#include "scaling.hpp"
+
+namespace ganit {
+
+namespace {
+void chk(int rc) {
+    if (rc != GANIT_OK) throw std::runtime_error(std::string("HIP backend: ") + ganit_hip_last_error());
+}
+using Clock = std::chrono::steady_clock;
+double since(Clock::time_point t) { return std::chrono::duration<double>(Clock::now() - t).count(); }
+
+// device status codes of ganit_hip_batch_results
+enum { kRunning = 0, kConverged = 1, kInfeasible = 2, kIterLimit = 3, kNumerical = 4, kStalled = 5, kTimeLimit = 6 };
+
+Status status_of(int code) {
+    switch (code) {
+        case kConverged: return Status::Optimal;
+        case kInfeasible: return Status::PrimalInfeasible;
+        case kTimeLimit: return Status::TimeLimit;
+        case kNumerical: return Status::NumericalError;
+        default: return Status::IterationLimit;  // iteration limit, stalled (pdhg_impl.hpp:232-240)
+    }
+}
+bool converged6(const double* k, double tol) { return k[2] <= tol && k[3] <= tol && k[4] <= tol; }
+}  // namespace
+
+struct BatchSolver::Impl {
+    const Options opt;
+    ganit_hip_ctx* ctx = nullptr;
+    ganit_hip_batch* bt = nullptr;
+    std::string device;
+    std::vector<ScaledLP> s;
+    std::vector<double> norm_a, w0;
+    std::vector<int> row0, col0;
+    int rows = 0, cols = 0;
+    double setup_seconds = 0, autotune_ms[8] = {0};
+    int autotune_pick = -1;
+    bool mixed = false;
+    int low_mdt = GANIT_F32;
+    BatchStats stats;
+    explicit Impl(const Options& o) : opt(o) {}
+    ~Impl() {
+        if (bt) ganit_hip_batch_destroy(bt);
+        if (ctx) ganit_hip_ctx_destroy(ctx);
+    }
+};
+
+BatchSolver::BatchSolver(const std::vector<const LP*>& lps, const Options& opt) : p_(new Impl(opt)) {
+    Impl& I = *p_;
+    if (lps.empty()) throw std::runtime_error("empty batch");
+    if (opt.check_every < 2 || opt.check_every % 2) throw std::runtime_error("batched check_every must be even");
+    for (const LP* lp : lps)
+        if (!lp || lp->has_q()) throw std::runtime_error("batched instances must be LPs");
+    chk(ganit_hip_ctx_create(opt.device, &I.ctx));
+    char name[256] = {0};
+    chk(ganit_hip_ctx_device_info(I.ctx, name, sizeof(name), nullptr));
+    I.device = std::string("gpu:") + name;
+    const auto t0 = Clock::now();
+    const int K = (int)lps.size();
+    I.s.resize(K);
+    I.norm_a.resize(K);
+    I.w0.resize(K);
+    // independent set-ups, one instance per thread (the per-instance arithmetic is the single solve's)
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int k = 0; k < K; ++k) {
+        I.s[k] = scale_lp(*lps[k], opt.ruiz_iters, opt.pock_chambolle);
+        I.norm_a[k] = estimate_norm(I.s[k]);
+        const double cn = norm2(I.s[k].c), bn = finite_bound_norm(I.s[k].lo, I.s[k].hi);
+        I.w0[k] = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;
+    }
+    I.row0.resize(K + 1);
+    I.col0.resize(K + 1);
+    I.row0[0] = I.col0[0] = 0;
+    for (int k = 0; k < K; ++k) {
+        I.row0[k + 1] = I.row0[k] + I.s[k].m;
+        I.col0[k + 1] = I.col0[k] + I.s[k].n;
+    }
+    I.rows = I.row0[K];
+    I.cols = I.col0[K];
+    I.mixed = opt.precision != Precision::Double;
+    I.low_mdt = opt.values == Values::BF16 ? GANIT_BF16 : GANIT_F32;
+    I.setup_seconds = since(t0);
+}
+
+BatchSolver::~BatchSolver() = default;
+
+int BatchSolver::size() const { return (int)p_->s.size(); }
+double BatchSolver::setup_seconds() const { return p_->setup_seconds; }
+const ScaledLP& BatchSolver::scaled(int k) const { return p_->s.at(k); }
+const BatchStats& BatchSolver::stats() const { return p_->stats; }
+
+void BatchSolver::upload() {
+    Impl& I = *p_;
+    if (I.bt) return;
+    const int K = size();
+    std::vector<ganit_hip_lp_desc> d(K);
+    std::vector<const ganit_hip_lp_desc*> dp(K);
+    std::vector<std::array<double, 7>> sc(K);
+    std::vector<const double*> scp(K);
+    for (int k = 0; k < K; ++k) {
+        const ScaledLP& s = I.s[k];
+        ganit_hip_lp_desc& e = d[k];
+        e = ganit_hip_lp_desc{};
+        e.m = s.m; e.n = s.n;
+        e.a_ptr = s.A.ptr.data(); e.a_idx = s.A.idx.data(); e.a_val = s.A.val.data();
+        e.at_ptr = s.AT.ptr.data(); e.at_idx = s.AT.idx.data(); e.at_val = s.AT.val.data();
+        e.c = s.c.data(); e.l = s.l.data(); e.u = s.u.data(); e.lo = s.lo.data(); e.hi = s.hi.data();
+        e.rres = s.Rres.data(); e.cres = s.Cres.data();
+        dp[k] = &d[k];
+        sc[k] = {0.998 / I.norm_a[k], I.norm_a[k], s.cscale * s.bscale, s.obj_const, s.bnorm, s.cnorm, I.w0[k]};
+        scp[k] = sc[k].data();
+    }
+    const unsigned mmask = (1u << GANIT_F64) | (I.mixed ? (1u << I.low_mdt) : 0u);
+    const unsigned vmask = (1u << GANIT_F64) | (I.mixed ? (1u << GANIT_F32) : 0u);
+    chk(ganit_hip_batch_create(I.ctx, K, dp.data(), scp.data(), mmask, vmask, &I.bt));
+    // pick the row-block partition for the level that runs first (the batch's bulk of iterations)
+    const char* at = std::getenv("GANIT_HIP_AUTOTUNE");
+    if (!(at && at[0] == '0')) {
+        const int vdt = I.mixed ? GANIT_F32 : GANIT_F64, mdt = I.mixed ? I.low_mdt : GANIT_F64;
+        std::vector<double> x0(I.cols), y0(I.rows, 0.0);
+        for (int k = 0; k < K; ++k)
+            for (int j = 0; j < I.s[k].n; ++j) x0[I.col0[k] + j] = std::min(std::max(0.0, I.s[k].l[j]), I.s[k].u[j]);
+        chk(ganit_hip_batch_load(I.bt, vdt, x0.data(), y0.data(), I.w0.data()));
+        chk(ganit_hip_batch_autotune(I.bt, mdt, vdt, 20, I.autotune_ms, &I.autotune_pick));
+    }
+}
+
+std::vector<Result> BatchSolver::run() {
+    Impl& I = *p_;
+    const Options& opt = I.opt;
+    const int K = size();
+    const auto t_solve = Clock::now();
+    upload();
+    ganit_hip_batch* bt = I.bt;
+    BatchStats& S = I.stats;
+    S = BatchStats{};
+    S.autotune_pick = I.autotune_pick;
+    std::copy(I.autotune_ms, I.autotune_ms + 8, S.autotune_ms);
+    int64_t dims[6];
+    chk(ganit_hip_batch_dims(bt, dims));
+    S.rows = dims[1]; S.cols = dims[2]; S.nnz = dims[3]; S.blocks_a = dims[4]; S.blocks_at = dims[5];
+
+    std::vector<double> x0(I.cols), y0(I.rows, 0.0);
+    for (int k = 0; k < K; ++k)
+        for (int j = 0; j < I.s[k].n; ++j) x0[I.col0[k] + j] = std::min(std::max(0.0, I.s[k].l[j]), I.s[k].u[j]);
+    const int first_vdt = I.mixed ? GANIT_F32 : GANIT_F64;
+    chk(ganit_hip_batch_load(bt, first_vdt, x0.data(), y0.data(), I.w0.data()));
+    S.upload_seconds = since(t_solve);
+
+    std::vector<int> ints(5 * K);
+    std::vector<double> dbl(13 * K);
+    auto fetch = [&]() { chk(ganit_hip_batch_results(bt, ints.data(), dbl.data())); };
+    std::vector<int> done(K, 0), sel(K, 1);
+    std::vector<long> l1_it(K, 0), l2_it(K, 0);
+    int last_vdt = first_vdt;
+
+    auto phase = [&](int slot, int mdt, int vdt, bool stall_exit, int stall_checks) {
+        const auto t0 = Clock::now();
+        const double left = std::max(0.0, opt.time_limit - since(t_solve));
+        const double params[7] = {opt.tol, opt.infeas_tol, (double)opt.max_iter, (double)opt.check_every,
+                                  (double)stall_checks, stall_exit ? 1.0 : 0.0, left};
+        if (last_vdt != vdt) chk(ganit_hip_batch_handover(bt, last_vdt, vdt));
+        else chk(ganit_hip_batch_handover(bt, vdt, vdt));  // fresh running sums
+        last_vdt = vdt;
+        int running = 0;
+        chk(ganit_hip_batch_phase(bt, mdt, vdt, params, &running));
+        S.level_seconds[slot] += since(t0);
+        S.level_instances[slot] += (int)std::count(sel.begin(), sel.end(), 1);
+    };
+    // FP64 certification of every instance's current result (pdhg_evaluate<Bd>)
+    auto certify = [&]() {
+        const auto t0 = Clock::now();
+        if (last_vdt != GANIT_F64) {
+            chk(ganit_hip_batch_handover(bt, last_vdt, GANIT_F64));
+            last_vdt = GANIT_F64;
+        }
+        chk(ganit_hip_batch_certify(bt, GANIT_F64));
+        fetch();
+        S.certify_seconds += since(t0);
+    };
+    // the ladder's rule for leaving after a reduced-precision level (pdhg_impl.hpp:283-289)
+    auto settle = [&]() {
+        int left = 0;
+        for (int k = 0; k < K; ++k) {
+            if (done[k]) { sel[k] = 0; continue; }
+            const int e = ints[5 * k];
+            const double* k64 = &dbl[13 * k + 6];
+            done[k] = (e == kConverged && converged6(k64, opt.tol)) || e == kTimeLimit || e == kIterLimit ||
+                      e == kInfeasible;
+            sel[k] = done[k] ? 0 : 1;
+            left += sel[k];
+        }
+        return left;
+    };
+
+    if (I.mixed) {
+        phase(0, I.low_mdt, GANIT_F32, true, opt.stall_checks);
+        certify();
+        for (int k = 0; k < K; ++k) l1_it[k] = ints[5 * k + 1];
+        int left = settle();
+        if (left > 0 && opt.precision == Precision::Dynamic) {
+            chk(ganit_hip_batch_select(bt, sel.data()));
+            phase(1, I.low_mdt, GANIT_F64, true, opt.stall_checks_l2);
+            certify();
+            for (int k = 0; k < K; ++k) l2_it[k] = sel[k] ? ints[5 * k + 1] - l1_it[k] : 0;
+            left = settle();
+        }
+        if (left > 0) {
+            chk(ganit_hip_batch_select(bt, sel.data()));
+            phase(2, GANIT_F64, GANIT_F64, false, 0);
+        }
+    } else {
+        phase(2, GANIT_F64, GANIT_F64, false, 0);
+    }
+    certify();  // final FP64 certification (pdhg_impl.hpp:300-303)
+
+    std::vector<double> xs(I.cols), ys(I.rows);
+    chk(ganit_hip_batch_store(bt, GANIT_F64, xs.data(), ys.data()));
+    const double solve_s = since(t_solve);
+    S.solve_seconds = solve_s;
+    std::vector<Result> out(K);
+    for (int k = 0; k < K; ++k) {
+        const ScaledLP& s = I.s[k];
+        Result& r = out[k];
+        r.device = I.device;
+        r.norm_a = I.norm_a[k];
+        r.setup_seconds = I.setup_seconds;
+        r.solve_seconds = solve_s;
+        r.upload_seconds = S.upload_seconds;
+        for (int q = 0; q < 3; ++q) r.iterate_seconds[q] = S.level_seconds[q];
+        const double* k64 = &dbl[13 * k + 6];
+        Status st = status_of(ints[5 * k]);
+        if (st == Status::Optimal && !converged6(k64, opt.tol * 1.0000001)) st = Status::NumericalError;
+        r.status = st;
+        r.iterations = ints[5 * k + 1];
+        r.fp32_iterations = I.mixed ? l1_it[k] : 0;
+        r.l2_iterations = l2_it[k];
+        r.restarts = ints[5 * k + 2];
+        r.checks = r.iterations / opt.check_every;
+        const double sign = s.maximize ? -1.0 : 1.0;
+        r.x.resize(s.n);
+        r.y.resize(s.m);
+        for (int j = 0; j < s.n; ++j) r.x[j] = xs[I.col0[k] + j] * s.C[j] * s.bscale;  // pdhg_impl.hpp:313-314
+        for (int i = 0; i < s.m; ++i) r.y[i] = sign * ys[I.row0[k] + i] * s.R[i] * s.cscale;
+        r.pobj = sign * k64[0];
+        r.dobj = sign * k64[1];
+        r.rel_primal = k64[2];
+        r.rel_dual = k64[3];
+        r.rel_gap = k64[4];
+        S.max_iterations = std::max(S.max_iterations, r.iterations);
+        S.total_iterations += r.iterations;
+    }
+    if (opt.verbose) {
+        int opt_n = 0;
+        for (const Result& r : out) opt_n += r.status == Status::Optimal;
+        std::printf("batch of %d LPs (%lld rows, %lld cols, %lld nnz): %d optimal, max %ld iterations, "
+                    "solve %.3fs (L1 %.3fs, L2 %.3fs, L3 %.3fs, certification %.3fs)\n",
+                    K, (long long)S.rows, (long long)S.cols, (long long)S.nnz, opt_n, S.max_iterations, solve_s,
+                    S.level_seconds[0], S.level_seconds[1], S.level_seconds[2], S.certify_seconds);
+    }
+    return out;
+}
+
+}  // namespace ganit

@@ -93,6 +93,11 @@ ganit_lp* ganit_lp_generate(int kind, const double* p, uint64_t seed) {
                 h->lp = ganit::gen_portfolio((int)p[0], (int)p[1], p[2], seed);
             } else if (kind == 3) {
                 h->lp = ganit::gen_diff_qp((int)p[0], (int)p[1], (int)p[2], p[3], p[4], seed, &h->x0);
+            } else if (kind == 4) {
+                ganit::BatchLpSpec sp;
+                sp.m_lo = (int)p[0]; sp.m_hi = (int)p[1]; sp.n_lo = (int)p[2]; sp.n_hi = (int)p[3];
+                sp.nnz_min = (int)p[4]; sp.nnz_max = (int)p[5]; sp.tighten_frac = p[6];
+                h->lp = ganit::gen_batch_instance(sp, seed, &h->x0, &h->y0);
             } else {
                 throw std::runtime_error("unknown generator kind");
             }
@@ -203,5 +208,56 @@ int ganit_solver_kernel_times(ganit_solver* h, int level, int reps, double* ms) 
 }
 
 void ganit_solver_free(ganit_solver* h) { delete reinterpret_cast<ganit::GpuSolver*>(h); }
+
+ganit_batch* ganit_batch_create(const ganit_lp* const* lps, int ninst, const double* o) {
+    ganit::BatchSolver* b = nullptr;
+    if (guard([&] {
+            if (ninst <= 0 || !lps) throw std::runtime_error("empty batch");
+            std::vector<const ganit::LP*> v(ninst);
+            for (int k = 0; k < ninst; ++k) {
+                if (!lps[k]) throw std::runtime_error("null instance");
+                v[k] = &reinterpret_cast<const Handle*>(lps[k])->lp;
+            }
+            b = new ganit::BatchSolver(v, options_from(o));
+        })) {
+        delete b;
+        return nullptr;
+    }
+    return reinterpret_cast<ganit_batch*>(b);
+}
+
+int ganit_batch_upload(ganit_batch* h) {
+    return guard([&] { reinterpret_cast<ganit::BatchSolver*>(h)->upload(); });
+}
+
+int ganit_batch_run(ganit_batch* h, double* out, double* x, double* y) {
+    return guard([&] {
+        auto* b = reinterpret_cast<ganit::BatchSolver*>(h);
+        const std::vector<ganit::Result> rs = b->run();
+        size_t xo = 0, yo = 0;
+        for (size_t k = 0; k < rs.size(); ++k) {
+            result_out(rs[k], out ? out + 18 * k : nullptr, x ? x + xo : nullptr, y ? y + yo : nullptr);
+            xo += rs[k].x.size();
+            yo += rs[k].y.size();
+        }
+    });
+}
+
+int ganit_batch_stats(const ganit_batch* h, double* st) {
+    return guard([&] {
+        const ganit::BatchStats& s = reinterpret_cast<const ganit::BatchSolver*>(h)->stats();
+        const double v[24] = {(double)s.rows, (double)s.cols, (double)s.nnz, (double)s.blocks_a,
+                              (double)s.blocks_at, s.upload_seconds, s.solve_seconds, s.certify_seconds,
+                              s.level_seconds[0], s.level_seconds[1], s.level_seconds[2],
+                              (double)s.level_instances[0], (double)s.level_instances[1],
+                              (double)s.level_instances[2], (double)s.max_iterations, (double)s.total_iterations,
+                              (double)s.autotune_pick, s.autotune_ms[0], s.autotune_ms[1], s.autotune_ms[2],
+                              s.autotune_ms[3],
+                              reinterpret_cast<const ganit::BatchSolver*>(h)->setup_seconds(), 0.0, 0.0};
+        std::copy(v, v + 24, st);
+    });
+}
+
+void ganit_batch_free(ganit_batch* h) { delete reinterpret_cast<ganit::BatchSolver*>(h); }
 
 }  // extern "C"

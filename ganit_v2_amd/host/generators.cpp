@@ -112,6 +112,25 @@ LP gen_random_lp(const RandomLpSpec& sp, uint64_t seed, std::vector<double>* x0_
     return lp;
 }
 
+LP gen_batch_instance(const BatchLpSpec& b, uint64_t seed, std::vector<double>* x0, std::vector<double>* y0) {
+    // sizes from a stream separate from the instance's own (seed mixed with a fixed odd constant)
+    Rng r(seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+    auto log_uniform = [&](int lo, int hi) {
+        const double v = std::exp(r.uniform(std::log((double)lo), std::log((double)hi)));
+        return std::min(hi, std::max(lo, (int)std::lround(v)));
+    };
+    RandomLpSpec sp;
+    sp.m = log_uniform(b.m_lo, b.m_hi);
+    sp.n = log_uniform(b.n_lo, b.n_hi);
+    sp.m_eq = sp.m / 2;
+    sp.nnz_min = b.nnz_min;
+    sp.nnz_max = b.nnz_max;
+    sp.tighten_frac = b.tighten_frac;
+    LP lp = gen_random_lp(sp, seed, x0, y0);
+    lp.name = "BATCHLP_S" + std::to_string(seed);
+    return lp;
+}
+
 LP gen_blending(int C, int P, int K, int T, uint64_t seed) {
     Rng rng(seed);
     LP lp;

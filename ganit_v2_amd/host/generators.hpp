@@ -37,6 +37,17 @@ struct RandomLpSpec {
 LP gen_random_lp(const RandomLpSpec& spec, uint64_t seed, std::vector<double>* x0 = nullptr,
                  std::vector<double>* y0 = nullptr);
 
+// One instance of the batched workload (uneven node relaxations): sizes drawn from the seed's own stream,
+// m log-uniform in [m_lo, m_hi], n log-uniform in [n_lo, n_hi] (rounded), half the rows equalities, then
+// gen_random_lp with column counts U{nnz_min..nnz_max}, the given tightening fraction and the same seed.
+struct BatchLpSpec {
+    int m_lo = 200, m_hi = 4000, n_lo = 500, n_hi = 10000;
+    int nnz_min = 5, nnz_max = 15;
+    double tighten_frac = 0.05;
+};
+LP gen_batch_instance(const BatchLpSpec& spec, uint64_t seed, std::vector<double>* x0 = nullptr,
+                      std::vector<double>* y0 = nullptr);
+
 // Multi-period crude-blending LP (C crudes, P products, K quality specs, T periods).  Per period t:
 //   variables  x[c,p,t] >= 0 blend flow, r[c,t] >= 0 crude received, ci[c,t] >= 0 crude inventory,
 //              pi[p,t] >= 0 product inventory;

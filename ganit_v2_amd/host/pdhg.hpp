@@ -2,6 +2,7 @@
 // Mirrors the reference's include/ganit/pdhg.hpp:10-47 (Precision, Options, Status, Result,
 // solve_pdhg_gpu); additions are marked "extension".
 #pragma once
+#include <cstdint>
 #include <memory>
 #include <string>
 #include <vector>
@@ -78,6 +79,38 @@ class GpuSolver {
     void kernel_times(int level, int reps, double* ms);
     const ScaledLP& scaled() const;
     double norm_a() const;
+
+   private:
+    struct Impl;
+    std::unique_ptr<Impl> p_;
+};
+
+// extension: many LPs solved together by the block-diagonal batched kernels.  Each instance gets the single
+// solve's scaling, initial point, step sizes and restart rules; a precision level runs for all selected
+// instances at once, every result is certified in FP64, and only the instances the ladder sends on take
+// part in the next level.
+struct BatchStats {
+    int64_t rows = 0, cols = 0, nnz = 0, blocks_a = 0, blocks_at = 0;
+    double upload_seconds = 0, solve_seconds = 0, certify_seconds = 0;
+    double level_seconds[3] = {0, 0, 0};  // wall time of the L1 / L2 / L3 batched phases
+    int level_instances[3] = {0, 0, 0};   // instances taking part in each level
+    long max_iterations = 0, total_iterations = 0;
+    int autotune_pick = -1;               // chosen row-block partition candidate (-1: not tuned)
+    double autotune_ms[8] = {0};          // ms per batched iteration of each candidate
+};
+
+class BatchSolver {
+   public:
+    BatchSolver(const std::vector<const LP*>& lps, const Options& opt);  // context + per-instance set-up
+    ~BatchSolver();
+    BatchSolver(const BatchSolver&) = delete;
+    BatchSolver& operator=(const BatchSolver&) = delete;
+    void upload();              // pack + upload every instance, autotune the partition; idempotent
+    std::vector<Result> run();  // one solve of every instance from its initial point
+    int size() const;
+    double setup_seconds() const;
+    const ScaledLP& scaled(int k) const;
+    const BatchStats& stats() const;  // of the last run
 
    private:
     struct Impl;

@@ -173,6 +173,42 @@ int ganit_hip_pdhg_adopt_average(ganit_hip_pdhg* pd, int vdt);
  * any pointer may be NULL.  Synchronises. */
 int ganit_hip_pdhg_store(ganit_hip_pdhg* pd, int vdt, double* x, double* y, double* xr, double* yr);
 
+/* ------------------------------------------------------------------ batched PDHG (many uneven LPs)
+ * No reference counterpart: the reference solves one LP per call and its branch-and-cut uses only the
+ * dual simplex (src/mip.cpp:116, 312-323).  The instances (scaled LPs, no Q) are packed block-diagonally;
+ * each keeps its own restart state, step sizes, primal weight and status on the device, and the check
+ * logic of src/pdhg_impl.hpp:144-189 is applied per instance by a device kernel. */
+typedef struct ganit_hip_batch ganit_hip_batch;
+/* inst_scalars[k] = {eta, ||A||, obj_scale, obj_const, bnorm, cnorm, w0} of instance k. */
+int ganit_hip_batch_create(ganit_hip_ctx* ctx, int ninst, const ganit_hip_lp_desc* const* lps,
+                           const double* const* inst_scalars, unsigned mdt_mask, unsigned vdt_mask,
+                           ganit_hip_batch** out);
+int ganit_hip_batch_destroy(ganit_hip_batch* bt);
+/* dims = {instances, rows, columns, nnz, row blocks of A, row blocks of A^T} */
+int ganit_hip_batch_dims(const ganit_hip_batch* bt, int64_t* dims);
+/* Initial iterate (concatenated x, y) and primal weights; every instance starts running. */
+int ganit_hip_batch_load(ganit_hip_batch* bt, int vdt, const double* x, const double* y, const double* w);
+/* Convert every instance's (x, y, xr, yr) between working sets and clear the target's running sums. */
+int ganit_hip_batch_handover(ganit_hip_batch* bt, int from_vdt, int to_vdt);
+/* The instances with sel[k] != 0 run in the next phase; their iteration count, restarts, weight and
+ * restart error carry over (the PdhgState hand-over, src/pdhg_impl.hpp:55-63). */
+int ganit_hip_batch_select(ganit_hip_batch* bt, const int* sel);
+/* Run one precision phase for the selected instances until each converges, stalls (stall_exit),
+ * is certified infeasible or hits the iteration limit; params = {tol, infeas_tol, max_iter,
+ * check_every (even), stall_checks, stall_exit, time_limit_s}. */
+int ganit_hip_batch_phase(ganit_hip_batch* bt, int mdt, int vdt, const double* params, int* running_left);
+/* Autotuner of the batched kernels: times `reps` iterations at (mdt, vdt) for each row-block partition
+ * candidate (rows, nonzeros per workgroup) and keeps the fastest; ms[c] = ms per iteration of candidate c
+ * (up to 8, may be NULL), *chosen = its index.  Advances the loaded iterate: load again afterwards. */
+int ganit_hip_batch_autotune(ganit_hip_batch* bt, int mdt, int vdt, int reps, double* ms, int* chosen);
+/* FP64 KKT of every instance's FP64 iterate with matrix values mdt (the certification). */
+int ganit_hip_batch_certify(ganit_hip_batch* bt, int mdt);
+/* ints[5k..] = status (0 running, 1 converged, 2 infeasible, 3 iteration limit, 4 numerical, 5 stalled,
+ * 6 time limit), iterations, restarts, best_is_avg, active; dbl[13k..] = phase-result KKT (pobj, dobj,
+ * rel_p, rel_d, rel_gap, err), certified FP64 KKT (same six), primal weight. */
+int ganit_hip_batch_results(ganit_hip_batch* bt, int* ints, double* dbl);
+int ganit_hip_batch_store(ganit_hip_batch* bt, int vdt, double* x, double* y);
+
 /* Average device time in milliseconds of each fused iteration kernel on the current iterate, measured with
  * HIP events on the context stream over `reps` back-to-back launches: ms[0] = primal kernel
  * (A^T y [+ Q x] + primal prox), ms[1] = dual kernel (A xbar + dual prox).  Advances the iterate; no

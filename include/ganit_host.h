@@ -31,7 +31,9 @@ ganit_lp* ganit_lp_from_triplets(int m, int n, int64_t nnz, const int* rows, con
  *   kind 0 random LP        p = [m, n, m_eq, nnz_min, nnz_max, tighten_frac]
  *   kind 1 blending LP      p = [crudes, products, specs, periods]
  *   kind 2 portfolio QP     p = [assets, factors, gamma]
- *   kind 3 differentiable QP p = [n, m_eq, m_le, density, qdensity]                                    */
+ *   kind 3 differentiable QP p = [n, m_eq, m_le, density, qdensity]
+ *   kind 4 batched-LP instance p = [m_lo, m_hi, n_lo, n_hi, nnz_min, nnz_max, tighten_frac]: m, n
+ *                           log-uniform from the seed, half the rows equalities, then kind 0 with that seed */
 ganit_lp* ganit_lp_generate(int kind, const double* p, uint64_t seed);
 void ganit_lp_free(ganit_lp* lp);
 /* dims[0..4] = m, n, nnz(A), nnz(Q), maximize */
@@ -73,6 +75,22 @@ int ganit_solver_upload(ganit_solver* s);
 int ganit_solver_run(ganit_solver* s, double* out, double* x, double* y);
 int ganit_solver_kernel_times(ganit_solver* s, int level, int reps, double* ms);
 void ganit_solver_free(ganit_solver* s);
+
+/* Many LPs solved together by the block-diagonal batched kernels (no reference counterpart: the reference
+ * solves one problem per call; this serves branch-and-bound node relaxations and strong-branching probes).
+ * Each instance gets the single solve's scaling, initial point, step sizes, restart rules and FP64
+ * certification; opts as ganit_solve_gpu (check_every must be even; eval_fp64 is ignored).
+ * run: out[18*k..] = instance k's results in ganit_solve_gpu's layout (solve_s is the whole batch's);
+ * x, y = the instances' original-space solutions concatenated in order (either may be NULL).
+ * stats[0..23] = rows, cols, nnz, row blocks of A, row blocks of A^T, upload_s, solve_s, certify_s,
+ *                L1_s, L2_s, L3_s, instances in L1, L2, L3, max iterations, total iterations,
+ *                autotuned partition (-1 none), ms per batched iteration of the 4 candidates, setup_s. */
+typedef struct ganit_batch ganit_batch;
+ganit_batch* ganit_batch_create(const ganit_lp* const* lps, int ninst, const double* opts);
+int ganit_batch_upload(ganit_batch* b);
+int ganit_batch_run(ganit_batch* b, double* out, double* x, double* y);
+int ganit_batch_stats(const ganit_batch* b, double* stats);
+void ganit_batch_free(ganit_batch* b);
 
 #ifdef __cplusplus
 }

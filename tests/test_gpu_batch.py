@@ -1,0 +1,122 @@
+"""Batched PDHG (many uneven LPs in one block-diagonal launch sequence) against the oracle, per instance.
+
+The batched kernels sum every row in the same order as the single-instance row-block kernels and the
+oracle's CPU row loop (rows here stay under 256 nonzeros), and each instance's decisions are the single
+solve's, taken on the device.  So each instance should follow the oracle's own trajectory: same status, and
+in most instances the same iteration count and restart count (FP64 check sums are added in a different
+order, which can flip a discontinuous restart test now and then).  Where the iteration counts agree the
+objectives agree to round-off; elsewhere both points are 1e-4-accurate and agree to a tolerance-sized bound.
+"""
+import numpy as np
+import pytest
+
+import ganit_v2_amd as g
+from conftest import golden, instance_path
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(m_lo=100, m_hi=800, n_lo=300, n_hi=2000, nnz_min=5, nnz_max=15, tighten_frac=0.05)
+
+
+def _problems(count, seed0=0, **spec):
+    sp = dict(SMALL)
+    sp.update(spec)
+    return [g.Problem.generate("batch_lp", seed=seed0 + k, **sp) for k in range(count)]
+
+
+def _certified(r, tol):
+    return max(r["rel_p"], r["rel_d"], r["rel_gap"]) <= tol * (1 + 1e-7)
+
+
+def _compare(res, refs, tol, same_iter_frac):
+    same = 0
+    for k, (r, o) in enumerate(zip(res, refs)):
+        print(f"instance {k}: gpu {r['status']} it {r['iters']} obj {r['pobj']:.10g} | "
+              f"oracle {o['status']} it {o['iters']} obj {o['pobj']:.10g}", flush=True)
+        assert r["status"] == o["status"], k
+        if r["status"] == "OPTIMAL":
+            assert _certified(r, tol), k
+            if r["iters"] == o["iters"]:
+                assert abs(r["pobj"] - o["pobj"]) <= 1e-9 * (1 + abs(o["pobj"])), k
+            else:
+                assert abs(r["pobj"] - o["pobj"]) <= 5e-4 * (1 + abs(o["pobj"])), k
+        same += r["iters"] == o["iters"]
+    assert same >= same_iter_frac * len(res)
+
+
+def test_batch_fp64_matches_oracle_per_instance(oracle):
+    probs = _problems(32)
+    b = g.Batch(probs, precision="double", tol=1e-4)
+    res = b.run(return_solution=True)
+    oracle.set_threads(4)
+    refs = [oracle.OracleLP.from_export(p.export()).solve(precision="double", tol=1e-4) for p in probs]
+    _compare(res, refs, 1e-4, 0.8)
+    # where the trajectories agree, so do the solutions
+    for r, o in zip(res, refs):
+        if r["iters"] == o["iters"] and r["status"] == "OPTIMAL":
+            assert np.max(np.abs(r["x"] - o["x"])) <= 1e-7 * (1 + np.max(np.abs(o["x"])))
+    st = b.stats()
+    assert st["l3_instances"] == 32 and st["l1_instances"] == 0
+    assert st["max_iters"] == max(r["iters"] for r in res)
+    assert 0 <= st["autotune_pick"] < 4 and st["autotune_ms0"] > 0
+
+
+def test_batch_mixed_matches_oracle_ladder(oracle):
+    """FP32 matrix + FP32 vectors in L1, FP64 certification of every instance, L3 for those not certified:
+    the oracle runs the same ladder per instance."""
+    probs = _problems(32, seed0=100)
+    res = g.Batch(probs, precision="mixed", tol=1e-4).run()
+    oracle.set_threads(4)
+    refs = [oracle.OracleLP.from_export(p.export()).solve(precision="mixed", tol=1e-4) for p in probs]
+    _compare(res, refs, 1e-4, 0.7)
+    for r, o in zip(res, refs):
+        if r["iters"] == o["iters"]:
+            assert r["fp32_iters"] == o["fp32_iters"]
+
+
+def test_batch_matches_sequential_gpu_solves():
+    probs = _problems(16, seed0=200)
+    res = g.Batch(probs, precision="mixed").run()
+    for k, (p, r) in enumerate(zip(probs, res)):
+        s = p.solve_gpu(precision="mixed")
+        assert r["status"] == s["status"], k
+        if s["status"] == "OPTIMAL":
+            assert abs(r["pobj"] - s["pobj"]) <= 5e-4 * (1 + abs(s["pobj"])), k
+
+
+def test_batch_is_deterministic():
+    probs = _problems(12, seed0=300)
+    b = g.Batch(probs, precision="mixed")
+    r1 = b.run(return_solution=True)
+    r2 = b.run(return_solution=True)
+    for a, c in zip(r1, r2):
+        assert a["iters"] == c["iters"] and a["restarts"] == c["restarts"]
+        assert a["pobj"] == c["pobj"]
+        assert np.array_equal(a["x"], c["x"]) and np.array_equal(a["y"], c["y"])
+
+
+def test_batch_with_netlib_and_infeasible_instances():
+    """Uneven mix: Netlib LPs (bgetam is primal infeasible) between random node relaxations."""
+    names = ["afiro", "bgetam", "adlittle", "stair"]
+    probs = [g.Problem.read_mps(instance_path(n)) for n in names] + _problems(6, seed0=400)
+    res = g.Batch(probs, precision="double", time_limit=120).run()
+    for n, r in zip(names, res):
+        ref = golden(n, ["--device", "cpu", "--precision", "double", "--time", "120"], 1)
+        assert r["status"] == ref["status"], n
+        if ref["status"] == "OPTIMAL":
+            assert _certified(r, 1e-4)
+            assert abs(r["pobj"] - ref["pobj"]) <= 5e-4 * (1 + abs(ref["pobj"])), n
+
+
+def test_batch_iteration_limit():
+    probs = _problems(8, seed0=500)
+    res = g.Batch(probs, precision="double", max_iter=256).run()
+    for r in res:
+        assert r["iters"] <= 256
+        assert r["status"] in ("OPTIMAL", "ITERATION_LIMIT", "PRIMAL_INFEASIBLE")
+    assert any(r["status"] == "ITERATION_LIMIT" for r in res)
+
+
+def test_batch_rejects_odd_check_interval():
+    with pytest.raises(RuntimeError):
+        g.Batch(_problems(2), check_every=63)

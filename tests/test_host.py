@@ -195,3 +195,29 @@ def test_diff_qp_generator_convex():
     ax = e["A"].matvec(x0)
     assert np.allclose(ax[:40], e["lo"][:40], atol=1e-9)
     assert np.all(e["hi"][40:] - ax[40:] >= -1e-9)
+
+
+def test_batch_workload_generator_sizes():
+    """The batched-LP workload: sizes log-uniform in [200, 4000] x [500, 10000], half the rows equalities,
+    column counts 5..15, deterministic per seed, and each instance the random-LP generator at its seed."""
+    import math
+    ms, ns = [], []
+    for s in range(64):
+        p = g.Problem.generate("batch_lp", seed=s)
+        d = p.dims()
+        assert 200 <= d["m"] <= 4000 and 500 <= d["n"] <= 10000
+        e = p.export()
+        col_counts = np.bincount(e["A"].idx, minlength=d["n"])
+        assert col_counts.min() >= 5 and col_counts.max() <= 15
+        assert np.sum(np.isneginf(e["lo"])) == d["m"] - d["m"] // 2  # the <= rows
+        ms.append(math.log(d["m"]))
+        ns.append(math.log(d["n"]))
+        q = g.Problem.generate("random_lp", seed=s, m=d["m"], n=d["n"], m_eq=d["m"] // 2, nnz_min=5, nnz_max=15,
+                               tighten_frac=0.05)
+        assert np.array_equal(q.export()["A"].val, e["A"].val) and np.array_equal(q.export()["c"], e["c"])
+    # log-uniform: mean of log within a few standard errors of the interval's midpoint
+    assert abs(np.mean(ms) - 0.5 * (math.log(200) + math.log(4000))) < 0.35
+    assert abs(np.mean(ns) - 0.5 * (math.log(500) + math.log(10000))) < 0.35
+    a = g.Problem.generate("batch_lp", seed=7).export()
+    b = g.Problem.generate("batch_lp", seed=7).export()
+    assert np.array_equal(a["A"].val, b["A"].val) and np.array_equal(a["u"], b["u"])
