@@ -71,6 +71,7 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_solver_run.argtypes = [_vp, _f64p, _vp, _vp]
     lib.ganit_solver_kernel_times.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p]
     lib.ganit_solver_free.argtypes = [_vp]
+    lib.ganit_solver_adjoint.argtypes = [_vp, _f64p, _f64p, _vp, _vp, _vp, _vp]
     lib.ganit_batch_create.restype = _vp
     lib.ganit_batch_create.argtypes = [_vp, ctypes.c_int, _f64p]
     lib.ganit_batch_upload.argtypes = [_vp]
@@ -285,6 +286,25 @@ class Solver:
         y = np.zeros(self._dims["m"]) if return_solution else None
         _check_host(host_lib().ganit_solver_run(self._h, out, _ptr(x), _ptr(y)), "solver_run")
         return _result(out, x, y)
+
+    def adjoint(self, gx, precision: str = "double", tol: float = 1e-8, max_iter: int = 20_000,
+                method: str = "gmres", every: int = 0, mask_tol: float = -1.0) -> dict:
+        """Gradient of L(x*) with respect to c, the active row bounds and diag Q at the last run()'s solution,
+        given gx = dL/dx*: the adjoint of the PDHG iteration on the GPU (precision "double" or "single";
+        method "gmres" = Krylov-accelerated, "fixed_point" = the plain transposed iteration)."""
+        d = self._dims
+        gx = np.ascontiguousarray(gx, dtype=np.float64)
+        if gx.shape != (d["n"],):
+            raise ValueError(f"gx must have shape ({d['n']},)")
+        if method not in ("gmres", "fixed_point"):
+            raise ValueError("method must be 'gmres' or 'fixed_point'")
+        ao = np.array([1.0 if precision == "single" else 0.0, tol, max_iter, 0.0 if method == "gmres" else 1.0,
+                       every, mask_tol], dtype=np.float64)
+        dc, dq, db, info = np.zeros(d["n"]), np.zeros(d["n"]), np.zeros(d["m"]), np.zeros(5)
+        _check_host(host_lib().ganit_solver_adjoint(self._h, gx, ao, _ptr(dc), _ptr(db), _ptr(dq), _ptr(info)),
+                    "solver_adjoint")
+        return dict(dc=dc, db=db, dq=dq, iters=int(info[0]), rel_change=float(info[1]), free_vars=int(info[2]),
+                    active_rows=int(info[3]), seconds=float(info[4]))
 
     def kernel_times(self, level: int, reps: int = 50):
         ms = np.zeros(2)

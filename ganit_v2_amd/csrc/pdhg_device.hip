@@ -17,6 +17,7 @@
 
 #include "internal.hpp"
 #include "pdhg_math.hpp"
+#include "pdhg_state.hpp"
 
 namespace ganit_hip {
 
@@ -410,53 +411,7 @@ __global__ __launch_bounds__(kBlock) void k_handover(int64_t n, const S* __restr
         d[i] = D(s[i]);
 }
 
-// ================================================================ host object
-struct VecSet {
-    bool on = false;
-    int vdt = 0;
-    void *c = nullptr, *l = nullptr, *u = nullptr, *lo = nullptr, *hi = nullptr, *rres = nullptr, *cres = nullptr;
-    void *X[2] = {nullptr, nullptr}, *Y[2] = {nullptr, nullptr};
-    void *xbar = nullptr, *sumx = nullptr, *sumy = nullptr, *xa = nullptr, *ya = nullptr, *xr = nullptr,
-         *yr = nullptr;
-    int cur = 0;
-};
-
-struct Pdhg {
-    Ctx* ctx = nullptr;
-    int m = 0, n = 0;
-    bool hasq = false;
-    Csr *A = nullptr, *AT = nullptr, *Q = nullptr;
-    VecSet vs[2];
-    double* prm = nullptr;
-    double* part_n = nullptr;
-    double* part_m = nullptr;
-    size_t part_cap = 0;          // blocks the partial buffers hold
-    double* out = nullptr;        // [GANIT_CHECK_NOUT]
-    std::vector<void*> owned;
-    std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
-    bool use_graphs = true;
-
-    void* alloc(size_t bytes) {
-        void* p = nullptr;
-        GH_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
-        owned.push_back(p);
-        return p;
-    }
-    VecSet& set(int vdt) {
-        GH_REQUIRE(vdt == GANIT_F64 || vdt == GANIT_F32, "vector dtype must be GANIT_F64 or GANIT_F32");
-        GH_REQUIRE(vs[vdt].on, "vector dtype not resident in this problem (vdt_mask)");
-        return vs[vdt];
-    }
-    ~Pdhg() {
-        if (ctx) (void)hipStreamSynchronize(ctx->stream);
-        for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
-        for (void* p : owned) (void)hipFree(p);
-        csr_destroy(A);
-        csr_destroy(AT);
-        csr_destroy(Q);
-    }
-};
-
+// ================================================================ host object (pdhg_state.hpp)
 void upload_as(Ctx* ctx, int vdt, const double* h, int64_t n, void* d) {
     if (n == 0) return;
     if (vdt == GANIT_F64) {

@@ -207,6 +207,34 @@ int ganit_solver_kernel_times(ganit_solver* h, int level, int reps, double* ms) 
     return guard([&] { reinterpret_cast<ganit::GpuSolver*>(h)->kernel_times(level, reps, ms); });
 }
 
+int ganit_solver_adjoint(ganit_solver* h, const double* gx, const double* ao, double* dc, double* db, double* dq,
+                         double* info) {
+    return guard([&] {
+        auto* sv = reinterpret_cast<ganit::GpuSolver*>(h);
+        if (!gx || !ao) throw std::runtime_error("null argument");
+        ganit::AdjointOptions o;
+        o.fp32 = ao[0] != 0;
+        o.tol = ao[1];
+        o.max_iter = (int)ao[2];
+        o.gmres = ao[3] == 0;
+        if (ao[4] > 0) {
+            if (o.gmres) o.restart = (int)ao[4];
+            else o.check_every = (int)ao[4];
+        }
+        o.mask_tol = ao[5];
+        const int n = sv->scaled().n;
+        const ganit::AdjointResult r = sv->adjoint(std::vector<double>(gx, gx + n), o);
+        put(dc, r.dc);
+        put(db, r.db);
+        put(dq, r.dq);
+        if (info) {
+            const double v[5] = {(double)r.iterations, r.rel_change, (double)r.free_vars, (double)r.active_rows,
+                                 r.seconds};
+            std::copy(v, v + 5, info);
+        }
+    });
+}
+
 void ganit_solver_free(ganit_solver* h) { delete reinterpret_cast<ganit::GpuSolver*>(h); }
 
 ganit_batch* ganit_batch_create(const ganit_lp* const* lps, int ninst, const double* o) {

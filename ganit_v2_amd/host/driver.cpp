@@ -205,6 +205,11 @@ struct GpuSolver::Impl {
     int low_mdt = GANIT_F32;
     std::vector<double> x0, y0;
     double w0 = 1.0;
+    // the last run's final primal weight and scaled solution (the adjoint linearises there)
+    bool solved = false;
+    bool final_fp32 = false;  // the last run's solution came out of an FP32-vector level
+    double w_final = 1.0;
+    std::vector<double> xs_final;
 
     explicit Impl(const Options& o) : opt(o) {}
     ~Impl() {
@@ -301,6 +306,7 @@ Result GpuSolver::run() {
 
     const Level L1{low_mdt, GANIT_F32, "L1"}, L2{low_mdt, GANIT_F64, "L2"}, L3{GANIT_F64, GANIT_F64, "L3"};
     int last_vdt = first_vdt;
+    int solution_vdt = first_vdt;  // working set of the level that produced the current iterate
     // FP64 KKT of the phase's result (pdhg_evaluate<Bd>): hand the iterate to the FP64 set if needed.
     auto fp64_eval = [&]() {
         if (last_vdt != GANIT_F64) {
@@ -323,6 +329,7 @@ Result GpuSolver::run() {
         const Exit e = phase(sh, pd, lv, st, opt.tol, stall_exit, stall_checks);
         res.iterate_seconds[slot] += sh.elapsed() - t0;
         last_vdt = lv.vdt;
+        solution_vdt = lv.vdt;
         return e;
     };
 
@@ -349,6 +356,10 @@ Result GpuSolver::run() {
     std::vector<double> xs(s.n), ys(s.m);
     chk(ganit_hip_pdhg_store(pd, GANIT_F64, xs.data(), ys.data(), nullptr, nullptr));
     res.solve_seconds = sh.elapsed();
+    I.solved = true;
+    I.final_fp32 = solution_vdt == GANIT_F32;
+    I.w_final = st.w;
+    I.xs_final = xs;
     res.status = status;
     res.iterations = st.it;
     res.fp32_iterations = fp32_iters;
@@ -372,6 +383,50 @@ Result GpuSolver::run() {
                     res.pobj, res.dobj, fin.rel_p, fin.rel_d, fin.rel_gap);
     }
     return res;
+}
+
+AdjointResult GpuSolver::adjoint(const std::vector<double>& gx, const AdjointOptions& ao) {
+    Impl& I = *p_;
+    const ScaledLP& s = I.s;
+    if (!I.solved) throw std::runtime_error("adjoint needs a completed run()");
+    if ((int)gx.size() != s.n) throw std::runtime_error("adjoint: dL/dx has the wrong length");
+    const bool fp32 = ao.fp32;
+    if (fp32 && !I.mixed) throw std::runtime_error("FP32 adjoint needs a mixed or dynamic precision solver");
+    const auto t0 = Clock::now();
+    // steps of the forward solve's last phase (reference pdhg_impl.hpp:87-91)
+    const double eta = 0.998 / I.normA;
+    const double sigma = eta * I.w_final;
+    double tau = eta / I.w_final;
+    if (s.has_q()) tau = std::min(tau, 0.998 / (0.5 * s.normQ + sigma * I.normA * I.normA));
+    // x = C x' bscale: dL/dx' = dL/dx * C * bscale
+    std::vector<double> g(s.n), p(s.n), mu(s.m);
+    for (int j = 0; j < s.n; ++j) g[j] = gx[j] * s.C[j] * s.bscale;
+    double info[4] = {0, 0, 0, 0};
+    const int mdt = fp32 ? I.low_mdt : GANIT_F64, vdt = fp32 ? GANIT_F32 : GANIT_F64;
+    // an iterate that ended in FP32 sits at its bounds only up to FP32 rounding of the bound
+    const double mask_tol = ao.mask_tol >= 0 ? ao.mask_tol : (I.final_fp32 ? 1e-6 : 1e-12);
+    const double params[5] = {ao.gmres ? 1.0 : 0.0, (double)ao.max_iter, ao.tol,
+                              (double)(ao.gmres ? ao.restart : ao.check_every), mask_tol};
+    chk(ganit_hip_pdhg_adjoint(I.pd, mdt, vdt, g.data(), tau, sigma, params, p.data(), mu.data(), info));
+    AdjointResult r;
+    const double sign = s.maximize ? -1.0 : 1.0;
+    r.dc.resize(s.n);
+    r.dq.resize(s.n);
+    r.db.resize(s.m);
+    // back to the original data: c' = sign c C / cscale, Q'_jj = sign Q_jj C_j^2 bscale / cscale,
+    // row bounds' = bounds R / bscale (scaling.cpp:36-129)
+    for (int j = 0; j < s.n; ++j) {
+        const double dcs = -tau * p[j];
+        r.dc[j] = sign * dcs * s.C[j] / s.cscale;
+        r.dq[j] = sign * dcs * I.xs_final[j] * s.C[j] * s.C[j] * s.bscale / s.cscale;
+    }
+    for (int i = 0; i < s.m; ++i) r.db[i] = sigma * mu[i] * s.R[i] / s.bscale;
+    r.iterations = (long)info[0];
+    r.rel_change = info[1];
+    r.free_vars = (int)info[2];
+    r.active_rows = (int)info[3];
+    r.seconds = std::chrono::duration<double>(Clock::now() - t0).count();
+    return r;
 }
 
 Result solve_pdhg_gpu(const LP& lp, const Options& opt) {

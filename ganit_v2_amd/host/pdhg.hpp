@@ -64,6 +64,27 @@ struct Result {
 Result solve_pdhg_gpu(const LP& lp, const Options& opt);
 
 struct ScaledLP;
+
+// extension: options and result of the adjoint (differentiation through a solve).
+struct AdjointOptions {
+    bool fp32 = false;     // FP32 matrix values and vectors (needs a mixed/dynamic solver), else FP64
+    bool gmres = true;     // GMRES on (I - J^T) lambda = g; false: the plain fixed-point iteration
+    double tol = 1e-8;     // GMRES: relative residual; fixed point: relative change of lambda
+    int max_iter = 20000;  // operator applications
+    int restart = 200;     // GMRES restart length
+    int check_every = 64;  // fixed point: iterations between convergence checks
+    double mask_tol = -1;  // bound-activity tolerance; < 0: 1e-12, or 1e-6 after an FP32-vector level
+};
+struct AdjointResult {
+    std::vector<double> dc;  // dL/dc (n)
+    std::vector<double> db;  // dL/d(active bound of each row) (m): dL/db for equality and one-sided rows
+    std::vector<double> dq;  // dL/dQ_jj (n)
+    long iterations = 0;
+    double rel_change = 0;   // relative change of lambda at the last check
+    int free_vars = 0, active_rows = 0;
+    double seconds = 0;
+};
+
 // extension: the same solve split into set-up (scaling, norm), device upload and repeated runs from the
 // initial point, so a caller can time solves with the problem already resident in HBM.
 class GpuSolver {
@@ -77,6 +98,9 @@ class GpuSolver {
     // Average device milliseconds of the two fused iteration kernels at a precision level
     // (0 = L1, 1 = L2, 2 = L3): ms[0] primal kernel, ms[1] dual kernel.
     void kernel_times(int level, int reps, double* ms);
+    // Gradient of L(x*) with respect to c, the active row bounds and diag Q, given gx = dL/dx* at the last
+    // run()'s solution: the adjoint PDHG iteration of libganit_hip.so at that solution.
+    AdjointResult adjoint(const std::vector<double>& gx, const AdjointOptions& opt);
     const ScaledLP& scaled() const;
     double norm_a() const;
 

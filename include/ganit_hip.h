@@ -209,6 +209,21 @@ int ganit_hip_batch_certify(ganit_hip_batch* bt, int mdt);
 int ganit_hip_batch_results(ganit_hip_batch* bt, int* ints, double* dbl);
 int ganit_hip_batch_store(ganit_hip_batch* bt, int vdt, double* x, double* y);
 
+/* Adjoint of the solution map at the FP64 working set's iterate (a converged forward solve; no reference
+ * counterpart).  gx = dL/dx in the scaled space (n).  Solves lambda = J^T lambda + (gx, 0), J the PDHG
+ * iteration's Jacobian with the projections frozen at the iterate (D_j = 0 when x_j lies within
+ * mask_tol (1 + |bound|) of l_j or u_j; F_i = 1 for equality rows and rows with y_i != 0), in matrix values mdt
+ * and working precision vdt, with steps (tau, sigma).  params = {method, max_iter, tol, every, mask_tol}:
+ * method 0 iterates lambda <- J^T lambda + (gx, 0) and stops when ||lambda' - lambda|| / ||lambda'|| <= tol
+ * (checked every `every` iterations); method 1 runs restarted GMRES(every) on (I - J^T) lambda = (gx, 0)
+ * (the same operator, Krylov-accelerated) until ||residual|| / ||gx|| <= tol.  Outputs
+ * p = D o (lx - 2 sigma A^T (F o ly)) (n) and mu = F o ly (m), from which dL/dc = -tau p,
+ * dL/dQ_jj = -tau p_j x_j and dL/d(active bound of row i) = sigma mu_i (scaled space).
+ * info[0..3] = iterations (operator applications), final relative change / residual, free variables
+ * (sum D), active rows (sum F). */
+int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* pd, int mdt, int vdt, const double* gx, double tau, double sigma,
+                           const double* params, double* p, double* mu, double* info);
+
 /* Average device time in milliseconds of each fused iteration kernel on the current iterate, measured with
  * HIP events on the context stream over `reps` back-to-back launches: ms[0] = primal kernel
  * (A^T y [+ Q x] + primal prox), ms[1] = dual kernel (A xbar + dual prox).  Advances the iterate; no

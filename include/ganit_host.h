@@ -74,6 +74,16 @@ ganit_solver* ganit_solver_create(const ganit_lp* lp, const double* opts);
 int ganit_solver_upload(ganit_solver* s);
 int ganit_solver_run(ganit_solver* s, double* out, double* x, double* y);
 int ganit_solver_kernel_times(ganit_solver* s, int level, int reps, double* ms);
+/* Differentiation through the last ganit_solver_run (no reference counterpart): gx = dL/dx* (n);
+ * aopts[0..5] = precision (0 FP64, 1 FP32 matrix + vectors, which needs a mixed/dynamic solver), tol,
+ * max operator applications, method (0 GMRES on (I - J^T) lambda = g, tol on the relative residual;
+ * 1 plain fixed-point iteration, tol on the relative change), restart length (GMRES) or check interval
+ * (fixed point) (0: 200 / 64), bound-activity tolerance (< 0: automatic).  Outputs (any may be NULL):
+ * dc = dL/dc (n), db = dL/d(active bound of each row) (m; dL/db of equality and one-sided rows, 0 for inactive
+ * rows), dq = dL/dQ_jj (n); info[0..4] = iterations, final relative change, free variables, active rows,
+ * seconds. */
+int ganit_solver_adjoint(ganit_solver* s, const double* gx, const double* aopts, double* dc, double* db, double* dq,
+                         double* info);
 void ganit_solver_free(ganit_solver* s);
 
 /* Many LPs solved together by the block-diagonal batched kernels (no reference counterpart: the reference

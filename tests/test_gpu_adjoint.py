@@ -1,0 +1,127 @@
+"""The adjoint PDHG kernel (ganit_hip_pdhg_adjoint) and the torch.autograd.Function over it, against the
+closed form of a separable QP, the active-set KKT reference gradient (oracle/adjoint_ref.py, itself pinned
+by central differences of oracle solves) and FP64 central differences (h = 1e-6) of the oracle's solves."""
+import numpy as np
+import pytest
+import torch
+
+import ganit_v2_amd as g
+from ganit_v2_amd.autograd import QPLayer
+from oracle import adjoint_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def _solve(p, **opts):
+    s = p.solver(**opts)
+    s.upload()
+    r = s.run(return_solution=True)
+    assert r["status"] == "OPTIMAL", r["status"]
+    return s, r
+
+
+def test_adjoint_closed_form_separable_qp():
+    rng = np.random.default_rng(3)
+    n = 50
+    q = rng.uniform(0.5, 2.0, n)
+    c = rng.normal(size=n)
+    w = rng.normal(size=n)
+    b = 2.0
+    x_exact, exact = adjoint_ref.separable_qp_closed_form(q, c, b, w)
+    idx = np.arange(n, dtype=np.int32)
+    p = g.Problem.from_triplets(1, n, np.zeros(n, np.int32), idx, np.ones(n), c, np.full(n, -100.0),
+                                np.full(n, 100.0), [b], [b], qrows=idx, qcols=idx, qvals=q)
+    s, r = _solve(p, tol=1e-10)
+    assert np.max(np.abs(r["x"] - x_exact)) < 1e-7
+    for method in ("gmres", "fixed_point"):
+        a = s.adjoint(w, tol=1e-11, method=method, max_iter=200_000)
+        print(method, "iters", a["iters"], "rel", a["rel_change"], flush=True)
+        assert a["free_vars"] == n and a["active_rows"] == 1
+        assert _rel(a["dc"], exact["dc"]) < 1e-6
+        assert _rel(a["db"], exact["db"]) < 1e-6
+        assert _rel(a["dq"], exact["dq"]) < 1e-6
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_adjoint_fp64_vs_active_set_gradient_c4(seed):
+    """C4 shape: n = 2000, 300 equality + 200 <= rows, A 1% dense, Q = diag + 0.1% symmetric."""
+    p = g.Problem.generate("diff_qp", seed=seed)
+    d = p.export()
+    s, r = _solve(p, tol=1e-9, time_limit=120)
+    w = np.random.default_rng(100 + seed).normal(size=d["A"].cols)
+    a = s.adjoint(w, tol=1e-10)
+    ref = adjoint_ref.active_set_gradient(d["A"], d["Q"], r["x"], r["y"], d["l"], d["u"], d["lo"], d["hi"], w)
+    errs = {k: _rel(a[k], ref[k]) for k in ("dc", "db", "dq")}
+    print(f"seed {seed}: iters {a['iters']} free {a['free_vars']}/{ref['free']} active {a['active_rows']}/"
+          f"{ref['active_rows']} rel errors {errs} adjoint {a['seconds']:.3f}s", flush=True)
+    assert a["free_vars"] == ref["free"] and a["active_rows"] == ref["active_rows"]
+    for k, e in errs.items():
+        assert e < 1e-5, (k, e)
+
+
+def test_adjoint_fp32_vs_active_set_gradient_c4():
+    """FP32 forward (mixed ladder, FP64-certified) and FP32 adjoint; error stated against the FP64 reference."""
+    p = g.Problem.generate("diff_qp", seed=2)
+    d = p.export()
+    s, r = _solve(p, tol=1e-6, precision="mixed", time_limit=120)
+    w = np.random.default_rng(7).normal(size=d["A"].cols)
+    a = s.adjoint(w, precision="single", tol=1e-5)
+    _, r64 = _solve(p, tol=1e-9, time_limit=120)
+    ref = adjoint_ref.active_set_gradient(d["A"], d["Q"], r64["x"], r64["y"], d["l"], d["u"], d["lo"], d["hi"], w)
+    errs = {k: _rel(a[k], ref[k]) for k in ("dc", "db", "dq")}
+    print(f"fp32 adjoint: iters {a['iters']} rel errors {errs}", flush=True)
+    for k, e in errs.items():
+        assert e < 1e-2, (k, e)
+
+
+def test_adjoint_vs_oracle_central_differences(oracle):
+    """Directional derivatives from FP64 central differences (h = 1e-6) of the oracle's CPU solves."""
+    oracle.set_threads(8)
+    p = g.Problem.generate("diff_qp", seed=5, n=300, m_eq=45, m_le=30, density=0.03, qdensity=0.005)
+    d = p.export()
+    s, r = _solve(p, tol=1e-10, time_limit=120)
+    rng = np.random.default_rng(11)
+    w = rng.normal(size=d["A"].cols)
+    a = s.adjoint(w, tol=1e-11)
+    h = 1e-6
+
+    def loss(dd):
+        o = oracle.OracleLP.from_export(dd).solve(tol=1e-12, time_limit=120)
+        assert o["status"] == "OPTIMAL"
+        return w @ o["x"]
+
+    for trial in range(2):
+        vc = rng.normal(size=d["A"].cols)
+        vb = rng.normal(size=d["A"].rows)
+        plus = dict(d, c=d["c"] + h * vc, lo=np.where(np.isfinite(d["lo"]), d["lo"] + h * vb, d["lo"]),
+                    hi=d["hi"] + h * vb)
+        minus = dict(d, c=d["c"] - h * vc, lo=np.where(np.isfinite(d["lo"]), d["lo"] - h * vb, d["lo"]),
+                     hi=d["hi"] - h * vb)
+        fd = (loss(plus) - loss(minus)) / (2 * h)
+        an = a["dc"] @ vc + a["db"] @ vb
+        print(f"trial {trial}: fd {fd:.10g} adjoint {an:.10g}", flush=True)
+        assert abs(fd - an) <= 1e-5 * (1 + abs(fd))
+
+
+def test_autograd_function_matches_adjoint():
+    p = g.Problem.generate("diff_qp", seed=4, n=500, m_eq=75, m_le=50, density=0.02, qdensity=0.002)
+    layer = QPLayer.from_problem(p, precision="double", tol=1e-9, adjoint_tol=1e-10)
+    c, b, q = (t.clone().requires_grad_(True) for t in layer.data_of(p))
+    w = torch.from_numpy(np.random.default_rng(9).normal(size=layer.n))
+    x = layer(c, b, q)
+    loss = (w * x).sum()
+    loss.backward()
+    d = p.export()
+    r, a = layer.last
+    ref = adjoint_ref.active_set_gradient(d["A"], d["Q"], r["x"], r["y"], d["l"], d["u"], d["lo"], d["hi"],
+                                          w.numpy())
+    assert np.array_equal(c.grad.numpy(), a["dc"]) and np.array_equal(q.grad.numpy(), a["dq"])
+    assert np.array_equal(b.grad.numpy(), a["db"])
+    for k, t in (("dc", c.grad), ("db", b.grad), ("dq", q.grad)):
+        assert _rel(t.numpy(), ref[k]) < 1e-5, k
+    # the layer's forward is the problem's own solution
+    assert np.allclose(x.detach().numpy(), r["x"])
