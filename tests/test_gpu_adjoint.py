@@ -90,7 +90,8 @@ def test_adjoint_vs_oracle_central_differences(oracle):
     h = 1e-6
 
     def loss(dd):
-        o = oracle.OracleLP.from_export(dd).solve(tol=1e-12, time_limit=120)
+        # 1e-11: tight enough for h = 1e-6 differences, loose enough for the FP64 certification to pass
+        o = oracle.OracleLP.from_export(dd).solve(tol=1e-11, time_limit=120)
         assert o["status"] == "OPTIMAL"
         return w @ o["x"]
 
