@@ -258,7 +258,8 @@ int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* h, int mdt, int vdt, const double* gx
         GH_REQUIRE((n == 0 || gat > 0) && (m == 0 || ga > 0), "row-block partition missing");
         DevBuf D(n), F(m), g(e * n), r(e * n), p(e * n), g64(8 * (size_t)n), lam(e * L);
         const unsigned gx_grid = grid_n(n), gl = grid_n(L);
-        DevBuf part_x(sizeof(double) * 2 * gx_grid), part_y(sizeof(double) * 2 * std::max(ga, 1u)),
+        const unsigned nsa = (unsigned)pd->A->nsplit, nsat = (unsigned)pd->AT->nsplit;
+        DevBuf part_x(sizeof(double) * 2 * gx_grid), part_y(sizeof(double) * 2 * std::max(ga + nsa, 1u)),
             out(sizeof(double) * 4), p64(8 * (size_t)n), mu64(8 * (size_t)std::max(m, 1));
         GH_CHECK(hipMemcpyAsync(g64.p, gx, 8 * (size_t)n, hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_adj_masks, dim3(grid_n(L)), dim3(kBlock), 0, st, n, m, (const double*)s64.X[s64.cur],
@@ -280,14 +281,21 @@ int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* h, int mdt, int vdt, const double* gx
                 // out = J^T in + g (RES false) or out = in - J^T in (RES true); in / out are L-vectors
                 auto apply = [&](const V* in, V* o, auto MS, auto RS) {
                     constexpr bool ms = decltype(MS)::value, rs = decltype(RS)::value;
+                    // each row-block kernel: main pass, then the split-row finish pass (partials after the main's)
                     if (n > 0)
-                        hipLaunchKernelGGL((k_adj_at<M, V>), dim3(gat), dim3(kBlock), 0, st, pd->AT->view<M>(),
-                                           pd->AT->blocks, pd->AT->split(), in + n, F.as<uint8_t>(), in,
-                                           D.as<uint8_t>(), V(2.0 * sigma), r.as<V>(), p.as<V>());
+                        for (int pass = 0; pass < (nsat ? 2 : 1); ++pass)
+                            hipLaunchKernelGGL((k_adj_at<M, V>), dim3(pass ? nsat : gat), dim3(kBlock), 0, st,
+                                               pd->AT->view<M>(), pd->AT->blocks,
+                                               pass ? pd->AT->split_finish() : pd->AT->split(), in + n,
+                                               F.as<uint8_t>(), in, D.as<uint8_t>(), V(2.0 * sigma), r.as<V>(),
+                                               p.as<V>());
                     if (m > 0)
-                        hipLaunchKernelGGL((k_adj_a<M, V, ms, rs>), dim3(ga), dim3(kBlock), 0, st, pd->A->view<M>(),
-                                           pd->A->blocks, pd->A->split(), p.as<V>(), F.as<uint8_t>(), V(tau), in + n,
-                                           o + n, part_y.as<double>());
+                        for (int pass = 0; pass < (nsa ? 2 : 1); ++pass)
+                            hipLaunchKernelGGL((k_adj_a<M, V, ms, rs>), dim3(pass ? nsa : ga), dim3(kBlock), 0, st,
+                                               pd->A->view<M>(), pd->A->blocks,
+                                               pass ? pd->A->split_finish() : pd->A->split(), p.as<V>(),
+                                               F.as<uint8_t>(), V(tau), in + n, o + n,
+                                               part_y.as<double>() + (pass ? 2 * (size_t)ga : 0));
                     if (n > 0) {
                         if (pd->hasq)
                             hipLaunchKernelGGL((k_adj_x<M, V, true, ms, rs>), dim3(gx_grid), dim3(kBlock), 0, st, n,
@@ -305,7 +313,7 @@ int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* h, int mdt, int vdt, const double* gx
                         for (int i = 0; i + 1 < k; ++i) apply(lx, lx, Fa{}, Fa{});
                         apply(lx, lx, T{}, Fa{});
                         hipLaunchKernelGGL(k_adj_finish, dim3(4), dim3(kBlock), 0, st, part_x.as<double>(),
-                                           n > 0 ? (int)gx_grid : 0, part_y.as<double>(), m > 0 ? (int)ga : 0,
+                                           n > 0 ? (int)gx_grid : 0, part_y.as<double>(), m > 0 ? (int)(ga + nsa) : 0,
                                            out.as<double>());
                     };
                     hipGraphExec_t ex = nullptr;
@@ -438,9 +446,11 @@ int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* h, int mdt, int vdt, const double* gx
                 }
                 // p and mu of the final iterate
                 if (n > 0)
-                    hipLaunchKernelGGL((k_adj_at<M, V>), dim3(gat), dim3(kBlock), 0, st, pd->AT->view<M>(),
-                                       pd->AT->blocks, pd->AT->split(), ly, F.as<uint8_t>(), lx, D.as<uint8_t>(),
-                                       V(2.0 * sigma), r.as<V>(), p.as<V>());
+                    for (int pass = 0; pass < (nsat ? 2 : 1); ++pass)
+                        hipLaunchKernelGGL((k_adj_at<M, V>), dim3(pass ? nsat : gat), dim3(kBlock), 0, st,
+                                           pd->AT->view<M>(), pd->AT->blocks,
+                                           pass ? pd->AT->split_finish() : pd->AT->split(), ly, F.as<uint8_t>(), lx,
+                                           D.as<uint8_t>(), V(2.0 * sigma), r.as<V>(), p.as<V>());
                 hipLaunchKernelGGL(k_adj_out<V>, dim3(grid_n(L)), dim3(kBlock), 0, st, n, m, p.as<V>(), ly,
                                    F.as<uint8_t>(), p64.as<double>(), mu64.as<double>());
             });

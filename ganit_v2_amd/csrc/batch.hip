@@ -69,7 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
     const BatchState& s = st[blk_inst[b]];
     if (!s.active) return;
     const V tau = V(s.tau);
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
     stream_block<M, V, 1>(
         AT, blocks, b, [&](int, int i) { return y[i]; },
         [&](int j, V (&aty)[1]) {
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
     const BatchState& s = st[blk_inst[b]];
     if (!s.active) return;
     const V sigma = V(s.sigma);
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
     stream_block<M, V, 1>(
         A, blocks, b, [&](int, int j) { return xbar[j]; },
         [&](int i, V (&axb)[1]) {
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
     double acc[kBN];
 #pragma unroll
     for (int k = 0; k < kBN; ++k) acc[k] = 0.0;
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
     stream_block_seq<M, V, 3>(
         AT, blocks, b,
         [&](int k, int i) { return k == 0 ? y[i] : k == 1 ? (avg ? ya[i] : V(0)) : V(y[i] - yr[i]); },
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* 
     double acc[kBM];
 #pragma unroll
     for (int k = 0; k < kBM; ++k) acc[k] = 0.0;
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, nullptr};
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
     stream_block_seq<M, V, 2>(
         A, blocks, b, [&](int k, int j) { return k == 0 ? x[j] : (avg ? xa[j] : V(0)); },
         [&](int i, V (&t)[2]) {

@@ -107,14 +107,17 @@ __device__ __forceinline__ void block_sum_v(V (&v)[NX], V* lds /* (kBlock / kWav
 
 // ---------------------------------------------------------------- split rows
 // chunk[4c .. 4c+3] = (row, first nonzero, end nonzero, split-row index s); first[s] .. first[s+1] are the
-// chunks of split row s; ticket[s] counts its finished chunks (zero between launches); part holds NX
-// partial sums per chunk (stored as double, formed in the vector type).
+// chunks of split row s; part holds NX partial sums per chunk (stored as double, formed in the vector type).
+// A kernel over a matrix with split rows runs twice: the main pass (finish = 0) over the row blocks and the
+// chunks, which only write their partials, then the finish pass (finish = 1, one workgroup per split row)
+// that folds each row's partials in a fixed order and applies the row function.  The kernel boundary
+// orders the two, so no workgroup waits for or fences against another.
 struct SplitPlan {
-    int nblocks;  // workgroups [0, nblocks) are row blocks, [nblocks, nblocks + nchunks) are chunks
+    int nblocks;  // main pass: workgroups [0, nblocks) are row blocks, [nblocks, nblocks + nchunks) chunks
     const int* __restrict__ chunk;
     const int* __restrict__ first;
-    unsigned int* ticket;
     double* part;
+    int finish;   // 1: workgroup b folds split row b
 };
 
 // ---------------------------------------------------------------- LDS-staged row blocks
@@ -127,12 +130,28 @@ struct SplitPlan {
 template <class M, class V, int NX, class Gather, class RowFn>
 __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __restrict__ blocks, int b, Gather gather,
                                              RowFn row_fn, V* lds /* NX * kTile */, const SplitPlan& sp) {
-    if (b >= sp.nblocks) {  // one chunk of a split row
+    if (sp.finish) {  // finish pass: fold split row b's chunk partials, fixed-order tree
+        const int q0 = sp.first[b], nch = sp.first[b + 1] - q0;
+        V tot[NX];
+#pragma unroll
+        for (int k = 0; k < NX; ++k) tot[k] = V(0);
+        for (int q = threadIdx.x; q < nch; q += kBlock) {
+#pragma unroll
+            for (int k = 0; k < NX; ++k) tot[k] += (V)sp.part[(size_t)(q0 + q) * NX + k];
+        }
+        block_sum_v<V, NX>(tot, lds);
+        if (threadIdx.x == 0) row_fn(sp.chunk[4 * q0], tot);
+        return;
+    }
+    if (b >= sp.nblocks) {  // one chunk of a split row: its partial only
         const int c = b - sp.nblocks;
-        const int row = sp.chunk[4 * c], p0 = sp.chunk[4 * c + 1], p1 = sp.chunk[4 * c + 2], sr = sp.chunk[4 * c + 3];
+        const int p0 = sp.chunk[4 * c + 1], p1 = sp.chunk[4 * c + 2];
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
+        // unrolled so that several index / value / gather loads are in flight; each thread still adds its
+        // products in increasing position order
+#pragma unroll 8
         for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
             const V av = widen<V>(a.val[p]);
             const int i = a.idx[p];
@@ -140,27 +159,9 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
             for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
         }
         block_sum_v<V, NX>(s, lds);
-        __shared__ bool last_chunk;
         if (threadIdx.x == 0) {
 #pragma unroll
             for (int k = 0; k < NX; ++k) sp.part[(size_t)c * NX + k] = (double)s[k];
-            __threadfence();
-            const int nch = sp.first[sr + 1] - sp.first[sr];
-            const unsigned t = __hip_atomic_fetch_add(&sp.ticket[sr], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            last_chunk = (t == (unsigned)nch - 1);
-        }
-        __syncthreads();
-        if (last_chunk && threadIdx.x == 0) {
-            __threadfence();
-            V tot[NX];
-#pragma unroll
-            for (int k = 0; k < NX; ++k) tot[k] = V(0);
-            for (int q = sp.first[sr]; q < sp.first[sr + 1]; ++q) {
-#pragma unroll
-                for (int k = 0; k < NX; ++k) tot[k] += (V)sp.part[(size_t)q * NX + k];
-            }
-            __hip_atomic_store(&sp.ticket[sr], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            row_fn(row, tot);
         }
         return;
     }
@@ -172,6 +173,7 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
+#pragma unroll 8
         for (int p = base + threadIdx.x; p < base + cnt; p += kBlock) {
             const V av = widen<V>(a.val[p]);
             const int i = a.idx[p];
@@ -196,6 +198,7 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
+#pragma unroll 8
         for (int q = beg; q < end; ++q) {
 #pragma unroll
             for (int k = 0; k < NX; ++k) s[k] += lds[k * kTile + q];
@@ -207,14 +210,15 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
 // stream_block for NX gathered vectors with ONE tile of LDS: on a staged block the NX products are staged and
 // summed one vector after another (the block's indices and values come from cache on the later passes), so a
 // check pass needs kTile * sizeof(V) bytes of LDS instead of NX times that.  Same row sums, same bits.  Long
-// rows and chunks of split rows need only (kBlock / kWave) * NX elements of LDS and take the NX-wide path
-// (a chunk's per-row ticket must be taken once per chunk).
+// rows, chunks of split rows and the finish pass need only (kBlock / kWave) * NX elements of LDS and take the
+// NX-wide path.
 template <class M, class V, int NX, class Gather, class RowFn>
 __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int* __restrict__ blocks, int b,
                                                  Gather gather, RowFn row_fn, V* lds /* kTile */,
                                                  const SplitPlan& sp) {
     static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
-    if (b >= sp.nblocks || (blocks[b + 1] - blocks[b] == 1 && a.ptr[blocks[b + 1]] - a.ptr[blocks[b]] > kLong)) {
+    if (sp.finish || b >= sp.nblocks ||
+        (blocks[b + 1] - blocks[b] == 1 && a.ptr[blocks[b + 1]] - a.ptr[blocks[b]] > kLong)) {
         stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
         return;
     }

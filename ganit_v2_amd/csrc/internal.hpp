@@ -87,9 +87,10 @@ struct Csr {
     int nsplit = 0, nchunks = 0;
     int* chunk = nullptr;
     int* split_first = nullptr;
-    unsigned int* split_ticket = nullptr;
     double* split_part = nullptr;  // 3 doubles per chunk
-    SplitPlan split() const { return {nblocks, chunk, split_first, split_ticket, split_part}; }
+    SplitPlan split() const { return {nblocks, chunk, split_first, split_part, 0}; }
+    // second launch of a row-block kernel when nsplit > 0: grid nsplit, folds the chunk partials
+    SplitPlan split_finish() const { return {nblocks, chunk, split_first, split_part, 1}; }
     unsigned rb_grid() const { return (unsigned)(nblocks + nchunks); }
     template <class M>
     CsrView<M> view() const;

@@ -466,15 +466,19 @@ void launch_one(Pdhg* pd, VecSet& s, int which) {
     V* yn = (V*)s.Y[1 - s.cur];
     if (which == 0 && pd->n > 0 && pd->AT->lanes == 0) {
         const CsrView<M> at = pd->AT->view<M>();
-        const dim3 g(pd->AT->rb_grid());
-        if (pd->hasq)
-            hipLaunchKernelGGL((k_iter_primal_rb<M, V, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
-                               pd->AT->split(), pd->Q->view<M>(), y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn,
-                               (V*)s.xbar, (V*)s.sumx);
-        else
-            hipLaunchKernelGGL((k_iter_primal_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
-                               pd->AT->split(), at, y, x,
-                               (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar, (V*)s.sumx);
+        // main pass, then (split rows only) the finish pass that folds the chunk partials
+        for (int pass = 0; pass < (pd->AT->nsplit > 0 ? 2 : 1); ++pass) {
+            const dim3 g(pass ? (unsigned)pd->AT->nsplit : pd->AT->rb_grid());
+            const SplitPlan sp = pass ? pd->AT->split_finish() : pd->AT->split();
+            if (pd->hasq)
+                hipLaunchKernelGGL((k_iter_primal_rb<M, V, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
+                                   pd->Q->view<M>(), y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn,
+                                   (V*)s.xbar, (V*)s.sumx);
+            else
+                hipLaunchKernelGGL((k_iter_primal_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp, at,
+                                   y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar,
+                                   (V*)s.sumx);
+        }
     } else if (which == 0 && pd->n > 0) {
         const CsrView<M> at = pd->AT->view<M>();
         with_lanes(pd->AT->lanes, [&](auto L) {
@@ -490,9 +494,11 @@ void launch_one(Pdhg* pd, VecSet& s, int which) {
         });
     }
     if (which == 1 && pd->m > 0 && pd->A->lanes == 0) {
-        hipLaunchKernelGGL((k_iter_dual_rb<M, V>), dim3(pd->A->rb_grid()), dim3(kBlock), 0, st, pd->A->view<M>(),
-                           pd->A->blocks, pd->A->split(), (const V*)s.xbar, y, (const V*)s.lo, (const V*)s.hi, pd->prm, yn,
-                           (V*)s.sumy);
+        for (int pass = 0; pass < (pd->A->nsplit > 0 ? 2 : 1); ++pass)
+            hipLaunchKernelGGL((k_iter_dual_rb<M, V>), dim3(pass ? (unsigned)pd->A->nsplit : pd->A->rb_grid()),
+                               dim3(kBlock), 0, st, pd->A->view<M>(), pd->A->blocks,
+                               pass ? pd->A->split_finish() : pd->A->split(), (const V*)s.xbar, y, (const V*)s.lo,
+                               (const V*)s.hi, pd->prm, yn, (V*)s.sumy);
     } else if (which == 1 && pd->m > 0) {
         const CsrView<M> a = pd->A->view<M>();
         with_lanes(pd->A->lanes, [&](auto L) {
@@ -522,22 +528,30 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
     }
     ReduceSlot sn{pd->part_n, pd->out};
     ReduceSlot sm{pd->part_m, pd->out + kNOutN};
-    unsigned nb_n = 0, nb_m = 0;  // grids of the two passes (number of partials)
+    unsigned nb_n = 0, nb_m = 0;  // partials written by the passes over A^T and A
     const CsrView<M> at = pd->AT->view<M>();
     const CsrView<M> a = pd->A->view<M>();
+    // the row-block passes write rb_grid partials, then their finish passes one per split row
+    if (pd->AT->lanes == 0 && pd->n > 0)
+        GH_REQUIRE(pd->AT->rb_grid() + (unsigned)pd->AT->nsplit <= pd->part_cap, "check grid exceeds the partial buffers");
+    if (pd->A->lanes == 0 && pd->m > 0)
+        GH_REQUIRE(pd->A->rb_grid() + (unsigned)pd->A->nsplit <= pd->part_cap, "check grid exceeds the partial buffers");
     if (pd->AT->lanes == 0 && pd->n > 0) {
-        const dim3 g(pd->AT->rb_grid());
-        nb_n = g.x;
-        if (pd->hasq)
-            hipLaunchKernelGGL((k_check_n_rb<M, V, C, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
-                               pd->AT->split(), pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
-                               (const V*)s.yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags,
-                               sn);
-        else
-            hipLaunchKernelGGL((k_check_n_rb<M, V, C, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks,
-                               pd->AT->split(), at, x, y,
-                               (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
-                               (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
+        for (int pass = 0; pass < (pd->AT->nsplit > 0 ? 2 : 1); ++pass) {
+            const dim3 g(pass ? (unsigned)pd->AT->nsplit : pd->AT->rb_grid());
+            const SplitPlan sp = pass ? pd->AT->split_finish() : pd->AT->split();
+            const ReduceSlot slot{pd->part_n + (size_t)nb_n * kNOutN, pd->out};
+            if (pd->hasq)
+                hipLaunchKernelGGL((k_check_n_rb<M, V, C, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
+                                   pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
+                                   (const V*)s.yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres,
+                                   flags, slot);
+            else
+                hipLaunchKernelGGL((k_check_n_rb<M, V, C, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp, at,
+                                   x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr,
+                                   (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, slot);
+            nb_n += g.x;
+        }
     } else with_lanes(pd->AT->lanes == 0 ? 1 : pd->AT->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         const int64_t groups = std::max<int64_t>(pd->n, 1);
@@ -553,11 +567,14 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
                                (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     });
     if (pd->A->lanes == 0 && pd->m > 0) {
-        nb_m = pd->A->rb_grid();
-        hipLaunchKernelGGL((k_check_m_rb<M, V, C>), dim3(pd->A->rb_grid()), dim3(kBlock), 0, st, a, pd->A->blocks,
-                           pd->A->split(), x,
-                           (const V*)s.xa, y, (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi,
-                           (const V*)s.rres, flags, sm);
+        for (int pass = 0; pass < (pd->A->nsplit > 0 ? 2 : 1); ++pass) {
+            const dim3 g(pass ? (unsigned)pd->A->nsplit : pd->A->rb_grid());
+            const ReduceSlot slot{pd->part_m + (size_t)nb_m * kNOutM, pd->out + kNOutN};
+            hipLaunchKernelGGL((k_check_m_rb<M, V, C>), g, dim3(kBlock), 0, st, a, pd->A->blocks,
+                               pass ? pd->A->split_finish() : pd->A->split(), x, (const V*)s.xa, y, (const V*)s.ya,
+                               (const V*)s.yr, (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags, slot);
+            nb_m += g.x;
+        }
     } else with_lanes(pd->A->lanes == 0 ? 1 : pd->A->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
         const int64_t groups = std::max<int64_t>(pd->m, 1);
@@ -663,7 +680,8 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
             }
             // partials for the widest check grid of the chosen variants
-            const size_t gmax = (size_t)std::max({(unsigned)kCheckGridMax, pd->A->rb_grid(), pd->AT->rb_grid()});
+            const size_t gmax = (size_t)std::max({(unsigned)kCheckGridMax, pd->A->rb_grid() + (unsigned)pd->A->nsplit,
+                                                  pd->AT->rb_grid() + (unsigned)pd->AT->nsplit});
             pd->part_cap = gmax;
             pd->part_n = (double*)pd->alloc(sizeof(double) * gmax * kNOutN);
             pd->part_m = (double*)pd->alloc(sizeof(double) * gmax * kNOutM);

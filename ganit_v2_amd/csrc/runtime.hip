@@ -272,12 +272,10 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
             if (a->nsplit > 0) {
                 GH_CHECK(hipMalloc(&a->chunk, sizeof(int) * ch.size()));
                 GH_CHECK(hipMalloc(&a->split_first, sizeof(int) * first.size()));
-                GH_CHECK(hipMalloc(&a->split_ticket, sizeof(unsigned) * a->nsplit));
                 GH_CHECK(hipMalloc(&a->split_part, sizeof(double) * 3 * a->nchunks));
                 GH_CHECK(hipMemcpyAsync(a->chunk, ch.data(), sizeof(int) * ch.size(), hipMemcpyHostToDevice, ctx->stream));
                 GH_CHECK(hipMemcpyAsync(a->split_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice,
                                         ctx->stream));
-                GH_CHECK(hipMemsetAsync(a->split_ticket, 0, sizeof(unsigned) * a->nsplit, ctx->stream));
             }
         }
         // short rows (nnz-weighted median under 16 entries) start on the row-block variant
@@ -323,7 +321,6 @@ void csr_destroy(Csr* a) {
     (void)hipFree(a->blocks);
     (void)hipFree(a->chunk);
     (void)hipFree(a->split_first);
-    (void)hipFree(a->split_ticket);
     (void)hipFree(a->split_part);
     delete a;
 }
@@ -335,6 +332,9 @@ void launch_spmv(Ctx* ctx, const Csr* a, const V* x, V* y) {
     if (a->lanes == 0) {
         hipLaunchKernelGGL((k_spmv_rb<M, V>), dim3(a->rb_grid()), dim3(kBlock), 0, ctx->stream, v, a->blocks,
                            a->split(), x, y);
+        if (a->nsplit > 0)
+            hipLaunchKernelGGL((k_spmv_rb<M, V>), dim3(a->nsplit), dim3(kBlock), 0, ctx->stream, v, a->blocks,
+                               a->split_finish(), x, y);
         GH_CHECK(hipGetLastError());
         return;
     }
