@@ -70,11 +70,15 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
     if (!s.active) return;
     const V tau = V(s.tau);
     const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
-    stream_block<M, V, 1>(
+    struct Ops {
+        V x, c, l, u, s;
+    };
+    stream_block_pf<M, V, 1>(
         AT, blocks, b, [&](int, int i) { return y[i]; },
-        [&](int j, V (&aty)[1]) {
-            V sm = sumx[j];
-            primal_update<V>(x[j], aty[0], V(0), c[j], l[j], u[j], tau, xn[j], xbar[j], sm);
+        [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j]}; },
+        [&](int j, V (&aty)[1], Ops& o) {
+            V sm = o.s;
+            primal_update<V>(o.x, aty[0], V(0), o.c, o.l, o.u, tau, xn[j], xbar[j], sm);
             sumx[j] = sm;
         },
         lds, none);
@@ -93,11 +97,15 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
     if (!s.active) return;
     const V sigma = V(s.sigma);
     const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
-    stream_block<M, V, 1>(
+    struct Ops {
+        V y, lo, hi, s;
+    };
+    stream_block_pf<M, V, 1>(
         A, blocks, b, [&](int, int j) { return xbar[j]; },
-        [&](int i, V (&axb)[1]) {
-            V sm = sumy[i];
-            dual_update<V>(y[i], axb[0], lo[i], hi[i], sigma, yn[i], sm);
+        [&](int i) { return Ops{y[i], lo[i], hi[i], sumy[i]}; },
+        [&](int i, V (&axb)[1], Ops& o) {
+            V sm = o.s;
+            dual_update<V>(o.y, axb[0], o.lo, o.hi, sigma, yn[i], sm);
             sumy[i] = sm;
         },
         lds, none);

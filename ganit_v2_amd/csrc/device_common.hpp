@@ -126,10 +126,12 @@ struct SplitPlan {
 // nonzeros and stored in LDS (rounded to V there); then thread t adds up row r0 + t sequentially in row
 // order -- the same operation sequence as a scalar CPU row loop, so these row sums are bit-identical to it.
 // A single row longer than kLong is instead swept by all threads and reduced with a fixed tree.
-// `row_fn(r, sums)` runs once per row, in the thread that owns it (thread 0 for a long row).
-template <class M, class V, int NX, class Gather, class RowFn>
-__device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __restrict__ blocks, int b, Gather gather,
-                                             RowFn row_fn, V* lds /* NX * kTile */, const SplitPlan& sp) {
+// `row_fn(r, sums, ops)` runs once per row, in the thread that owns it (thread 0 for a long row), with
+// ops = pref(r): the row's own operands, loaded early (stream_block_pf); stream_block has none.
+template <class M, class V, int NX, class Gather, class Pref, class RowFn>
+__device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* __restrict__ blocks, int b,
+                                                Gather gather, Pref pref, RowFn row_fn, V* lds /* NX * kTile */,
+                                                const SplitPlan& sp) {
     if (sp.finish) {  // finish pass: fold split row b's chunk partials, fixed-order tree
         const int q0 = sp.first[b], nch = sp.first[b + 1] - q0;
         V tot[NX];
@@ -140,7 +142,11 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
             for (int k = 0; k < NX; ++k) tot[k] += (V)sp.part[(size_t)(q0 + q) * NX + k];
         }
         block_sum_v<V, NX>(tot, lds);
-        if (threadIdx.x == 0) row_fn(sp.chunk[4 * q0], tot);
+        if (threadIdx.x == 0) {
+            const int row = sp.chunk[4 * q0];
+            auto pv = pref(row);
+            row_fn(row, tot, pv);
+        }
         return;
     }
     if (b >= sp.nblocks) {  // one chunk of a split row: its partial only
@@ -181,8 +187,22 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
             for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
         }
         block_sum_v<V, NX>(s, lds);
-        if (threadIdx.x == 0) row_fn(r0, s);
+        if (threadIdx.x == 0) {
+            auto pv = pref(r0);
+            row_fn(r0, s, pv);
+        }
         return;
+    }
+    // the owning thread's row extent and row operands are loaded before the staging sweep, so their
+    // latency overlaps it instead of following the barrier
+    const int r = r0 + (int)threadIdx.x;
+    const bool own = r < r1;
+    int beg = 0, end = 0;
+    decltype(pref(0)) pv{};
+    if (own) {
+        beg = a.ptr[r] - base;
+        end = a.ptr[r + 1] - base;
+        pv = pref(r);
     }
 #pragma unroll 4
     for (int q = threadIdx.x; q < cnt; q += kBlock) {
@@ -192,9 +212,7 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
         for (int k = 0; k < NX; ++k) lds[k * kTile + q] = av * gather(k, i);
     }
     __syncthreads();
-    const int r = r0 + (int)threadIdx.x;
-    if (r < r1) {
-        const int beg = a.ptr[r] - base, end = a.ptr[r + 1] - base;
+    if (own) {
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
@@ -203,7 +221,7 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
 #pragma unroll
             for (int k = 0; k < NX; ++k) s[k] += lds[k * kTile + q];
         }
-        row_fn(r, s);
+        row_fn(r, s, pv);
     }
 }
 
@@ -212,6 +230,18 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
 // check pass needs kTile * sizeof(V) bytes of LDS instead of NX times that.  Same row sums, same bits.  Long
 // rows, chunks of split rows and the finish pass need only (kBlock / kWave) * NX elements of LDS and take the
 // NX-wide path.
+
+struct NoRowOperands {};
+
+// stream_block without row operands: row_fn(r, sums).
+template <class M, class V, int NX, class Gather, class RowFn>
+__device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __restrict__ blocks, int b, Gather gather,
+                                             RowFn row_fn, V* lds /* NX * kTile */, const SplitPlan& sp) {
+    stream_block_pf<M, V, NX>(
+        a, blocks, b, gather, [](int) { return NoRowOperands{}; },
+        [&](int r, V (&t)[NX], NoRowOperands&) { row_fn(r, t); }, lds, sp);
+}
+
 template <class M, class V, int NX, class Gather, class RowFn>
 __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int* __restrict__ blocks, int b,
                                                  Gather gather, RowFn row_fn, V* lds /* kTile */,

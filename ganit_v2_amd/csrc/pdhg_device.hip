@@ -68,9 +68,13 @@ __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const 
                                                            const double* __restrict__ prm, V* __restrict__ xn,
                                                            V* __restrict__ xbar, V* __restrict__ sumx) {
     __shared__ V lds[kTile];
-    stream_block<M, V, 1>(
+    struct Ops {
+        V x, c, l, u, s;
+    };
+    stream_block_pf<M, V, 1>(
         AT, blocks, xcd_block_id(), [&](int, int i) { return y[i]; },
-        [&](int j, V (&aty)[1]) {
+        [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j]}; },
+        [&](int j, V (&aty)[1], Ops& o) {
 #pragma clang fp contract(off)
             V qx = V(0);
             if (HASQ) {  // Q row summed sequentially by the owning thread
@@ -79,9 +83,9 @@ __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const 
                     qx = qx + prod;
                 }
             }
-            V s = sumx[j];
-            primal_update<V>(x[j], aty[0], qx, c[j], l[j], u[j], V(prm[0]), xn[j], xbar[j], s);
-            sumx[j] = s;
+            V sm = o.s;
+            primal_update<V>(o.x, aty[0], qx, o.c, o.l, o.u, V(prm[0]), xn[j], xbar[j], sm);
+            sumx[j] = sm;
         },
         lds, sp);
 }
@@ -93,12 +97,16 @@ __global__ __launch_bounds__(kBlock) void k_iter_dual_rb(CsrView<M> A, const int
                                                          const double* __restrict__ prm, V* __restrict__ yn,
                                                          V* __restrict__ sumy) {
     __shared__ V lds[kTile];
-    stream_block<M, V, 1>(
+    struct Ops {
+        V y, lo, hi, s;
+    };
+    stream_block_pf<M, V, 1>(
         A, blocks, xcd_block_id(), [&](int, int j) { return xbar[j]; },
-        [&](int i, V (&axb)[1]) {
-            V s = sumy[i];
-            dual_update<V>(y[i], axb[0], lo[i], hi[i], V(prm[1]), yn[i], s);
-            sumy[i] = s;
+        [&](int i) { return Ops{y[i], lo[i], hi[i], sumy[i]}; },
+        [&](int i, V (&axb)[1], Ops& o) {
+            V sm = o.s;
+            dual_update<V>(o.y, axb[0], o.lo, o.hi, V(prm[1]), yn[i], sm);
+            sumy[i] = sm;
         },
         lds, sp);
 }
@@ -144,6 +152,39 @@ __device__ __forceinline__ double row_obj_term(double yi, double lw, double hg) 
 // One pass over A^T (and Q): out[0..11] as documented for ganit_hip_pdhg_check.  Vectors are stored in V;
 // the row sums are formed in C (C = V reproduces the level's own evaluation, C = double with FP64 matrix
 // values is the FP64 evaluation of a reduced-precision iterate).
+// Per-thread accumulator of the check sums: FP64, as the reference's reductions; or, with GANIT_CHECK_KAHAN,
+// every term rounded to FP32 and added with Kahan compensation in FP32.  Block partials and the fold are
+// FP64 either way: compensated FP32 within a thread, FP64 across threads (the oracle's Kahan reductions).
+template <int N>
+struct CheckAcc {
+    double d[N];
+    float s[N], c[N];
+    bool kahan;
+    __device__ explicit CheckAcc(bool k) : kahan(k) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            d[i] = 0.0;
+            s[i] = 0.0f;
+            c[i] = 0.0f;
+        }
+    }
+    __device__ __forceinline__ void add(int i, double v) {
+#pragma clang fp contract(off)
+        if (kahan) {
+            const float yv = float(v) - c[i];
+            const float t = s[i] + yv;
+            c[i] = (t - s[i]) - yv;
+            s[i] = t;
+        } else {
+            d[i] += v;
+        }
+    }
+    __device__ __forceinline__ void totals(double (&o)[N]) const {
+#pragma unroll
+        for (int i = 0; i < N; ++i) o[i] = kahan ? double(s[i]) : d[i];
+    }
+};
+
 template <int LPR, class M, class V, class C, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q, const V* __restrict__ x,
                                                     const V* __restrict__ y, const V* __restrict__ xa,
@@ -152,9 +193,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
                                                     const V* __restrict__ l, const V* __restrict__ u,
                                                     const V* __restrict__ cres, int flags, ReduceSlot slot) {
     __shared__ double lds[(kBlock / kWave) * kNOutN];
-    double acc[kNOutN];
-#pragma unroll
-    for (int k = 0; k < kNOutN; ++k) acc[k] = 0.0;
+    CheckAcc<kNOutN> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
     constexpr int kGroups = kBlock / LPR;
     const int lane = threadIdx.x % LPR;
@@ -189,29 +228,33 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
             const double xj = double(x[j]);
             double r2, ob;
             dual_terms(cj - double(sc) + (HASQ ? double(qc) : 0.0), lj, uj, cr, r2, ob);
-            acc[0] += r2;
-            acc[1] += ob;
-            acc[2] += cj * xj;
-            if (HASQ) acc[3] += xj * double(qc);
+            acc.add(0, r2);
+            acc.add(1, ob);
+            acc.add(2, cj * xj);
+            if (HASQ) acc.add(3, xj * double(qc));
             if (avg) {
                 const double xaj = double(xa[j]), xrj = double(xr[j]);
                 dual_terms(cj - double(sa) + (HASQ ? double(qa) : 0.0), lj, uj, cr, r2, ob);
-                acc[4] += r2;
-                acc[5] += ob;
-                acc[6] += cj * xaj;
-                if (HASQ) acc[7] += xaj * double(qa);
+                acc.add(4, r2);
+                acc.add(5, ob);
+                acc.add(6, cj * xaj);
+                if (HASQ) acc.add(7, xaj * double(qa));
                 const double d0 = xj - xrj, d1 = xaj - xrj;
-                acc[10] += d0 * d0;
-                acc[11] += d1 * d1;
+                acc.add(10, d0 * d0);
+                acc.add(11, d1 * d1);
             }
             if (ray) {  // homogeneous dual: cw = 0, no Qx (src/pdhg_impl.hpp:114)
                 dual_terms(0.0 * cj - double(sr) + 0.0, lj, uj, cr, r2, ob);
-                acc[8] += r2;
-                acc[9] += ob;
+                acc.add(8, r2);
+                acc.add(9, ob);
             }
         }
     }
-    block_partial<kNOutN>(acc, slot, lds);
+    {
+        double tot[kNOutN];
+        acc.totals(tot);
+        block_partial<kNOutN>(tot, slot, lds);
+    }
 }
 
 // One pass over A: out[12..18] as documented for ganit_hip_pdhg_check (sums in C as for k_check_n).
@@ -222,9 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
                                                     const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                     ReduceSlot slot) {
     __shared__ double lds[(kBlock / kWave) * kNOutM];
-    double acc[kNOutM];
-#pragma unroll
-    for (int k = 0; k < kNOutM; ++k) acc[k] = 0.0;
+    CheckAcc<kNOutM> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
     constexpr int kGroups = kBlock / LPR;
     const int lane = threadIdx.x % LPR;
@@ -247,20 +288,24 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
                 const double r = (a - p) / rr;
                 return r * r;
             };
-            acc[0] += pres(double(sc));
-            acc[1] += row_obj_term(yi, lw, hg);
+            acc.add(0, pres(double(sc)));
+            acc.add(1, row_obj_term(yi, lw, hg));
             if (avg) {
                 const double yai = double(ya[i]), yri = double(yr[i]);
-                acc[2] += pres(double(sa));
-                acc[3] += row_obj_term(yai, lw, hg);
+                acc.add(2, pres(double(sa)));
+                acc.add(3, row_obj_term(yai, lw, hg));
                 const double d0 = yi - yri, d1 = yai - yri;
-                acc[5] += d0 * d0;
-                acc[6] += d1 * d1;
+                acc.add(5, d0 * d0);
+                acc.add(6, d1 * d1);
             }
-            if (ray) acc[4] += row_obj_term(double(V(y[i] - yr[i])), lw, hg);
+            if (ray) acc.add(4, row_obj_term(double(V(y[i] - yr[i])), lw, hg));
         }
     }
-    block_partial<kNOutM>(acc, slot, lds);
+    {
+        double tot[kNOutM];
+        acc.totals(tot);
+        block_partial<kNOutM>(tot, slot, lds);
+    }
 }
 
 // The two check passes on LDS-staged row blocks (grid = number of row blocks); sums in C as above.
@@ -275,9 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
                                                        ReduceSlot slot) {
     __shared__ C lds[kTile];
     __shared__ double red[(kBlock / kWave) * kNOutN];
-    double acc[kNOutN];
-#pragma unroll
-    for (int k = 0; k < kNOutN; ++k) acc[k] = 0.0;
+    CheckAcc<kNOutN> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
     stream_block_seq<M, C, 3>(
         AT, blocks, xcd_block_id(),
@@ -302,30 +345,34 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
             const double xj = double(x[j]);
             double r2, ob;
             dual_terms(cj - double(t[0]) + (HASQ ? double(qc) : 0.0), lj, uj, cr, r2, ob);
-            acc[0] += r2;
-            acc[1] += ob;
-            acc[2] += cj * xj;
-            if (HASQ) acc[3] += xj * double(qc);
+            acc.add(0, r2);
+            acc.add(1, ob);
+            acc.add(2, cj * xj);
+            if (HASQ) acc.add(3, xj * double(qc));
             if (avg) {
                 const double xaj = double(xa[j]), xrj = double(xr[j]);
                 dual_terms(cj - double(t[1]) + (HASQ ? double(qa) : 0.0), lj, uj, cr, r2, ob);
-                acc[4] += r2;
-                acc[5] += ob;
-                acc[6] += cj * xaj;
-                if (HASQ) acc[7] += xaj * double(qa);
+                acc.add(4, r2);
+                acc.add(5, ob);
+                acc.add(6, cj * xaj);
+                if (HASQ) acc.add(7, xaj * double(qa));
                 const double d0 = xj - xrj, d1 = xaj - xrj;
-                acc[10] += d0 * d0;
-                acc[11] += d1 * d1;
+                acc.add(10, d0 * d0);
+                acc.add(11, d1 * d1);
             }
             if (ray) {
                 dual_terms(0.0 * cj - double(t[2]) + 0.0, lj, uj, cr, r2, ob);
-                acc[8] += r2;
-                acc[9] += ob;
+                acc.add(8, r2);
+                acc.add(9, ob);
             }
         },
         lds, sp);
     __syncthreads();
-    block_partial<kNOutN>(acc, slot, red);
+    {
+        double tot[kNOutN];
+        acc.totals(tot);
+        block_partial<kNOutN>(tot, slot, red);
+    }
 }
 
 template <class M, class V, class C>
@@ -337,9 +384,7 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
                                                        ReduceSlot slot) {
     __shared__ C lds[kTile];
     __shared__ double red[(kBlock / kWave) * kNOutM];
-    double acc[kNOutM];
-#pragma unroll
-    for (int k = 0; k < kNOutM; ++k) acc[k] = 0.0;
+    CheckAcc<kNOutM> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
     stream_block_seq<M, C, 2>(
         A, blocks, xcd_block_id(), [&](int k, int j) { return k == 0 ? C(x[j]) : (avg ? C(xa[j]) : C(0)); },
@@ -350,21 +395,25 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
                 const double r = (a - p) / rr;
                 return r * r;
             };
-            acc[0] += pres(double(t[0]));
-            acc[1] += row_obj_term(yi, lw, hg);
+            acc.add(0, pres(double(t[0])));
+            acc.add(1, row_obj_term(yi, lw, hg));
             if (avg) {
                 const double yai = double(ya[i]), yri = double(yr[i]);
-                acc[2] += pres(double(t[1]));
-                acc[3] += row_obj_term(yai, lw, hg);
+                acc.add(2, pres(double(t[1])));
+                acc.add(3, row_obj_term(yai, lw, hg));
                 const double d0 = yi - yri, d1 = yai - yri;
-                acc[5] += d0 * d0;
-                acc[6] += d1 * d1;
+                acc.add(5, d0 * d0);
+                acc.add(6, d1 * d1);
             }
-            if (ray) acc[4] += row_obj_term(double(V(y[i] - yr[i])), lw, hg);
+            if (ray) acc.add(4, row_obj_term(double(V(y[i] - yr[i])), lw, hg));
         },
         lds, sp);
     __syncthreads();
-    block_partial<kNOutM>(acc, slot, red);
+    {
+        double tot[kNOutM];
+        acc.totals(tot);
+        block_partial<kNOutM>(tot, slot, red);
+    }
 }
 
 // Folds the per-block partials of both check passes: block 0 the A^T pass, block 1 the A pass.
@@ -816,7 +865,8 @@ int ganit_hip_pdhg_check(ganit_hip_pdhg* h, int mdt, int vdt, int flags, double 
     return guarded([&] {
         auto* pd = reinterpret_cast<Pdhg*>(h);
         GH_REQUIRE(pd && out, "null argument");
-        GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY | GANIT_CHECK_FP64)) == 0, "unknown check flag");
+        GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY | GANIT_CHECK_FP64 | GANIT_CHECK_KAHAN)) == 0,
+                   "unknown check flag");
         VecSet& s = pd->set(vdt);
         if ((flags & GANIT_CHECK_FP64) && vdt == GANIT_F32) {
             // FP64 evaluation of the FP32 iterate: form the average at FP32 (as the iteration would adopt

@@ -107,7 +107,10 @@ Exit phase(Shared& sh, ganit_hip_pdhg* pd, const Level& lv, State& st, double to
     chk(ganit_hip_pdhg_handover(pd, lv.vdt, lv.vdt));  // fresh running sums
     // checks of a reduced-precision level: at the level's precision (the reference's rule) or in FP64
     const bool e64 = opt.eval_fp64 && (lv.mdt != GANIT_F64 || lv.vdt != GANIT_F64);
-    const int ev_mdt = e64 ? GANIT_F64 : lv.mdt, ev_flags = e64 ? GANIT_CHECK_FP64 : 0;
+    // Kahan-compensated FP32 check reductions apply to L1's own-precision checks (as the oracle's kahan option)
+    const bool kahan = opt.kahan && !e64 && lv.vdt == GANIT_F32;
+    const int ev_mdt = e64 ? GANIT_F64 : lv.mdt;
+    const int ev_flags = (e64 ? GANIT_CHECK_FP64 : 0) | (kahan ? GANIT_CHECK_KAHAN : 0);
 
     KKT k_restart = evaluate(sh, pd, ev_mdt, lv.vdt, ev_flags);
     if (st.restart_err > 0) k_restart.err = std::max(k_restart.err, st.restart_err);

@@ -34,6 +34,8 @@ struct Options {
     bool eval_fp64 = false;       // L1/L2 checks (restart candidates, stall and convergence tests) evaluated
                                   // in FP64 with the FP64 matrix instead of at the level's own precision
     int device = 0;               // HIP device ordinal
+    bool kahan = false;           // L1 checks at FP32: FP32-rounded terms, Kahan-compensated FP32 sums per
+                                  // thread, FP64 across threads (instead of FP64 sums)
 };
 
 enum class Status { Optimal, PrimalInfeasible, IterationLimit, TimeLimit, NumericalError };

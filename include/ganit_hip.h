@@ -151,6 +151,8 @@ int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, doub
                               in FP64 -- widened into the FP64 working set and checked there against
                               the FP64 problem data with matrix values mdt (GANIT_F64 for a full
                               FP64 evaluation).  Requires the FP64 working set (vdt_mask bit 0). */
+#define GANIT_CHECK_KAHAN 8 /* sums formed from FP32-rounded terms with Kahan-compensated FP32 accumulation per
+                             * thread, combined in FP64 (extension; without it every sum is FP64) */
 #define GANIT_CHECK_NOUT 19
 /* out[ 0.. 3]  current point : rd2, vobj, c'x, x'Qx      (dual_res, dot_c, dot with Qx)
  * out[ 4.. 7]  average point : rd2, vobj, c'x, x'Qx

@@ -453,3 +453,30 @@ def test_handover_and_restart_semantics(hipctx):
     out = pd.check(BF16, F32, hip.CHECK_AVG, 1.0)
     assert out[10] == 0.0 and out[17] == 0.0      # iterate == restart point
     assert out[4] == 0 or np.isfinite(out[4])
+
+
+@pytest.mark.parametrize("name", ["25fv47", "random_lp"])
+def test_pdhg_check_sums_kahan_fp32(hipctx, name):
+    """GANIT_CHECK_KAHAN: the same 19 sums from FP32-rounded terms with Kahan-compensated FP32 accumulation,
+    against the FP64-accumulated sums of the same FP32 iterate: within a few FP32 roundings of sum |terms|."""
+    import ganit_v2_amd as g
+    from ganit_v2_amd import hip
+    p = g.Problem.generate("random_lp", seed=2) if name == "random_lp" else g.Problem.read_mps(instance_path(name))
+    s = p.scaled()
+    pd = hip.DevicePdhg(hipctx, s, mdt_mask=2, vdt_mask=2)
+    rng = np.random.default_rng(8)
+    m, n = s["m"], s["n"]
+    pd.load(F32, rng.standard_normal(n), rng.standard_normal(m), rng.standard_normal(n), rng.standard_normal(m))
+    pd.iterate(F32, F32, 4, 0.1, 0.2)
+    flags = hip.CHECK_AVG | hip.CHECK_RAY
+    # the check forms the average from the running sums; both calls below see the same sums and iterate
+    f64 = pd.check(F32, F32, flags, 0.25)
+    kah = pd.check(F32, F32, flags | hip.CHECK_KAHAN, 0.25)
+    xc, yc, xr, yr = pd.store(F32)
+    pd.adopt_average(F32)
+    xa, ya, _, _ = pd.store(F32)
+    terms = _check_terms(s, xc, yc, xa, ya, xr, yr)
+    for k in range(19):
+        scale = math.fsum(np.abs(terms[k])) + 1e-30
+        assert abs(kah[k] - f64[k]) <= 1e-6 * scale, (k, kah[k], f64[k], scale)
+    assert not np.array_equal(kah, f64)  # the FP32 path is really taken

@@ -189,3 +189,18 @@ def test_cli_json_line(tmp_path):
     assert lines[0] == "# status OPTIMAL" and len(lines) == 2 + 32
     p = subprocess.run([exe, instance_path("bgetam"), "--quiet"], stdout=subprocess.PIPE, timeout=120)
     assert p.returncode == 2  # not optimal
+
+
+@pytest.mark.parametrize("name", ["afiro", "adlittle", "25fv47", "random_lp"])
+def test_kahan_ladder_matches_oracle_kahan(oracle, name):
+    """The Kahan-compensated FP32 check reductions in L1 (--kahan): same status as the oracle's kahan mode and
+    as the plain mixed run, FP64-certified, objective within a tolerance-sized bound of FP64."""
+    p = g.Problem.generate("random_lp", seed=0) if name == "random_lp" else g.Problem.read_mps(instance_path(name))
+    r = p.solve_gpu(precision="mixed", kahan=True, time_limit=120)
+    plain = p.solve_gpu(precision="mixed", time_limit=120)
+    oracle.set_threads(4)
+    o = oracle.OracleLP.from_export(p.export()).solve(precision="mixed", kahan=1, time_limit=120)
+    assert r["status"] == plain["status"] == o["status"]
+    if r["status"] == "OPTIMAL":
+        assert _certified(r, 1e-4)
+        assert abs(r["pobj"] - o["pobj"]) <= 5e-4 * (1 + abs(o["pobj"]))
