@@ -463,9 +463,7 @@ void bt_partition(Batch* bt, int rows, int nnz) {
         }
         if (a->blocks) GH_CHECK(hipFree(a->blocks));
         a->blocks = nullptr;
-        GH_CHECK(hipMalloc(&a->blocks, sizeof(int) * blocks.size()));
-        GH_CHECK(hipMemcpyAsync(a->blocks, blocks.data(), sizeof(int) * blocks.size(), hipMemcpyHostToDevice,
-                                ctx->stream));
+        a->blocks = upload_row_blocks(ctx, ptr.data(), blocks);
         a->nblocks = (int)blocks.size() - 1;
         a->lanes = 0;
         a->rb_rows = rows;

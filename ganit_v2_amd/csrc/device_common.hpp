@@ -171,9 +171,11 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
         }
         return;
     }
-    const int r0 = blocks[b], r1 = blocks[b + 1];
-    const int base = a.ptr[r0];
-    const int cnt = a.ptr[r1] - base;
+    // block descriptor: first row, end row, first nonzero, end nonzero (one 16-byte load)
+    const int4 d = reinterpret_cast<const int4*>(blocks)[b];
+    const int r0 = d.x, r1 = d.y;
+    const int base = d.z;
+    const int cnt = d.w - d.z;
     if (r1 - r0 == 1 && cnt > kSplit) return;  // a split row: its chunks do the work
     if (r1 - r0 == 1 && cnt > kLong) {
         V s[NX];
@@ -247,8 +249,10 @@ __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int*
                                                  Gather gather, RowFn row_fn, V* lds /* kTile */,
                                                  const SplitPlan& sp) {
     static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
-    if (sp.finish || b >= sp.nblocks ||
-        (blocks[b + 1] - blocks[b] == 1 && a.ptr[blocks[b + 1]] - a.ptr[blocks[b]] > kLong)) {
+    if (sp.finish || b >= sp.nblocks || (reinterpret_cast<const int4*>(blocks)[b].y -
+                                                 reinterpret_cast<const int4*>(blocks)[b].x == 1 &&
+                                             reinterpret_cast<const int4*>(blocks)[b].w -
+                                                 reinterpret_cast<const int4*>(blocks)[b].z > kLong)) {
         stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
         return;
     }

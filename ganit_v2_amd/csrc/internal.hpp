@@ -80,7 +80,8 @@ struct Csr {
     bf16_t* v16 = nullptr;
     int lanes = 8;           // row-group variant: lanes per row (1..64); 0 selects the LDS-staged row blocks
     int64_t hist[9] = {0};
-    int* blocks = nullptr;   // row-block starts of the LDS-staged variant (nblocks + 1 entries, device)
+    int* blocks = nullptr;   // row-block descriptors of the LDS-staged variant: 4 ints per block (first row,
+                             // end row, first nonzero, end nonzero), device; see upload_row_blocks
     int nblocks = 0;
     int rb_rows = kBlock, rb_nnz = kTile;  // row-block caps the partition was built with
     // rows longer than kSplit nonzeros, cut into kChunk chunks (device arrays; see SplitPlan)
@@ -117,6 +118,8 @@ void csr_destroy(Csr* a);
 int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz);
 bool valid_variant(int lanes);
 std::vector<int> row_blocks(const int* ptr, int rows, int max_rows = kBlock, int max_nnz = kTile);
+// Device array of block descriptors {r0, r1, ptr[r0], ptr[r1]} for the row-block boundaries rb (host ptr).
+int* upload_row_blocks(Ctx* ctx, const int* ptr, const std::vector<int>& rb);
 // Rebuild the row-block partition with at most `rows` rows and `nnz` nonzeros per block.
 void csr_set_partition(Ctx* ctx, Csr* a, int rows, int nnz);
 // Candidate variants: (lanes, rb_rows, rb_nnz); lanes 0 = row blocks with that partition.

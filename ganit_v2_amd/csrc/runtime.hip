@@ -232,6 +232,22 @@ std::vector<int> row_blocks(const int* ptr, int rows, int max_rows, int max_nnz)
     return b;
 }
 
+int* upload_row_blocks(Ctx* ctx, const int* ptr, const std::vector<int>& rb) {
+    const size_t nb = rb.empty() ? 0 : rb.size() - 1;
+    std::vector<int> d(4 * std::max<size_t>(nb, 1), 0);
+    for (size_t q = 0; q < nb; ++q) {
+        d[4 * q + 0] = rb[q];
+        d[4 * q + 1] = rb[q + 1];
+        d[4 * q + 2] = ptr[rb[q]];
+        d[4 * q + 3] = ptr[rb[q + 1]];
+    }
+    int* dev = nullptr;
+    GH_CHECK(hipMalloc(&dev, sizeof(int) * d.size()));
+    GH_CHECK(hipMemcpyAsync(dev, d.data(), sizeof(int) * d.size(), hipMemcpyHostToDevice, ctx->stream));
+    GH_CHECK(hipStreamSynchronize(ctx->stream));
+    return dev;
+}
+
 Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, const double* val, unsigned mask) {
     GH_REQUIRE(rows >= 0 && cols >= 0, "negative matrix dimension");
     GH_REQUIRE(rows == 0 || ptr, "null row pointer");
@@ -255,8 +271,7 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
         a->lanes = choose_lanes_from_lengths(len, a->nnz);
         const std::vector<int> rb = row_blocks(ptr, rows);
         a->nblocks = (int)rb.size() - 1;
-        GH_CHECK(hipMalloc(&a->blocks, sizeof(int) * rb.size()));
-        GH_CHECK(hipMemcpyAsync(a->blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, ctx->stream));
+        a->blocks = upload_row_blocks(ctx, ptr, rb);
         {  // chunks of the rows longer than kSplit
             std::vector<int> ch, first{0};
             for (int i = 0; i < rows; ++i) {
@@ -356,10 +371,7 @@ void csr_set_partition(Ctx* ctx, Csr* a, int rows, int nnz) {
     GH_CHECK(hipMemcpyAsync(ptr.data(), a->ptr, sizeof(int) * ptr.size(), hipMemcpyDeviceToHost, ctx->stream));
     GH_CHECK(hipStreamSynchronize(ctx->stream));
     const std::vector<int> rb = row_blocks(ptr.data(), a->rows, rows, nnz);
-    int* blocks = nullptr;
-    GH_CHECK(hipMalloc(&blocks, sizeof(int) * rb.size()));
-    GH_CHECK(hipMemcpyAsync(blocks, rb.data(), sizeof(int) * rb.size(), hipMemcpyHostToDevice, ctx->stream));
-    GH_CHECK(hipStreamSynchronize(ctx->stream));
+    int* blocks = upload_row_blocks(ctx, ptr.data(), rb);
     GH_CHECK(hipFree(a->blocks));
     a->blocks = blocks;
     a->nblocks = (int)rb.size() - 1;
