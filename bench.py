@@ -69,6 +69,103 @@ def cpu_baseline(problem, gpu_run, sample_s):
     }
 
 
+def portfolio_ladder():
+    """Factor-model portfolio QP (100,000 assets x 50 factors, tol 1e-6): FP64 vs the FP32-matrix ladder, each
+    FP64-certified; one timed solve after a warm-up, problem resident."""
+    import ganit_v2_amd as g
+    p = g.Problem.generate("portfolio")
+    d = p.dims()
+    out = dict(workload="portfolio QP 100000 assets x 50 factors, seed 0, tol 1e-6", m=d["m"], n=d["n"], nnz=d["nnz"])
+    runs = {}
+    for name, cfg in (("fp64", dict(precision="double")), ("mixed", dict(precision="mixed")),
+                      ("dynamic", dict(precision="dynamic"))):
+        s = p.solver(tol=1e-6, time_limit=120, **cfg)
+        s.upload()
+        s.run()
+        r = s.run()
+        runs[name] = dict(solve_s=round(r["solve_s"], 5), status=r["status"], iters=r["iters"],
+                          l1_iters=r["fp32_iters"], obj=r["pobj"],
+                          certified_rel=[r["rel_p"], r["rel_d"], r["rel_gap"]])
+        print(f"[bench] portfolio {name}: {runs[name]}", file=sys.stderr, flush=True)
+    f = runs["fp64"]
+    for name in ("mixed", "dynamic"):
+        runs[name]["speedup_vs_fp64"] = round(f["solve_s"] / runs[name]["solve_s"], 3)
+        runs[name]["obj_rel_err_vs_fp64"] = abs(runs[name]["obj"] - f["obj"]) / (1 + abs(f["obj"]))
+    out["runs"] = runs
+    return out
+
+
+def batched_lps(count=512):
+    """512 uneven random LPs (node-relaxation shapes): the batched kernels (FP64, and FP32 + FP64
+    certification) against sequential single-instance FP64 solves with each problem resident."""
+    import numpy as np
+    import ganit_v2_amd as g
+    probs = [g.Problem.generate("batch_lp", seed=k) for k in range(count)]
+    out = dict(workload=f"{count} random LPs, m log-U[200,4000], n log-U[500,10000], 5-15 nonzeros per column, "
+                        f"5% tightened bounds, seeds 0..{count - 1}, tol 1e-4")
+    res = {}
+    for name, prec in (("batched_fp64", "double"), ("batched_mixed", "mixed")):
+        b = g.Batch(probs, precision=prec, tol=1e-4)
+        b.upload()
+        b.run()
+        t = time.perf_counter()
+        rs = b.run()
+        wall = time.perf_counter() - t
+        st = b.stats()
+        res[name] = dict(solve_s=round(st["solve_s"], 4), wall_s=round(wall, 4), max_iters=st["max_iters"],
+                         total_iters=st["total_iters"], l3_instances=st["l3_instances"],
+                         statuses={k: sum(1 for r in rs if r["status"] == k) for k in {r["status"] for r in rs}})
+        res[name + "_results"] = rs
+        print(f"[bench] batched {name}: {res[name]}", file=sys.stderr, flush=True)
+    seq, tsum = [], 0.0
+    t = time.perf_counter()
+    for p in probs:
+        s = p.solver(precision="double", tol=1e-4)
+        s.upload()
+        r = s.run()
+        seq.append(r)
+        tsum += r["solve_s"]
+    wall = time.perf_counter() - t
+    res["sequential_fp64"] = dict(solve_s_sum=round(tsum, 4), wall_s=round(wall, 3))
+    for name in ("batched_fp64", "batched_mixed"):
+        rs = res.pop(name + "_results")
+        errs = [abs(a["pobj"] - b["pobj"]) / (1 + abs(b["pobj"])) for a, b in zip(rs, seq)
+                if a["status"] == b["status"] == "OPTIMAL"]
+        res[name]["status_mismatches"] = sum(1 for a, b in zip(rs, seq) if a["status"] != b["status"])
+        res[name]["same_iterations"] = sum(1 for a, b in zip(rs, seq) if a["iters"] == b["iters"])
+        res[name]["obj_rel_err_vs_sequential_fp64"] = dict(max=max(errs), median=float(np.median(errs)))
+        res[name]["speedup_vs_sequential_fp64"] = round(tsum / res[name]["solve_s"], 2)
+    print(f"[bench] batched summary: {res}", file=sys.stderr, flush=True)
+    out.update(res)
+    return out
+
+
+def diff_qps(count):
+    """Differentiable QPs (n=2000, 300 equality + 200 <= rows): FP32 forward (mixed ladder, tol 1e-6) + FP32
+    adjoint, and FP64 forward (tol 1e-9) + FP64 adjoint; gradient errors against the FP64 one."""
+    import numpy as np
+    import ganit_v2_amd as g
+    rows = []
+    for seed in range(count):
+        p = g.Problem.generate("diff_qp", seed=seed)
+        w = np.random.default_rng(1000 + seed).normal(size=p.dims()["n"])
+        s32 = p.solver(precision="mixed", tol=1e-6, time_limit=300)
+        s32.upload()
+        r32 = s32.run(return_solution=True)
+        a32 = s32.adjoint(w, precision="single", tol=1e-5)
+        s64 = p.solver(precision="double", tol=1e-9, time_limit=300)
+        s64.upload()
+        r64 = s64.run(return_solution=True)
+        a64 = s64.adjoint(w, tol=1e-10)
+        rel = lambda u, v: float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300))
+        rows.append(dict(seed=seed, fwd32_s=round(r32["solve_s"], 3), fwd32_iters=r32["iters"],
+                         adj32_s=round(a32["seconds"], 3), fwd64_s=round(r64["solve_s"], 3),
+                         adj64_s=round(a64["seconds"], 3), adj64_iters=a64["iters"],
+                         grad_err_fp32={k: rel(a32[k], a64[k]) for k in ("dc", "db", "dq")}))
+        print(f"[bench] diff_qp {rows[-1]}", file=sys.stderr, flush=True)
+    return dict(workload=f"{count} differentiable QPs n=2000, 300 E + 200 L rows, seeds 0..{count - 1}", rows=rows)
+
+
 def kernels_only(prob, staged, a, m, n, nnz):
     """Back-to-back launches of the fused kernels at L1 and L3 (HIP events on the solver's stream), then one
     torch copy of 1 GiB: a plain copy kernel of known size whose FETCH_SIZE / WRITE_SIZE calibrate the
@@ -107,6 +204,10 @@ def main():
                     help="checks without 20%% progress before leaving L1 (the reference's default is 12)")
     ap.add_argument("--eval", default="fp64", choices=["fp64", "level"],
                     help="evaluate L1 restart candidates in FP64 (default) or at L1 precision")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the portfolio-QP ladder and the batched-LP comparison reported beside the headline")
+    ap.add_argument("--c4", type=int, default=0,
+                    help="also run N differentiable QPs (forward + adjoint) and report their times and errors")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the fused iteration kernels (the launches the roofline figure uses), plus one "
                          "plain 1 GiB device copy as the counter calibration; for profiler runs")
@@ -192,6 +293,13 @@ def main():
         "traffic": (traffic or {}).get(f"k_iter_{dom}_{level_dom}"),
     }
 
+    extras = {}
+    if not a.no_extras:
+        extras["portfolio_qp"] = portfolio_ladder()
+        extras["batched_lp"] = batched_lps()
+    if a.c4 > 0:
+        extras["diff_qp"] = diff_qps(a.c4)
+
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(prob, last, a.cpu_sample_s)
@@ -223,6 +331,7 @@ def main():
         "kernels": kern,
         "roofline": roof,
         "cpu_baseline": cpu,
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
 
