@@ -386,6 +386,7 @@ int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* h, int mdt, int vdt, const double* gx
                     const double bnorm = std::sqrt(hh[2 * k + 2]);
                     std::vector<double> H((size_t)k * k), cs(k), sn(k), sv(k + 1), y(k);
                     rel = bnorm > 0 ? INFINITY : 0.0;
+                    double prev_beta = INFINITY;  // true residual at the previous restart
                     while (bnorm > 0 && iters < max_iter) {
                         // r0 = b - (I - J^T) lambda, into basis column 0
                         apply(lx, W, Fa{}, T{});
@@ -395,6 +396,9 @@ int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* h, int mdt, int vdt, const double* gx
                         const double beta = std::sqrt(hh[2 * k + 2]);
                         rel = beta / bnorm;
                         if (!std::isfinite(rel) || rel <= tol) break;
+                        // a whole restart cycle that gained less than 1% has met the working precision's floor
+                        if (beta > 0.99 * prev_beta) break;
+                        prev_beta = beta;
                         hipLaunchKernelGGL(k_gm_scale<V>, dim3(gl), dim3(kBlock), 0, st, L, Vb, nr2, Vb);
                         std::fill(sv.begin(), sv.end(), 0.0);
                         sv[0] = beta;

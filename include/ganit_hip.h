@@ -218,7 +218,8 @@ int ganit_hip_batch_store(ganit_hip_batch* bt, int vdt, double* x, double* y);
  * and working precision vdt, with steps (tau, sigma).  params = {method, max_iter, tol, every, mask_tol}:
  * method 0 iterates lambda <- J^T lambda + (gx, 0) and stops when ||lambda' - lambda|| / ||lambda'|| <= tol
  * (checked every `every` iterations); method 1 runs restarted GMRES(every) on (I - J^T) lambda = (gx, 0)
- * (the same operator, Krylov-accelerated) until ||residual|| / ||gx|| <= tol.  Outputs
+ * (the same operator, Krylov-accelerated) until ||residual|| / ||gx|| <= tol, or until a restart cycle lowers the
+ * residual by less than 1% (the working precision's floor).  Outputs
  * p = D o (lx - 2 sigma A^T (F o ly)) (n) and mu = F o ly (m), from which dL/dc = -tau p,
  * dL/dQ_jj = -tau p_j x_j and dL/d(active bound of row i) = sigma mu_i (scaled space).
  * info[0..3] = iterations (operator applications), final relative change / residual, free variables
