@@ -253,7 +253,7 @@ def main():
           f"iters {f_last['iters']}", file=sys.stderr, flush=True)
     # other ladder configurations, one timed solve each after a warm-up (reported, not the headline)
     ladder = []
-    for cfg in (dict(precision="mixed"), dict(precision="dynamic"),
+    for cfg in (dict(precision="mixed"), dict(precision="dynamic"), dict(precision="mixed", kahan=True),
                 dict(precision="mixed", values="bf16", stall_checks=a.stall_checks, eval_fp64=True)):
         sv = staged(**cfg)
         sv.run()

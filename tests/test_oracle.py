@@ -216,3 +216,15 @@ def test_oracle_objective_matches_reference_simplex(oracle, name):
     r = oracle.OracleLP.read_mps(instance_path(name)).solve(tol=1e-8, time_limit=60)
     assert r["status"] == "OPTIMAL"
     assert abs(r["pobj"] - simplex["pobj"]) <= 1e-6 * (1 + abs(simplex["pobj"]))
+
+
+@pytest.mark.parametrize("name", ["afiro", "adlittle"])
+def test_oracle_kahan_mode_reaches_the_same_answer(oracle, name):
+    """The oracle's Kahan-compensated FP32 reductions in L1 (an extension the GPU option is checked against):
+    same status and FP64-certified objective as the FP64-accumulating mixed run, within tolerance."""
+    oracle.set_threads(1)
+    lp = oracle.OracleLP.read_mps(instance_path(name))
+    a = lp.solve(precision="mixed", kahan=0, time_limit=60)
+    b = lp.solve(precision="mixed", kahan=1, time_limit=60)
+    assert a["status"] == b["status"] == "OPTIMAL"
+    assert abs(a["pobj"] - b["pobj"]) <= 5e-4 * (1 + abs(a["pobj"]))
