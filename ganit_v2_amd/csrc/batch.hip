@@ -367,7 +367,7 @@ struct Batch {
     int* running = nullptr;
     int* host_running = nullptr;  // pinned
     // host copies for re-partitioning (autotuner)
-    std::vector<int> aptr_h, atptr_h;
+    std::vector<int> aptr_h, atptr_h, aidx_h, atidx_h;
     std::vector<BatchInst> inst_h;
     int part_rows = kBlock, part_nnz = kTile;
     size_t part_n_cap = 0, part_m_cap = 0;
@@ -444,8 +444,8 @@ std::vector<char> bt_active(Batch* bt) {
 void bt_partition(Batch* bt, int rows, int nnz) {
     Ctx* ctx = bt->ctx;
     const int ninst = bt->ninst;
-    auto build = [&](Csr* a, const std::vector<int>& ptr, bool rows_of_a, int** blk_inst, int** first,
-                     std::vector<int>& first_h, int** list) {
+    auto build = [&](Csr* a, const std::vector<int>& ptr, const std::vector<int>& idx, bool rows_of_a,
+                     int** blk_inst, int** first, std::vector<int>& first_h, int** list) {
         std::vector<int> blocks{0}, bi, fi{0};
         std::vector<int> lp;
         for (int k = 0; k < ninst; ++k) {
@@ -464,6 +464,7 @@ void bt_partition(Batch* bt, int rows, int nnz) {
         if (a->blocks) GH_CHECK(hipFree(a->blocks));
         a->blocks = nullptr;
         a->blocks = upload_row_blocks(ctx, ptr.data(), blocks);
+        build_compact_indices(ctx, a, ptr.data(), idx.data(), blocks);
         a->nblocks = (int)blocks.size() - 1;
         a->lanes = 0;
         a->rb_rows = rows;
@@ -486,8 +487,9 @@ void bt_partition(Batch* bt, int rows, int nnz) {
     };
     GH_CHECK(hipStreamSynchronize(ctx->stream));
     bt->drop_graphs();
-    build(bt->A, bt->aptr_h, true, &bt->blk_inst_a, &bt->first_a, bt->first_a_h, &bt->list_a);
-    build(bt->AT, bt->atptr_h, false, &bt->blk_inst_at, &bt->first_at, bt->first_at_h, &bt->list_at);
+    build(bt->A, bt->aptr_h, bt->aidx_h, true, &bt->blk_inst_a, &bt->first_a, bt->first_a_h, &bt->list_a);
+    build(bt->AT, bt->atptr_h, bt->atidx_h, false, &bt->blk_inst_at, &bt->first_at, bt->first_at_h,
+          &bt->list_at);
     bt->listed.assign(ninst, 0);
     bt_list(bt, std::vector<char>(ninst, 1));
     const size_t need_n = (size_t)kBN * std::max(bt->AT->nblocks, 1), need_m = (size_t)kBM * std::max(bt->A->nblocks, 1);
@@ -620,6 +622,8 @@ int ganit_hip_batch_create(ganit_hip_ctx* h, int ninst, const ganit_hip_lp_desc*
             bt->AT = csr_create(ctx, bt->N, bt->Mr, atptr.data(), atidx.data(), atval.data(), mdt_mask);
             bt->aptr_h = std::move(aptr);
             bt->atptr_h = std::move(atptr);
+            bt->aidx_h = std::move(aidx);
+            bt->atidx_h = std::move(atidx);
             bt->inst_h = inst;
             bt_partition(bt, kBlock, kTile);
             bt->col_inst = (int*)bt->alloc(sizeof(int) * std::max<size_t>(col_inst.size(), 1));

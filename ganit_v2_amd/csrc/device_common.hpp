@@ -49,6 +49,9 @@ struct CsrView {
     const int* __restrict__ ptr;
     const int* __restrict__ idx;
     const M* __restrict__ val;
+    // optional compact indices for the LDS-staged row blocks: idx = bcol[block] + idx16 (null: use idx)
+    const uint16_t* __restrict__ idx16 = nullptr;
+    const int* __restrict__ bcol = nullptr;
 };
 
 // ---------------------------------------------------------------- block numbering
@@ -171,8 +174,11 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
         }
         return;
     }
-    // block descriptor: first row, end row, first nonzero, end nonzero (one 16-byte load)
+    // block descriptor: first row, end row, first nonzero, end nonzero (one 16-byte load); with compact
+    // indices, the block's base column (an independent load, in flight at the same time)
     const int4 d = reinterpret_cast<const int4*>(blocks)[b];
+    const uint16_t* __restrict__ i16 = a.idx16;
+    const int cb = i16 ? a.bcol[b] : 0;
     const int r0 = d.x, r1 = d.y;
     const int base = d.z;
     const int cnt = d.w - d.z;
@@ -209,7 +215,7 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
 #pragma unroll 4
     for (int q = threadIdx.x; q < cnt; q += kBlock) {
         const V av = widen<V>(a.val[base + q]);
-        const int i = a.idx[base + q];
+        const int i = i16 ? cb + (int)i16[base + q] : a.idx[base + q];
 #pragma unroll
         for (int k = 0; k < NX; ++k) lds[k * kTile + q] = av * gather(k, i);
     }

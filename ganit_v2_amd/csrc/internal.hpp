@@ -80,6 +80,8 @@ struct Csr {
     bf16_t* v16 = nullptr;
     int lanes = 8;           // row-group variant: lanes per row (1..64); 0 selects the LDS-staged row blocks
     int64_t hist[9] = {0};
+    uint16_t* idx16 = nullptr;  // compact indices of the staged row blocks (null when a block spans >= 2^16
+    int* bcol = nullptr;        // columns); bcol[b] = block b's smallest column
     int* blocks = nullptr;   // row-block descriptors of the LDS-staged variant: 4 ints per block (first row,
                              // end row, first nonzero, end nonzero), device; see upload_row_blocks
     int nblocks = 0;
@@ -100,17 +102,17 @@ struct Csr {
 template <>
 inline CsrView<double> Csr::view<double>() const {
     GH_REQUIRE(rows == 0 || v64, "matrix has no FP64 values resident");
-    return {rows, ptr, idx, v64};
+    return {rows, ptr, idx, v64, idx16, bcol};
 }
 template <>
 inline CsrView<float> Csr::view<float>() const {
     GH_REQUIRE(rows == 0 || v32, "matrix has no FP32 values resident");
-    return {rows, ptr, idx, v32};
+    return {rows, ptr, idx, v32, idx16, bcol};
 }
 template <>
 inline CsrView<bf16_t> Csr::view<bf16_t>() const {
     GH_REQUIRE(rows == 0 || v16, "matrix has no BF16 values resident");
-    return {rows, ptr, idx, v16};
+    return {rows, ptr, idx, v16, idx16, bcol};
 }
 
 Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, const double* val, unsigned mask);
@@ -120,6 +122,9 @@ bool valid_variant(int lanes);
 std::vector<int> row_blocks(const int* ptr, int rows, int max_rows = kBlock, int max_nnz = kTile);
 // Device array of block descriptors {r0, r1, ptr[r0], ptr[r1]} for the row-block boundaries rb (host ptr).
 int* upload_row_blocks(Ctx* ctx, const int* ptr, const std::vector<int>& rb);
+// (Re)build a->idx16 / a->bcol for the partition rb when every staged block spans fewer than 2^16 columns;
+// otherwise leave them null (the kernels then read the 32-bit indices).
+void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, const std::vector<int>& rb);
 // Rebuild the row-block partition with at most `rows` rows and `nnz` nonzeros per block.
 void csr_set_partition(Ctx* ctx, Csr* a, int rows, int nnz);
 // Candidate variants: (lanes, rb_rows, rb_nnz); lanes 0 = row blocks with that partition.

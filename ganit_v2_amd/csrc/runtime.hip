@@ -248,6 +248,35 @@ int* upload_row_blocks(Ctx* ctx, const int* ptr, const std::vector<int>& rb) {
     return dev;
 }
 
+void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, const std::vector<int>& rb) {
+    if (a->idx16) GH_CHECK(hipFree(a->idx16));
+    if (a->bcol) GH_CHECK(hipFree(a->bcol));
+    a->idx16 = nullptr;
+    a->bcol = nullptr;
+    const char* env = std::getenv("GANIT_HIP_COMPACT_IDX");  // 0: keep the 32-bit indices only
+    if ((env && env[0] == '0') || a->nnz == 0 || rb.size() < 2) return;
+    const size_t nb = rb.size() - 1;
+    std::vector<int> bc(nb, 0);
+    std::vector<uint16_t> c16((size_t)a->nnz, 0);
+    for (size_t q = 0; q < nb; ++q) {
+        const int r0 = rb[q], r1 = rb[q + 1], p0 = ptr[r0], p1 = ptr[r1];
+        if ((r1 - r0 == 1 && p1 - p0 > kLong) || p1 == p0) continue;  // long and split rows: 32-bit indices
+        int mn = idx[p0], mx = idx[p0];
+        for (int p = p0 + 1; p < p1; ++p) {
+            mn = std::min(mn, idx[p]);
+            mx = std::max(mx, idx[p]);
+        }
+        if (mx - mn > 65535) return;  // a block too wide: keep the 32-bit path for this matrix
+        bc[q] = mn;
+        for (int p = p0; p < p1; ++p) c16[p] = (uint16_t)(idx[p] - mn);
+    }
+    GH_CHECK(hipMalloc(&a->idx16, sizeof(uint16_t) * c16.size()));
+    GH_CHECK(hipMalloc(&a->bcol, sizeof(int) * bc.size()));
+    GH_CHECK(hipMemcpyAsync(a->idx16, c16.data(), sizeof(uint16_t) * c16.size(), hipMemcpyHostToDevice, ctx->stream));
+    GH_CHECK(hipMemcpyAsync(a->bcol, bc.data(), sizeof(int) * bc.size(), hipMemcpyHostToDevice, ctx->stream));
+    GH_CHECK(hipStreamSynchronize(ctx->stream));
+}
+
 Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, const double* val, unsigned mask) {
     GH_REQUIRE(rows >= 0 && cols >= 0, "negative matrix dimension");
     GH_REQUIRE(rows == 0 || ptr, "null row pointer");
@@ -272,6 +301,7 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
         const std::vector<int> rb = row_blocks(ptr, rows);
         a->nblocks = (int)rb.size() - 1;
         a->blocks = upload_row_blocks(ctx, ptr, rb);
+        if (a->nnz) build_compact_indices(ctx, a, ptr, idx, rb);
         {  // chunks of the rows longer than kSplit
             std::vector<int> ch, first{0};
             for (int i = 0; i < rows; ++i) {
@@ -334,6 +364,8 @@ void csr_destroy(Csr* a) {
     (void)hipFree(a->v32);
     (void)hipFree(a->v16);
     (void)hipFree(a->blocks);
+    (void)hipFree(a->idx16);
+    (void)hipFree(a->bcol);
     (void)hipFree(a->chunk);
     (void)hipFree(a->split_first);
     (void)hipFree(a->split_part);
@@ -374,6 +406,12 @@ void csr_set_partition(Ctx* ctx, Csr* a, int rows, int nnz) {
     int* blocks = upload_row_blocks(ctx, ptr.data(), rb);
     GH_CHECK(hipFree(a->blocks));
     a->blocks = blocks;
+    std::vector<int> idx((size_t)a->nnz);
+    if (a->nnz) {
+        GH_CHECK(hipMemcpyAsync(idx.data(), a->idx, sizeof(int) * idx.size(), hipMemcpyDeviceToHost, ctx->stream));
+        GH_CHECK(hipStreamSynchronize(ctx->stream));
+    }
+    build_compact_indices(ctx, a, ptr.data(), idx.data(), rb);
     a->nblocks = (int)rb.size() - 1;
     a->rb_rows = rows;
     a->rb_nnz = nnz;
