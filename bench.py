@@ -21,12 +21,13 @@ PEAK_HBM_GBS = 8000.0
 WORKLOAD = dict(crudes=60, products=16, specs=5, periods=365)
 
 
-def kernel_bytes(m, n, nnz, s_m, s_v):
-    """Algorithmic bytes per launch of the two fused iteration kernels.
+def kernel_bytes(m, n, nnz, s_m, s_v, ib_at=4, ib_a=4):
+    """Algorithmic bytes per launch of the two fused iteration kernels, with the column-index width each
+    matrix is read with (2 with compact indices, 4 otherwise).
     primal: A^T (row pointers + nnz index/value) + gathered y + x, c, l, u, sumx read + xn, xbar, sumx written
     dual:   A   (row pointers + nnz index/value) + gathered xbar + y, lo, hi, sumy read + yn, sumy written"""
-    primal = 4 * (n + 1) + nnz * (4 + s_m) + m * s_v + 8 * n * s_v
-    dual = 4 * (m + 1) + nnz * (4 + s_m) + n * s_v + 6 * m * s_v
+    primal = 4 * (n + 1) + nnz * (ib_at + s_m) + m * s_v + 8 * n * s_v
+    dual = 4 * (m + 1) + nnz * (ib_a + s_m) + n * s_v + 6 * m * s_v
     return primal, dual
 
 
@@ -176,7 +177,7 @@ def kernels_only(prob, staged, a, m, n, nnz):
     for level, (sm, sv, sv_obj) in {"L1": (s_m_low, 4, staged("mixed", values=a.values)),
                                     "L3": (8, 8, staged("double"))}.items():
         ms_p, ms_d = sv_obj.kernel_times(0 if level == "L1" else 2, a.reps)
-        bp, bd = kernel_bytes(m, n, nnz, sm, sv)
+        bp, bd = kernel_bytes(m, n, nnz, sm, sv, sv_obj.index_bytes(1), sv_obj.index_bytes(0))
         out[level] = dict(primal_us=ms_p * 1e3, dual_us=ms_d * 1e3, primal_gbs=bp / (ms_p * 1e6),
                           dual_gbs=bd / (ms_d * 1e6), primal_bytes=bp, dual_bytes=bd)
         del sv_obj
@@ -273,9 +274,10 @@ def main():
         if level == "L1" and a.precision == "double":
             continue
         ms_p, ms_d = solver.kernel_times(0 if level == "L1" else 2, a.reps)
-        bp, bd = kernel_bytes(m, n, nnz, sm, sv)
+        ib_at, ib_a = solver.index_bytes(1), solver.index_bytes(0)
+        bp, bd = kernel_bytes(m, n, nnz, sm, sv, ib_at, ib_a)
         kern[level] = dict(primal_ms=ms_p, dual_ms=ms_d, primal_gbs=bp / (ms_p * 1e6), dual_gbs=bd / (ms_d * 1e6),
-                           primal_bytes=bp, dual_bytes=bd,
+                           primal_bytes=bp, dual_bytes=bd, index_bytes_at=ib_at, index_bytes_a=ib_a,
                            iteration_us=(ms_p + ms_d) * 1e3,
                            ref_sequence_gbs=reference_sequence_bytes(m, n, nnz, sm, sv) / ((ms_p + ms_d) * 1e6))
     print(f"[bench] kernels {json.dumps(kern)}", file=sys.stderr, flush=True)

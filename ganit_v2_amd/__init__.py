@@ -71,6 +71,7 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_solver_run.argtypes = [_vp, _f64p, _vp, _vp]
     lib.ganit_solver_kernel_times.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p]
     lib.ganit_solver_free.argtypes = [_vp]
+    lib.ganit_solver_index_bytes.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.ganit_solver_adjoint.argtypes = [_vp, _f64p, _f64p, _vp, _vp, _vp, _vp]
     lib.ganit_batch_create.restype = _vp
     lib.ganit_batch_create.argtypes = [_vp, ctypes.c_int, _f64p]
@@ -305,6 +306,12 @@ class Solver:
                     "solver_adjoint")
         return dict(dc=dc, db=db, dq=dq, iters=int(info[0]), rel_change=float(info[1]), free_vars=int(info[2]),
                     active_rows=int(info[3]), seconds=float(info[4]))
+
+    def index_bytes(self, which: int) -> int:
+        """Bytes per column index the fused kernels read for A (0), A^T (1), Q (2): 2 (compact) or 4."""
+        b = ctypes.c_int(4)
+        _check_host(host_lib().ganit_solver_index_bytes(self._h, int(which), ctypes.byref(b)), "index_bytes")
+        return int(b.value)
 
     def kernel_times(self, level: int, reps: int = 50):
         ms = np.zeros(2)

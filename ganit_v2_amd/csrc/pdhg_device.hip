@@ -22,6 +22,7 @@
 namespace ganit_hip {
 
 constexpr int kCheckGridMax = 2048;
+constexpr int64_t kGraphMaxNnz = 1000000;  // replay iteration blocks from graphs below this many nonzeros
 constexpr int kNOutN = 12, kNOutM = 7;
 
 // ================================================================ iteration kernels
@@ -690,8 +691,12 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
             pd->out = (double*)pd->alloc(sizeof(double) * GANIT_CHECK_NOUT);
             GH_CHECK(hipMemsetAsync(pd->out, 0, sizeof(double) * GANIT_CHECK_NOUT, ctx->stream));
             GH_CHECK(hipStreamSynchronize(ctx->stream));
+            // Graph replay pays for itself when an iteration is short (it saves the host launch cost, ~3 us per
+            // iteration on C0-sized problems); on large matrices plain stream launches keep up with the GPU and
+            // were measured ~2% faster than replay.  GANIT_HIP_NO_GRAPH=1 / GANIT_HIP_GRAPHS=1 force either.
             const char* ng = std::getenv("GANIT_HIP_NO_GRAPH");
-            pd->use_graphs = !(ng && ng[0] == '1');
+            const char* fg = std::getenv("GANIT_HIP_GRAPHS");
+            pd->use_graphs = (fg && fg[0] == '1') || (!(ng && ng[0] == '1') && pd->A->nnz < kGraphMaxNnz);
             const char* at = std::getenv("GANIT_HIP_AUTOTUNE");
             if (!(at && at[0] == '0')) {
                 // Per-matrix variant, timed on the fused iteration kernels that will run, at the lowest resident
@@ -754,6 +759,16 @@ int ganit_hip_pdhg_plan(const ganit_hip_pdhg* h, int which, int* lanes, int64_t*
         GH_REQUIRE(a, "problem has no Q");
         if (lanes) *lanes = a->lanes;
         if (hist) std::copy(a->hist, a->hist + 9, hist);
+    });
+}
+
+int ganit_hip_pdhg_index_bytes(const ganit_hip_pdhg* h, int which, int* bytes) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<const Pdhg*>(h);
+        GH_REQUIRE(pd && bytes && which >= 0 && which <= 2, "bad index-width query");
+        const Csr* a = which == 0 ? pd->A : which == 1 ? pd->AT : pd->Q;
+        GH_REQUIRE(a, "problem has no Q");
+        *bytes = (a->lanes == 0 && a->idx16) ? 2 : 4;
     });
 }
 

@@ -236,6 +236,10 @@ int ganit_solver_adjoint(ganit_solver* h, const double* gx, const double* ao, do
     });
 }
 
+int ganit_solver_index_bytes(ganit_solver* h, int which, int* bytes) {
+    return guard([&] { *bytes = reinterpret_cast<ganit::GpuSolver*>(h)->index_bytes(which); });
+}
+
 void ganit_solver_free(ganit_solver* h) { delete reinterpret_cast<ganit::GpuSolver*>(h); }
 
 ganit_batch* ganit_batch_create(const ganit_lp* const* lps, int ninst, const double* o) {

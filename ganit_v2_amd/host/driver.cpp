@@ -273,6 +273,13 @@ void GpuSolver::kernel_times(int level, int reps, double* ms) {
     chk(ganit_hip_pdhg_time_kernels(I.pd, mdt, vdt, reps, tau, sigma, ms));
 }
 
+int GpuSolver::index_bytes(int which) {
+    upload();
+    int b = 4;
+    chk(ganit_hip_pdhg_index_bytes(p_->pd, which, &b));
+    return b;
+}
+
 Result GpuSolver::run() {
     Impl& I = *p_;
     const Options& opt = I.opt;

@@ -100,6 +100,8 @@ class GpuSolver {
     // Average device milliseconds of the two fused iteration kernels at a precision level
     // (0 = L1, 1 = L2, 2 = L3): ms[0] primal kernel, ms[1] dual kernel.
     void kernel_times(int level, int reps, double* ms);
+    // Bytes per column index the fused kernels read for A (0), A^T (1) or Q (2): 2 with compact indices, else 4.
+    int index_bytes(int which);
     // Gradient of L(x*) with respect to c, the active row bounds and diag Q, given gx = dL/dx* at the last
     // run()'s solution: the adjoint PDHG iteration of libganit_hip.so at that solution.
     AdjointResult adjoint(const std::vector<double>& gx, const AdjointOptions& opt);

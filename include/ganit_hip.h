@@ -125,6 +125,10 @@ typedef struct {
 int ganit_hip_pdhg_create(ganit_hip_ctx* ctx, const ganit_hip_lp_desc* lp, unsigned mdt_mask, unsigned vdt_mask,
                           ganit_hip_pdhg** out);
 int ganit_hip_pdhg_destroy(ganit_hip_pdhg* pd);
+/* Bytes per column index the fused kernels read for matrix `which` (0 A, 1 A^T, 2 Q): 2 when the matrix runs
+ * on LDS-staged row blocks with compact indices (each block's columns = a base column + 16-bit offsets), else 4.
+ * No reference counterpart (the reference always reads 32-bit indices). */
+int ganit_hip_pdhg_index_bytes(const ganit_hip_pdhg* pd, int which, int* bytes);
 /* The matrices' variant plans (A, AT, Q); the variant may be overridden per matrix (which: 0 = A, 1 = AT,
  * 2 = Q), with the lanes_per_row values of ganit_hip_csr_set_lanes.  ganit_hip_pdhg_create picks each
  * matrix's variant by timing them (ganit_hip_csr_autotune) unless GANIT_HIP_AUTOTUNE=0. */

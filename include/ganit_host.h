@@ -75,6 +75,8 @@ ganit_solver* ganit_solver_create(const ganit_lp* lp, const double* opts);
 int ganit_solver_upload(ganit_solver* s);
 int ganit_solver_run(ganit_solver* s, double* out, double* x, double* y);
 int ganit_solver_kernel_times(ganit_solver* s, int level, int reps, double* ms);
+/* bytes per column index the fused kernels read for A (0), A^T (1), Q (2): 2 (compact) or 4 */
+int ganit_solver_index_bytes(ganit_solver* s, int which, int* bytes);
 /* Differentiation through the last ganit_solver_run (no reference counterpart): gx = dL/dx* (n);
  * aopts[0..5] = precision (0 FP64, 1 FP32 matrix + vectors, which needs a mixed/dynamic solver), tol,
  * max operator applications, method (0 GMRES on (I - J^T) lambda = g, tol on the relative residual;
