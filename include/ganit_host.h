@@ -97,7 +97,8 @@ void ganit_solver_free(ganit_solver* s);
  * x, y = the instances' original-space solutions concatenated in order (either may be NULL).
  * stats[0..23] = rows, cols, nnz, row blocks of A, row blocks of A^T, upload_s, solve_s, certify_s,
  *                L1_s, L2_s, L3_s, instances in L1, L2, L3, max iterations, total iterations,
- *                autotuned partition (-1 none), ms per batched iteration of the 4 candidates, setup_s. */
+ *                autotuned partition (-1 none), ms per batched iteration of the 4 candidates, setup_s,
+ *                2 reserved (0). */
 typedef struct ganit_batch ganit_batch;
 ganit_batch* ganit_batch_create(const ganit_lp* const* lps, int ninst, const double* opts);
 int ganit_batch_upload(ganit_batch* b);
