@@ -22,10 +22,14 @@ constexpr int kNumXcd = 8;
 // longer than kLong is a block of its own and is reduced by the whole block instead.
 constexpr int kTile = 2048;
 constexpr int kLong = 256;
-// Rows longer than kSplit are cut into chunks of kChunk nonzeros, one workgroup each; the partial sums are
-// combined in chunk order by the last chunk to finish (so very long rows spread over many CUs).
+// Rows longer than kSplit are cut into chunks of kChunk nonzeros, one workgroup each; a second launch (one
+// workgroup per split row) combines the chunk partials in chunk order (so very long rows spread over many CUs).
 constexpr int kSplit = 8192;
 constexpr int kChunk = 4096;
+// A staged block whose columns lie within kWin consecutive entries of the gathered vector may stage that window
+// of the vector in LDS with coalesced loads and gather from LDS instead of global memory (windowed blocks).
+constexpr int kWin = 2048;
+static_assert(kTile % kBlock == 0 && kWin % kBlock == 0 && kLong <= kTile, "staging is whole slots per thread");
 
 // ---------------------------------------------------------------- BF16 storage
 struct bf16_t {
@@ -49,7 +53,8 @@ struct CsrView {
     const int* __restrict__ ptr;
     const int* __restrict__ idx;
     const M* __restrict__ val;
-    // optional compact indices for the LDS-staged row blocks: idx = bcol[block] + idx16 (null: use idx)
+    // optional compact indices for the LDS-staged row blocks: idx = bcol[2 * block] + idx16 (null: use idx);
+    // bcol[2 * block + 1] = the block's window length (columns bcol[2b] .. + length - 1), 0 when not windowed
     const uint16_t* __restrict__ idx16 = nullptr;
     const int* __restrict__ bcol = nullptr;
 };
@@ -131,12 +136,19 @@ struct SplitPlan {
 // A single row longer than kLong is instead swept by all threads and reduced with a fixed tree.
 // `row_fn(r, sums, ops)` runs once per row, in the thread that owns it (thread 0 for a long row), with
 // ops = pref(r): the row's own operands, loaded early (stream_block_pf); stream_block has none.
-template <class M, class V, int NX, class Gather, class Pref, class RowFn>
+// WIN (NX = 1 only): windowed blocks gather from a copy of their window of the vector in LDS; lds then holds
+// kTile + kWin elements.
+template <class M, class V, int NX, bool WIN = false, class Gather, class Pref, class RowFn>
 __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* __restrict__ blocks, int b,
-                                                Gather gather, Pref pref, RowFn row_fn, V* lds /* NX * kTile */,
-                                                const SplitPlan& sp) {
+                                                Gather gather, Pref pref, RowFn row_fn,
+                                                V* lds /* NX * kTile (+ kWin with WIN) */, const SplitPlan& sp) {
+    static_assert(!WIN || NX == 1, "windowed gathers stage one vector");
     if (sp.finish) {  // finish pass: fold split row b's chunk partials, fixed-order tree
         const int q0 = sp.first[b], nch = sp.first[b + 1] - q0;
+        // the row and its operands are loaded before the fold (by every thread, unconditionally: one cache
+        // line, and no branch that would force a wait), so their latency overlaps it
+        const int row = sp.chunk[4 * q0];
+        auto pv = pref(row);
         V tot[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) tot[k] = V(0);
@@ -145,11 +157,7 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
             for (int k = 0; k < NX; ++k) tot[k] += (V)sp.part[(size_t)(q0 + q) * NX + k];
         }
         block_sum_v<V, NX>(tot, lds);
-        if (threadIdx.x == 0) {
-            const int row = sp.chunk[4 * q0];
-            auto pv = pref(row);
-            row_fn(row, tot, pv);
-        }
+        if (threadIdx.x == 0) row_fn(row, tot, pv);
         return;
     }
     if (b >= sp.nblocks) {  // one chunk of a split row: its partial only
@@ -175,15 +183,23 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
         return;
     }
     // block descriptor: first row, end row, first nonzero, end nonzero (one 16-byte load); with compact
-    // indices, the block's base column (an independent load, in flight at the same time)
+    // indices, the block's base column and window length (an independent load, in flight at the same time)
     const int4 d = reinterpret_cast<const int4*>(blocks)[b];
     const uint16_t* __restrict__ i16 = a.idx16;
-    const int cb = i16 ? a.bcol[b] : 0;
+    int cb = 0, wl = 0;
+    if (i16) {
+        const int2 bw = reinterpret_cast<const int2*>(a.bcol)[b];
+        cb = bw.x;
+        wl = WIN ? bw.y : 0;
+    }
     const int r0 = d.x, r1 = d.y;
     const int base = d.z;
     const int cnt = d.w - d.z;
     if (r1 - r0 == 1 && cnt > kSplit) return;  // a split row: its chunks do the work
     if (r1 - r0 == 1 && cnt > kLong) {
+        // a long row: the whole block reduces it; the row's operands are loaded before the sweep (by every
+        // thread, unconditionally) so that their latency overlaps it instead of following the reduction
+        auto pv = pref(r0);
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
@@ -195,32 +211,78 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
             for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
         }
         block_sum_v<V, NX>(s, lds);
-        if (threadIdx.x == 0) {
-            auto pv = pref(r0);
-            row_fn(r0, s, pv);
-        }
+        if (threadIdx.x == 0) row_fn(r0, s, pv);
         return;
     }
     // the owning thread's row extent and row operands are loaded before the staging sweep, so their
-    // latency overlaps it instead of following the barrier
+    // latency overlaps it instead of following the barrier (the extents are only used after it).  Threads
+    // without a row load row r0's (always a row of the block): unconditional loads leave no branch whose
+    // merge would make the sweep wait for them.
     const int r = r0 + (int)threadIdx.x;
     const bool own = r < r1;
-    int beg = 0, end = 0;
-    decltype(pref(0)) pv{};
-    if (own) {
-        beg = a.ptr[r] - base;
-        end = a.ptr[r + 1] - base;
-        pv = pref(r);
-    }
-#pragma unroll 4
-    for (int q = threadIdx.x; q < cnt; q += kBlock) {
-        const V av = widen<V>(a.val[base + q]);
-        const int i = i16 ? cb + (int)i16[base + q] : a.idx[base + q];
+    const int rl = own ? r : r0;
+    const int pbeg = a.ptr[rl], pend = a.ptr[rl + 1];
+    auto pv = pref(rl);
+    // Staging in two straight-line phases: every index/value load of the thread's (at most kPer) slots is
+    // issued first, then every gather, so a block costs one index round trip and one gather round trip
+    // instead of one of each per slot.  Slots past the block's end load its last entry (always in bounds)
+    // and store into the tile's unused tail [cnt, kTile), which no row reads; the store is unconditional so
+    // that the gathers are not sunk into a branch one slot at a time.  The products are the same as one slot
+    // at a time, so the row sums are too.
+    constexpr int kPer = kTile / kBlock;
+    if (WIN && wl > 0) {
+        // windowed block: the window's values, the offsets and the matrix values are all loaded in one
+        // phase (the window loads are coalesced), the window goes to LDS, and the products read it there.
+        // Offsets index the window directly; window slots past wl (<= kWin) are filled but never read.
+        constexpr int kWPer = kWin / kBlock;
+        V* win = lds + kTile;
+        const int wlast = wl - 1, last = cnt - 1;
+        V wv[kWPer];
+        M mv[kPer];
+        int off[kPer];
 #pragma unroll
-        for (int k = 0; k < NX; ++k) lds[k * kTile + q] = av * gather(k, i);
+        for (int k = 0; k < kWPer; ++k) wv[k] = gather(0, cb + min((int)threadIdx.x + k * kBlock, wlast));
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = min((int)threadIdx.x + k * kBlock, last);
+            mv[k] = a.val[base + q];
+            off[k] = (int)i16[base + q];
+        }
+#pragma unroll
+        for (int k = 0; k < kWPer; ++k) win[(int)threadIdx.x + k * kBlock] = wv[k];
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * win[off[k]];
+    } else if (cnt > 0) {
+        const int last = cnt - 1;
+        M mv[kPer];
+        int col[kPer];
+        if (i16) {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int q = min((int)threadIdx.x + k * kBlock, last);
+                mv[k] = a.val[base + q];
+                col[k] = cb + (int)i16[base + q];
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) {
+                const int q = min((int)threadIdx.x + k * kBlock, last);
+                mv[k] = a.val[base + q];
+                col[k] = a.idx[base + q];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = (int)threadIdx.x + k * kBlock;  // < kTile
+            const V av = widen<V>(mv[k]);
+#pragma unroll
+            for (int j = 0; j < NX; ++j) lds[j * kTile + q] = av * gather(j, col[k]);
+        }
     }
     __syncthreads();
     if (own) {
+        const int beg = pbeg - base, end = pend - base;
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
@@ -242,10 +304,11 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
 struct NoRowOperands {};
 
 // stream_block without row operands: row_fn(r, sums).
-template <class M, class V, int NX, class Gather, class RowFn>
+template <class M, class V, int NX, bool WIN = false, class Gather, class RowFn>
 __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __restrict__ blocks, int b, Gather gather,
-                                             RowFn row_fn, V* lds /* NX * kTile */, const SplitPlan& sp) {
-    stream_block_pf<M, V, NX>(
+                                             RowFn row_fn, V* lds /* NX * kTile (+ kWin with WIN) */,
+                                             const SplitPlan& sp) {
+    stream_block_pf<M, V, NX, WIN>(
         a, blocks, b, gather, [](int) { return NoRowOperands{}; },
         [&](int r, V (&t)[NX], NoRowOperands&) { row_fn(r, t); }, lds, sp);
 }

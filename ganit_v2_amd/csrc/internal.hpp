@@ -81,7 +81,7 @@ struct Csr {
     int lanes = 8;           // row-group variant: lanes per row (1..64); 0 selects the LDS-staged row blocks
     int64_t hist[9] = {0};
     uint16_t* idx16 = nullptr;  // compact indices of the staged row blocks (null when a block spans >= 2^16
-    int* bcol = nullptr;        // columns); bcol[b] = block b's smallest column
+    int* bcol = nullptr;        // columns); bcol[2b] = block b's smallest column, bcol[2b + 1] = its window length
     int* blocks = nullptr;   // row-block descriptors of the LDS-staged variant: 4 ints per block (first row,
                              // end row, first nonzero, end nonzero), device; see upload_row_blocks
     int nblocks = 0;

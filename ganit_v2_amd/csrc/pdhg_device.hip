@@ -25,6 +25,18 @@ constexpr int kCheckGridMax = 2048;
 constexpr int64_t kGraphMaxNnz = 1000000;  // replay iteration blocks from graphs below this many nonzeros
 constexpr int kNOutN = 12, kNOutM = 7;
 
+// Step sizes on the device: prm[0] = tau, prm[1] = sigma (FP64), and the same two rounded to FP32 in the next
+// slot, so an FP32 kernel reads its step size without a conversion (same rounding as converting in place).
+constexpr int kPrmDoubles = 3;
+template <class V>
+__device__ __forceinline__ V step_as(const double* __restrict__ prm, int k);
+template <>
+__device__ __forceinline__ double step_as<double>(const double* __restrict__ prm, int k) { return prm[k]; }
+template <>
+__device__ __forceinline__ float step_as<float>(const double* __restrict__ prm, int k) {
+    return reinterpret_cast<const float*>(prm + 2)[k];
+}
+
 // ================================================================ iteration kernels
 template <int LPR, class M, class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_iter_primal(CsrView<M> AT, CsrView<M> Q, const V* __restrict__ y,
@@ -68,13 +80,16 @@ __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const 
                                                            const V* __restrict__ l, const V* __restrict__ u,
                                                            const double* __restrict__ prm, V* __restrict__ xn,
                                                            V* __restrict__ xbar, V* __restrict__ sumx) {
-    __shared__ V lds[kTile];
+    __shared__ V lds[kTile + kWin];
+    // the row's operands, and the step size with them: prm is written by the previous launch, so its load is
+    // issued once the block is known, in flight together with the block's index / value loads (and it is
+    // read in V already, so nothing waits for it before the row update)
     struct Ops {
-        V x, c, l, u, s;
+        V x, c, l, u, s, tau;
     };
-    stream_block_pf<M, V, 1>(
+    stream_block_pf<M, V, 1, true>(
         AT, blocks, xcd_block_id(), [&](int, int i) { return y[i]; },
-        [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j]}; },
+        [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j], step_as<V>(prm, 0)}; },
         [&](int j, V (&aty)[1], Ops& o) {
 #pragma clang fp contract(off)
             V qx = V(0);
@@ -85,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const 
                 }
             }
             V sm = o.s;
-            primal_update<V>(o.x, aty[0], qx, o.c, o.l, o.u, V(prm[0]), xn[j], xbar[j], sm);
+            primal_update<V>(o.x, aty[0], qx, o.c, o.l, o.u, o.tau, xn[j], xbar[j], sm);
             sumx[j] = sm;
         },
         lds, sp);
@@ -97,16 +112,16 @@ __global__ __launch_bounds__(kBlock) void k_iter_dual_rb(CsrView<M> A, const int
                                                          const V* __restrict__ lo, const V* __restrict__ hi,
                                                          const double* __restrict__ prm, V* __restrict__ yn,
                                                          V* __restrict__ sumy) {
-    __shared__ V lds[kTile];
+    __shared__ V lds[kTile + kWin];
     struct Ops {
-        V y, lo, hi, s;
+        V y, lo, hi, s, sigma;
     };
-    stream_block_pf<M, V, 1>(
+    stream_block_pf<M, V, 1, true>(
         A, blocks, xcd_block_id(), [&](int, int j) { return xbar[j]; },
-        [&](int i) { return Ops{y[i], lo[i], hi[i], sumy[i]}; },
+        [&](int i) { return Ops{y[i], lo[i], hi[i], sumy[i], step_as<V>(prm, 1)}; },
         [&](int i, V (&axb)[1], Ops& o) {
             V sm = o.s;
-            dual_update<V>(o.y, axb[0], o.lo, o.hi, V(prm[1]), yn[i], sm);
+            dual_update<V>(o.y, axb[0], o.lo, o.hi, o.sigma, yn[i], sm);
             sumy[i] = sm;
         },
         lds, sp);
@@ -116,6 +131,9 @@ __global__ void k_set_steps(double* prm, double tau, double sigma) {
     if (threadIdx.x == 0) {
         prm[0] = tau;
         prm[1] = sigma;
+        float* f = reinterpret_cast<float*>(prm + 2);
+        f[0] = (float)tau;
+        f[1] = (float)sigma;
     }
 }
 
@@ -687,7 +705,7 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 s.xbar = pd->alloc(e * n); s.sumx = pd->alloc(e * n); s.xa = pd->alloc(e * n); s.xr = pd->alloc(e * n);
                 s.sumy = pd->alloc(e * m); s.ya = pd->alloc(e * m); s.yr = pd->alloc(e * m);
             }
-            pd->prm = (double*)pd->alloc(sizeof(double) * 2);
+            pd->prm = (double*)pd->alloc(sizeof(double) * kPrmDoubles);
             pd->out = (double*)pd->alloc(sizeof(double) * GANIT_CHECK_NOUT);
             GH_CHECK(hipMemsetAsync(pd->out, 0, sizeof(double) * GANIT_CHECK_NOUT, ctx->stream));
             GH_CHECK(hipStreamSynchronize(ctx->stream));

@@ -59,8 +59,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<M> a, const V* __restri
 template <class M, class V>
 __global__ __launch_bounds__(kBlock) void k_spmv_rb(CsrView<M> a, const int* __restrict__ blocks, SplitPlan sp,
                                                     const V* __restrict__ x, V* __restrict__ y) {
-    __shared__ V lds[kTile];
-    stream_block<M, V, 1>(
+    __shared__ V lds[kTile + kWin];
+    stream_block<M, V, 1, true>(
         a, blocks, xcd_block_id(), [&](int, int j) { return x[j]; }, [&](int r, V (&t)[1]) { y[r] = t[0]; }, lds,
         sp);
 }
@@ -255,8 +255,10 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
     a->bcol = nullptr;
     const char* env = std::getenv("GANIT_HIP_COMPACT_IDX");  // 0: keep the 32-bit indices only
     if ((env && env[0] == '0') || a->nnz == 0 || rb.size() < 2) return;
+    const char* wenv = std::getenv("GANIT_HIP_WINDOW");  // 0: no windowed blocks (every gather from HBM / L2)
+    const bool windows = !(wenv && wenv[0] == '0');
     const size_t nb = rb.size() - 1;
-    std::vector<int> bc(nb, 0);
+    std::vector<int> bc(2 * nb, 0);  // per block: base column, window length (0: not windowed)
     std::vector<uint16_t> c16((size_t)a->nnz, 0);
     for (size_t q = 0; q < nb; ++q) {
         const int r0 = rb[q], r1 = rb[q + 1], p0 = ptr[r0], p1 = ptr[r1];
@@ -267,7 +269,10 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
             mx = std::max(mx, idx[p]);
         }
         if (mx - mn > 65535) return;  // a block too wide: keep the 32-bit path for this matrix
-        bc[q] = mn;
+        bc[2 * q] = mn;
+        // windowed when the span fits the LDS window and loading it costs at most two entries per nonzero
+        const int span = mx - mn + 1;
+        bc[2 * q + 1] = (windows && span <= kWin && span <= 2 * (p1 - p0)) ? span : 0;
         for (int p = p0; p < p1; ++p) c16[p] = (uint16_t)(idx[p] - mn);
     }
     GH_CHECK(hipMalloc(&a->idx16, sizeof(uint16_t) * c16.size()));

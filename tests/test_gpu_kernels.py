@@ -108,6 +108,53 @@ def test_rowblock_spmv_bitwise_vs_oracle(hipctx, oracle, mdt, vdt):
     assert np.array_equal(yd.numpy(), oracle.spmv(mdt, vdt, m, n, ptr, idx, val, x))
 
 
+def _windowed_blocks(ptr, idx, max_rows=256, max_nnz=2048, klong=256, kwin=2048):
+    """Staged row blocks of the default partition whose columns qualify for an LDS window (span <= kwin and
+    span <= 2 * block nonzeros): (qualifying, staged)."""
+    m, r, hit, staged = len(ptr) - 1, 0, 0, 0
+    while r < m:
+        r0, nz = r, 0
+        if ptr[r + 1] - ptr[r] > klong:
+            r += 1
+            continue
+        while r < m and r - r0 < max_rows and ptr[r + 1] - ptr[r] <= klong and nz + ptr[r + 1] - ptr[r] <= max_nnz:
+            nz += ptr[r + 1] - ptr[r]
+            r += 1
+        if nz:
+            cols = idx[ptr[r0]:ptr[r]]
+            span = int(cols.max() - cols.min()) + 1
+            staged += 1
+            hit += span <= kwin and span <= 2 * nz
+    return hit, staged
+
+
+@pytest.mark.parametrize("window", [True, False])
+@pytest.mark.parametrize("mdt,vdt", PAIRS)
+def test_windowed_rowblock_spmv_bitwise(hipctx, oracle, monkeypatch, mdt, vdt, window):
+    """Banded rows: most blocks gather from a copy of their window of x in LDS; with windows on (and with
+    them switched off) the row sums are the oracle's bits exactly."""
+    from ganit_v2_amd import hip
+    if not window:
+        monkeypatch.setenv("GANIT_HIP_WINDOW", "0")
+    rng = np.random.default_rng(40 + 3 * mdt + vdt)
+    m, n = 4000, 9000
+    lens = rng.integers(0, 120, m)
+    lens[rng.integers(0, m, 7)] = 0
+    starts = (np.arange(m) * 2 + rng.integers(0, 50, m)).clip(0, n - 300)
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.concatenate([np.sort(s0 + rng.choice(300, k, replace=False)) for s0, k in zip(starts, lens)])
+    idx = idx.astype(np.int32)
+    hit, staged = _windowed_blocks(ptr, idx)
+    assert hit >= staged // 2 and hit > 20, (hit, staged)
+    val = rng.standard_normal(ptr[-1])
+    x = rng.standard_normal(n)
+    A = hip.DeviceCsr(hipctx, m, n, ptr, idx, val)
+    A.set_lanes(0)
+    xd, yd = hipctx.vector(x, vdt), hipctx.empty(m, vdt)
+    A.spmv(xd, yd, mdt)
+    assert np.array_equal(yd.numpy(), oracle.spmv(mdt, vdt, m, n, ptr, idx, val, x))
+
+
 @pytest.mark.parametrize("mdt,vdt", [(F64, F64), (F32, F32), (BF16, F32)])
 def test_split_rows_spmv(hipctx, oracle, mdt, vdt):
     """Rows longer than 8192 entries are cut into 4096-entry chunks on separate workgroups and combined in
@@ -355,12 +402,18 @@ def test_pdhg_iteration_matches_oracle_steps(hipctx, oracle):
     assert np.max(np.abs(y1 - yn)) <= 1e-11 * (1 + np.max(np.abs(yn)))
 
 
-def test_rowblock_iteration_is_bit_exact(hipctx, oracle):
+@pytest.mark.parametrize("kind", ["random_lp", "blending"])
+def test_rowblock_iteration_is_bit_exact(hipctx, oracle, kind):
     """With A and A^T on row blocks, one fused iteration reproduces the oracle's spmv_AT -> primal_step ->
-    spmv_A -> dual_step bit for bit (FP64 and FP32)."""
+    spmv_A -> dual_step bit for bit (FP64 and FP32); the blending LP's blocks gather through LDS windows."""
     import ganit_v2_amd as g
     from ganit_v2_amd import hip
-    s = g.Problem.generate("random_lp", seed=4).scaled()
+    if kind == "random_lp":
+        s = g.Problem.generate("random_lp", seed=4).scaled()
+    else:
+        s = g.Problem.generate("blending", crudes=20, products=6, specs=5, periods=30, seed=0).scaled()
+        hit, staged = _windowed_blocks(s["A"].ptr, s["A"].idx)
+        assert hit == staged, (hit, staged)
     assert max(np.diff(s["A"].ptr).max(), np.diff(s["AT"].ptr).max()) <= 256
     m, n = s["m"], s["n"]
     pd = hip.DevicePdhg(hipctx, s, mdt_mask=3, vdt_mask=3)
