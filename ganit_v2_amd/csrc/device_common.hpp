@@ -295,11 +295,11 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
     }
 }
 
-// stream_block for NX gathered vectors with ONE tile of LDS: on a staged block the NX products are staged and
-// summed one vector after another (the block's indices and values come from cache on the later passes), so a
-// check pass needs kTile * sizeof(V) bytes of LDS instead of NX times that.  Same row sums, same bits.  Long
-// rows, chunks of split rows and the finish pass need only (kBlock / kWave) * NX elements of LDS and take the
-// NX-wide path.
+// stream_block for NX gathered vectors with ONE tile of LDS: on a staged block the indices and values are
+// loaded once (into registers) and the NX products are staged and summed one vector after another, so a check
+// pass needs kTile * sizeof(V) bytes of LDS (+ one window) instead of NX times that.  Same row sums, same bits.
+// Long rows, chunks of split rows and the finish pass need only (kBlock / kWave) * NX elements of LDS and take
+// the NX-wide path.
 
 struct NoRowOperands {};
 
@@ -313,37 +313,99 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
         [&](int r, V (&t)[NX], NoRowOperands&) { row_fn(r, t); }, lds, sp);
 }
 
-template <class M, class V, int NX, class Gather, class RowFn>
+template <class M, class V, int NX, bool WIN = false, class Gather, class RowFn>
 __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int* __restrict__ blocks, int b,
-                                                 Gather gather, RowFn row_fn, V* lds /* kTile */,
+                                                 Gather gather, RowFn row_fn, V* lds /* kTile (+ kWin with WIN) */,
                                                  const SplitPlan& sp) {
     static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
-    if (sp.finish || b >= sp.nblocks || (reinterpret_cast<const int4*>(blocks)[b].y -
-                                                 reinterpret_cast<const int4*>(blocks)[b].x == 1 &&
-                                             reinterpret_cast<const int4*>(blocks)[b].w -
-                                                 reinterpret_cast<const int4*>(blocks)[b].z > kLong)) {
+    if (sp.finish || b >= sp.nblocks) {
         stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
         return;
     }
-    V acc[NX];
-    bool have = false;
-    int row = -1;
-#pragma unroll
-    for (int k = 0; k < NX; ++k) {
-        V got = V(0);
-        int r_got = -1;
-        stream_block<M, V, 1>(
-            a, blocks, b, [&](int, int i) { return gather(k, i); },
-            [&](int r, V (&t)[1]) {
-                got = t[0];
-                r_got = r;
-            },
-            lds, sp);
-        __syncthreads();  // the next vector reuses the tile
-        acc[k] = got;
-        if (r_got >= 0) { have = true; row = r_got; }
+    const int4 d = reinterpret_cast<const int4*>(blocks)[b];
+    const int r0 = d.x, r1 = d.y, base = d.z, cnt = d.w - d.z;
+    if (r1 - r0 == 1 && cnt > kLong) {  // long (or split) row: the NX-wide block-reduction path
+        stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
+        return;
     }
-    if (have) row_fn(row, acc);
+    // A staged block: its indices and values are loaded once (and, for a windowed block, the NX windows of the
+    // gathered vectors), then for each vector the products go through the one tile and each row is summed in
+    // order -- the same row sums as NX separate passes.
+    const uint16_t* __restrict__ i16 = a.idx16;
+    int cb = 0, wl = 0;
+    if (i16) {
+        const int2 bw = reinterpret_cast<const int2*>(a.bcol)[b];
+        cb = bw.x;
+        wl = WIN ? bw.y : 0;
+    }
+    const int r = r0 + (int)threadIdx.x;
+    const bool own = r < r1;
+    const int rl = own ? r : r0;
+    const int pbeg = a.ptr[rl], pend = a.ptr[rl + 1];
+    constexpr int kPer = kTile / kBlock, kWPer = kWin / kBlock;
+    const int last = cnt > 0 ? cnt - 1 : 0;
+    M mv[kPer];
+    int col[kPer];  // the window offset (windowed block) or the column
+    if (cnt > 0 && i16) {
+        const int add = (WIN && wl > 0) ? 0 : cb;
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = min((int)threadIdx.x + k * kBlock, last);
+            mv[k] = a.val[base + q];
+            col[k] = (int)i16[base + q] + add;
+        }
+    } else if (cnt > 0) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int q = min((int)threadIdx.x + k * kBlock, last);
+            mv[k] = a.val[base + q];
+            col[k] = a.idx[base + q];
+        }
+    }
+    V acc[NX];
+    if (WIN && wl > 0) {
+        V* win = lds + kTile;
+        const int wlast = wl - 1;
+        V wv[NX][kWPer];
+#pragma unroll
+        for (int v = 0; v < NX; ++v) {
+#pragma unroll
+            for (int k = 0; k < kWPer; ++k) wv[v][k] = gather(v, cb + min((int)threadIdx.x + k * kBlock, wlast));
+        }
+#pragma unroll
+        for (int v = 0; v < NX; ++v) {
+#pragma unroll
+            for (int k = 0; k < kWPer; ++k) win[(int)threadIdx.x + k * kBlock] = wv[v][k];
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * win[col[k]];
+            __syncthreads();
+            V t = V(0);
+            if (own) {
+#pragma unroll 8
+                for (int q = pbeg - base; q < pend - base; ++q) t += lds[q];
+            }
+            acc[v] = t;
+            __syncthreads();  // the next vector reuses the window and the tile
+        }
+    } else {
+#pragma unroll
+        for (int v = 0; v < NX; ++v) {
+            if (cnt > 0) {
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * gather(v, col[k]);
+            }
+            __syncthreads();
+            V t = V(0);
+            if (own) {
+#pragma unroll 8
+                for (int q = pbeg - base; q < pend - base; ++q) t += lds[q];
+            }
+            acc[v] = t;
+            __syncthreads();  // the next vector reuses the tile
+        }
+    }
+    if (own) row_fn(r, acc);
 }
 
 // ---------------------------------------------------------------- deterministic grid reduction

@@ -204,11 +204,11 @@ struct CheckAcc {
     }
 };
 
-template <int LPR, class M, class V, class C, bool HASQ>
-__global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q, const V* __restrict__ x,
-                                                    const V* __restrict__ y, const V* __restrict__ xa,
-                                                    const V* __restrict__ ya, const V* __restrict__ xr,
-                                                    const V* __restrict__ yr, const V* __restrict__ c,
+template <int LPR, class M, class V, class C, bool HASQ, class VI = V>
+__global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q, const VI* __restrict__ x,
+                                                    const VI* __restrict__ y, const VI* __restrict__ xa,
+                                                    const VI* __restrict__ ya, const VI* __restrict__ xr,
+                                                    const VI* __restrict__ yr, const V* __restrict__ c,
                                                     const V* __restrict__ l, const V* __restrict__ u,
                                                     const V* __restrict__ cres, int flags, ReduceSlot slot) {
     __shared__ double lds[(kBlock / kWave) * kNOutN];
@@ -225,7 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
             const int i = AT.idx[p];
             sc += a * C(y[i]);
             if (avg) sa += a * C(ya[i]);
-            if (ray) sr += a * C(V(y[i] - yr[i]));
+            if (ray) sr += a * C(V(V(y[i]) - V(yr[i])));
         }
         if (HASQ) {
             for (int p = Q.ptr[j] + lane; p < Q.ptr[j + 1]; p += LPR) {
@@ -277,10 +277,11 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
 }
 
 // One pass over A: out[12..18] as documented for ganit_hip_pdhg_check (sums in C as for k_check_n).
-template <int LPR, class M, class V, class C>
-__global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __restrict__ x, const V* __restrict__ xa,
-                                                    const V* __restrict__ y, const V* __restrict__ ya,
-                                                    const V* __restrict__ yr, const V* __restrict__ lo,
+template <int LPR, class M, class V, class C, class VI = V>
+__global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const VI* __restrict__ x,
+                                                    const VI* __restrict__ xa, const VI* __restrict__ y,
+                                                    const VI* __restrict__ ya, const VI* __restrict__ yr,
+                                                    const V* __restrict__ lo,
                                                     const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                     ReduceSlot slot) {
     __shared__ double lds[(kBlock / kWave) * kNOutM];
@@ -317,7 +318,7 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
                 acc.add(5, d0 * d0);
                 acc.add(6, d1 * d1);
             }
-            if (ray) acc.add(4, row_obj_term(double(V(y[i] - yr[i])), lw, hg));
+            if (ray) acc.add(4, row_obj_term(double(V(V(y[i]) - V(yr[i]))), lw, hg));
         }
     }
     {
@@ -328,23 +329,23 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const V* __res
 }
 
 // The two check passes on LDS-staged row blocks (grid = number of row blocks); sums in C as above.
-template <class M, class V, class C, bool HASQ>
+template <class M, class V, class C, bool HASQ, class VI = V>
 __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int* __restrict__ blocks, SplitPlan sp,
                                                        CsrView<M> Q,
-                                                       const V* __restrict__ x, const V* __restrict__ y,
-                                                       const V* __restrict__ xa, const V* __restrict__ ya,
-                                                       const V* __restrict__ xr, const V* __restrict__ yr,
+                                                       const VI* __restrict__ x, const VI* __restrict__ y,
+                                                       const VI* __restrict__ xa, const VI* __restrict__ ya,
+                                                       const VI* __restrict__ xr, const VI* __restrict__ yr,
                                                        const V* __restrict__ c, const V* __restrict__ l,
                                                        const V* __restrict__ u, const V* __restrict__ cres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ C lds[kTile];
+    __shared__ C lds[kTile + kWin];
     __shared__ double red[(kBlock / kWave) * kNOutN];
     CheckAcc<kNOutN> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
-    stream_block_seq<M, C, 3>(
+    stream_block_seq<M, C, 3, true>(
         AT, blocks, xcd_block_id(),
         [&](int k, int i) {
-            return k == 0 ? C(y[i]) : k == 1 ? (avg ? C(ya[i]) : C(0)) : (ray ? C(V(y[i] - yr[i])) : C(0));
+            return k == 0 ? C(y[i]) : k == 1 ? (avg ? C(ya[i]) : C(0)) : (ray ? C(V(V(y[i]) - V(yr[i]))) : C(0));
         },
         [&](int j, C (&t)[3]) {
 #pragma clang fp contract(off)
@@ -394,18 +395,18 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
     }
 }
 
-template <class M, class V, class C>
+template <class M, class V, class C, class VI = V>
 __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* __restrict__ blocks, SplitPlan sp,
-                                                       const V* __restrict__ x, const V* __restrict__ xa,
-                                                       const V* __restrict__ y, const V* __restrict__ ya,
-                                                       const V* __restrict__ yr, const V* __restrict__ lo,
+                                                       const VI* __restrict__ x, const VI* __restrict__ xa,
+                                                       const VI* __restrict__ y, const VI* __restrict__ ya,
+                                                       const VI* __restrict__ yr, const V* __restrict__ lo,
                                                        const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ C lds[kTile];
+    __shared__ C lds[kTile + kWin];
     __shared__ double red[(kBlock / kWave) * kNOutM];
     CheckAcc<kNOutM> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
-    stream_block_seq<M, C, 2>(
+    stream_block_seq<M, C, 2, true>(
         A, blocks, xcd_block_id(), [&](int k, int j) { return k == 0 ? C(x[j]) : (avg ? C(xa[j]) : C(0)); },
         [&](int i, C (&t)[2]) {
             const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
@@ -424,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
                 acc.add(5, d0 * d0);
                 acc.add(6, d1 * d1);
             }
-            if (ray) acc.add(4, row_obj_term(double(V(y[i] - yr[i])), lw, hg));
+            if (ray) acc.add(4, row_obj_term(double(V(V(y[i]) - V(yr[i]))), lw, hg));
         },
         lds, sp);
     __syncthreads();
@@ -585,14 +586,18 @@ void launch_iteration(Pdhg* pd, VecSet& s) {
     s.cur ^= 1;
 }
 
-template <class M, class V, class C>
-void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, bool form_avg = true) {
+// Iterates (x, y, averages, restart point) of type VI from working set `it`, problem data of type V from
+// working set `s` (the same set unless an FP32 iterate is evaluated against the FP64 data: VI = float,
+// V = double, each iterate entry widened exactly where it is read).
+template <class M, class V, class C, class VI = V>
+void run_check(Pdhg* pd, VecSet& it, const VecSet& s, int flags, double inv_k, double* host_out) {
     hipStream_t st = pd->ctx->stream;
-    const V* x = (const V*)s.X[s.cur];
-    const V* y = (const V*)s.Y[s.cur];
-    if ((flags & GANIT_CHECK_AVG) && form_avg) {
-        hipLaunchKernelGGL(k_average<V>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st, pd->n, pd->m,
-                           inv_k, (const V*)s.sumx, (const V*)s.sumy, (V*)s.xa, (V*)s.ya);
+    const VI* x = (const VI*)it.X[it.cur];
+    const VI* y = (const VI*)it.Y[it.cur];
+    const VI *xa = (const VI*)it.xa, *ya = (const VI*)it.ya, *xr = (const VI*)it.xr, *yr = (const VI*)it.yr;
+    if (flags & GANIT_CHECK_AVG) {
+        hipLaunchKernelGGL(k_average<VI>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st, pd->n, pd->m,
+                           inv_k, (const VI*)it.sumx, (const VI*)it.sumy, (VI*)it.xa, (VI*)it.ya);
     }
     ReduceSlot sn{pd->part_n, pd->out};
     ReduceSlot sm{pd->part_m, pd->out + kNOutN};
@@ -610,14 +615,13 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
             const SplitPlan sp = pass ? pd->AT->split_finish() : pd->AT->split();
             const ReduceSlot slot{pd->part_n + (size_t)nb_n * kNOutN, pd->out};
             if (pd->hasq)
-                hipLaunchKernelGGL((k_check_n_rb<M, V, C, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
-                                   pd->Q->view<M>(), x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr,
-                                   (const V*)s.yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres,
-                                   flags, slot);
+                hipLaunchKernelGGL((k_check_n_rb<M, V, C, true, VI>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
+                                   pd->Q->view<M>(), x, y, xa, ya, xr, yr, (const V*)s.c, (const V*)s.l,
+                                   (const V*)s.u, (const V*)s.cres, flags, slot);
             else
-                hipLaunchKernelGGL((k_check_n_rb<M, V, C, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp, at,
-                                   x, y, (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr,
-                                   (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, slot);
+                hipLaunchKernelGGL((k_check_n_rb<M, V, C, false, VI>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
+                                   at, x, y, xa, ya, xr, yr, (const V*)s.c, (const V*)s.l, (const V*)s.u,
+                                   (const V*)s.cres, flags, slot);
             nb_n += g.x;
         }
     } else with_lanes(pd->AT->lanes == 0 ? 1 : pd->AT->lanes, [&](auto L) {
@@ -626,21 +630,20 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
         nb_n = g;
         if (pd->hasq)
-            hipLaunchKernelGGL((k_check_n<LPR, M, V, C, true>), dim3(g), dim3(kBlock), 0, st, at, pd->Q->view<M>(), x, y,
-                               (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
-                               (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
+            hipLaunchKernelGGL((k_check_n<LPR, M, V, C, true, VI>), dim3(g), dim3(kBlock), 0, st, at,
+                               pd->Q->view<M>(), x, y, xa, ya, xr, yr, (const V*)s.c, (const V*)s.l, (const V*)s.u,
+                               (const V*)s.cres, flags, sn);
         else
-            hipLaunchKernelGGL((k_check_n<LPR, M, V, C, false>), dim3(g), dim3(kBlock), 0, st, at, at, x, y,
-                               (const V*)s.xa, (const V*)s.ya, (const V*)s.xr, (const V*)s.yr, (const V*)s.c,
-                               (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
+            hipLaunchKernelGGL((k_check_n<LPR, M, V, C, false, VI>), dim3(g), dim3(kBlock), 0, st, at, at, x, y, xa,
+                               ya, xr, yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     });
     if (pd->A->lanes == 0 && pd->m > 0) {
         for (int pass = 0; pass < (pd->A->nsplit > 0 ? 2 : 1); ++pass) {
             const dim3 g(pass ? (unsigned)pd->A->nsplit : pd->A->rb_grid());
             const ReduceSlot slot{pd->part_m + (size_t)nb_m * kNOutM, pd->out + kNOutN};
-            hipLaunchKernelGGL((k_check_m_rb<M, V, C>), g, dim3(kBlock), 0, st, a, pd->A->blocks,
-                               pass ? pd->A->split_finish() : pd->A->split(), x, (const V*)s.xa, y, (const V*)s.ya,
-                               (const V*)s.yr, (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags, slot);
+            hipLaunchKernelGGL((k_check_m_rb<M, V, C, VI>), g, dim3(kBlock), 0, st, a, pd->A->blocks,
+                               pass ? pd->A->split_finish() : pd->A->split(), x, xa, y, ya, yr, (const V*)s.lo,
+                               (const V*)s.hi, (const V*)s.rres, flags, slot);
             nb_m += g.x;
         }
     } else with_lanes(pd->A->lanes == 0 ? 1 : pd->A->lanes, [&](auto L) {
@@ -648,9 +651,8 @@ void run_check(Pdhg* pd, VecSet& s, int flags, double inv_k, double* host_out, b
         const int64_t groups = std::max<int64_t>(pd->m, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
         nb_m = g;
-        hipLaunchKernelGGL((k_check_m<LPR, M, V, C>), dim3(g), dim3(kBlock), 0, st, a, x, (const V*)s.xa, y,
-                           (const V*)s.ya, (const V*)s.yr, (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags,
-                           sm);
+        hipLaunchKernelGGL((k_check_m<LPR, M, V, C, VI>), dim3(g), dim3(kBlock), 0, st, a, x, xa, y, ya, yr,
+                           (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags, sm);
     });
     GH_REQUIRE(nb_n <= pd->part_cap && nb_m <= pd->part_cap, "check grid exceeds the partial buffers");
     hipLaunchKernelGGL(k_check_finish, dim3(kNOutN + kNOutM), dim3(kBlock), 0, st, pd->part_n, (int)nb_n, pd->part_m,
@@ -902,38 +904,19 @@ int ganit_hip_pdhg_check(ganit_hip_pdhg* h, int mdt, int vdt, int flags, double 
                    "unknown check flag");
         VecSet& s = pd->set(vdt);
         if ((flags & GANIT_CHECK_FP64) && vdt == GANIT_F32) {
-            // FP64 evaluation of the FP32 iterate: form the average at FP32 (as the iteration would adopt
-            // it), widen iterate, average and restart point into the FP64 working set, and evaluate there
-            // against the FP64 problem data.
-            VecSet& d = pd->set(GANIT_F64);
-            hipStream_t st = pd->ctx->stream;
-            if (flags & GANIT_CHECK_AVG)
-                hipLaunchKernelGGL(k_average<float>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st,
-                                   pd->n, pd->m, inv_k, (const float*)s.sumx, (const float*)s.sumy, (float*)s.xa,
-                                   (float*)s.ya);
-            auto widen_vec = [&](const void* src, void* dst, int64_t k) {
-                if (k > 0)
-                    hipLaunchKernelGGL((k_handover<float, double>), dim3(ew_grid2(k)), dim3(kBlock), 0, st, k,
-                                       (const float*)src, (double*)dst);
-            };
-            widen_vec(s.X[s.cur], d.X[d.cur], pd->n);
-            widen_vec(s.Y[s.cur], d.Y[d.cur], pd->m);
-            widen_vec(s.xr, d.xr, pd->n);
-            widen_vec(s.yr, d.yr, pd->m);
-            if (flags & GANIT_CHECK_AVG) {
-                widen_vec(s.xa, d.xa, pd->n);
-                widen_vec(s.ya, d.ya, pd->m);
-            }
-            GH_CHECK(hipGetLastError());
+            // FP64 evaluation of the FP32 iterate: the average is formed at FP32 (as the iteration would adopt
+            // it); the check passes read the FP32 iterate, average and restart point, widen each entry as they
+            // read it (exact), and evaluate against the FP64 problem data.
+            const VecSet& d = pd->set(GANIT_F64);
             with_mdt(mdt, [&](auto MT) {
-                run_check<typename decltype(MT)::type, double, double>(pd, d, flags, inv_k, out, false);
+                run_check<typename decltype(MT)::type, double, double, float>(pd, s, d, flags, inv_k, out);
             });
         } else {
             with_mdt(mdt, [&](auto MT) {
                 with_vdt(vdt, [&](auto VT) {
                     using M = typename decltype(MT)::type;
                     using V = typename decltype(VT)::type;
-                    run_check<M, V, V>(pd, s, flags, inv_k, out);
+                    run_check<M, V, V>(pd, s, s, flags, inv_k, out);
                 });
             });
         }

@@ -464,13 +464,14 @@ def test_graph_replay_matches_eager(hipctx, monkeypatch):
         assert np.array_equal(oa, ob)
 
 
-def test_fp64_check_of_fp32_iterate(hipctx):
+@pytest.mark.parametrize("lanes", [0, 4])
+def test_fp64_check_of_fp32_iterate(hipctx, lanes):
     """GANIT_CHECK_FP64 on an FP32 working set equals the FP64 check of the same (widened) iterate."""
     import ganit_v2_amd as g
     from ganit_v2_amd import hip
     s = g.Problem.read_mps(instance_path("25fv47")).scaled()
     pd = hip.DevicePdhg(hipctx, s, mdt_mask=3, vdt_mask=3)
-    for which, lanes in ((0, 0), (1, 0)):
+    for which in (0, 1):
         pd.set_lanes(which, lanes)
     rng = np.random.default_rng(11)
     x = np.clip(rng.standard_normal(s["n"]), s["l"], s["u"]).astype(np.float32).astype(np.float64)
@@ -482,6 +483,18 @@ def test_fp64_check_of_fp32_iterate(hipctx):
     assert np.array_equal(a, b)
     c = pd.check(F32, F32, hip.CHECK_RAY)   # the level's own evaluation differs (FP32 sums)
     assert not np.array_equal(a, c)
+    # with the average: after one FP32 iteration the running sums are the iterate itself, so with inv_k = 1 the
+    # FP32 average equals the iterate and its FP64 evaluation equals the current point's
+    pd.load(F32, x, y)
+    pd.iterate(F32, F32, 1, 0.3, 0.4)
+    x1, y1, xr, yr = pd.store(F32)
+    pd.load(F64, x1, y1, xr, yr)
+    a = pd.check(F64, F32, hip.CHECK_AVG | hip.CHECK_RAY | hip.CHECK_FP64, 1.0)
+    b = pd.check(F64, F64, hip.CHECK_RAY)
+    cur = [0, 1, 2, 3, 8, 9, 12, 13, 16]  # entries of the current point and the ray
+    assert np.array_equal(a[cur], b[cur])
+    assert np.array_equal(a[4:8], a[0:4]) and np.array_equal(a[14:16], a[12:14])
+    assert a[11] == a[10] and a[18] == a[17] and a[10] > 0
 
 
 def test_handover_and_restart_semantics(hipctx):
