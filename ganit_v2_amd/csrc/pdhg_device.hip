@@ -740,7 +740,7 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 for (int which = 0; which < 2; ++which) {
                     Csr* a = which == 0 ? pd->AT : pd->A;
                     if ((which == 0 && pd->n == 0) || (which == 1 && pd->m == 0)) continue;
-                    autotune_with(ctx, a, 5, [&](int k) {
+                    autotune_with(ctx, a, 12, [&](int k) {
                         if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
                         with_mdt(mdt, [&](auto MT) {
                             with_vdt(vdt, [&](auto VT) {

@@ -434,6 +434,8 @@ void apply_variant(Ctx* ctx, Csr* a, const Variant& v) {
     if (v.lanes == 0) csr_set_partition(ctx, a, v.rows, v.nnz);
 }
 
+constexpr float kRowGroupMargin = 0.9f;  // a row-group variant must be >= 10% faster than the row blocks
+
 void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& launch) {
     if (a->rows == 0 || a->nnz == 0) return;
     const auto& cand = variant_candidates();
@@ -463,9 +465,14 @@ void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& l
         (void)hipEventDestroy(e1);
         throw;
     }
-    size_t bi = 0;
-    for (size_t c = 1; c < cand.size(); ++c)
-        if (best[c] < best[bi]) bi = c;
+    // the fastest row-block partition, unless a row-group width beats it by a clear margin: the row blocks'
+    // row sums are the CPU row loop's bits, and a short timing window can rank near-equal variants either way
+    size_t bi = cand.size(), bl = cand.size();
+    for (size_t c = 0; c < cand.size(); ++c) {
+        size_t& slot = cand[c].lanes == 0 ? bi : bl;
+        if (slot == cand.size() || best[c] < best[slot]) slot = c;
+    }
+    if (bi == cand.size() || (bl != cand.size() && best[bl] < kRowGroupMargin * best[bi])) bi = bl;
     apply_variant(ctx, a, cand[bi]);
     GH_CHECK(hipEventDestroy(e0));
     GH_CHECK(hipEventDestroy(e1));
