@@ -452,6 +452,43 @@ __device__ __forceinline__ void block_partial(double (&v)[N], const ReduceSlot& 
     }
 }
 
+// block_partial for many sums through a large LDS scratch (`cap` doubles, free for the duration): every thread
+// stores its N values, 16-entry slices are added in parallel, and thread k adds value k's 16 slice sums and
+// writes partial k.  Fixed order, no shuffles: for N ~ 10 it replaces N six-step shuffle trees (two LDS
+// permutes and one dependent add per step and value) with one store, 16 + 16 adds and three barriers per
+// group of values.
+template <int N>
+__device__ __forceinline__ void block_partial_wide(const double (&v)[N], const ReduceSlot& slot, double* lds,
+                                                   int cap) {
+    constexpr int kSlices = 16, kPerSlice = kBlock / kSlices;
+    const int per = kBlock + kSlices;  // doubles per value: the block's values, then the slice sums
+    const int group = min(N, cap / per);
+    for (int k0 = 0; k0 < N; k0 += group) {
+        const int g = min(group, N - k0);
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+            if (k >= k0 && k < k0 + g) lds[(k - k0) * per + threadIdx.x] = v[k];
+        __syncthreads();
+        if ((int)threadIdx.x < g * kSlices) {
+            const int k = threadIdx.x / kSlices, j = threadIdx.x % kSlices;
+            const double* src = lds + k * per + j * kPerSlice;
+            double s = 0.0;
+#pragma unroll
+            for (int i = 0; i < kPerSlice; ++i) s += src[i];
+            lds[k * per + kBlock + j] = s;
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < g) {
+            const double* src = lds + threadIdx.x * per + kBlock;
+            double s = 0.0;
+#pragma unroll
+            for (int j = 0; j < kSlices; ++j) s += src[j];
+            slot.partials[(size_t)blockIdx.x * N + k0 + threadIdx.x] = s;
+        }
+        __syncthreads();  // the next group (or the caller) reuses the scratch
+    }
+}
+
 // Run by every thread of one block: out[k] = sum over b < nparts of partials[b][k] for the single output k,
 // fixed order (one block per output keeps the fold parallel).
 __device__ __forceinline__ void sum_partials_one(const double* __restrict__ partials, int N, int k, int nparts,

@@ -338,8 +338,8 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
                                                        const V* __restrict__ c, const V* __restrict__ l,
                                                        const V* __restrict__ u, const V* __restrict__ cres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ C lds[kTile + kWin];
-    __shared__ double red[(kBlock / kWave) * kNOutN];
+    // the staging tile and window; after the sweep, the scratch of the block reduction (hence 16-byte aligned)
+    __shared__ __align__(16) C lds[kTile + kWin];
     CheckAcc<kNOutN> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
     stream_block_seq<M, C, 3, true>(
@@ -391,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
     {
         double tot[kNOutN];
         acc.totals(tot);
-        block_partial<kNOutN>(tot, slot, red);
+        block_partial_wide<kNOutN>(tot, slot, reinterpret_cast<double*>(lds), (int)(sizeof(lds) / sizeof(double)));
     }
 }
 
@@ -402,8 +402,8 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
                                                        const VI* __restrict__ yr, const V* __restrict__ lo,
                                                        const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                        ReduceSlot slot) {
-    __shared__ C lds[kTile + kWin];
-    __shared__ double red[(kBlock / kWave) * kNOutM];
+    // the staging tile and window; after the sweep, the scratch of the block reduction (hence 16-byte aligned)
+    __shared__ __align__(16) C lds[kTile + kWin];
     CheckAcc<kNOutM> acc((flags & GANIT_CHECK_KAHAN) != 0);
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
     stream_block_seq<M, C, 2, true>(
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
     {
         double tot[kNOutM];
         acc.totals(tot);
-        block_partial<kNOutM>(tot, slot, red);
+        block_partial_wide<kNOutM>(tot, slot, reinterpret_cast<double*>(lds), (int)(sizeof(lds) / sizeof(double)));
     }
 }
 

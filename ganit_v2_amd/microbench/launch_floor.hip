@@ -41,11 +41,12 @@ int main() {
     hipEvent_t e0, e1; CK(hipEventCreate(&e0)); CK(hipEventCreate(&e1));
     auto timeit = [&](auto launch, int reps) -> float {
         for (int i = 0; i < 5; ++i) launch();
-        hipEventRecord(e0, st);
+        float ms = -1.f;  // a failed call leaves a negative time
+        if (hipEventRecord(e0, st) != hipSuccess) return ms;
         for (int i = 0; i < reps; ++i) launch();
-        hipEventRecord(e1, st);
-        hipEventSynchronize(e1);
-        float ms = 0; hipEventElapsedTime(&ms, e0, e1);
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            return -1.f;
         return ms * 1000.f / reps;
     };
     for (int g : {1, 256, 1450, 4096}) {
