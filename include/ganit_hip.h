@@ -152,9 +152,9 @@ int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, doub
 #define GANIT_CHECK_AVG 1  /* also form (xa, ya) = (sumx, sumy) * inv_k and evaluate them          */
 #define GANIT_CHECK_RAY 2  /* also the infeasibility ray y - yr (src/pdhg_impl.hpp:110-118)        */
 #define GANIT_CHECK_FP64 4 /* with vdt = GANIT_F32: evaluate the FP32 iterate (and its FP32 average)
-                              in FP64 -- widened into the FP64 working set and checked there against
-                              the FP64 problem data with matrix values mdt (GANIT_F64 for a full
-                              FP64 evaluation).  Requires the FP64 working set (vdt_mask bit 0). */
+                              in FP64 -- each entry widened as the check reads it, against the FP64
+                              problem data with matrix values mdt (GANIT_F64 for a full FP64
+                              evaluation).  Requires the FP64 working set (vdt_mask bit 0). */
 #define GANIT_CHECK_KAHAN 8 /* sums formed from FP32-rounded terms with Kahan-compensated FP32 accumulation per
                              * thread, combined in FP64 (extension; without it every sum is FP64) */
 #define GANIT_CHECK_NOUT 19
