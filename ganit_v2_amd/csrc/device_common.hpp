@@ -53,8 +53,9 @@ struct CsrView {
     const int* __restrict__ ptr;
     const int* __restrict__ idx;
     const M* __restrict__ val;
-    // optional compact indices for the LDS-staged row blocks: idx = bcol[2 * block] + idx16 (null: use idx);
-    // bcol[2 * block + 1] = the block's window length (columns bcol[2b] .. + length - 1), 0 when not windowed
+    // optional compact indices: idx = bcol[2 * block] + idx16 (null: use idx); bcol[2 * block + 1] = the
+    // block's window length (columns bcol[2b] .. + length - 1), 0 when not windowed, -1 for a long row whose
+    // span needs 32-bit indices (split-row chunks carry their own bases, SplitPlan::cbase)
     const uint16_t* __restrict__ idx16 = nullptr;
     const int* __restrict__ bcol = nullptr;
 };
@@ -126,6 +127,8 @@ struct SplitPlan {
     const int* __restrict__ first;
     double* part;
     int finish;   // 1: workgroup b folds split row b
+    // compact indices of the chunks: chunk c's columns = cbase[c] + idx16 (cbase[c] < 0 or null: 32-bit)
+    const int* __restrict__ cbase = nullptr;
 };
 
 // ---------------------------------------------------------------- LDS-staged row blocks
@@ -163,17 +166,28 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
     if (b >= sp.nblocks) {  // one chunk of a split row: its partial only
         const int c = b - sp.nblocks;
         const int p0 = sp.chunk[4 * c + 1], p1 = sp.chunk[4 * c + 2];
+        const int ccb = (a.idx16 && sp.cbase) ? sp.cbase[c] : -1;
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
         // unrolled so that several index / value / gather loads are in flight; each thread still adds its
-        // products in increasing position order
+        // products in increasing position order (the compact / plain index choice is outside the loops)
+        if (ccb >= 0) {
 #pragma unroll 8
-        for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
-            const V av = widen<V>(a.val[p]);
-            const int i = a.idx[p];
+            for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+                const V av = widen<V>(a.val[p]);
+                const int i = ccb + (int)a.idx16[p];
 #pragma unroll
-            for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+                for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+            }
+        } else {
+#pragma unroll 8
+            for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+                const V av = widen<V>(a.val[p]);
+                const int i = a.idx[p];
+#pragma unroll
+                for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+            }
         }
         block_sum_v<V, NX>(s, lds);
         if (threadIdx.x == 0) {
@@ -186,12 +200,13 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
     // indices, the block's base column and window length (an independent load, in flight at the same time)
     const int4 d = reinterpret_cast<const int4*>(blocks)[b];
     const uint16_t* __restrict__ i16 = a.idx16;
-    int cb = 0, wl = 0;
+    int cb = 0, wraw = -1;  // base column; window length (0: none; -1: a long row with 32-bit indices)
     if (i16) {
         const int2 bw = reinterpret_cast<const int2*>(a.bcol)[b];
         cb = bw.x;
-        wl = WIN ? bw.y : 0;
+        wraw = bw.y;
     }
+    const int wl = WIN ? max(wraw, 0) : 0;
     const int r0 = d.x, r1 = d.y;
     const int base = d.z;
     const int cnt = d.w - d.z;
@@ -203,12 +218,22 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
+        if (i16 && wraw >= 0) {
 #pragma unroll 8
-        for (int p = base + threadIdx.x; p < base + cnt; p += kBlock) {
-            const V av = widen<V>(a.val[p]);
-            const int i = a.idx[p];
+            for (int p = base + threadIdx.x; p < base + cnt; p += kBlock) {
+                const V av = widen<V>(a.val[p]);
+                const int i = cb + (int)i16[p];
 #pragma unroll
-            for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+                for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+            }
+        } else {
+#pragma unroll 8
+            for (int p = base + threadIdx.x; p < base + cnt; p += kBlock) {
+                const V av = widen<V>(a.val[p]);
+                const int i = a.idx[p];
+#pragma unroll
+                for (int k = 0; k < NX; ++k) s[k] += av * gather(k, i);
+            }
         }
         block_sum_v<V, NX>(s, lds);
         if (threadIdx.x == 0) row_fn(r0, s, pv);

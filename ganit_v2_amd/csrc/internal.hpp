@@ -80,8 +80,10 @@ struct Csr {
     bf16_t* v16 = nullptr;
     int lanes = 8;           // row-group variant: lanes per row (1..64); 0 selects the LDS-staged row blocks
     int64_t hist[9] = {0};
-    uint16_t* idx16 = nullptr;  // compact indices of the staged row blocks (null when a block spans >= 2^16
-    int* bcol = nullptr;        // columns); bcol[2b] = block b's smallest column, bcol[2b + 1] = its window length
+    uint16_t* idx16 = nullptr;  // compact indices (null when a staged block spans >= 2^16 columns); bcol[2b] =
+    int* bcol = nullptr;        // block b's smallest column, bcol[2b + 1] = its window length (-1: long row, 32-bit)
+    int* chunk_base = nullptr;  // per split-row chunk: its smallest column, or -1 (32-bit indices)
+    int64_t idx32_entries = 0;  // entries still read through 32-bit indices when idx16 is present
     int* blocks = nullptr;   // row-block descriptors of the LDS-staged variant: 4 ints per block (first row,
                              // end row, first nonzero, end nonzero), device; see upload_row_blocks
     int nblocks = 0;
@@ -91,9 +93,9 @@ struct Csr {
     int* chunk = nullptr;
     int* split_first = nullptr;
     double* split_part = nullptr;  // 3 doubles per chunk
-    SplitPlan split() const { return {nblocks, chunk, split_first, split_part, 0}; }
+    SplitPlan split() const { return {nblocks, chunk, split_first, split_part, 0, chunk_base}; }
     // second launch of a row-block kernel when nsplit > 0: grid nsplit, folds the chunk partials
-    SplitPlan split_finish() const { return {nblocks, chunk, split_first, split_part, 1}; }
+    SplitPlan split_finish() const { return {nblocks, chunk, split_first, split_part, 1, chunk_base}; }
     unsigned rb_grid() const { return (unsigned)(nblocks + nchunks); }
     template <class M>
     CsrView<M> view() const;

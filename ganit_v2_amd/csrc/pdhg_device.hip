@@ -788,7 +788,10 @@ int ganit_hip_pdhg_index_bytes(const ganit_hip_pdhg* h, int which, int* bytes) {
         GH_REQUIRE(pd && bytes && which >= 0 && which <= 2, "bad index-width query");
         const Csr* a = which == 0 ? pd->A : which == 1 ? pd->AT : pd->Q;
         GH_REQUIRE(a, "problem has no Q");
-        *bytes = (a->lanes == 0 && a->idx16) ? 2 : 4;
+        // the average width, rounded down (long rows or chunks too wide for 16-bit offsets read 4 bytes)
+        *bytes = (a->lanes == 0 && a->idx16 && a->nnz > 0)
+                     ? (int)((2 * (a->nnz - a->idx32_entries) + 4 * a->idx32_entries) / a->nnz)
+                     : 4;
     });
 }
 

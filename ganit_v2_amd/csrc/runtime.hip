@@ -251,8 +251,11 @@ int* upload_row_blocks(Ctx* ctx, const int* ptr, const std::vector<int>& rb) {
 void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, const std::vector<int>& rb) {
     if (a->idx16) GH_CHECK(hipFree(a->idx16));
     if (a->bcol) GH_CHECK(hipFree(a->bcol));
+    if (a->chunk_base) GH_CHECK(hipFree(a->chunk_base));
     a->idx16 = nullptr;
     a->bcol = nullptr;
+    a->chunk_base = nullptr;
+    a->idx32_entries = 0;
     const char* env = std::getenv("GANIT_HIP_COMPACT_IDX");  // 0: keep the 32-bit indices only
     if ((env && env[0] == '0') || a->nnz == 0 || rb.size() < 2) return;
     const char* wenv = std::getenv("GANIT_HIP_WINDOW");  // 0: no windowed blocks (every gather from HBM / L2)
@@ -260,25 +263,68 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
     const size_t nb = rb.size() - 1;
     std::vector<int> bc(2 * nb, 0);  // per block: base column, window length (0: not windowed)
     std::vector<uint16_t> c16((size_t)a->nnz, 0);
+    auto span_of = [&](int p0, int p1, int& mn) {
+        int lo = idx[p0], hi = idx[p0];
+        for (int p = p0 + 1; p < p1; ++p) {
+            lo = std::min(lo, idx[p]);
+            hi = std::max(hi, idx[p]);
+        }
+        mn = lo;
+        return hi - lo;
+    };
+    auto fill = [&](int p0, int p1, int mn) {
+        for (int p = p0; p < p1; ++p) c16[p] = (uint16_t)(idx[p] - mn);
+    };
+    int64_t n32 = 0;
     for (size_t q = 0; q < nb; ++q) {
         const int r0 = rb[q], r1 = rb[q + 1], p0 = ptr[r0], p1 = ptr[r1];
-        if ((r1 - r0 == 1 && p1 - p0 > kLong) || p1 == p0) continue;  // long and split rows: 32-bit indices
-        int mn = idx[p0], mx = idx[p0];
-        for (int p = p0 + 1; p < p1; ++p) {
-            mn = std::min(mn, idx[p]);
-            mx = std::max(mx, idx[p]);
+        if (p1 == p0) continue;
+        int mn = 0;
+        const int span = span_of(p0, p1, mn);
+        if (r1 - r0 == 1 && p1 - p0 > kLong) {
+            if (p1 - p0 > kSplit) continue;  // a split row: its chunks carry their own bases (below)
+            // a long row: compact when its span fits 16 bits, else 32-bit indices for this row only
+            if (span > 65535) {
+                bc[2 * q + 1] = -1;
+                n32 += p1 - p0;
+            } else {
+                bc[2 * q] = mn;
+                fill(p0, p1, mn);
+            }
+            continue;
         }
-        if (mx - mn > 65535) return;  // a block too wide: keep the 32-bit path for this matrix
+        if (span > 65535) return;  // a staged block too wide: keep the 32-bit path for this matrix
         bc[2 * q] = mn;
         // windowed when the span fits the LDS window and loading it costs at most two entries per nonzero
-        const int span = mx - mn + 1;
-        bc[2 * q + 1] = (windows && span <= kWin && span <= 2 * (p1 - p0)) ? span : 0;
-        for (int p = p0; p < p1; ++p) c16[p] = (uint16_t)(idx[p] - mn);
+        bc[2 * q + 1] = (windows && span + 1 <= kWin && span + 1 <= 2 * (p1 - p0)) ? span + 1 : 0;
+        fill(p0, p1, mn);
+    }
+    // chunks of the split rows, enumerated as the chunk plan enumerates them (rows ascending, kChunk apart)
+    std::vector<int> cbase;
+    for (int i = 0; i < a->rows; ++i) {
+        if (ptr[i + 1] - ptr[i] <= kSplit) continue;
+        for (int p0 = ptr[i]; p0 < ptr[i + 1]; p0 += kChunk) {
+            const int p1 = std::min(p0 + kChunk, ptr[i + 1]);
+            int mn = 0;
+            if (span_of(p0, p1, mn) > 65535) {
+                cbase.push_back(-1);
+                n32 += p1 - p0;
+            } else {
+                cbase.push_back(mn);
+                fill(p0, p1, mn);
+            }
+        }
     }
     GH_CHECK(hipMalloc(&a->idx16, sizeof(uint16_t) * c16.size()));
     GH_CHECK(hipMalloc(&a->bcol, sizeof(int) * bc.size()));
     GH_CHECK(hipMemcpyAsync(a->idx16, c16.data(), sizeof(uint16_t) * c16.size(), hipMemcpyHostToDevice, ctx->stream));
     GH_CHECK(hipMemcpyAsync(a->bcol, bc.data(), sizeof(int) * bc.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (!cbase.empty()) {
+        GH_CHECK(hipMalloc(&a->chunk_base, sizeof(int) * cbase.size()));
+        GH_CHECK(hipMemcpyAsync(a->chunk_base, cbase.data(), sizeof(int) * cbase.size(), hipMemcpyHostToDevice,
+                                ctx->stream));
+    }
+    a->idx32_entries = n32;
     GH_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -371,6 +417,7 @@ void csr_destroy(Csr* a) {
     (void)hipFree(a->blocks);
     (void)hipFree(a->idx16);
     (void)hipFree(a->bcol);
+    (void)hipFree(a->chunk_base);
     (void)hipFree(a->chunk);
     (void)hipFree(a->split_first);
     (void)hipFree(a->split_part);

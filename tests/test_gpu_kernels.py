@@ -156,6 +156,43 @@ def test_windowed_rowblock_spmv_bitwise(hipctx, oracle, monkeypatch, mdt, vdt, w
 
 
 @pytest.mark.parametrize("mdt,vdt", [(F64, F64), (F32, F32), (BF16, F32)])
+def test_compact_indices_match_plain(hipctx, oracle, monkeypatch, mdt, vdt):
+    """16-bit column offsets (per staged block, long row and split-row chunk, each falling back to 32-bit
+    indices where its span does not fit) give the same bits as the plain 32-bit indices, on one matrix with
+    banded short rows, long rows (compact and too wide) and split rows (compact and too-wide chunks)."""
+    from ganit_v2_amd import hip
+    rng = np.random.default_rng(60 + mdt)
+    n = 300000
+    rows = []
+    for i in range(1500):  # short rows, banded: staged blocks, most of them windowed
+        k = int(rng.integers(0, 60))
+        s0 = min(i * 150, n - 1000)
+        rows.append(np.sort(s0 + rng.choice(1000, k, replace=False)))
+    rows.append(np.sort(rng.choice(2000, 1000, replace=False)))         # long row, compact
+    rows.append(np.sort(rng.choice(n, 1000, replace=False)))            # long row, span > 65535
+    rows.append(np.sort(rng.choice(60000, 20000, replace=False)))       # split row, compact chunks
+    rows.append(np.sort(rng.choice(n, 9000, replace=False)))            # split row: two too-wide chunks, one compact
+    rows.append(np.sort(100000 + rng.choice(200000, 30000, replace=False)))  # split row, compact chunks
+    lens = np.array([len(r) for r in rows])
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    idx = np.concatenate(rows).astype(np.int32)
+    val = rng.standard_normal(ptr[-1])
+    x = rng.standard_normal(n)
+    ys = []
+    for compact in ("1", "0"):
+        monkeypatch.setenv("GANIT_HIP_COMPACT_IDX", compact)
+        A = hip.DeviceCsr(hipctx, len(lens), n, ptr, idx, val)
+        A.set_lanes(0)
+        xd, yd = hipctx.vector(x, vdt), hipctx.empty(len(lens), vdt)
+        A.spmv(xd, yd, mdt)
+        ys.append(yd.numpy())
+    assert np.array_equal(ys[0], ys[1])
+    xr = x if vdt == F64 else x.astype(np.float32).astype(np.float64)
+    exact, bound = _exact_and_bound(ptr, idx, rounded(val, mdt, oracle), xr, 2.0 ** (-53 if vdt == F64 else -24))
+    assert np.all(np.abs(ys[0] - exact) <= bound + 1e-300)
+
+
+@pytest.mark.parametrize("mdt,vdt", [(F64, F64), (F32, F32), (BF16, F32)])
 def test_split_rows_spmv(hipctx, oracle, mdt, vdt):
     """Rows longer than 8192 entries are cut into 4096-entry chunks on separate workgroups and combined in
     chunk order by the last one: within the SpMV error bound, and deterministic."""
