@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
                                                       const V* __restrict__ l, const V* __restrict__ u,
                                                       V* __restrict__ xn, V* __restrict__ xbar,
                                                       V* __restrict__ sumx) {
-    __shared__ V lds[kTile];
+    __shared__ V lds[kTile + kWin];
     const int b = list[xcd_block_id()];  // a workgroup of a running instance
     const BatchState& s = st[blk_inst[b]];
     if (!s.active) return;
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
     struct Ops {
         V x, c, l, u, s;
     };
-    stream_block_pf<M, V, 1>(
+    stream_block_pf<M, V, 1, true>(
         AT, blocks, b, [&](int, int i) { return y[i]; },
         [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j]}; },
         [&](int j, V (&aty)[1], Ops& o) {
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
                                                     const V* __restrict__ xbar, const V* __restrict__ y,
                                                     const V* __restrict__ lo, const V* __restrict__ hi,
                                                     V* __restrict__ yn, V* __restrict__ sumy) {
-    __shared__ V lds[kTile];
+    __shared__ V lds[kTile + kWin];
     const int b = list[xcd_block_id()];  // a workgroup of a running instance
     const BatchState& s = st[blk_inst[b]];
     if (!s.active) return;
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
     struct Ops {
         V y, lo, hi, s;
     };
-    stream_block_pf<M, V, 1>(
+    stream_block_pf<M, V, 1, true>(
         A, blocks, b, [&](int, int j) { return xbar[j]; },
         [&](int i) { return Ops{y[i], lo[i], hi[i], sumy[i]}; },
         [&](int i, V (&axb)[1], Ops& o) {
