@@ -10,6 +10,7 @@
 #include <fstream>
 #include <iostream>
 #include <stdexcept>
+#include <string_view>
 #include <unordered_map>
 #include <unordered_set>
 
@@ -109,6 +110,68 @@ std::string load_file(const std::string& path) {
     return out;
 }
 
+// Name -> index for the reader: open addressing over (hash, index) slots, power-of-two capacity, linear
+// probing; names are views into the file buffer (alive for the whole parse), compared only on a hash match.
+// A node-based map chases a pointer per probe; at millions of COLUMNS entries those misses dominated the parse.
+class NameIndex {
+  public:
+    static uint64_t hash(std::string_view s) {
+        uint64_t h = 1469598103934665603ull;  // FNV-1a, 64-bit
+        for (const char ch : s) h = (h ^ (unsigned char)ch) * 1099511628211ull;
+        return h ^ (h >> 29);
+    }
+    // the index stored for `s`, or -1
+    int find(std::string_view s) const {
+        if (keys_.empty()) return -1;
+        const uint64_t h = hash(s);
+        for (size_t i = h & mask_;; i = (i + 1) & mask_) {
+            if (slot_[i].idx < 0) return -1;
+            if (slot_[i].h == h && keys_[(size_t)slot_[i].key] == s) return slot_[i].idx;
+        }
+    }
+    // map `s` to `v` (replacing an existing mapping, as map[s] = v does)
+    void assign(std::string_view s, int v) {
+        if (2 * (keys_.size() + 1) > slot_.size()) grow();
+        const uint64_t h = hash(s);
+        for (size_t i = h & mask_;; i = (i + 1) & mask_) {
+            if (slot_[i].idx < 0) {
+                slot_[i] = {h, v, (int)keys_.size()};
+                keys_.push_back(s);
+                return;
+            }
+            if (slot_[i].h == h && keys_[(size_t)slot_[i].key] == s) {
+                slot_[i].idx = v;
+                return;
+            }
+        }
+    }
+
+  private:
+    struct Slot {
+        uint64_t h;
+        int idx;  // -1: empty
+        int key;
+    };
+    void grow() {
+        std::vector<Slot> old;
+        old.swap(slot_);
+        slot_.assign(std::max<size_t>(64, 2 * old.size()), Slot{0, -1, 0});
+        mask_ = slot_.size() - 1;
+        for (const Slot& o : old) {
+            if (o.idx < 0) continue;
+            size_t i = o.h & mask_;
+            while (slot_[i].idx >= 0) i = (i + 1) & mask_;
+            slot_[i] = o;
+        }
+    }
+    std::vector<Slot> slot_;
+    std::vector<std::string_view> keys_;
+    size_t mask_ = 0;
+};
+
+// The C locale's isspace set (space, \t, \n, \v, \f, \r), inline.
+inline bool is_ws(char ch) { return ch == ' ' || (ch >= '\t' && ch <= '\r'); }
+
 // Whitespace tokens of one line as (pointer, length) views into the file buffer.
 struct Tok {
     const char* p;
@@ -117,13 +180,63 @@ struct Tok {
     std::string str() const { return std::string(p, n); }
 };
 
+// Plain decimals ([+-]digits[.digits][(e|E)[+-]digits]) with at most 19 significant digits, a mantissa below
+// 2^53 and a decimal exponent within [-22, 22]: mantissa and power of ten are both exact doubles, so one IEEE
+// multiply or divide gives the correctly rounded value -- the same double strtod returns (Clinger's fast
+// path).  Anything else returns false and goes to strtod.
+bool fast_decimal(const char* p, size_t n, double& out) {
+    static const double kPow10[23] = {1e0,  1e1,  1e2,  1e3,  1e4,  1e5,  1e6,  1e7,  1e8,  1e9,  1e10, 1e11,
+                                      1e12, 1e13, 1e14, 1e15, 1e16, 1e17, 1e18, 1e19, 1e20, 1e21, 1e22};
+    size_t i = 0;
+    bool neg = false;
+    if (i < n && (p[i] == '+' || p[i] == '-')) neg = p[i++] == '-';
+    uint64_t mant = 0;
+    int digits = 0, frac = 0;
+    bool any = false;
+    for (; i < n && p[i] >= '0' && p[i] <= '9'; ++i, any = true) {
+        if (mant == 0 && p[i] == '0') continue;  // leading zeros are not significant
+        if (++digits > 19) return false;
+        mant = mant * 10 + (uint64_t)(p[i] - '0');
+    }
+    if (i < n && p[i] == '.') {
+        for (++i; i < n && p[i] >= '0' && p[i] <= '9'; ++i, any = true) {
+            ++frac;
+            if (mant == 0 && p[i] == '0') continue;
+            if (++digits > 19) return false;
+            mant = mant * 10 + (uint64_t)(p[i] - '0');
+        }
+    }
+    if (!any) return false;
+    int e10 = 0;
+    if (i < n && (p[i] == 'e' || p[i] == 'E')) {
+        ++i;
+        bool eneg = false;
+        if (i < n && (p[i] == '+' || p[i] == '-')) eneg = p[i++] == '-';
+        if (i >= n || p[i] < '0' || p[i] > '9') return false;
+        for (; i < n && p[i] >= '0' && p[i] <= '9'; ++i) {
+            e10 = e10 * 10 + (p[i] - '0');
+            if (e10 > 1000) return false;
+        }
+        if (eneg) e10 = -e10;
+    }
+    if (i != n) return false;
+    e10 -= frac;
+    if (mant > (uint64_t(1) << 53) || e10 < -22 || e10 > 22) return false;
+    double v = (double)mant;
+    v = e10 >= 0 ? v * kPow10[e10] : v / kPow10[-e10];
+    out = neg ? -v : v;
+    return true;
+}
+
 double to_num(const Tok& t, int line) {
+    double v;
+    if (fast_decimal(t.p, t.n, v)) return v;
     char tmp[128];
     const size_t k = std::min(t.n, sizeof(tmp) - 1);
     std::memcpy(tmp, t.p, k);
     tmp[k] = 0;
     char* end = nullptr;
-    const double v = std::strtod(tmp, &end);
+    v = std::strtod(tmp, &end);
     if (end == tmp) throw std::runtime_error("MPS line " + std::to_string(line) + ": bad number '" + t.str() + "'");
     return v;
 }
@@ -134,9 +247,10 @@ LP read_mps(const std::string& path) {
     const std::string text = load_file(path);
     enum Sec { None, Name, ObjSense, Rows, Columns, Rhs, Ranges, Bounds, QuadObj, QMatrix };
     LP lp;
-    std::unordered_map<std::string, int> row_of, col_of;
-    std::unordered_set<std::string> extra_n;
-    std::string obj_name, last_col;
+    // names are views into `text`, which outlives the parse
+    NameIndex row_of, col_of;
+    std::unordered_set<std::string_view> extra_n;
+    std::string_view obj_name, last_col;
     std::vector<char> rtype, lower_set;
     std::vector<double> rhs, range;
     std::vector<int> tr, tc, qr, qc;
@@ -145,18 +259,18 @@ LP read_mps(const std::string& path) {
     int neg_up = 0, cur = -1, line_no = 0;
     Sec sec = None;
     std::vector<Tok> tok;
+    auto view = [](const Tok& t) { return std::string_view(t.p, t.n); };
     auto col_index = [&](const Tok& t) {
-        auto it = col_of.find(t.str());
-        if (it == col_of.end())
-            throw std::runtime_error("MPS: unknown column '" + t.str() + "' at line " + std::to_string(line_no));
-        return it->second;
+        const int j = col_of.find(view(t));
+        if (j < 0) throw std::runtime_error("MPS: unknown column '" + t.str() + "' at line " + std::to_string(line_no));
+        return j;
     };
     auto row_index = [&](const Tok& t) -> int {  // -1 for a row of an ignored extra N row
-        const std::string s = t.str();
-        auto it = row_of.find(s);
-        if (it != row_of.end()) return it->second;
+        const std::string_view s = view(t);
+        const int r = row_of.find(s);
+        if (r >= 0) return r;
         if (extra_n.count(s)) return -1;
-        throw std::runtime_error("MPS: unknown row '" + s + "' at line " + std::to_string(line_no));
+        throw std::runtime_error("MPS: unknown row '" + std::string(s) + "' at line " + std::to_string(line_no));
     };
 
     const char* p = text.data();
@@ -172,15 +286,15 @@ LP read_mps(const std::string& path) {
         if (len == 0 || line[0] == '*') continue;
         tok.clear();
         for (size_t i = 0; i < len;) {
-            while (i < len && std::isspace((unsigned char)line[i])) ++i;
+            while (i < len && is_ws(line[i])) ++i;
             if (i >= len) break;
             size_t j = i;
-            while (j < len && !std::isspace((unsigned char)line[j])) ++j;
+            while (j < len && !is_ws(line[j])) ++j;
             tok.push_back({line + i, j - i});
             i = j;
         }
         if (tok.empty()) continue;
-        if (!std::isspace((unsigned char)line[0])) {  // section header
+        if (!is_ws(line[0])) {  // section header
             const Tok& h = tok[0];
             if (h.is("NAME")) { sec = Name; if (tok.size() > 1) lp.name = tok[1].str(); }
             else if (h.is("OBJSENSE")) {
@@ -207,9 +321,9 @@ LP read_mps(const std::string& path) {
                 if (tok.size() < 2) throw std::runtime_error("MPS: bad ROWS line " + std::to_string(line_no));
                 const char t = (char)std::toupper((unsigned char)tok[0].p[0]);
                 if (t == 'N') {
-                    if (obj_name.empty()) obj_name = tok[1].str(); else extra_n.insert(tok[1].str());
+                    if (obj_name.empty()) obj_name = view(tok[1]); else extra_n.insert(view(tok[1]));
                 } else {
-                    row_of[tok[1].str()] = lp.m++;
+                    row_of.assign(view(tok[1]), lp.m++);
                     lp.row_names.push_back(tok[1].str());
                     rtype.push_back(t);
                     rhs.push_back(0.0);
@@ -222,11 +336,17 @@ LP read_mps(const std::string& path) {
                     else if (tok[2].is("'INTEND'")) int_mode = false;
                     break;
                 }
-                if (!tok[0].is(last_col.c_str()) || cur < 0) {
-                    last_col = tok[0].str();
+                if (view(tok[0]) != last_col || cur < 0) {
+                    if (tr.empty()) {  // first column: size the triplet buffers from the section's bytes
+                        const size_t guess = (size_t)(end - p) / 24;
+                        tr.reserve(guess);
+                        tc.reserve(guess);
+                        tv.reserve(guess);
+                    }
+                    last_col = view(tok[0]);
                     cur = lp.n++;
-                    col_of[last_col] = cur;
-                    lp.col_names.push_back(last_col);
+                    col_of.assign(last_col, cur);
+                    lp.col_names.push_back(std::string(last_col));
                     lp.c.push_back(0.0);
                     lp.l.push_back(0.0);
                     lp.u.push_back(kInf);
@@ -235,7 +355,7 @@ LP read_mps(const std::string& path) {
                 }
                 for (size_t k = 1; k + 1 < tok.size(); k += 2) {
                     const double v = to_num(tok[k + 1], line_no);
-                    if (tok[k].is(obj_name.c_str())) { lp.c[cur] += v; continue; }
+                    if (view(tok[k]) == obj_name) { lp.c[cur] += v; continue; }
                     const int r = row_index(tok[k]);
                     if (r < 0) continue;
                     tr.push_back(r);
@@ -249,7 +369,7 @@ LP read_mps(const std::string& path) {
                 const size_t first = (tok.size() % 2 == 1) ? 1 : 0;  // optional set name
                 for (size_t k = first; k + 1 < tok.size(); k += 2) {
                     const double v = to_num(tok[k + 1], line_no);
-                    if (tok[k].is(obj_name.c_str())) {
+                    if (view(tok[k]) == obj_name) {
                         if (sec == Rhs) lp.obj_const = -v;  // objective RHS is minus the constant
                         continue;
                     }

@@ -60,6 +60,40 @@ def test_mps_write_read_round_trip(tmp_path, name):
     assert a["obj_const"] == b["obj_const"] and a["maximize"] == b["maximize"]
 
 
+def test_reader_number_formats(tmp_path, oracle):
+    """Coefficients in every decimal form the reader's fast path takes or hands to strtod (signs, leading and
+    trailing zeros, 1-25 significant digits, exponents up to +-330, 2^53 + 1, subnormals, '1.', '.5', '-0'):
+    the values equal Python's correctly rounded float() and the oracle's strtod, bit for bit."""
+    rng = np.random.default_rng(12)
+    forms = ["1", "1.", ".5", "-.5", "+2.5", "-0", "-0.0", "0.1", "0.30000000000000004", "1e22", "1e23", "1E-05",
+             "5e3", "1.0e-5", "123456789012345678", "1234567890123456789", "12345678901234567890",
+             "9007199254740992", "9007199254740993", "0.000000000000000000000000001", "4.9e-324", "2.2250738585072014e-308",
+             "1.7976931348623157e308", "000123.4500", "-00.00012e+3", "3.14159265358979323846264", "1e-22", "8.5e-23"]
+    for _ in range(400):
+        digits = "".join(rng.choice(list("0123456789"), int(rng.integers(1, 26))))
+        dot = int(rng.integers(0, len(digits) + 1))
+        num = digits[:dot] + "." + digits[dot:] if rng.random() < 0.7 else digits
+        if rng.random() < 0.5:
+            num += ("e" if rng.random() < 0.5 else "E") + str(int(rng.integers(-330, 330)))
+        forms.append(("-" if rng.random() < 0.3 else "") + num)
+    forms = [f for f in forms if float(f) != 0.0 and np.isfinite(float(f))] + ["-0", "-0.0"]
+    n = len(forms)
+    lines = ["NAME NUMBERS", "ROWS", " N OBJ", " E R1", "COLUMNS"]
+    for j, f in enumerate(forms):
+        lines.append(f"    X{j} OBJ {f} R1 {forms[(j + 1) % n]}")
+    lines += ["RHS", "    RHS R1 1", "BOUNDS", " UP BND X0 4", "ENDATA", ""]
+    path = tmp_path / "numbers.mps"
+    path.write_text("\n".join(lines))
+    p = g.Problem.read_mps(str(path))
+    _same_lp(p, oracle.OracleLP.read_mps(str(path)))
+    d = p.export()
+    want_c = np.array([float(f) for f in forms]) + 0.0
+    assert np.array_equal(d["c"], want_c)
+    want_a = np.array([float(forms[(j + 1) % n]) for j in range(n)])
+    keep = want_a != 0.0
+    assert np.array_equal(d["A"].val, want_a[keep])
+
+
 def test_reader_edge_cases(tmp_path, oracle):
     """RANGES on E/L/G rows, negative UP on a default lower bound, objective constant, extra N rows,
     MARKER lines, OBJSENSE MAX, duplicate entries and explicit zeros -- reader vs oracle restatement."""
