@@ -14,6 +14,8 @@
 #include <unordered_map>
 #include <unordered_set>
 
+#include <omp.h>
+
 namespace ganit {
 
 Csr empty_csr(int rows, int cols) {
@@ -42,21 +44,35 @@ Csr csr_from_triplets(int rows, int cols, const std::vector<int>& r, const std::
         for (size_t k = 0; k < v.size(); ++k) bucket[at[r[k]]++] = {c[k], v[k]};
     }
     Csr a = empty_csr(rows, cols);
-    a.idx.reserve(v.size());
-    a.val.reserve(v.size());
+    // rows are independent: each is ordered and merged in place (in parallel), then copied out.  A row whose
+    // columns already increase strictly (column-major input) skips the sort -- its sorted order is unique, so
+    // the result is the same; with equal columns the sort runs as always.
+    std::vector<int> kept(rows, 0);
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < rows; ++i) {
         auto b = bucket.begin() + head[i], e = bucket.begin() + head[i + 1];
-        std::sort(b, e, [](const auto& x, const auto& y) { return x.first < y.first; });
+        bool strict = true;
+        for (auto q = b; q + 1 < e && strict; ++q) strict = q->first < (q + 1)->first;
+        if (!strict) std::sort(b, e, [](const auto& x, const auto& y) { return x.first < y.first; });
+        auto w = b;
         for (auto p = b; p != e;) {
             const int col = p->first;
             double s = 0.0;
             for (; p != e && p->first == col; ++p) s += p->second;
-            if (s != 0.0) {
-                a.idx.push_back(col);
-                a.val.push_back(s);
-            }
+            if (s != 0.0) *w++ = {col, s};
         }
-        a.ptr[i + 1] = static_cast<int>(a.idx.size());
+        kept[i] = (int)(w - b);
+    }
+    for (int i = 0; i < rows; ++i) a.ptr[i + 1] = a.ptr[i] + kept[i];
+    a.idx.resize((size_t)a.ptr[rows]);
+    a.val.resize((size_t)a.ptr[rows]);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < rows; ++i) {
+        const auto b = bucket.begin() + head[i];
+        for (int k = 0; k < kept[i]; ++k) {
+            a.idx[(size_t)a.ptr[i] + k] = b[k].first;
+            a.val[(size_t)a.ptr[i] + k] = b[k].second;
+        }
     }
     return a;
 }
@@ -128,6 +144,11 @@ class NameIndex {
             if (slot_[i].idx < 0) return -1;
             if (slot_[i].h == h && keys_[(size_t)slot_[i].key] == s) return slot_[i].idx;
         }
+    }
+    // room for `n` names without rehashing
+    void reserve(size_t n) {
+        while (2 * n > slot_.size()) grow();
+        keys_.reserve(n);
     }
     // map `s` to `v` (replacing an existing mapping, as map[s] = v does)
     void assign(std::string_view s, int v) {
@@ -275,6 +296,190 @@ LP read_mps(const std::string& path) {
 
     const char* p = text.data();
     const char* const end = p + text.size();
+
+    // The COLUMNS section, parsed in parallel: it is cut at line starts into chunks that threads tokenise and
+    // look up independently (the row index is read-only by then), each chunk recording its column runs,
+    // MARKER lines and entries in file order; one sequential pass then numbers the columns, applies the
+    // integer markers and adds objective entries in line order, and the triplets are copied out in file order
+    // -- so the result, and the first error reported, are the sequential loop's.
+    auto parse_columns = [&]() {
+        const int64_t line0 = line_no;  // the COLUMNS header line
+        const char* const s0 = p;
+        // the section ends at the next header line (first character not whitespace, not a comment)
+        const char* s1 = p;
+        while (s1 < end) {
+            const char* eol = static_cast<const char*>(std::memchr(s1, '\n', (size_t)(end - s1)));
+            if (!eol) eol = end;
+            if (eol > s1 && !is_ws(*s1) && *s1 != '*' && !(eol - s1 == 1 && *s1 == '\r')) break;
+            s1 = eol + (eol < end ? 1 : 0);
+        }
+        struct Item {
+            std::string_view name;  // empty: a MARKER line
+            int marker;             // +1 INTORG, -1 INTEND, 0 other marker
+            size_t e0, e1;          // entries [e0, e1) of the chunk
+            int col;
+        };
+        struct Err {
+            int64_t line = -1;  // chunk-local line index of the first error, -1: none
+            bool bad_number = false;
+            std::string what;  // the offending token
+        };
+        struct Chunk {
+            const char *a, *b;
+            std::vector<Item> items;
+            std::vector<int> row;  // kObjRow: an objective entry
+            std::vector<double> val;
+            std::vector<size_t> obj;  // positions of the objective entries, in order
+            size_t ntrip = 0;         // entries that are not objective entries
+            int64_t lines = 0;
+            Err err;
+        };
+        constexpr int kObjRow = -2;
+        const int nthreads = std::max(1, omp_get_max_threads());
+        const size_t bytes = (size_t)(s1 - s0);
+        const int nchunks = bytes < (1u << 20) ? 1 : nthreads * 4;
+        std::vector<Chunk> ch((size_t)nchunks);
+        {
+            const char* a = s0;
+            for (int k = 0; k < nchunks; ++k) {
+                const char* b = k + 1 == nchunks ? s1 : s0 + bytes * (size_t)(k + 1) / (size_t)nchunks;
+                if (b < a) b = a;
+                if (k + 1 < nchunks) {  // a chunk ends after a newline
+                    const char* nl = static_cast<const char*>(std::memchr(b, '\n', (size_t)(s1 - b)));
+                    b = nl ? nl + 1 : s1;
+                }
+                ch[(size_t)k].a = a;
+                ch[(size_t)k].b = b;
+                a = b;
+            }
+        }
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int k = 0; k < nchunks; ++k) {
+            Chunk& c = ch[(size_t)k];
+            c.row.reserve((size_t)(c.b - c.a) / 24);
+            c.val.reserve((size_t)(c.b - c.a) / 24);
+            std::vector<Tok> tk;
+            std::string_view prev;
+            for (const char* q = c.a; q < c.b;) {
+                const char* eol = static_cast<const char*>(std::memchr(q, '\n', (size_t)(c.b - q)));
+                if (!eol) eol = c.b;
+                const char* line = q;
+                size_t len = (size_t)(eol - q);
+                q = eol + (eol < c.b ? 1 : 0);
+                const int64_t ln = c.lines++;
+                if (c.err.line >= 0) continue;
+                if (len && line[len - 1] == '\r') --len;
+                if (len == 0 || line[0] == '*') continue;
+                tk.clear();
+                for (size_t i = 0; i < len;) {
+                    while (i < len && is_ws(line[i])) ++i;
+                    if (i >= len) break;
+                    size_t j = i;
+                    while (j < len && !is_ws(line[j])) ++j;
+                    tk.push_back({line + i, j - i});
+                    i = j;
+                }
+                if (tk.empty()) continue;
+                if (tk.size() >= 3 && tk[1].is("'MARKER'")) {
+                    c.items.push_back({std::string_view(), tk[2].is("'INTORG'") ? 1 : tk[2].is("'INTEND'") ? -1 : 0,
+                                       c.row.size(), c.row.size(), -1});
+                    continue;
+                }
+                const std::string_view name(tk[0].p, tk[0].n);
+                if (c.items.empty() || c.items.back().name.empty() || c.items.back().name != name)
+                    c.items.push_back({name, 0, c.row.size(), c.row.size(), -1});
+                for (size_t t = 1; t + 1 < tk.size(); t += 2) {
+                    double v;
+                    if (!fast_decimal(tk[t + 1].p, tk[t + 1].n, v)) {
+                        char tmp[128];
+                        const size_t n = std::min(tk[t + 1].n, sizeof(tmp) - 1);
+                        std::memcpy(tmp, tk[t + 1].p, n);
+                        tmp[n] = 0;
+                        char* e = nullptr;
+                        v = std::strtod(tmp, &e);
+                        if (e == tmp) {
+                            c.err = {ln, true, tk[t + 1].str()};
+                            break;
+                        }
+                    }
+                    const std::string_view rn(tk[t].p, tk[t].n);
+                    int r;
+                    if (rn == obj_name) r = kObjRow;
+                    else if ((r = row_of.find(rn)) < 0) {
+                        if (!extra_n.count(rn)) {
+                            c.err = {ln, false, std::string(rn)};
+                            break;
+                        }
+                        continue;  // an extra N row: ignored
+                    }
+                    if (r == kObjRow) c.obj.push_back(c.row.size());
+                    else ++c.ntrip;
+                    c.row.push_back(r);
+                    c.val.push_back(v);
+                }
+                c.items.back().e1 = c.row.size();
+            }
+        }
+        // sequential: column numbering, markers, objective sums, first error -- in file order
+        int64_t lbase = line0;
+        size_t ntrip = 0, nitems = 0;
+        for (const Chunk& c : ch) nitems += c.items.size();
+        col_of.reserve((size_t)lp.n + nitems);
+        lp.col_names.reserve((size_t)lp.n + nitems);
+        for (Chunk& c : ch) {
+            size_t ob = 0;  // next objective entry of the chunk
+            for (Item& it : c.items) {
+                if (it.name.empty()) {
+                    if (it.marker > 0) int_mode = true;
+                    else if (it.marker < 0) int_mode = false;
+                    continue;
+                }
+                if (it.name != last_col || cur < 0) {
+                    last_col = it.name;
+                    cur = lp.n++;
+                    col_of.assign(last_col, cur);
+                    lp.col_names.push_back(std::string(last_col));
+                    lp.c.push_back(0.0);
+                    lp.l.push_back(0.0);
+                    lp.u.push_back(kInf);
+                    lp.is_int.push_back(int_mode ? 1 : 0);
+                    lower_set.push_back(0);
+                }
+                it.col = cur;
+                for (; ob < c.obj.size() && c.obj[ob] < it.e1; ++ob) lp.c[(size_t)cur] += c.val[c.obj[ob]];
+            }
+            ntrip += c.ntrip;
+            if (c.err.line >= 0) {  // the sequential loop's messages
+                const std::string at = std::to_string(lbase + 1 + c.err.line);
+                if (c.err.bad_number) throw std::runtime_error("MPS line " + at + ": bad number '" + c.err.what + "'");
+                throw std::runtime_error("MPS: unknown row '" + c.err.what + "' at line " + at);
+            }
+            lbase += c.lines;
+        }
+        // triplets in file order
+        std::vector<size_t> off(ch.size() + 1, 0);
+        for (size_t k = 0; k < ch.size(); ++k) off[k + 1] = off[k] + ch[k].ntrip;
+        const size_t t0 = tr.size();
+        tr.resize(t0 + ntrip);
+        tc.resize(t0 + ntrip);
+        tv.resize(t0 + ntrip);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int k = 0; k < nchunks; ++k) {
+            const Chunk& c = ch[(size_t)k];
+            size_t o = t0 + off[(size_t)k];
+            for (const Item& it : c.items)
+                for (size_t e = it.e0; e < it.e1; ++e)
+                    if (c.row[e] != kObjRow) {
+                        tr[o] = c.row[e];
+                        tc[o] = it.col;
+                        tv[o] = c.val[e];
+                        ++o;
+                    }
+        }
+        p = s1;
+        line_no = (int)lbase;
+    };
+
     while (p < end) {
         const char* eol = static_cast<const char*>(std::memchr(p, '\n', (size_t)(end - p)));
         if (!eol) eol = end;
@@ -302,7 +507,10 @@ LP read_mps(const std::string& path) {
                 if (tok.size() > 1) lp.maximize = tok[1].is("MAX") || tok[1].is("MAXIMIZE");
             }
             else if (h.is("ROWS")) sec = Rows;
-            else if (h.is("COLUMNS")) sec = Columns;
+            else if (h.is("COLUMNS")) {
+                sec = Columns;
+                parse_columns();  // the whole section; p and line_no end at the next header line
+            }
             else if (h.is("RHS")) sec = Rhs;
             else if (h.is("RANGES")) sec = Ranges;
             else if (h.is("BOUNDS")) sec = Bounds;
@@ -327,40 +535,6 @@ LP read_mps(const std::string& path) {
                     lp.row_names.push_back(tok[1].str());
                     rtype.push_back(t);
                     rhs.push_back(0.0);
-                }
-                break;
-            }
-            case Columns: {
-                if (tok.size() >= 3 && tok[1].is("'MARKER'")) {
-                    if (tok[2].is("'INTORG'")) int_mode = true;
-                    else if (tok[2].is("'INTEND'")) int_mode = false;
-                    break;
-                }
-                if (view(tok[0]) != last_col || cur < 0) {
-                    if (tr.empty()) {  // first column: size the triplet buffers from the section's bytes
-                        const size_t guess = (size_t)(end - p) / 24;
-                        tr.reserve(guess);
-                        tc.reserve(guess);
-                        tv.reserve(guess);
-                    }
-                    last_col = view(tok[0]);
-                    cur = lp.n++;
-                    col_of.assign(last_col, cur);
-                    lp.col_names.push_back(std::string(last_col));
-                    lp.c.push_back(0.0);
-                    lp.l.push_back(0.0);
-                    lp.u.push_back(kInf);
-                    lp.is_int.push_back(int_mode ? 1 : 0);
-                    lower_set.push_back(0);
-                }
-                for (size_t k = 1; k + 1 < tok.size(); k += 2) {
-                    const double v = to_num(tok[k + 1], line_no);
-                    if (view(tok[k]) == obj_name) { lp.c[cur] += v; continue; }
-                    const int r = row_index(tok[k]);
-                    if (r < 0) continue;
-                    tr.push_back(r);
-                    tc.push_back(cur);
-                    tv.push_back(v);
                 }
                 break;
             }

@@ -94,6 +94,66 @@ def test_reader_number_formats(tmp_path, oracle):
     assert np.array_equal(d["A"].val, want_a[keep])
 
 
+def _big_mps(rng, n_rows=2000, n_cols=60000):
+    """A multi-megabyte free-MPS text: columns spread over several lines, repeated objective entries, entries in
+    an extra N row, duplicate entries, integer MARKER ranges, ranged rows and bounds."""
+    lines = ["NAME BIG", "ROWS", " N COST", " N COST2"]
+    types = rng.choice(list("ELG"), n_rows)
+    lines += [f" {t} R{i}" for i, t in enumerate(types)]
+    lines.append("COLUMNS")
+    in_int = False
+    for j in range(n_cols):
+        if j % 997 == 0:
+            lines.append(f"    MARKER                 'MARKER'                 '{'INTEND' if in_int else 'INTORG'}'")
+            in_int = not in_int
+        k = int(rng.integers(1, 7))
+        rows = list(rng.integers(0, n_rows, k))
+        if rng.random() < 0.05:
+            rows.append(rows[0])  # a duplicate entry: summed
+        ent = [(f"R{r}", f"{rng.normal():.15g}") for r in rows]
+        if rng.random() < 0.7:
+            ent.insert(int(rng.integers(0, len(ent) + 1)), ("COST", f"{rng.uniform(1, 10):.12g}"))
+        if rng.random() < 0.1:
+            ent.append(("COST", f"{rng.normal():.9g}"))  # a second objective entry, added in line order
+        if rng.random() < 0.1:
+            ent.append(("COST2", "3.5"))  # extra N row: ignored
+        for a in range(0, len(ent), 2):
+            pair = ent[a:a + 2]
+            lines.append(f"    X{j}  " + "  ".join(f"{r} {v}" for r, v in pair))
+    lines.append("RHS")
+    lines += [f"    RHS R{i} {rng.normal():.12g}" for i in range(0, n_rows, 3)]
+    lines.append("RANGES")
+    lines += [f"    RNG R{i} {abs(rng.normal()):.6g}" for i in range(1, n_rows, 7)]
+    lines.append("BOUNDS")
+    lines += [f" UP BND X{j} {rng.uniform(1, 5):.6g}" for j in range(0, n_cols, 11)]
+    lines += [f" LO BND X{j} -2" for j in range(5, n_cols, 13)]
+    lines.append("ENDATA")
+    return lines
+
+
+def test_reader_large_file_matches_oracle(tmp_path, oracle):
+    """A file large enough for the parallel COLUMNS parse (many chunks, columns split across chunk boundaries):
+    the same LP as the oracle's sequential restatement, and the sequential loop's first error and line."""
+    rng = np.random.default_rng(21)
+    lines = _big_mps(rng)
+    path = tmp_path / "big.mps"
+    path.write_text("\n".join(lines) + "\n")
+    assert path.stat().st_size > 4 << 20
+    _same_lp(g.Problem.read_mps(str(path)), oracle.OracleLP.read_mps(str(path)))
+    # an unknown row and, later, a bad number deep in COLUMNS: the first one in file order is reported
+    col0 = lines.index("COLUMNS")
+    bad_at = col0 + 1 + (len(lines) - col0) * 2 // 3
+    lines.insert(bad_at, "    X999999  NOPE 1.0")
+    lines.insert(bad_at + 5000, "    X999998  R1 1.0x")
+    path.write_text("\n".join(lines) + "\n")
+    with pytest.raises(RuntimeError, match=f"unknown row 'NOPE' at line {bad_at + 1}"):
+        g.Problem.read_mps(str(path))
+    del lines[bad_at]
+    path.write_text("\n".join(lines) + "\n")
+    with pytest.raises(RuntimeError, match=f"MPS line {bad_at + 5000}: bad number '1.0x'"):
+        g.Problem.read_mps(str(path))
+
+
 def test_reader_edge_cases(tmp_path, oracle):
     """RANGES on E/L/G rows, negative UP on a default lower bound, objective constant, extra N rows,
     MARKER lines, OBJSENSE MAX, duplicate entries and explicit zeros -- reader vs oracle restatement."""
