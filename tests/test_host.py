@@ -144,13 +144,13 @@ def test_reader_large_file_matches_oracle(tmp_path, oracle):
     col0 = lines.index("COLUMNS")
     bad_at = col0 + 1 + (len(lines) - col0) * 2 // 3
     lines.insert(bad_at, "    X999999  NOPE 1.0")
-    lines.insert(bad_at + 5000, "    X999998  R1 1.0x")
+    lines.insert(bad_at + 5000, "    X999998  R1 ABC")  # no numeric prefix (strtod would take "1.0x" as 1.0)
     path.write_text("\n".join(lines) + "\n")
     with pytest.raises(RuntimeError, match=f"unknown row 'NOPE' at line {bad_at + 1}"):
         g.Problem.read_mps(str(path))
     del lines[bad_at]
     path.write_text("\n".join(lines) + "\n")
-    with pytest.raises(RuntimeError, match=f"MPS line {bad_at + 5000}: bad number '1.0x'"):
+    with pytest.raises(RuntimeError, match=f"MPS line {bad_at + 5000}: bad number 'ABC'"):
         g.Problem.read_mps(str(path))
 
 
