@@ -44,6 +44,17 @@ def load_traffic():
     return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def reference_cpu_baseline(problem, threads, time_limit):
     """The reference's own CPU path: its CLI built from its sources (oracle/_ref/ganit), --device cpu
     --precision mixed with its defaults, solving the same LP written as MPS.  Returns None when the binary is
@@ -60,13 +71,23 @@ def reference_cpu_baseline(problem, threads, time_limit):
         out = subprocess.run([exe, path, "--device", "cpu", "--precision", "mixed", "--tol", "1e-4", "--time",
                               str(time_limit), "--json", "--quiet"], capture_output=True, text=True, env=env,
                              timeout=time_limit + 120)
+        # one core, a bounded sample (640 iterations): seconds per iteration, for single-core comparisons
+        one = subprocess.run([exe, path, "--device", "cpu", "--precision", "mixed", "--iters", "640", "--time",
+                              str(time_limit), "--json", "--quiet"], capture_output=True, text=True,
+                             env=dict(os.environ, OMP_NUM_THREADS="1"), timeout=time_limit + 120)
+    one_lines = [l for l in one.stdout.splitlines() if l.startswith("{")]
+    one_rate = None
+    if one_lines:
+        o = json.loads(one_lines[-1])
+        one_rate = o["solve_s"] / max(o["iters"], 1)
     lines = [l for l in out.stdout.splitlines() if l.startswith("{")]
     if out.returncode not in (0, 2) or not lines:
         print(f"[bench] reference CPU run failed ({out.returncode}): {out.stderr[-300:]}", file=sys.stderr)
         return None
     r = json.loads(lines[-1])
     return {
-        "value": r["solve_s"], "unit": "s", "cores": threads, "kind": "reference",
+        "value": r["solve_s"], "unit": "s", "cores": threads, "kind": "reference", "cpu_model": cpu_model(),
+        "single_core_s_per_iter": one_rate,
         "sample": (f"the reference CLI built from its sources, --device cpu --precision mixed (its defaults: stall "
                    f"12, level-precision checks), full solve of the same LP written as MPS: {r['status']} in "
                    f"{r['iters']} iterations ({r['fp32_iters']} L1), solve_s as it reports it (read "
