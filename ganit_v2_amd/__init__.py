@@ -319,6 +319,24 @@ class Solver:
         return float(ms[0]), float(ms[1])
 
 
+def solve_many(problems, threads: int = 8, **opts) -> list:
+    """Solve independent problems concurrently from host threads, one staged solver (HIP context and stream)
+    each: for small, launch-bound LPs/QPs the GPU overlaps the solves, so the batch takes about as long as its
+    slowest member.  Returns one result dict per problem (with x and y), in order; each is that problem's own
+    solve (bitwise whenever the per-solver autotuner picks the same kernel variants).  (For many LPs, Batch
+    runs them in one set of block-diagonal kernels instead.)"""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def one(p):
+        s = p.solver(**opts)
+        s.upload()
+        return s.run(return_solution=True)
+
+    problems = list(problems)
+    with ThreadPoolExecutor(max(1, min(threads, len(problems) or 1))) as ex:
+        return list(ex.map(one, problems))
+
+
 _BATCH_STATS_KEYS = ["rows", "cols", "nnz", "blocks_a", "blocks_at", "upload_s", "solve_s", "certify_s", "l1_s",
                      "l2_s", "l3_s", "l1_instances", "l2_instances", "l3_instances", "max_iters", "total_iters",
                      "autotune_pick", "autotune_ms0", "autotune_ms1", "autotune_ms2", "autotune_ms3", "setup_s"]

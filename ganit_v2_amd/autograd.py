@@ -12,8 +12,16 @@ C ABI through ctypes; tensors are converted to float64 host arrays at the bounda
     layer = QPLayer.from_problem(Problem.generate("diff_qp", seed=0))
     x = layer(c, b, q)              # torch tensors with requires_grad
     (w * x).sum().backward()        # c.grad, b.grad, q.grad
+
+A batch (c, b, q of shape (B, n), (B, m), (B, n)) solves its B instances concurrently from host threads, each
+on its own HIP context and stream: these solves are launch-bound, so the GPU overlaps them and a batch takes
+about as long as its slowest instance.  Each instance runs the single-instance solve (bitwise the same
+whenever the autotuner, which times the kernel variants for every solver, picks the same variants).
 """
 from __future__ import annotations
+
+import os
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -87,7 +95,25 @@ class QPLayer:
                                      qvals=qv)
 
     def __call__(self, c: torch.Tensor, b: torch.Tensor, q: torch.Tensor) -> torch.Tensor:
+        if c.dim() == 2:
+            if not (b.dim() == 2 and q.dim() == 2 and len(b) == len(c) == len(q)):
+                raise ValueError("a batch needs c, b, q of shapes (B, n), (B, m), (B, n)")
+            return QPBatchFunction.apply(c, b, q, self)
         return QPFunction.apply(c, b, q, self)
+
+    def threads(self, count: int) -> int:
+        """Host threads for a batch of `count` instances (GANIT_QP_THREADS overrides; default up to 8)."""
+        env = os.environ.get("GANIT_QP_THREADS")
+        return max(1, min(count, int(env) if env else 8))
+
+    def _forward_one(self, c: np.ndarray, b: np.ndarray, q: np.ndarray):
+        p = self.problem(c, b, q)
+        solver = p.solver(**self.opts)
+        solver.upload()
+        r = solver.run(return_solution=True)
+        if r["status"] != "OPTIMAL":
+            raise RuntimeError(f"QP layer forward solve ended {r['status']}")
+        return solver, p, r
 
 
 def _host(t: torch.Tensor) -> np.ndarray:
@@ -99,12 +125,7 @@ class QPFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, c, b, q, layer: QPLayer):
-        p = layer.problem(_host(c), _host(b), _host(q))
-        solver = p.solver(**layer.opts)
-        solver.upload()
-        r = solver.run(return_solution=True)
-        if r["status"] != "OPTIMAL":
-            raise RuntimeError(f"QP layer forward solve ended {r['status']}")
+        solver, p, r = layer._forward_one(_host(c), _host(b), _host(q))
         ctx.solver, ctx.layer, ctx.problem = solver, layer, p
         layer.last = (r, None)
         return torch.from_numpy(r["x"]).to(device=c.device, dtype=c.dtype)
@@ -120,3 +141,29 @@ class QPFunction(torch.autograd.Function):
 
 
 __all__ = ["QPLayer", "QPFunction", "ROW_EQ", "ROW_LE", "ROW_GE"]
+
+
+class QPBatchFunction(torch.autograd.Function):
+    """Row k of x* = QP(c[k], b[k], diag q[k]); the B forward solves and the B adjoints run concurrently."""
+
+    @staticmethod
+    def forward(ctx, c, b, q, layer: QPLayer):
+        C, B, Qd = _host(c), _host(b), _host(q)
+        with ThreadPoolExecutor(layer.threads(len(C))) as ex:
+            outs = list(ex.map(layer._forward_one, C, B, Qd))
+        ctx.solvers = [o[0] for o in outs]
+        ctx.problems = [o[1] for o in outs]
+        ctx.layer = layer
+        layer.last = ([o[2] for o in outs], None)
+        x = np.stack([o[2]["x"] for o in outs])
+        return torch.from_numpy(x).to(device=c.device, dtype=c.dtype)
+
+    @staticmethod
+    def backward(ctx, gx):
+        layer: QPLayer = ctx.layer
+        G = _host(gx)
+        with ThreadPoolExecutor(layer.threads(len(G))) as ex:
+            adj = list(ex.map(lambda sg: sg[0].adjoint(sg[1], **layer.adjoint_opts), zip(ctx.solvers, G)))
+        layer.last = (layer.last[0] if layer.last else None, adj)
+        conv = lambda key: torch.from_numpy(np.stack([a[key] for a in adj])).to(device=gx.device, dtype=gx.dtype)
+        return conv("dc"), conv("db"), conv("dq"), None

@@ -126,3 +126,39 @@ def test_autograd_function_matches_adjoint():
         assert _rel(t.numpy(), ref[k]) < 1e-5, k
     # the layer's forward is the problem's own solution
     assert np.allclose(x.detach().numpy(), r["x"])
+
+
+def test_batched_layer_matches_single_instances():
+    """A batch (B x n data) runs its instances concurrently; each row's solution and gradients agree with the
+    same instance through the single-instance layer."""
+    p = g.Problem.generate("diff_qp", seed=4, n=500, m_eq=75, m_le=50, density=0.02, qdensity=0.002)
+    layer = QPLayer.from_problem(p, precision="double", tol=1e-9, adjoint_tol=1e-10)
+    c0, b0, q0 = layer.data_of(p)
+    rng = np.random.default_rng(3)
+    k = 3
+    C = torch.stack([c0 + 0.1 * torch.from_numpy(rng.normal(size=layer.n)) for _ in range(k)])
+    B = torch.stack([b0] * k)
+    Q = torch.stack([q0 * torch.from_numpy(rng.uniform(0.9, 1.1, layer.n)) for _ in range(k)])
+    W = torch.from_numpy(rng.normal(size=(k, layer.n)))
+    Cb, Bb, Qb = (t.clone().requires_grad_(True) for t in (C, B, Q))
+    X = layer(Cb, Bb, Qb)
+    assert X.shape == (k, layer.n)
+    (W * X).sum().backward()
+    for i in range(k):
+        c, b, q = (t[i].clone().requires_grad_(True) for t in (C, B, Q))
+        x = layer(c, b, q)
+        (W[i] * x).sum().backward()
+        assert _rel(X[i].detach().numpy(), x.detach().numpy()) < 1e-7, i
+        for bt, st in ((Cb.grad[i], c.grad), (Bb.grad[i], b.grad), (Qb.grad[i], q.grad)):
+            assert _rel(bt.numpy(), st.numpy()) < 1e-5, i
+
+
+def test_solve_many_matches_single_solves():
+    probs = [g.Problem.generate("random_lp", seed=s, m=400, n=900, m_eq=200) for s in range(6)]
+    many = g.solve_many(probs, threads=6, tol=1e-6, precision="double")
+    for p, r in zip(probs, many):
+        s = p.solver(tol=1e-6, precision="double")
+        s.upload()
+        one = s.run(return_solution=True)
+        assert r["status"] == one["status"]
+        assert abs(r["pobj"] - one["pobj"]) <= 1e-5 * (1 + abs(one["pobj"]))
