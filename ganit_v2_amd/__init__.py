@@ -21,7 +21,7 @@ PKG_DIR = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(PKG_DIR, "lib")
 BIN_DIR = os.path.join(PKG_DIR, "bin")
 
-STATUS = ["OPTIMAL", "PRIMAL_INFEASIBLE", "ITERATION_LIMIT", "TIME_LIMIT", "NUMERICAL_ERROR"]
+STATUS = ["OPTIMAL", "PRIMAL_INFEASIBLE", "ITERATION_LIMIT", "TIME_LIMIT", "NUMERICAL_ERROR", "DUAL_INFEASIBLE"]
 PRECISION = {"double": 0, "mixed": 1, "dynamic": 2}
 VALUES = {"f32": 0, "bf16": 1}
 
@@ -247,10 +247,10 @@ _RESULT_KEYS = ["status", "pobj", "dobj", "rel_p", "rel_d", "rel_gap", "iters", 
 
 def _options(tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10, pock_chambolle=True,
              verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12, stall_checks_l2=24, values="f32",
-             device=0, eval_fp64=False, kahan=False) -> np.ndarray:
+             device=0, eval_fp64=False, kahan=False, detect_unbounded=False) -> np.ndarray:
     return np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
                      infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device,
-                     int(bool(eval_fp64)), int(bool(kahan))], dtype=np.float64)
+                     int(bool(eval_fp64)), int(bool(kahan)) | (2 if detect_unbounded else 0)], dtype=np.float64)
 
 
 def _result(out, x, y) -> dict:

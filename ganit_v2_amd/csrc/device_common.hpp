@@ -338,10 +338,12 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
         [&](int r, V (&t)[NX], NoRowOperands&) { row_fn(r, t); }, lds, sp);
 }
 
+// act: bit v set = vector v is wanted; an unwanted vector's sums are 0 and, on a staged block, it costs no
+// pass (the long-row and chunk paths still gather it, and the gather returns 0 for it).
 template <class M, class V, int NX, bool WIN = false, class Gather, class RowFn>
 __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int* __restrict__ blocks, int b,
                                                  Gather gather, RowFn row_fn, V* lds /* kTile (+ kWin with WIN) */,
-                                                 const SplitPlan& sp) {
+                                                 const SplitPlan& sp, unsigned act = ~0u) {
     static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
     if (sp.finish || b >= sp.nblocks) {
         stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
@@ -394,11 +396,16 @@ __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int*
         V wv[NX][kWPer];
 #pragma unroll
         for (int v = 0; v < NX; ++v) {
+            if (!((act >> v) & 1u)) continue;
 #pragma unroll
             for (int k = 0; k < kWPer; ++k) wv[v][k] = gather(v, cb + min((int)threadIdx.x + k * kBlock, wlast));
         }
 #pragma unroll
         for (int v = 0; v < NX; ++v) {
+            if (!((act >> v) & 1u)) {
+                acc[v] = V(0);
+                continue;
+            }
 #pragma unroll
             for (int k = 0; k < kWPer; ++k) win[(int)threadIdx.x + k * kBlock] = wv[v][k];
             __syncthreads();
@@ -416,6 +423,10 @@ __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int*
     } else {
 #pragma unroll
         for (int v = 0; v < NX; ++v) {
+            if (!((act >> v) & 1u)) {
+                acc[v] = V(0);
+                continue;
+            }
             if (cnt > 0) {
 #pragma unroll
                 for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * gather(v, col[k]);

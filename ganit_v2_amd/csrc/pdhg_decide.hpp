@@ -8,7 +8,7 @@
 
 namespace ganit_hip {
 
-constexpr int kBOut = GANIT_CHECK_NOUT;
+constexpr int kBOut = 19;  // per-instance sums of the batch: the single check's first 19 outputs
 
 enum BatchStatus { kRunning = 0, kConverged = 1, kInfeasible = 2, kIterLimit = 3, kNumerical = 4, kStalled = 5,
                    kTimeLimit = 6 };

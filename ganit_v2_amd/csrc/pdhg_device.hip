@@ -8,7 +8,7 @@
 //                   (src/backend_cpu.cpp:85-91).
 // A check (src/pdhg_impl.hpp:144-147 + :110-118 + :179) is one averaging kernel, one pass over A^T/Q and one
 // pass over A that evaluate the current point, the average and the infeasibility ray together, and reduce
-// all 19 sums deterministically; the host then reads them with a single copy.
+// all 22 sums deterministically; the host then reads them with a single copy.
 #include <algorithm>
 #include <cstdlib>
 #include <map>
@@ -23,7 +23,10 @@ namespace ganit_hip {
 
 constexpr int kCheckGridMax = 2048;
 constexpr int64_t kGraphMaxNnz = 1000000;  // replay iteration blocks from graphs below this many nonzeros
-constexpr int kNOutN = 12, kNOutM = 7;
+// per-block sums of the two check passes: A^T pass outputs 0..11 and, with GANIT_CHECK_DRAY, c'd and the
+// column cone excess (12, 13); A pass outputs 12..18 of the check and, with GANIT_CHECK_DRAY, the row cone
+// excess (its slot 7)
+constexpr int kNOutN = 14, kNOutM = 8;
 
 // Step sizes on the device: prm[0] = tau, prm[1] = sigma (FP64), and the same two rounded to FP32 in the next
 // slot, so an FP32 kernel reads its step size without a conversion (same rounding as converting in place).
@@ -161,6 +164,17 @@ __device__ __forceinline__ void dual_terms(double z, double lj, double uj, doubl
     ob = o;
 }
 
+// The distance of a ray component d to the recession cone of the interval [lo, hi]: {0} when both ends are
+// finite, [0, inf) when only lo is, (-inf, 0] when only hi is, everything when neither (the dual-infeasibility
+// test of a primal ray, for rows with d = (A d)_i and for variables with d = d_j).
+__device__ __forceinline__ double cone_excess(double d, double lo, double hi) {
+    const bool fl = isfinite(lo), fh = isfinite(hi);
+    if (fl && fh) return d;
+    if (fl) return d < 0 ? d : 0.0;
+    if (fh) return d > 0 ? d : 0.0;
+    return 0.0;
+}
+
 // Row-bound dual objective term (src/backend_cpu.cpp:137-139).
 __device__ __forceinline__ double row_obj_term(double yi, double lw, double hg) {
     if (yi > 0 && isfinite(lw)) return yi * lw;
@@ -213,7 +227,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
                                                     const V* __restrict__ cres, int flags, ReduceSlot slot) {
     __shared__ double lds[(kBlock / kWave) * kNOutN];
     CheckAcc<kNOutN> acc((flags & GANIT_CHECK_KAHAN) != 0);
-    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY, dray = flags & GANIT_CHECK_DRAY;
     constexpr int kGroups = kBlock / LPR;
     const int lane = threadIdx.x % LPR;
     for (int64_t g = (int64_t)xcd_block_id() * kGroups + threadIdx.x / LPR; g < AT.rows;
@@ -267,6 +281,12 @@ __global__ __launch_bounds__(kBlock) void k_check_n(CsrView<M> AT, CsrView<M> Q,
                 acc.add(8, r2);
                 acc.add(9, ob);
             }
+            if (dray) {  // the primal ray d = x - xr: c'd and d's excess over the bounds' recession cone
+                const double d = xj - double(xr[j]);
+                acc.add(12, cj * d);
+                const double e = cone_excess(d, lj, uj) * cr;
+                acc.add(13, e * e);
+            }
         }
     }
     {
@@ -281,26 +301,28 @@ template <int LPR, class M, class V, class C, class VI = V>
 __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const VI* __restrict__ x,
                                                     const VI* __restrict__ xa, const VI* __restrict__ y,
                                                     const VI* __restrict__ ya, const VI* __restrict__ yr,
-                                                    const V* __restrict__ lo,
+                                                    const VI* __restrict__ xr, const V* __restrict__ lo,
                                                     const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                     ReduceSlot slot) {
     __shared__ double lds[(kBlock / kWave) * kNOutM];
     CheckAcc<kNOutM> acc((flags & GANIT_CHECK_KAHAN) != 0);
-    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY, dray = flags & GANIT_CHECK_DRAY;
     constexpr int kGroups = kBlock / LPR;
     const int lane = threadIdx.x % LPR;
     for (int64_t g = (int64_t)xcd_block_id() * kGroups + threadIdx.x / LPR; g < A.rows;
          g += (int64_t)gridDim.x * kGroups) {
         const int i = (int)g;
-        C sc = C(0), sa = C(0);
+        C sc = C(0), sa = C(0), sd = C(0);
         for (int p = A.ptr[i] + lane; p < A.ptr[i + 1]; p += LPR) {
             const C a = widen<C>(A.val[p]);
             const int j = A.idx[p];
             sc += a * C(x[j]);
             if (avg) sa += a * C(xa[j]);
+            if (dray) sd += a * C(V(V(x[j]) - V(xr[j])));
         }
         sc = group_sum<LPR>(sc);
         if (avg) sa = group_sum<LPR>(sa);
+        if (dray) sd = group_sum<LPR>(sd);
         if (lane == 0) {
             const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
             auto pres = [&](double a) {  // src/backend_cpu.cpp:106-109
@@ -319,6 +341,10 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const VI* __re
                 acc.add(6, d1 * d1);
             }
             if (ray) acc.add(4, row_obj_term(double(V(V(y[i]) - V(yr[i]))), lw, hg));
+            if (dray) {  // (A d)_i's excess over the row bounds' recession cone, in the residual's units
+                const double e = cone_excess(double(sd), lw, hg) / rr;
+                acc.add(7, e * e);
+            }
         }
     }
     {
@@ -341,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
     // the staging tile and window; after the sweep, the scratch of the block reduction (hence 16-byte aligned)
     __shared__ __align__(16) C lds[kTile + kWin];
     CheckAcc<kNOutN> acc((flags & GANIT_CHECK_KAHAN) != 0);
-    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY, dray = flags & GANIT_CHECK_DRAY;
     stream_block_seq<M, C, 3, true>(
         AT, blocks, xcd_block_id(),
         [&](int k, int i) {
@@ -385,8 +411,14 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
                 acc.add(8, r2);
                 acc.add(9, ob);
             }
+            if (dray) {  // the primal ray d = x - xr: c'd and d's excess over the bounds' recession cone
+                const double d = xj - double(xr[j]);
+                acc.add(12, cj * d);
+                const double e = cone_excess(d, lj, uj) * cr;
+                acc.add(13, e * e);
+            }
         },
-        lds, sp);
+        lds, sp, 1u | (avg ? 2u : 0u) | (ray ? 4u : 0u));
     __syncthreads();
     {
         double tot[kNOutN];
@@ -399,16 +431,20 @@ template <class M, class V, class C, class VI = V>
 __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* __restrict__ blocks, SplitPlan sp,
                                                        const VI* __restrict__ x, const VI* __restrict__ xa,
                                                        const VI* __restrict__ y, const VI* __restrict__ ya,
-                                                       const VI* __restrict__ yr, const V* __restrict__ lo,
+                                                       const VI* __restrict__ yr, const VI* __restrict__ xr,
+                                                       const V* __restrict__ lo,
                                                        const V* __restrict__ hi, const V* __restrict__ rres, int flags,
                                                        ReduceSlot slot) {
     // the staging tile and window; after the sweep, the scratch of the block reduction (hence 16-byte aligned)
     __shared__ __align__(16) C lds[kTile + kWin];
     CheckAcc<kNOutM> acc((flags & GANIT_CHECK_KAHAN) != 0);
-    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY;
-    stream_block_seq<M, C, 2, true>(
-        A, blocks, xcd_block_id(), [&](int k, int j) { return k == 0 ? C(x[j]) : (avg ? C(xa[j]) : C(0)); },
-        [&](int i, C (&t)[2]) {
+    const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY, dray = flags & GANIT_CHECK_DRAY;
+    stream_block_seq<M, C, 3, true>(
+        A, blocks, xcd_block_id(),
+        [&](int k, int j) {
+            return k == 0 ? C(x[j]) : k == 1 ? (avg ? C(xa[j]) : C(0)) : (dray ? C(V(V(x[j]) - V(xr[j]))) : C(0));
+        },
+        [&](int i, C (&t)[3]) {
             const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
             auto pres = [&](double a) {
                 const double p = fmin(fmax(a, lw), hg);
@@ -426,8 +462,12 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
                 acc.add(6, d1 * d1);
             }
             if (ray) acc.add(4, row_obj_term(double(V(V(y[i]) - V(yr[i]))), lw, hg));
+            if (dray) {  // (A d)_i's excess over the row bounds' recession cone, in the residual's units
+                const double e = cone_excess(double(t[2]), lw, hg) / rr;
+                acc.add(7, e * e);
+            }
         },
-        lds, sp);
+        lds, sp, 1u | (avg ? 2u : 0u) | (dray ? 4u : 0u));
     __syncthreads();
     {
         double tot[kNOutM];
@@ -441,9 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_check_finish(const double* __restric
                                                          const double* __restrict__ part_m, int nb_m,
                                                          double* __restrict__ out) {
     __shared__ double lds[kBlock / kWave];
+    // outputs 0..11: A^T pass 0..11; 12..18: A pass 0..6; 19: A pass 7; 20, 21: A^T pass 12, 13
     const int k = blockIdx.x;  // one block per output
-    if (k < kNOutN) sum_partials_one(part_n, kNOutN, k, nb_n, out + k, lds);
-    else sum_partials_one(part_m, kNOutM, k - kNOutN, nb_m, out + k, lds);
+    if (k < 12) sum_partials_one(part_n, kNOutN, k, nb_n, out + k, lds);
+    else if (k < 20) sum_partials_one(part_m, kNOutM, k - 12, nb_m, out + k, lds);
+    else sum_partials_one(part_n, kNOutN, k - 8, nb_n, out + k, lds);
 }
 
 // ================================================================ restart / hand-over kernels
@@ -642,7 +684,7 @@ void run_check(Pdhg* pd, VecSet& it, const VecSet& s, int flags, double inv_k, d
             const dim3 g(pass ? (unsigned)pd->A->nsplit : pd->A->rb_grid());
             const ReduceSlot slot{pd->part_m + (size_t)nb_m * kNOutM, pd->out + kNOutN};
             hipLaunchKernelGGL((k_check_m_rb<M, V, C, VI>), g, dim3(kBlock), 0, st, a, pd->A->blocks,
-                               pass ? pd->A->split_finish() : pd->A->split(), x, xa, y, ya, yr, (const V*)s.lo,
+                               pass ? pd->A->split_finish() : pd->A->split(), x, xa, y, ya, yr, xr, (const V*)s.lo,
                                (const V*)s.hi, (const V*)s.rres, flags, slot);
             nb_m += g.x;
         }
@@ -651,7 +693,7 @@ void run_check(Pdhg* pd, VecSet& it, const VecSet& s, int flags, double inv_k, d
         const int64_t groups = std::max<int64_t>(pd->m, 1);
         const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
         nb_m = g;
-        hipLaunchKernelGGL((k_check_m<LPR, M, V, C, VI>), dim3(g), dim3(kBlock), 0, st, a, x, xa, y, ya, yr,
+        hipLaunchKernelGGL((k_check_m<LPR, M, V, C, VI>), dim3(g), dim3(kBlock), 0, st, a, x, xa, y, ya, yr, xr,
                            (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags, sm);
     });
     GH_REQUIRE(nb_n <= pd->part_cap && nb_m <= pd->part_cap, "check grid exceeds the partial buffers");
@@ -903,7 +945,8 @@ int ganit_hip_pdhg_check(ganit_hip_pdhg* h, int mdt, int vdt, int flags, double 
     return guarded([&] {
         auto* pd = reinterpret_cast<Pdhg*>(h);
         GH_REQUIRE(pd && out, "null argument");
-        GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY | GANIT_CHECK_FP64 | GANIT_CHECK_KAHAN)) == 0,
+        GH_REQUIRE((flags & ~(GANIT_CHECK_AVG | GANIT_CHECK_RAY | GANIT_CHECK_FP64 | GANIT_CHECK_KAHAN |
+                              GANIT_CHECK_DRAY)) == 0,
                    "unknown check flag");
         VecSet& s = pd->set(vdt);
         if ((flags & GANIT_CHECK_FP64) && vdt == GANIT_F32) {
@@ -925,6 +968,7 @@ int ganit_hip_pdhg_check(ganit_hip_pdhg* h, int mdt, int vdt, int flags, double 
         }
         if (!(flags & GANIT_CHECK_AVG)) for (int k : {4, 5, 6, 7, 10, 11, 14, 15, 17, 18}) out[k] = 0.0;
         if (!(flags & GANIT_CHECK_RAY)) for (int k : {8, 9, 16}) out[k] = 0.0;
+        if (!(flags & GANIT_CHECK_DRAY)) for (int k : {19, 20, 21}) out[k] = 0.0;
     });
 }
 

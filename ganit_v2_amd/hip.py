@@ -12,7 +12,7 @@ import numpy as np
 from . import _load
 
 F64, F32, BF16 = 0, 1, 2
-CHECK_AVG, CHECK_RAY, CHECK_FP64, CHECK_KAHAN, CHECK_NOUT = 1, 2, 4, 8, 19
+CHECK_AVG, CHECK_RAY, CHECK_FP64, CHECK_KAHAN, CHECK_DRAY, CHECK_NOUT = 1, 2, 4, 8, 16, 22
 _NP = {F64: np.float64, F32: np.float32}
 _SIZE = {F64: 8, F32: 4, BF16: 2}
 

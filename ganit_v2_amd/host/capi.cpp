@@ -166,7 +166,9 @@ ganit::Options options_from(const double* o) {
     opt.values = (int)o[11] == 1 ? ganit::Values::BF16 : ganit::Values::F32;
     opt.device = (int)o[12];
     opt.eval_fp64 = o[13] != 0;
-    opt.kahan = o[14] != 0;
+    const int bits = (int)o[14];  // bit 0: kahan, bit 1: detect_unbounded
+    opt.kahan = (bits & 1) != 0;
+    opt.detect_unbounded = (bits & 2) != 0;
     return opt;
 }
 void result_out(const ganit::Result& r, double* out, double* x, double* y) {

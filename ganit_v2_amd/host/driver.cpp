@@ -27,6 +27,7 @@ const char* to_string(Status s) {
         case Status::PrimalInfeasible: return "PRIMAL_INFEASIBLE";
         case Status::IterationLimit: return "ITERATION_LIMIT";
         case Status::TimeLimit: return "TIME_LIMIT";
+        case Status::DualInfeasible: return "DUAL_INFEASIBLE";
         default: return "NUMERICAL_ERROR";
     }
 }
@@ -65,7 +66,7 @@ struct State {
     KKT kkt;
 };
 
-enum class Exit { Converged, Stalled, IterationLimit, TimeLimit, Infeasible, Numerical };
+enum class Exit { Converged, Stalled, IterationLimit, TimeLimit, Infeasible, DualInfeasible, Numerical };
 
 struct Level {
     int mdt, vdt;
@@ -133,7 +134,9 @@ Exit phase(Shared& sh, ganit_hip_pdhg* pd, const Level& lv, State& st, double to
         st.it += to_check;
         it_since += to_check;
 
-        const int flags = GANIT_CHECK_AVG | (it_since >= 4L * ce ? GANIT_CHECK_RAY : 0);
+        const bool rays = it_since >= 4L * ce;
+        const bool dray_test = rays && opt.detect_unbounded && !hasq;
+        const int flags = GANIT_CHECK_AVG | (rays ? GANIT_CHECK_RAY : 0) | (dray_test ? GANIT_CHECK_DRAY : 0);
         chk(ganit_hip_pdhg_check(pd, ev_mdt, lv.vdt, flags | ev_flags, 1.0 / static_cast<double>(it_since), o));
         ++st.checks;
         const KKT kc = kkt_current(sh, o);
@@ -156,6 +159,16 @@ Exit phase(Shared& sh, ganit_hip_pdhg* pd, const Level& lv, State& st, double to
             const double dray = (o[16] + o[9]) * sh.obj_scale;
             if (dray > 0 && std::sqrt(o[8]) * (1.0 + s.bnorm) / dray <= opt.infeas_tol) {
                 why = Exit::Infeasible; best = kcand; best_is_avg = avg_better; break;
+            }
+        }
+        if (dray_test) {
+            // The mirror test on the primal ray d = x - xr (an extension: the reference has no dual-infeasibility
+            // test): c'd < 0 while A d and d violate the recession cones of the row and variable bounds by a
+            // small multiple of -c'd.  o[19] is in row units; o[20] and o[21] carry a factor 1/obj_scale.
+            const double cd = o[20] * sh.obj_scale;
+            const double viol = std::sqrt(o[19] + o[21] * sh.obj_scale * sh.obj_scale);
+            if (cd < 0 && viol * (1.0 + s.cnorm) / -cd <= opt.infeas_tol) {
+                why = Exit::DualInfeasible; best = kcand; best_is_avg = avg_better; break;
             }
         }
         if (kcand.err < 0.8 * best_err) { best_err = kcand.err; no_progress = 0; }
@@ -189,6 +202,7 @@ Status status_from(Exit e) {
     switch (e) {
         case Exit::Converged: return Status::Optimal;
         case Exit::Infeasible: return Status::PrimalInfeasible;
+        case Exit::DualInfeasible: return Status::DualInfeasible;
         case Exit::TimeLimit: return Status::TimeLimit;
         case Exit::Numerical: return Status::NumericalError;
         default: return Status::IterationLimit;
@@ -328,7 +342,7 @@ Result GpuSolver::run() {
     auto finished = [&](Exit e, const char* level, const char* next) {
         const KKT k64 = fp64_eval();
         const bool done = (e == Exit::Converged && k64.converged(opt.tol)) || e == Exit::TimeLimit ||
-                          e == Exit::IterationLimit || e == Exit::Infeasible;
+                          e == Exit::IterationLimit || e == Exit::Infeasible || e == Exit::DualInfeasible;
         if (opt.verbose)
             std::printf("  %s phase: %ld iters total, %.3fs, FP64-checked error %.2e -> %s\n", level, st.it,
                         sh.elapsed(), k64.err, done ? "done" : next);

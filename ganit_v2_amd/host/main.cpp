@@ -2,7 +2,7 @@
 // summary line and the exit codes follow the reference's src/main.cpp:41-211 for the PDHG path; keys after
 // "nnz" in the JSON line are additions.
 //   ganit model.mps [--device gpu] [--tol 1e-4] [--precision double|mixed|dynamic] [--time S] [--iters N]
-//                   [--values f32|bf16] [--eval level|fp64] [--kahan] [--gpu K] [--sol FILE] [--json] [--quiet]
+//                   [--values f32|bf16] [--eval level|fp64] [--kahan] [--detect-unbounded] [--gpu K] [--sol FILE] [--json] [--quiet]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -27,6 +27,7 @@ static void usage() {
         "  --eval level|fp64  KKT checks of the reduced-precision levels at their own precision (default)\n"
         "                     or in FP64 with the FP64 matrix\n"
         "  --kahan            L1 checks: Kahan-compensated FP32 sums (instead of FP64 sums)\n"
+        "  --detect-unbounded LPs: also test for a dual-infeasibility (unboundedness) ray\n"
         "  --time S           time limit in seconds (default 3600)\n"
         "  --iters N          iteration limit\n"
         "  --stall-checks K   checks without progress before leaving L1 (default 12)\n"
@@ -74,6 +75,7 @@ int main(int argc, char** argv) {
             else { std::fprintf(stderr, "--eval must be level or fp64\n"); return 1; }
         }
         else if (s == "--kahan") opt.kahan = true;
+        else if (s == "--detect-unbounded") opt.detect_unbounded = true;
         else if (s == "--time") opt.time_limit = std::stod(next());
         else if (s == "--iters") opt.max_iter = std::stol(next());
         else if (s == "--gpu") opt.device = std::stoi(next());

@@ -36,9 +36,13 @@ struct Options {
     int device = 0;               // HIP device ordinal
     bool kahan = false;           // L1 checks at FP32: FP32-rounded terms, Kahan-compensated FP32 sums per
                                   // thread, FP64 across threads (instead of FP64 sums)
+    bool detect_unbounded = false;  // LPs: also test the primal ray x - xr as a dual-infeasibility
+                                    // (unboundedness) certificate, reported as Status::DualInfeasible
 };
 
-enum class Status { Optimal, PrimalInfeasible, IterationLimit, TimeLimit, NumericalError };
+// DualInfeasible (an extension, only with Options::detect_unbounded) comes last so the reference's codes keep
+// their values.
+enum class Status { Optimal, PrimalInfeasible, IterationLimit, TimeLimit, NumericalError, DualInfeasible };
 const char* to_string(Status s);
 
 struct Result {

@@ -157,7 +157,11 @@ int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, doub
                               evaluation).  Requires the FP64 working set (vdt_mask bit 0). */
 #define GANIT_CHECK_KAHAN 8 /* sums formed from FP32-rounded terms with Kahan-compensated FP32 accumulation per
                              * thread, combined in FP64 (extension; without it every sum is FP64) */
-#define GANIT_CHECK_NOUT 19
+#define GANIT_CHECK_DRAY 16 /* also the primal ray d = x - xr of an LP: c'd and d's excess over the recession cones
+                             * of the row and variable bounds (extension: the dual-infeasibility test that the
+                             * reference lists as missing, README.md:197; its src/pdhg_impl.hpp:110-118,163
+                             * tests only primal infeasibility, on the ray y - yr) */
+#define GANIT_CHECK_NOUT 22
 /* out[ 0.. 3]  current point : rd2, vobj, c'x, x'Qx      (dual_res, dot_c, dot with Qx)
  * out[ 4.. 7]  average point : rd2, vobj, c'x, x'Qx
  * out[ 8.. 9]  ray           : rd2, vobj with cw = 0 and no Qx
@@ -166,6 +170,8 @@ int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, doub
  * out[14..15]  average point : primal_res2, row_dual_obj
  * out[16]      ray           : row_dual_obj(y - yr)
  * out[17..18]  ||y - yr||^2, ||ya - yr||^2
+ * out[19]      primal ray    : sum_i ((A d)_i's distance to its row's recession cone / row scale)^2
+ * out[20..21]  primal ray    : c'd, sum_j (d_j's distance to its bounds' recession cone * column scale)^2
  * Entries whose flag is off are returned as 0.  Replaces the 8-12 separate reductions and host syncs of
  * one check (src/pdhg_impl.hpp:92-118, 144-147, 179) with three kernels and one synchronisation. */
 int ganit_hip_pdhg_check(ganit_hip_pdhg* pd, int mdt, int vdt, int flags, double inv_k, double* out);

@@ -58,8 +58,11 @@ int ganit_lp_scaled(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, doub
  *               precision (0 double, 1 mixed, 2 dynamic), stall_checks, stall_checks_l2,
  *               values (0 FP32, 1 BF16 matrix values in L1/L2), device,
  *               eval_fp64 (1: L1/L2 checks evaluated in FP64 with the FP64 matrix),
- *               kahan (1: L1's FP32 checks sum FP32-rounded terms with Kahan compensation)
- * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical),
+ *               flags: bit 0 kahan (L1's FP32 checks sum FP32-rounded terms with Kahan compensation),
+ *                      bit 1 detect_unbounded (LPs: test the primal ray x - xr as a dual-infeasibility
+ *                      certificate; ignored by ganit_batch_*)
+ * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical,
+ *                       5 dual infeasible -- only with detect_unbounded),
  *               pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,
  *               setup_s, solve_s, normA, upload_s, L1_s, L2_s, L3_s, checks
  * x (n) and y (m) receive the original-space solution; either may be NULL. */

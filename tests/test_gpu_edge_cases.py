@@ -104,3 +104,43 @@ def test_every_spmv_path_in_one_matrix(oracle, precision):
     c = rng.uniform(0.1, 1.0, n)
     p = _lp(m, n, trip, c, np.zeros(n), np.full(n, 2.0), np.full(m, -INF), b)
     _compare(oracle, p, precision=precision, tol=1e-5)
+
+
+def _unbounded(maximize=False):
+    """min -x0 + x2 + x3 (or max x0 - x2 - x3) with x0 - x1 = 0, x2 + x3 <= 5, x0 + x2 >= 1, x >= 0: unbounded
+    along d = (1, 1, 0, 0), which keeps every row in its cone (A d = (0, 0, 1)) and every bound (d >= 0)."""
+    trip = [(0, 0, 1.0), (0, 1, -1.0), (1, 2, 1.0), (1, 3, 1.0), (2, 0, 1.0), (2, 2, 1.0)]
+    sign = -1.0 if maximize else 1.0
+    return _lp(3, 4, trip, [-sign, 0.0, sign, sign], [0] * 4, [INF] * 4, [0, -INF, 1], [0, 5, INF],
+               maximize=maximize)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("maximize", [False, True])
+def test_unbounded_lp_certificate(oracle, precision, maximize):
+    """With detect_unbounded the primal ray x - xr certifies dual infeasibility; without it the solve runs as the
+    reference's would (no such test) and ends the way the oracle's does."""
+    p = _unbounded(maximize)
+    r = p.solve_gpu(tol=1e-6, max_iter=200_000, time_limit=60, precision=precision, detect_unbounded=True)
+    print(f"{precision} max={maximize}: {r['status']} after {r['iters']} iterations", flush=True)
+    assert r["status"] == "DUAL_INFEASIBLE"
+    assert r["iters"] < 200_000
+    o = oracle.OracleLP.from_export(p.export()).solve(tol=1e-6, max_iter=20_000, time_limit=60)
+    q = p.solve_gpu(tol=1e-6, max_iter=20_000, time_limit=60, precision=precision)
+    assert q["status"] == o["status"] != "DUAL_INFEASIBLE"
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_detect_unbounded_leaves_bounded_solves_unchanged(oracle, precision, monkeypatch):
+    """On a bounded LP and an infeasible LP the extra test never fires: same status, objective and iterations.
+    The per-matrix variant is pinned (no autotuning), so the two solves run the same kernels."""
+    monkeypatch.setenv("GANIT_HIP_AUTOTUNE", "0")
+    trip = [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, -1.0), (2, 0, 3.0), (2, 2, 1.0)]
+    bounded = _lp(3, 3, trip, [1.0, 2.0, -0.5], [0, 0, 0], [3, 3, 10], [1, -INF, 2], [4, INF, INF])
+    trip = [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 1.0)]
+    infeasible = _lp(2, 2, trip, [1.0, 1.0], [0, 0], [10, 10], [1, 3], [1, 3])
+    for p in (bounded, infeasible, g.Problem.generate("random_lp", seed=2)):
+        a = p.solve_gpu(tol=1e-6, time_limit=60, precision=precision)
+        b = p.solve_gpu(tol=1e-6, time_limit=60, precision=precision, detect_unbounded=True)
+        assert a["status"] == b["status"] != "DUAL_INFEASIBLE"
+        assert a["iters"] == b["iters"] and a["pobj"] == b["pobj"]

@@ -349,7 +349,7 @@ def test_reductions_vs_fsum(hipctx, vdt):
 
 
 def _check_terms(s, x, y, xa, ya, xr, yr, oracle_like=True):
-    """The 19 sums of ganit_hip_pdhg_check computed in numpy (math.fsum)."""
+    """The 22 sums of ganit_hip_pdhg_check computed in numpy (math.fsum)."""
     A, AT, Q = s["A"], s["AT"], s["Q"]
     c, l, u, lo, hi, Rr, Cr = (s[k] for k in ("c", "l", "u", "lo", "hi", "Rres", "Cres"))
 
@@ -380,12 +380,21 @@ def _check_terms(s, x, y, xa, ya, xr, yr, oracle_like=True):
     t[14], t[15] = pres(A.matvec(xa)), rowobj(ya)
     t[16] = rowobj(y - yr)
     t[17], t[18] = (y - yr) ** 2, (ya - yr) ** 2
+    d = x - xr
+    t[19] = (_cone_excess(A.matvec(d), lo, hi) / Rr) ** 2
+    t[20], t[21] = c * d, (_cone_excess(d, l, u) * Cr) ** 2
     return t
+
+
+def _cone_excess(d, lo, hi):
+    """d's distance, entry by entry, to the recession cone of [lo, hi]: {0}, [0, inf), (-inf, 0] or R."""
+    fl, fh = np.isfinite(lo), np.isfinite(hi)
+    return np.where(fl & fh, d, np.where(fl, np.minimum(d, 0.0), np.where(fh, np.maximum(d, 0.0), 0.0)))
 
 
 @pytest.mark.parametrize("name", ["afiro", "25fv47", "port_small", "random_lp"])
 def test_pdhg_check_sums_fp64(hipctx, name):
-    """All 19 sums of one fused check (current, average, ray) against numpy on the same FP64 data."""
+    """All 22 sums of one fused check (current, average, both rays) against numpy on the same FP64 data."""
     import ganit_v2_amd as g
     from ganit_v2_amd import hip
     p = g.Problem.generate("random_lp", seed=1) if name == "random_lp" else g.Problem.read_mps(instance_path(name))
@@ -398,7 +407,7 @@ def test_pdhg_check_sums_fp64(hipctx, name):
     pd.load(F64, x, y, xr, yr)
     # build running sums with one block of iterations, then read the average back through the check
     pd.iterate(F64, F64, 3, 0.1, 0.2)
-    out = pd.check(F64, F64, hip.CHECK_AVG | hip.CHECK_RAY, 1.0 / 3)
+    out = pd.check(F64, F64, hip.CHECK_AVG | hip.CHECK_RAY | hip.CHECK_DRAY, 1.0 / 3)
     xc, yc, xr2, yr2 = pd.store(F64)
     # the average the kernel formed: re-derive it by adopting it
     pd.adopt_average(F64)
@@ -406,7 +415,7 @@ def test_pdhg_check_sums_fp64(hipctx, name):
     assert np.array_equal(xr2, xr) and np.array_equal(yr2, yr)
     terms = _check_terms(s, xc, yc, xa, ya, xr, yr)
     k_row = max(np.diff(s["A"].ptr).max(), np.diff(s["AT"].ptr).max(), 1)
-    for k in range(19):
+    for k in range(22):
         want = math.fsum(terms[k])
         # SpMV-derived terms carry the row-sum rounding; bound generously by k_row * eps relative
         scale = math.fsum(np.abs(terms[k])) + 1e-300
@@ -526,9 +535,9 @@ def test_fp64_check_of_fp32_iterate(hipctx, lanes):
     pd.iterate(F32, F32, 1, 0.3, 0.4)
     x1, y1, xr, yr = pd.store(F32)
     pd.load(F64, x1, y1, xr, yr)
-    a = pd.check(F64, F32, hip.CHECK_AVG | hip.CHECK_RAY | hip.CHECK_FP64, 1.0)
-    b = pd.check(F64, F64, hip.CHECK_RAY)
-    cur = [0, 1, 2, 3, 8, 9, 12, 13, 16]  # entries of the current point and the ray
+    a = pd.check(F64, F32, hip.CHECK_AVG | hip.CHECK_RAY | hip.CHECK_DRAY | hip.CHECK_FP64, 1.0)
+    b = pd.check(F64, F64, hip.CHECK_RAY | hip.CHECK_DRAY)
+    cur = [0, 1, 2, 3, 8, 9, 12, 13, 16, 19, 20, 21]  # entries of the current point and the two rays
     assert np.array_equal(a[cur], b[cur])
     assert np.array_equal(a[4:8], a[0:4]) and np.array_equal(a[14:16], a[12:14])
     assert a[11] == a[10] and a[18] == a[17] and a[10] > 0
