@@ -124,7 +124,7 @@ __device__ __forceinline__ double bt_row_obj(double yi, double lw, double hg) {
     return 0.0;
 }
 
-// Same sums as k_check_n_rb of the single-instance path, one partial row per workgroup.
+// Same sums as check_n_block of the single-instance path, one partial row per workgroup.
 template <class M, class V>
 __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int* __restrict__ blocks,
                                                        const int* __restrict__ list,

@@ -339,25 +339,29 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
 }
 
 // act: bit v set = vector v is wanted; an unwanted vector's sums are 0 and, on a staged block, it costs no
-// pass (the long-row and chunk paths still gather it, and the gather returns 0 for it).
-template <class M, class V, int NX, bool WIN = false, class Gather, class RowFn>
-__device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int* __restrict__ blocks, int b,
-                                                 Gather gather, RowFn row_fn, V* lds /* kTile (+ kWin with WIN) */,
-                                                 const SplitPlan& sp, unsigned act = ~0u) {
+// pass (the long-row and chunk paths still gather it, and the gather returns 0 for it).  pref(r) loads row r's
+// operands before the sweep (as in stream_block_pf), and row_fn(r, sums, operands) runs after it.
+template <class M, class V, int NX, bool WIN = false, class Gather, class Pref, class RowFn>
+__device__ __forceinline__ void stream_block_seq_pf(const CsrView<M>& a, const int* __restrict__ blocks, int b,
+                                                    Gather gather, Pref pref, RowFn row_fn,
+                                                    V* lds /* kTile (+ kWin with WIN) */, const SplitPlan& sp,
+                                                    unsigned act = ~0u) {
     static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
     if (sp.finish || b >= sp.nblocks) {
-        stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
+        stream_block_pf<M, V, NX>(a, blocks, b, gather, pref, row_fn, lds, sp);
         return;
     }
     const int4 d = reinterpret_cast<const int4*>(blocks)[b];
     const int r0 = d.x, r1 = d.y, base = d.z, cnt = d.w - d.z;
     if (r1 - r0 == 1 && cnt > kLong) {  // long (or split) row: the NX-wide block-reduction path
-        stream_block<M, V, NX>(a, blocks, b, gather, row_fn, lds, sp);
+        stream_block_pf<M, V, NX>(a, blocks, b, gather, pref, row_fn, lds, sp);
         return;
     }
-    // A staged block: its indices and values are loaded once (and, for a windowed block, the NX windows of the
-    // gathered vectors), then for each vector the products go through the one tile and each row is summed in
-    // order -- the same row sums as NX separate passes.
+    // A staged block: its indices and values are loaded once, every gather of every wanted vector is issued
+    // in one phase (the windows of a windowed block; the products of a plain one), then for each vector the
+    // products go through the one tile and each row is summed in order -- the same row sums as NX separate
+    // passes, with one gather round trip instead of NX.  The owning thread's row extent and row operands are
+    // loaded first (unconditionally: threads without a row load row r0's) so their latency overlaps the sweep.
     const uint16_t* __restrict__ i16 = a.idx16;
     int cb = 0, wl = 0;
     if (i16) {
@@ -369,6 +373,7 @@ __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int*
     const bool own = r < r1;
     const int rl = own ? r : r0;
     const int pbeg = a.ptr[rl], pend = a.ptr[rl + 1];
+    auto pv = pref(rl);
     constexpr int kPer = kTile / kBlock, kWPer = kWin / kBlock;
     const int last = cnt > 0 ? cnt - 1 : 0;
     M mv[kPer];
@@ -421,6 +426,15 @@ __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int*
             __syncthreads();  // the next vector reuses the window and the tile
         }
     } else {
+        V pr[NX][kPer];  // the products of every wanted vector, gathered in one phase
+        if (cnt > 0) {
+#pragma unroll
+            for (int v = 0; v < NX; ++v) {
+                if (!((act >> v) & 1u)) continue;
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) pr[v][k] = widen<V>(mv[k]) * gather(v, col[k]);
+            }
+        }
 #pragma unroll
         for (int v = 0; v < NX; ++v) {
             if (!((act >> v) & 1u)) {
@@ -429,7 +443,7 @@ __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int*
             }
             if (cnt > 0) {
 #pragma unroll
-                for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * gather(v, col[k]);
+                for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = pr[v][k];
             }
             __syncthreads();
             V t = V(0);
@@ -441,7 +455,17 @@ __device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int*
             __syncthreads();  // the next vector reuses the tile
         }
     }
-    if (own) row_fn(r, acc);
+    if (own) row_fn(r, acc, pv);
+}
+
+// stream_block_seq_pf without row operands: row_fn(r, sums).
+template <class M, class V, int NX, bool WIN = false, class Gather, class RowFn>
+__device__ __forceinline__ void stream_block_seq(const CsrView<M>& a, const int* __restrict__ blocks, int b,
+                                                 Gather gather, RowFn row_fn, V* lds /* kTile (+ kWin with WIN) */,
+                                                 const SplitPlan& sp, unsigned act = ~0u) {
+    stream_block_seq_pf<M, V, NX, WIN>(
+        a, blocks, b, gather, [](int) { return NoRowOperands{}; },
+        [&](int r, V (&t)[NX], NoRowOperands&) { row_fn(r, t); }, lds, sp, act);
 }
 
 // ---------------------------------------------------------------- deterministic grid reduction
@@ -495,7 +519,8 @@ __device__ __forceinline__ void block_partial(double (&v)[N], const ReduceSlot& 
 // group of values.
 template <int N>
 __device__ __forceinline__ void block_partial_wide(const double (&v)[N], const ReduceSlot& slot, double* lds,
-                                                   int cap) {
+                                                   int cap, int at = -1 /* partial index; -1: blockIdx.x */) {
+    const size_t row = at < 0 ? (size_t)blockIdx.x : (size_t)at;
     constexpr int kSlices = 16, kPerSlice = kBlock / kSlices;
     const int per = kBlock + kSlices;  // doubles per value: the block's values, then the slice sums
     const int group = min(N, cap / per);
@@ -519,7 +544,7 @@ __device__ __forceinline__ void block_partial_wide(const double (&v)[N], const R
             double s = 0.0;
 #pragma unroll
             for (int j = 0; j < kSlices; ++j) s += src[j];
-            slot.partials[(size_t)blockIdx.x * N + k0 + threadIdx.x] = s;
+            slot.partials[row * N + k0 + threadIdx.x] = s;
         }
         __syncthreads();  // the next group (or the caller) reuses the scratch
     }
@@ -529,8 +554,19 @@ __device__ __forceinline__ void block_partial_wide(const double (&v)[N], const R
 // fixed order (one block per output keeps the fold parallel).
 __device__ __forceinline__ void sum_partials_one(const double* __restrict__ partials, int N, int k, int nparts,
                                                  double* out, double* lds) {
+    // each thread adds its partials b, b + blockDim.x, ... in that order; eight loads are issued before
+    // their adds so that a large grid's fold costs few memory round trips
     double s[1] = {0.0};
-    for (int b = threadIdx.x; b < nparts; b += blockDim.x) s[0] += partials[(size_t)b * N + k];
+    const int step = blockDim.x;
+    int b = threadIdx.x;
+    for (; b + 7 * step < nparts; b += 8 * step) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = partials[(size_t)(b + u * step) * N + k];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s[0] += v[u];
+    }
+    for (; b < nparts; b += step) s[0] += partials[(size_t)b * N + k];
     block_sum<1>(s, lds);
     if (threadIdx.x == 0) *out = s[0];
 }

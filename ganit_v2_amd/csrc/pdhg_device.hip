@@ -11,6 +11,7 @@
 // all 22 sums deterministically; the host then reads them with a single copy.
 #include <algorithm>
 #include <cstdlib>
+#include <limits>
 #include <map>
 #include <tuple>
 #include <type_traits>
@@ -354,41 +355,66 @@ __global__ __launch_bounds__(kBlock) void k_check_m(CsrView<M> A, const VI* __re
     }
 }
 
-// The two check passes on LDS-staged row blocks (grid = number of row blocks); sums in C as above.
-template <class M, class V, class C, bool HASQ, class VI = V>
-__global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int* __restrict__ blocks, SplitPlan sp,
-                                                       CsrView<M> Q,
-                                                       const VI* __restrict__ x, const VI* __restrict__ y,
-                                                       const VI* __restrict__ xa, const VI* __restrict__ ya,
-                                                       const VI* __restrict__ xr, const VI* __restrict__ yr,
-                                                       const V* __restrict__ c, const V* __restrict__ l,
-                                                       const V* __restrict__ u, const V* __restrict__ cres, int flags,
-                                                       ReduceSlot slot) {
-    // the staging tile and window; after the sweep, the scratch of the block reduction (hence 16-byte aligned)
-    __shared__ __align__(16) C lds[kTile + kWin];
-    CheckAcc<kNOutN> acc((flags & GANIT_CHECK_KAHAN) != 0);
+// The two check passes on LDS-staged row blocks, both in one launch: sums in C as above.  The averages
+// (xa, ya) = (sumx, sumy) * inv_k are formed where they are read -- each gathered entry, and each owning row's
+// own entry, which that row's thread also stores for a later adopt/restart -- with k_average's arithmetic, so
+// the values are the same bits and no averaging launch precedes the passes.
+template <class M, class V, class VI>
+struct CheckArgs {
+    CsrView<M> AT, A, Q;
+    const int *bn, *bm;  // row-block descriptors of A^T and A
+    const VI *x, *y, *sumx, *sumy, *xr, *yr;
+    VI *xa, *ya;
+    const V *c, *l, *u, *cres, *lo, *hi, *rres;
+    double inv_k;
+    int flags;
+    double *part_n, *part_m;  // [blocks][kNOutN], [blocks][kNOutM]
+};
+
+// One block of the pass over A^T (and Q): per column, the reduced-cost terms of the current point, the
+// average and the ray y - yr, the objective terms and the distance terms; the block's sums go to partial `at`.
+template <class M, class V, class C, bool HASQ, class VI>
+__device__ __forceinline__ void check_n_block(const CheckArgs<M, V, VI>& g, int b, const SplitPlan& sp, int at,
+                                              C* lds, int cap) {
+    CheckAcc<kNOutN> acc((g.flags & GANIT_CHECK_KAHAN) != 0);
+    const int flags = g.flags;
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY, dray = flags & GANIT_CHECK_DRAY;
-    stream_block_seq<M, C, 3, true>(
-        AT, blocks, xcd_block_id(),
+    const double ik = g.inv_k;
+    const VI *__restrict__ x = g.x, *__restrict__ y = g.y, *__restrict__ sumx = g.sumx, *__restrict__ sumy = g.sumy,
+             *__restrict__ yr = g.yr;
+    // the column's operands, loaded before the sweep; an unrequested iterate reads x in its place (a valid
+    // address, so the loads need no branch)
+    const VI* __restrict__ sxp = avg ? sumx : x;
+    const VI* __restrict__ xrp = (avg || dray) ? g.xr : x;
+    struct Ops {
+        V c, l, u, cres;
+        VI x, sx, xr;
+    };
+    stream_block_seq_pf<M, C, 3, true>(
+        g.AT, g.bn, b,
         [&](int k, int i) {
-            return k == 0 ? C(y[i]) : k == 1 ? (avg ? C(ya[i]) : C(0)) : (ray ? C(V(V(y[i]) - V(yr[i]))) : C(0));
+            return k == 0   ? C(y[i])
+                   : k == 1 ? (avg ? C(VI(ik * double(sumy[i]))) : C(0))
+                            : (ray ? C(V(V(y[i]) - V(yr[i]))) : C(0));
         },
-        [&](int j, C (&t)[3]) {
+        [&](int j) { return Ops{g.c[j], g.l[j], g.u[j], g.cres[j], x[j], sxp[j], xrp[j]}; },
+        [&](int j, C (&t)[3], Ops& o) {
 #pragma clang fp contract(off)
             C qc = C(0), qa = C(0);
             if (HASQ) {
-                for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
-                    const C a = widen<C>(Q.val[p]);
-                    const C pc = a * C(x[Q.idx[p]]);
+                for (int p = g.Q.ptr[j]; p < g.Q.ptr[j + 1]; ++p) {
+                    const C a = widen<C>(g.Q.val[p]);
+                    const int q = g.Q.idx[p];
+                    const C pc = a * C(x[q]);
                     qc = qc + pc;
                     if (avg) {
-                        const C pa = a * C(xa[Q.idx[p]]);
+                        const C pa = a * C(VI(ik * double(sumx[q])));
                         qa = qa + pa;
                     }
                 }
             }
-            const double cj = double(c[j]), lj = double(l[j]), uj = double(u[j]), cr = double(cres[j]);
-            const double xj = double(x[j]);
+            const double cj = double(o.c), lj = double(o.l), uj = double(o.u), cr = double(o.cres);
+            const double xj = double(o.x);
             double r2, ob;
             dual_terms(cj - double(t[0]) + (HASQ ? double(qc) : 0.0), lj, uj, cr, r2, ob);
             acc.add(0, r2);
@@ -396,7 +422,9 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
             acc.add(2, cj * xj);
             if (HASQ) acc.add(3, xj * double(qc));
             if (avg) {
-                const double xaj = double(xa[j]), xrj = double(xr[j]);
+                const VI xav = VI(ik * double(o.sx));
+                g.xa[j] = xav;
+                const double xaj = double(xav), xrj = double(o.xr);
                 dual_terms(cj - double(t[1]) + (HASQ ? double(qa) : 0.0), lj, uj, cr, r2, ob);
                 acc.add(4, r2);
                 acc.add(5, ob);
@@ -412,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
                 acc.add(9, ob);
             }
             if (dray) {  // the primal ray d = x - xr: c'd and d's excess over the bounds' recession cone
-                const double d = xj - double(xr[j]);
+                const double d = xj - double(o.xr);
                 acc.add(12, cj * d);
                 const double e = cone_excess(d, lj, uj) * cr;
                 acc.add(13, e * e);
@@ -420,32 +448,38 @@ __global__ __launch_bounds__(kBlock) void k_check_n_rb(CsrView<M> AT, const int*
         },
         lds, sp, 1u | (avg ? 2u : 0u) | (ray ? 4u : 0u));
     __syncthreads();
-    {
-        double tot[kNOutN];
-        acc.totals(tot);
-        block_partial_wide<kNOutN>(tot, slot, reinterpret_cast<double*>(lds), (int)(sizeof(lds) / sizeof(double)));
-    }
+    double tot[kNOutN];
+    acc.totals(tot);
+    block_partial_wide<kNOutN>(tot, ReduceSlot{g.part_n, nullptr}, reinterpret_cast<double*>(lds), cap, at);
 }
 
-template <class M, class V, class C, class VI = V>
-__global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* __restrict__ blocks, SplitPlan sp,
-                                                       const VI* __restrict__ x, const VI* __restrict__ xa,
-                                                       const VI* __restrict__ y, const VI* __restrict__ ya,
-                                                       const VI* __restrict__ yr, const VI* __restrict__ xr,
-                                                       const V* __restrict__ lo,
-                                                       const V* __restrict__ hi, const V* __restrict__ rres, int flags,
-                                                       ReduceSlot slot) {
-    // the staging tile and window; after the sweep, the scratch of the block reduction (hence 16-byte aligned)
-    __shared__ __align__(16) C lds[kTile + kWin];
-    CheckAcc<kNOutM> acc((flags & GANIT_CHECK_KAHAN) != 0);
+// One block of the pass over A: per row, the primal residual of the current point and the average, the row
+// objective terms of y, ya and the ray, the distance terms and the primal-ray cone excess.
+template <class M, class V, class C, class VI>
+__device__ __forceinline__ void check_m_block(const CheckArgs<M, V, VI>& g, int b, const SplitPlan& sp, int at,
+                                              C* lds, int cap) {
+    CheckAcc<kNOutM> acc((g.flags & GANIT_CHECK_KAHAN) != 0);
+    const int flags = g.flags;
     const bool avg = flags & GANIT_CHECK_AVG, ray = flags & GANIT_CHECK_RAY, dray = flags & GANIT_CHECK_DRAY;
-    stream_block_seq<M, C, 3, true>(
-        A, blocks, xcd_block_id(),
+    const double ik = g.inv_k;
+    const VI *__restrict__ x = g.x, *__restrict__ y = g.y, *__restrict__ sumx = g.sumx, *__restrict__ xr = g.xr;
+    // the row's operands, loaded before the sweep (an unrequested iterate reads y in its place)
+    const VI* __restrict__ syp = avg ? g.sumy : y;
+    const VI* __restrict__ yrp = (avg || ray) ? g.yr : y;
+    struct Ops {
+        V lo, hi, rres;
+        VI y, sy, yr;
+    };
+    stream_block_seq_pf<M, C, 3, true>(
+        g.A, g.bm, b,
         [&](int k, int j) {
-            return k == 0 ? C(x[j]) : k == 1 ? (avg ? C(xa[j]) : C(0)) : (dray ? C(V(V(x[j]) - V(xr[j]))) : C(0));
+            return k == 0   ? C(x[j])
+                   : k == 1 ? (avg ? C(VI(ik * double(sumx[j]))) : C(0))
+                            : (dray ? C(V(V(x[j]) - V(xr[j]))) : C(0));
         },
-        [&](int i, C (&t)[3]) {
-            const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
+        [&](int i) { return Ops{g.lo[i], g.hi[i], g.rres[i], y[i], syp[i], yrp[i]}; },
+        [&](int i, C (&t)[3], Ops& o) {
+            const double lw = double(o.lo), hg = double(o.hi), rr = double(o.rres), yi = double(o.y);
             auto pres = [&](double a) {
                 const double p = fmin(fmax(a, lw), hg);
                 const double r = (a - p) / rr;
@@ -454,14 +488,16 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
             acc.add(0, pres(double(t[0])));
             acc.add(1, row_obj_term(yi, lw, hg));
             if (avg) {
-                const double yai = double(ya[i]), yri = double(yr[i]);
+                const VI yav = VI(ik * double(o.sy));
+                g.ya[i] = yav;
+                const double yai = double(yav), yri = double(o.yr);
                 acc.add(2, pres(double(t[1])));
                 acc.add(3, row_obj_term(yai, lw, hg));
                 const double d0 = yi - yri, d1 = yai - yri;
                 acc.add(5, d0 * d0);
                 acc.add(6, d1 * d1);
             }
-            if (ray) acc.add(4, row_obj_term(double(V(V(y[i]) - V(yr[i]))), lw, hg));
+            if (ray) acc.add(4, row_obj_term(double(V(V(o.y) - V(o.yr))), lw, hg));
             if (dray) {  // (A d)_i's excess over the row bounds' recession cone, in the residual's units
                 const double e = cone_excess(double(t[2]), lw, hg) / rr;
                 acc.add(7, e * e);
@@ -469,14 +505,26 @@ __global__ __launch_bounds__(kBlock) void k_check_m_rb(CsrView<M> A, const int* 
         },
         lds, sp, 1u | (avg ? 2u : 0u) | (dray ? 4u : 0u));
     __syncthreads();
-    {
-        double tot[kNOutM];
-        acc.totals(tot);
-        block_partial_wide<kNOutM>(tot, slot, reinterpret_cast<double*>(lds), (int)(sizeof(lds) / sizeof(double)));
-    }
+    double tot[kNOutM];
+    acc.totals(tot);
+    block_partial_wide<kNOutM>(tot, ReduceSlot{g.part_m, nullptr}, reinterpret_cast<double*>(lds), cap, at);
 }
 
-// Folds the per-block partials of both check passes: block 0 the A^T pass, block 1 the A pass.
+// Both row-block passes in one grid: logical blocks [0, gn) are blocks of A^T (plan spn, partials from
+// at_n), the rest blocks of A (plan spm, partials from at_m).  With the main plans this is the whole pass;
+// with the split_finish plans, the per-split-row folds that follow it.
+template <class M, class V, class C, bool HASQ, class VI>
+__global__ __launch_bounds__(kBlock) void k_check_rb(CheckArgs<M, V, VI> g, SplitPlan spn, SplitPlan spm, int gn,
+                                                     int at_n, int at_m) {
+    // the staging tile and window; after the sweep, the scratch of the block reduction (hence 16-byte aligned)
+    __shared__ __align__(16) C lds[kTile + kWin];
+    constexpr int cap = (int)(sizeof(lds) / sizeof(double));
+    const int id = xcd_block_id();
+    if (id < gn) check_n_block<M, V, C, HASQ, VI>(g, id, spn, at_n + id, lds, cap);
+    else check_m_block<M, V, C, VI>(g, id - gn, spm, at_m + id - gn, lds, cap);
+}
+
+// Folds the per-block partials of both check passes, one block per output, each in block order.
 __global__ __launch_bounds__(kBlock) void k_check_finish(const double* __restrict__ part_n, int nb_n,
                                                          const double* __restrict__ part_m, int nb_m,
                                                          double* __restrict__ out) {
@@ -637,40 +685,45 @@ void run_check(Pdhg* pd, VecSet& it, const VecSet& s, int flags, double inv_k, d
     const VI* x = (const VI*)it.X[it.cur];
     const VI* y = (const VI*)it.Y[it.cur];
     const VI *xa = (const VI*)it.xa, *ya = (const VI*)it.ya, *xr = (const VI*)it.xr, *yr = (const VI*)it.yr;
-    if (flags & GANIT_CHECK_AVG) {
+    // matrices on row blocks go through the fused pass (which forms the averages itself); a lanes-per-row
+    // matrix has its own pass, which reads the averages from memory
+    const bool rb_n = pd->n > 0 && pd->AT->lanes == 0, rb_m = pd->m > 0 && pd->A->lanes == 0;
+    const bool lpr_n = pd->n > 0 && !rb_n, lpr_m = pd->m > 0 && !rb_m;
+    if ((flags & GANIT_CHECK_AVG) && (lpr_n || lpr_m)) {
         hipLaunchKernelGGL(k_average<VI>, dim3(ew_grid2((int64_t)pd->n + pd->m)), dim3(kBlock), 0, st, pd->n, pd->m,
                            inv_k, (const VI*)it.sumx, (const VI*)it.sumy, (VI*)it.xa, (VI*)it.ya);
     }
-    ReduceSlot sn{pd->part_n, pd->out};
-    ReduceSlot sm{pd->part_m, pd->out + kNOutN};
     unsigned nb_n = 0, nb_m = 0;  // partials written by the passes over A^T and A
     const CsrView<M> at = pd->AT->view<M>();
     const CsrView<M> a = pd->A->view<M>();
-    // the row-block passes write rb_grid partials, then their finish passes one per split row
-    if (pd->AT->lanes == 0 && pd->n > 0)
-        GH_REQUIRE(pd->AT->rb_grid() + (unsigned)pd->AT->nsplit <= pd->part_cap, "check grid exceeds the partial buffers");
-    if (pd->A->lanes == 0 && pd->m > 0)
-        GH_REQUIRE(pd->A->rb_grid() + (unsigned)pd->A->nsplit <= pd->part_cap, "check grid exceeds the partial buffers");
-    if (pd->AT->lanes == 0 && pd->n > 0) {
-        for (int pass = 0; pass < (pd->AT->nsplit > 0 ? 2 : 1); ++pass) {
-            const dim3 g(pass ? (unsigned)pd->AT->nsplit : pd->AT->rb_grid());
-            const SplitPlan sp = pass ? pd->AT->split_finish() : pd->AT->split();
-            const ReduceSlot slot{pd->part_n + (size_t)nb_n * kNOutN, pd->out};
+    // the row-block passes: one launch over both matrices' blocks (rb_grid partials each), then, if either
+    // has split rows, one launch of their folds (one partial per split row)
+    const unsigned gn = rb_n ? pd->AT->rb_grid() : 0u, gm = rb_m ? pd->A->rb_grid() : 0u;
+    const unsigned fn = rb_n ? (unsigned)pd->AT->nsplit : 0u, fm = rb_m ? (unsigned)pd->A->nsplit : 0u;
+    GH_REQUIRE(gn + fn <= pd->part_cap && gm + fm <= pd->part_cap, "check grid exceeds the partial buffers");
+    if (gn + gm > 0) {
+        CheckArgs<M, V, VI> g{at, a, pd->hasq ? pd->Q->view<M>() : at, pd->AT->blocks, pd->A->blocks, x, y,
+                              (const VI*)it.sumx, (const VI*)it.sumy, xr, yr, (VI*)it.xa, (VI*)it.ya,
+                              (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, (const V*)s.lo,
+                              (const V*)s.hi, (const V*)s.rres, inv_k, flags, pd->part_n, pd->part_m};
+        auto launch = [&](unsigned grid, SplitPlan spn, SplitPlan spm, unsigned g_n, unsigned at_n, unsigned at_m) {
             if (pd->hasq)
-                hipLaunchKernelGGL((k_check_n_rb<M, V, C, true, VI>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
-                                   pd->Q->view<M>(), x, y, xa, ya, xr, yr, (const V*)s.c, (const V*)s.l,
-                                   (const V*)s.u, (const V*)s.cres, flags, slot);
+                hipLaunchKernelGGL((k_check_rb<M, V, C, true, VI>), dim3(grid), dim3(kBlock), 0, st, g, spn, spm,
+                                   (int)g_n, (int)at_n, (int)at_m);
             else
-                hipLaunchKernelGGL((k_check_n_rb<M, V, C, false, VI>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
-                                   at, x, y, xa, ya, xr, yr, (const V*)s.c, (const V*)s.l, (const V*)s.u,
-                                   (const V*)s.cres, flags, slot);
-            nb_n += g.x;
-        }
-    } else with_lanes(pd->AT->lanes == 0 ? 1 : pd->AT->lanes, [&](auto L) {
+                hipLaunchKernelGGL((k_check_rb<M, V, C, false, VI>), dim3(grid), dim3(kBlock), 0, st, g, spn, spm,
+                                   (int)g_n, (int)at_n, (int)at_m);
+        };
+        launch(gn + gm, pd->AT->split(), pd->A->split(), gn, 0, 0);
+        if (fn + fm > 0) launch(fn + fm, pd->AT->split_finish(), pd->A->split_finish(), fn, gn, gm);
+        nb_n = gn + fn;
+        nb_m = gm + fm;
+    }
+    if (lpr_n) with_lanes(pd->AT->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
-        const int64_t groups = std::max<int64_t>(pd->n, 1);
-        const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
+        const unsigned g = (unsigned)std::min<int64_t>(((int64_t)pd->n * LPR + kBlock - 1) / kBlock, kCheckGridMax);
         nb_n = g;
+        const ReduceSlot sn{pd->part_n, nullptr};
         if (pd->hasq)
             hipLaunchKernelGGL((k_check_n<LPR, M, V, C, true, VI>), dim3(g), dim3(kBlock), 0, st, at,
                                pd->Q->view<M>(), x, y, xa, ya, xr, yr, (const V*)s.c, (const V*)s.l, (const V*)s.u,
@@ -679,31 +732,24 @@ void run_check(Pdhg* pd, VecSet& it, const VecSet& s, int flags, double inv_k, d
             hipLaunchKernelGGL((k_check_n<LPR, M, V, C, false, VI>), dim3(g), dim3(kBlock), 0, st, at, at, x, y, xa,
                                ya, xr, yr, (const V*)s.c, (const V*)s.l, (const V*)s.u, (const V*)s.cres, flags, sn);
     });
-    if (pd->A->lanes == 0 && pd->m > 0) {
-        for (int pass = 0; pass < (pd->A->nsplit > 0 ? 2 : 1); ++pass) {
-            const dim3 g(pass ? (unsigned)pd->A->nsplit : pd->A->rb_grid());
-            const ReduceSlot slot{pd->part_m + (size_t)nb_m * kNOutM, pd->out + kNOutN};
-            hipLaunchKernelGGL((k_check_m_rb<M, V, C, VI>), g, dim3(kBlock), 0, st, a, pd->A->blocks,
-                               pass ? pd->A->split_finish() : pd->A->split(), x, xa, y, ya, yr, xr, (const V*)s.lo,
-                               (const V*)s.hi, (const V*)s.rres, flags, slot);
-            nb_m += g.x;
-        }
-    } else with_lanes(pd->A->lanes == 0 ? 1 : pd->A->lanes, [&](auto L) {
+    if (lpr_m) with_lanes(pd->A->lanes, [&](auto L) {
         constexpr int LPR = decltype(L)::value;
-        const int64_t groups = std::max<int64_t>(pd->m, 1);
-        const unsigned g = (unsigned)std::min<int64_t>((groups * LPR + kBlock - 1) / kBlock, kCheckGridMax);
+        const unsigned g = (unsigned)std::min<int64_t>(((int64_t)pd->m * LPR + kBlock - 1) / kBlock, kCheckGridMax);
         nb_m = g;
+        const ReduceSlot sm{pd->part_m, nullptr};
         hipLaunchKernelGGL((k_check_m<LPR, M, V, C, VI>), dim3(g), dim3(kBlock), 0, st, a, x, xa, y, ya, yr, xr,
                            (const V*)s.lo, (const V*)s.hi, (const V*)s.rres, flags, sm);
     });
     GH_REQUIRE(nb_n <= pd->part_cap && nb_m <= pd->part_cap, "check grid exceeds the partial buffers");
-    hipLaunchKernelGGL(k_check_finish, dim3(kNOutN + kNOutM), dim3(kBlock), 0, st, pd->part_n, (int)nb_n, pd->part_m,
-                       (int)nb_m,
-                       pd->out);
+    // the fold writes the sums straight into host-coherent pinned memory: no copy follows it.  The entries are
+    // set to NaN first, so a value the GPU did not write can never pass for a result.
+    double* res = pd->check_host;
+    std::fill(res, res + GANIT_CHECK_NOUT, std::numeric_limits<double>::quiet_NaN());
+    hipLaunchKernelGGL(k_check_finish, dim3(GANIT_CHECK_NOUT), dim3(kBlock), 0, st, pd->part_n, (int)nb_n,
+                       pd->part_m, (int)nb_m, res);
     GH_CHECK(hipGetLastError());
-    GH_CHECK(hipMemcpyAsync(pd->ctx->host_out, pd->out, sizeof(double) * GANIT_CHECK_NOUT, hipMemcpyDeviceToHost, st));
     GH_CHECK(hipStreamSynchronize(st));
-    std::copy(pd->ctx->host_out, pd->ctx->host_out + GANIT_CHECK_NOUT, host_out);
+    std::copy(res, res + GANIT_CHECK_NOUT, host_out);
 }
 
 }  // namespace ganit_hip
@@ -750,8 +796,7 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 s.sumy = pd->alloc(e * m); s.ya = pd->alloc(e * m); s.yr = pd->alloc(e * m);
             }
             pd->prm = (double*)pd->alloc(sizeof(double) * kPrmDoubles);
-            pd->out = (double*)pd->alloc(sizeof(double) * GANIT_CHECK_NOUT);
-            GH_CHECK(hipMemsetAsync(pd->out, 0, sizeof(double) * GANIT_CHECK_NOUT, ctx->stream));
+            GH_CHECK(hipHostMalloc((void**)&pd->check_host, sizeof(double) * GANIT_CHECK_NOUT, hipHostMallocCoherent));
             GH_CHECK(hipStreamSynchronize(ctx->stream));
             // Graph replay pays for itself when an iteration is short (it saves the host launch cost, ~3 us per
             // iteration on C0-sized problems); on large matrices plain stream launches keep up with the GPU and

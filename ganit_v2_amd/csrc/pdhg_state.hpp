@@ -29,7 +29,7 @@ struct Pdhg {
     double* part_n = nullptr;
     double* part_m = nullptr;
     size_t part_cap = 0;          // blocks the partial buffers hold
-    double* out = nullptr;        // [GANIT_CHECK_NOUT]
+    double* check_host = nullptr;  // [GANIT_CHECK_NOUT] check sums, host-coherent pinned memory
     std::vector<void*> owned;
     std::map<std::tuple<int, int, int, int>, hipGraphExec_t> graphs;
     bool use_graphs = true;
@@ -49,6 +49,7 @@ struct Pdhg {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
         for (void* p : owned) (void)hipFree(p);
+        if (check_host) (void)hipHostFree(check_host);
         csr_destroy(A);
         csr_destroy(AT);
         csr_destroy(Q);
