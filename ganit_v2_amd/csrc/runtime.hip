@@ -1,6 +1,7 @@
 // libganit_hip.so, part 1: context, device vectors, CSR upload with the row-length plan, and the
 // backend-concept kernels (SpMV, primal / dual step, the FP64 reductions) behind the C ABI.
 #include <algorithm>
+#include <exception>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -483,20 +484,96 @@ void apply_variant(Ctx* ctx, Csr* a, const Variant& v) {
 
 constexpr float kRowGroupMargin = 0.9f;  // a row-group variant must be >= 10% faster than the row blocks
 
+namespace {
+
+// The device state of one row-block partition (what csr_set_partition builds).  While the autotuner compares
+// partitions it keeps every plan it has built, so each partition is built once instead of once per round.
+struct PartitionPlan {
+    bool built = false;
+    int rows = 0, nnz = 0, nblocks = 0;
+    int* blocks = nullptr;
+    uint16_t* idx16 = nullptr;
+    int* bcol = nullptr;
+    int* chunk_base = nullptr;
+    int64_t idx32_entries = 0;
+};
+
+PartitionPlan plan_of(const Csr* a) {
+    return {true, a->rb_rows, a->rb_nnz, a->nblocks, a->blocks, a->idx16, a->bcol, a->chunk_base, a->idx32_entries};
+}
+
+void attach_plan(Csr* a, const PartitionPlan& p) {
+    a->rb_rows = p.rows;
+    a->rb_nnz = p.nnz;
+    a->nblocks = p.nblocks;
+    a->blocks = p.blocks;
+    a->idx16 = p.idx16;
+    a->bcol = p.bcol;
+    a->chunk_base = p.chunk_base;
+    a->idx32_entries = p.idx32_entries;
+}
+
+// `a` lets go of its plan (its owner keeps the buffers); the next csr_set_partition builds a fresh one
+void detach_plan(Csr* a) {
+    a->blocks = nullptr;
+    a->idx16 = nullptr;
+    a->bcol = nullptr;
+    a->chunk_base = nullptr;
+    a->rb_rows = a->rb_nnz = 0;
+}
+
+void free_plan(PartitionPlan& p) {
+    (void)hipFree(p.blocks);
+    (void)hipFree(p.idx16);
+    (void)hipFree(p.bcol);
+    (void)hipFree(p.chunk_base);
+    p = PartitionPlan{};
+}
+
+}  // namespace
+
 void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& launch) {
     if (a->rows == 0 || a->nnz == 0) return;
     const auto& cand = variant_candidates();
-    hipEvent_t e0, e1;
-    GH_CHECK(hipEventCreate(&e0));
-    GH_CHECK(hipEventCreate(&e1));
-    std::vector<float> best(cand.size(), 1e30f);
+    const size_t nc = cand.size();
+    // plans[c]: row-block candidate c's partition once built; `entry` holds the plan `a` came with when it is
+    // none of the candidates'.  Every plan belongs to this table until the end, when `a` keeps the chosen one.
+    std::vector<PartitionPlan> plans(nc);
+    PartitionPlan entry = plan_of(a);
+    for (size_t c = 0; c < nc; ++c) {
+        if (cand[c].lanes == 0 && cand[c].rows == a->rb_rows && cand[c].nnz == a->rb_nnz) {
+            plans[c] = entry;
+            entry = PartitionPlan{};
+            break;
+        }
+    }
+    auto use = [&](size_t c) {
+        a->lanes = cand[c].lanes;
+        if (cand[c].lanes != 0) return;
+        if (!plans[c].built) {
+            detach_plan(a);
+            csr_set_partition(ctx, a, cand[c].rows, cand[c].nnz);
+            plans[c] = plan_of(a);
+        } else {
+            attach_plan(a, plans[c]);
+        }
+    };
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    std::vector<float> best(nc, 1e30f);
+    size_t bi = nc, bl = nc;
+    std::exception_ptr err;
     try {
+        GH_CHECK(hipEventCreate(&e0));
+        GH_CHECK(hipEventCreate(&e1));
         // Warm the clocks and caches first, then time every candidate in two interleaved rounds and keep
-        // each one's best time, so a candidate timed early is not penalised by a cold GPU.
+        // each one's best time, so a candidate timed early is not penalised by a cold GPU.  A candidate more
+        // than twice as slow as the best after the first round cannot be chosen and is not timed again.
         launch(20);
         for (int round = 0; round < 2; ++round) {
-            for (size_t c = 0; c < cand.size(); ++c) {
-                apply_variant(ctx, a, cand[c]);
+            const float lead = *std::min_element(best.begin(), best.end());
+            for (size_t c = 0; c < nc; ++c) {
+                if (round > 0 && best[c] > 2.0f * lead) continue;
+                use(c);
                 launch(2);
                 GH_CHECK(hipEventRecord(e0, ctx->stream));
                 launch(reps);
@@ -507,22 +584,37 @@ void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& l
                 best[c] = std::min(best[c], ms);
             }
         }
+        // the fastest row-block partition, unless a row-group width beats it by a clear margin: the row
+        // blocks' row sums are the CPU row loop's bits, and a short timing window can rank near-equal variants
+        // either way
+        for (size_t c = 0; c < nc; ++c) {
+            size_t& slot = cand[c].lanes == 0 ? bi : bl;
+            if (slot == nc || best[c] < best[slot]) slot = c;
+        }
+        if (bi == nc || (bl != nc && best[bl] < kRowGroupMargin * best[bi])) bi = bl;
+        use(bi);
     } catch (...) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
-        throw;
+        err = std::current_exception();
+        bi = nc;  // keep whatever plan `a` holds now
     }
-    // the fastest row-block partition, unless a row-group width beats it by a clear margin: the row blocks'
-    // row sums are the CPU row loop's bits, and a short timing window can rank near-equal variants either way
-    size_t bi = cand.size(), bl = cand.size();
-    for (size_t c = 0; c < cand.size(); ++c) {
-        size_t& slot = cand[c].lanes == 0 ? bi : bl;
-        if (slot == cand.size() || best[c] < best[slot]) slot = c;
+    // `a` keeps the chosen partition (a row-group winner keeps the first row-block plan built, for the
+    // passes that still use row blocks); every other plan is freed
+    if (bi < nc && cand[bi].lanes != 0) {
+        const int lanes = a->lanes;
+        for (size_t c = 0; c < nc; ++c) {
+            if (plans[c].built) {
+                attach_plan(a, plans[c]);
+                break;
+            }
+        }
+        a->lanes = lanes;
     }
-    if (bi == cand.size() || (bl != cand.size() && best[bl] < kRowGroupMargin * best[bi])) bi = bl;
-    apply_variant(ctx, a, cand[bi]);
-    GH_CHECK(hipEventDestroy(e0));
-    GH_CHECK(hipEventDestroy(e1));
+    for (size_t c = 0; c < nc; ++c)
+        if (plans[c].built && plans[c].blocks != a->blocks) free_plan(plans[c]);
+    if (entry.built && entry.blocks != a->blocks) free_plan(entry);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (err) std::rethrow_exception(err);
 }
 
 void autotune_csr(Ctx* ctx, Csr* a, int mdt, int vdt, int reps) {
