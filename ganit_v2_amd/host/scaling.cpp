@@ -3,6 +3,8 @@
 // per-element arithmetic is sequential (a row's or a column's own sum in ascending index order, or a max).
 #include "scaling.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 
@@ -42,6 +44,55 @@ void apply(Csr& a, const std::vector<double>& rs, const std::vector<double>& cs,
     for (int j = 0; j < a.cols; ++j) C[j] *= cs[j];
 }
 
+// Column view of a's pattern: entries of column j are a's entries perm[cptr[j] .. cptr[j+1]), rows ascending,
+// and trow[q] is the row of entry perm[q] -- the order of the reference's stable transpose (lp.cpp:55-72),
+// built as a parallel counting sort: each thread counts the columns of its own contiguous range of rows, the
+// per-column offsets are laid out thread after thread, and each thread then places its rows' entries in order.
+void column_view(const Csr& a, std::vector<int>& cptr, std::vector<int>& perm, std::vector<int>& trow) {
+    const int m = a.rows, n = a.cols;
+    const int64_t nnz = (int64_t)a.val.size();
+    cptr.assign((size_t)n + 1, 0);
+    perm.resize((size_t)nnz);
+    trow.resize((size_t)nnz);
+    // threads, bounded so that the per-thread column counters stay within about 64 MB
+    const int64_t cap = std::max<int64_t>(1, (int64_t)16 * 1024 * 1024 / std::max(n, 1));
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)omp_get_max_threads(), cap, (int64_t)m}));
+    // row ranges of about nnz / nt entries each
+    std::vector<int> rb((size_t)nt + 1, m);
+    rb[0] = 0;
+    for (int t = 1; t < nt; ++t) {
+        const int64_t target = nnz * t / nt;
+        rb[t] = (int)(std::lower_bound(a.ptr.begin(), a.ptr.begin() + m + 1, (int)target) - a.ptr.begin());
+        rb[t] = std::max(rb[t], rb[t - 1]);
+    }
+    std::vector<int> cnt((size_t)nt * n, 0);
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+        int* c = cnt.data() + (size_t)t * n;
+        for (int p = a.ptr[rb[t]]; p < a.ptr[rb[t + 1]]; ++p) ++c[a.idx[p]];
+    }
+    for (int j = 0; j < n; ++j) {  // column starts, and each thread's offset within each column
+        int pos = cptr[j];
+        for (int t = 0; t < nt; ++t) {
+            int& c = cnt[(size_t)t * n + j];
+            const int k = c;
+            c = pos;
+            pos += k;
+        }
+        cptr[j + 1] = pos;
+    }
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+    for (int t = 0; t < nt; ++t) {
+        int* at = cnt.data() + (size_t)t * n;
+        for (int i = rb[t]; i < rb[t + 1]; ++i)
+            for (int p = a.ptr[i]; p < a.ptr[i + 1]; ++p) {
+                const int q = at[a.idx[p]]++;
+                perm[q] = p;
+                trow[q] = i;
+            }
+    }
+}
+
 // CSR SpMV with each row summed sequentially (the reference's host loops).
 void spmv_seq_rows(const Csr& a, const std::vector<double>& x, std::vector<double>& y) {
 #pragma omp parallel for schedule(dynamic, 1024)
@@ -64,15 +115,10 @@ ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_
     s.maximize = lp.maximize;
     const double sign = lp.maximize ? -1.0 : 1.0;
 
-    // Column view of A's pattern: entry q of column j is A entry perm[q]; rows ascending within a column.
-    std::vector<int> cptr(lp.n + 1, 0), perm(s.A.val.size());
-    for (int j : s.A.idx) ++cptr[j + 1];
-    for (int j = 0; j < lp.n; ++j) cptr[j + 1] += cptr[j];
-    {
-        std::vector<int> at(cptr.begin(), cptr.end() - 1);
-        for (int i = 0; i < lp.m; ++i)
-            for (int p = s.A.ptr[i]; p < s.A.ptr[i + 1]; ++p) perm[at[s.A.idx[p]]++] = p;
-    }
+    // Column view of A's pattern: entry q of column j is A entry perm[q] (row trow[q]); rows ascending within
+    // a column.  The scaled A^T is read off it at the end.
+    std::vector<int> cptr, perm, trow;
+    column_view(s.A, cptr, perm, trow);
 
     std::vector<double> rs(lp.m), cs(lp.n);
     for (int it = 0; it < ruiz_iters; ++it) {  // inf-norm Ruiz (reference scaling.cpp:45-57)
@@ -109,7 +155,13 @@ ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_
         inv_sqrt_or_one(cs);
         apply(s.A, rs, cs, s.R, s.C);
     }
-    s.AT = transpose(s.A);
+    // A^T: the column view with the scaled values (the same arrays as transpose(s.A), built in parallel)
+    s.AT = empty_csr(lp.n, lp.m);
+    s.AT.ptr = cptr;
+    s.AT.idx = std::move(trow);
+    s.AT.val.resize(perm.size());
+#pragma omp parallel for schedule(static)
+    for (long q = 0; q < (long)perm.size(); ++q) s.AT.val[q] = s.A.val[perm[q]];
 
     s.c.resize(lp.n);
     s.l.resize(lp.n);
@@ -175,6 +227,7 @@ double estimate_norm(const ScaledLP& s, int iters) {
         const double nz = norm2(z);
         if (nz == 0.0) break;
         lambda = nz;  // ||v|| = 1, so ||A'^T A' v|| -> sigma_max^2
+#pragma omp parallel for schedule(static)
         for (int j = 0; j < s.n; ++j) v[j] = z[j] / nz;
     }
     return std::sqrt(lambda) * 1.01;

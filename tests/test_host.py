@@ -45,6 +45,23 @@ def test_scaling_bitwise_equal_to_oracle(oracle, name):
         assert s[k] == o[k], k
 
 
+@pytest.mark.parametrize("kind,params", [("blending", dict(periods=24)), ("random_lp", dict(m=3000, n=7000)),
+                                         ("portfolio", dict(assets=3000, factors=20))])
+def test_scaling_of_generated_problems_bitwise_equal_to_oracle(oracle, kind, params):
+    """The parallel column view, A^T and power iteration of the scaling on larger generated problems (uneven row
+    lengths, many threads' row ranges): every array and scalar equals the oracle's serial computation."""
+    p = g.Problem.generate(kind, seed=3, **params)
+    s = p.scaled()
+    o = oracle.OracleLP.from_export(p.export()).scaled()
+    for a, b in [("A", "a"), ("AT", "t"), ("Q", "q")]:
+        assert np.array_equal(s[a].ptr, o[b + "ptr"]) and np.array_equal(s[a].idx, o[b + "idx"]), a
+        assert np.array_equal(s[a].val, o[b + "val"]), a
+    for k in ("c", "l", "u", "lo", "hi", "R", "C", "Rres", "Cres"):
+        assert np.array_equal(s[k], o[k]), k
+    for k in ("bscale", "cscale", "normQ", "obj_const", "bnorm", "cnorm", "normA"):
+        assert s[k] == o[k], k
+
+
 @pytest.mark.parametrize("name", ["afiro", "bgetam", "port_tiny", "qblend_small", "etamacro"])
 def test_mps_write_read_round_trip(tmp_path, name):
     p = g.Problem.read_mps(instance_path(name))
