@@ -343,7 +343,8 @@ _BATCH_STATS_KEYS = ["rows", "cols", "nnz", "blocks_a", "blocks_at", "upload_s",
 
 
 class Batch:
-    """Many LPs solved together by the block-diagonal batched kernels (one launch sequence for all).
+    """Many LPs and convex QPs solved together by the block-diagonal batched kernels (one launch sequence for
+    all).
 
     Options as Problem.solve_gpu (check_every must be even).  run() returns one result dict per instance
     (solve_s is the whole batch's wall time) and stats() the batch-level figures of that run."""

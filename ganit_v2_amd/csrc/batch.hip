@@ -1,8 +1,9 @@
-// libganit_hip.so, part 3: many uneven LPs solved together (branch-and-bound node relaxations, strong-
-// branching probes).
+// libganit_hip.so, part 3: many uneven LPs and convex QPs solved together (branch-and-bound node relaxations,
+// strong-branching probes, a batch of differentiable QP layers).
 //
 // Layout: the instances are packed into one block-diagonal problem -- A_all and its transpose are the
-// instances' matrices placed on the diagonal (global column indices), every vector is the concatenation of
+// instances' matrices placed on the diagonal (global column indices), and so is Q when an instance has one
+// (an LP's rows of it are empty); every vector is the concatenation of
 // the instances' vectors -- and the LDS-staged row-block partition is built per instance, so a workgroup
 // never mixes instances and carries its instance id.  Every kernel therefore runs over the whole batch
 // with full-GPU parallelism, whatever the instance sizes, instead of one instance per workgroup where the
@@ -34,10 +35,10 @@ namespace ganit_hip {
 constexpr int kBN = 12, kBM = 7;
 
 // ------------------------------------------------------------------ iteration kernels
-template <class M, class V>
+template <class M, class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* __restrict__ blocks,
                                                       const int* __restrict__ list,
-                                                      const int* __restrict__ blk_inst,
+                                                      const int* __restrict__ blk_inst, CsrView<M> Q,
                                                       const BatchState* __restrict__ st, const V* __restrict__ y,
                                                       const V* __restrict__ x, const V* __restrict__ c,
                                                       const V* __restrict__ l, const V* __restrict__ u,
@@ -56,8 +57,16 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
         AT, blocks, b, [&](int, int i) { return y[i]; },
         [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j]}; },
         [&](int j, V (&aty)[1], Ops& o) {
+#pragma clang fp contract(off)
+            V qx = V(0);
+            if (HASQ) {  // Q row summed sequentially by the owning thread, as in the single-instance kernel
+                for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
+                    const V prod = widen<V>(Q.val[p]) * x[Q.idx[p]];
+                    qx = qx + prod;
+                }
+            }
             V sm = o.s;
-            primal_update<V>(o.x, aty[0], V(0), o.c, o.l, o.u, tau, xn[j], xbar[j], sm);
+            primal_update<V>(o.x, aty[0], qx, o.c, o.l, o.u, tau, xn[j], xbar[j], sm);
             sumx[j] = sm;
         },
         lds, none);
@@ -125,10 +134,10 @@ __device__ __forceinline__ double bt_row_obj(double yi, double lw, double hg) {
 }
 
 // Same sums as check_n_block of the single-instance path, one partial row per workgroup.
-template <class M, class V>
+template <class M, class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int* __restrict__ blocks,
                                                        const int* __restrict__ list,
-                                                       const int* __restrict__ blk_inst,
+                                                       const int* __restrict__ blk_inst, CsrView<M> Q,
                                                        const BatchState* __restrict__ st, const V* __restrict__ x,
                                                        const V* __restrict__ y, const V* __restrict__ xa,
                                                        const V* __restrict__ ya, const V* __restrict__ xr,
@@ -148,19 +157,34 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
         AT, blocks, b,
         [&](int k, int i) { return k == 0 ? y[i] : k == 1 ? (avg ? ya[i] : V(0)) : V(y[i] - yr[i]); },
         [&](int j, V (&t)[3]) {
+#pragma clang fp contract(off)
+            V qc = V(0), qa = V(0);
+            if (HASQ) {
+                for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
+                    const V a = widen<V>(Q.val[p]);
+                    const V pc = a * x[Q.idx[p]];
+                    qc = qc + pc;
+                    if (avg) {
+                        const V pa = a * xa[Q.idx[p]];
+                        qa = qa + pa;
+                    }
+                }
+            }
             const double cj = double(c[j]), lj = double(l[j]), uj = double(u[j]), cr = double(cres[j]);
             const double xj = double(x[j]);
             double r2, ob;
-            bt_dual_terms(cj - double(t[0]) + 0.0, lj, uj, cr, r2, ob);
+            bt_dual_terms(cj - double(t[0]) + (HASQ ? double(qc) : 0.0), lj, uj, cr, r2, ob);
             acc[0] += r2;
             acc[1] += ob;
             acc[2] += cj * xj;
+            if (HASQ) acc[3] += xj * double(qc);
             if (avg) {
                 const double xaj = double(xa[j]), xrj = double(xr[j]);
-                bt_dual_terms(cj - double(t[1]) + 0.0, lj, uj, cr, r2, ob);
+                bt_dual_terms(cj - double(t[1]) + (HASQ ? double(qa) : 0.0), lj, uj, cr, r2, ob);
                 acc[4] += r2;
                 acc[5] += ob;
                 acc[6] += cj * xaj;
+                if (HASQ) acc[7] += xaj * double(qa);
                 const double d0 = xj - xrj, d1 = xaj - xrj;
                 acc[10] += d0 * d0;
                 acc[11] += d1 * d1;
@@ -359,6 +383,7 @@ struct Batch {
     Ctx* ctx = nullptr;
     int ninst = 0, N = 0, Mr = 0;
     Csr *A = nullptr, *AT = nullptr;
+    Csr* Q = nullptr;  // block-diagonal Q when an instance is a QP
     int *blk_inst_a = nullptr, *blk_inst_at = nullptr, *first_a = nullptr, *first_at = nullptr;
     int *col_inst = nullptr, *row_inst = nullptr;
     BatchInst* inst = nullptr;
@@ -402,6 +427,7 @@ struct Batch {
         if (host_running) (void)hipHostFree(host_running);
         csr_destroy(A);
         csr_destroy(AT);
+        csr_destroy(Q);
     }
 };
 
@@ -517,10 +543,18 @@ void bt_iteration(Batch* bt, int mdt_unused) {
     hipStream_t s = bt->ctx->stream;
     BatchVecs& v = bt->vs[std::is_same<V, double>::value ? 0 : 1];
     const int c = bt->cur;
-    if (bt->nact_at > 0)
-        hipLaunchKernelGGL((k_bt_primal<M, V>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                           bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
-                           (const V*)v.c, (const V*)v.l, (const V*)v.u, (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
+    if (bt->nact_at > 0) {
+        if (bt->Q)
+            hipLaunchKernelGGL((k_bt_primal<M, V, true>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
+                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->Q->view<M>(), bt->st,
+                               (const V*)v.Y[c], (const V*)v.X[c], (const V*)v.c, (const V*)v.l, (const V*)v.u,
+                               (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
+        else
+            hipLaunchKernelGGL((k_bt_primal<M, V, false>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
+                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->AT->view<M>(), bt->st,
+                               (const V*)v.Y[c], (const V*)v.X[c], (const V*)v.c, (const V*)v.l, (const V*)v.u,
+                               (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
+    }
     if (bt->nact_a > 0)
         hipLaunchKernelGGL((k_bt_dual<M, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<M>(), bt->A->blocks,
                            bt->list_a, bt->blk_inst_a, bt->st, (const V*)v.xbar, (const V*)v.Y[c], (const V*)v.lo,
@@ -537,11 +571,19 @@ void bt_check(Batch* bt, int avg, int mode) {
         hipLaunchKernelGGL(k_bt_average<V>, dim3(bt_grid((int64_t)bt->N + bt->Mr)), dim3(kBlock), 0, s, bt->N, bt->Mr,
                            bt->col_inst, bt->row_inst, bt->st, (const V*)v.sumx, (const V*)v.sumy, (V*)v.xa,
                            (V*)v.ya);
-    if (bt->nact_at > 0)
-    hipLaunchKernelGGL((k_bt_check_n<M, V>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                       bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->st, (const V*)v.X[c], (const V*)v.Y[c], (const V*)v.xa,
-                       (const V*)v.ya, (const V*)v.xr, (const V*)v.yr, (const V*)v.c, (const V*)v.l, (const V*)v.u,
-                       (const V*)v.cres, avg, bt->part_n);
+    if (bt->nact_at > 0) {
+        const CsrView<M> q = bt->Q ? bt->Q->view<M>() : bt->AT->view<M>();
+        if (bt->Q)
+            hipLaunchKernelGGL((k_bt_check_n<M, V, true>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
+                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, q, bt->st, (const V*)v.X[c],
+                               (const V*)v.Y[c], (const V*)v.xa, (const V*)v.ya, (const V*)v.xr, (const V*)v.yr,
+                               (const V*)v.c, (const V*)v.l, (const V*)v.u, (const V*)v.cres, avg, bt->part_n);
+        else
+            hipLaunchKernelGGL((k_bt_check_n<M, V, false>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
+                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, q, bt->st, (const V*)v.X[c],
+                               (const V*)v.Y[c], (const V*)v.xa, (const V*)v.ya, (const V*)v.xr, (const V*)v.yr,
+                               (const V*)v.c, (const V*)v.l, (const V*)v.u, (const V*)v.cres, avg, bt->part_n);
+    }
     if (bt->nact_a > 0)
     hipLaunchKernelGGL((k_bt_check_m<M, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<M>(),
                        bt->A->blocks, bt->list_a, bt->blk_inst_a, bt->st, (const V*)v.X[c], (const V*)v.xa, (const V*)v.Y[c],
@@ -583,18 +625,32 @@ int ganit_hip_batch_create(ganit_hip_ctx* h, int ninst, const ganit_hip_lp_desc*
             bt->ctx = ctx;
             bt->ninst = ninst;
             // ---- pack the block-diagonal A and A^T with global indices
-            std::vector<int> aptr{0}, aidx, atptr{0}, atidx;
-            std::vector<double> aval, atval;
+            std::vector<int> aptr{0}, aidx, atptr{0}, atidx, qptr{0}, qidx;
+            std::vector<double> aval, atval, qval;
+            bool anyq = false;
             std::vector<BatchInst> inst(ninst);
             std::vector<int> col_inst, row_inst;
             for (int k = 0; k < ninst; ++k) {
                 const ganit_hip_lp_desc& d = *lps[k];
-                GH_REQUIRE(d.m >= 0 && d.n >= 0 && !d.q_ptr, "batched instances must be LPs");
+                GH_REQUIRE(d.m >= 0 && d.n >= 0, "negative problem dimension");
                 BatchInst& in = inst[k];
                 in.m = d.m; in.n = d.n; in.row0 = bt->Mr; in.col0 = bt->N;
-                const double* sc = inst_scalars[k];  // eta, norm_a, obj_scale, obj_const, bnorm, cnorm, w0
+                // eta, norm_a, obj_scale, obj_const, bnorm, cnorm, w0 (and ||Q'|| for a QP)
+                const double* sc = inst_scalars[k];
                 in.eta = sc[0]; in.norm_a = sc[1]; in.obj_scale = sc[2]; in.obj_const = sc[3];
                 in.bnorm = sc[4]; in.cnorm = sc[5]; in.w0 = sc[6];
+                in.hasq = d.q_ptr != nullptr;
+                in.norm_q = in.hasq ? sc[7] : 0.0;
+                anyq = anyq || in.hasq;
+                for (int j = 0; j < d.n; ++j) {  // this instance's rows of the block-diagonal Q
+                    if (in.hasq)
+                        for (int p = d.q_ptr[j]; p < d.q_ptr[j + 1]; ++p) {
+                            GH_REQUIRE(d.q_idx[p] >= 0 && d.q_idx[p] < d.n, "Q column index out of range");
+                            qidx.push_back(in.col0 + d.q_idx[p]);
+                            qval.push_back(d.q_val[p]);
+                        }
+                    qptr.push_back((int)qidx.size());
+                }
                 for (int i = 0; i < d.m; ++i) {
                     GH_REQUIRE(d.a_ptr[i + 1] - d.a_ptr[i] <= kSplit, "batched rows are limited to 8192 entries");
                     for (int p = d.a_ptr[i]; p < d.a_ptr[i + 1]; ++p) {
@@ -620,6 +676,7 @@ int ganit_hip_batch_create(ganit_hip_ctx* h, int ninst, const ganit_hip_lp_desc*
             }
             bt->A = csr_create(ctx, bt->Mr, bt->N, aptr.data(), aidx.data(), aval.data(), mdt_mask);
             bt->AT = csr_create(ctx, bt->N, bt->Mr, atptr.data(), atidx.data(), atval.data(), mdt_mask);
+            if (anyq) bt->Q = csr_create(ctx, bt->N, bt->N, qptr.data(), qidx.data(), qval.data(), mdt_mask);
             bt->aptr_h = std::move(aptr);
             bt->atptr_h = std::move(atptr);
             bt->aidx_h = std::move(aidx);

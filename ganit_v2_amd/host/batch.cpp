@@ -70,7 +70,7 @@ BatchSolver::BatchSolver(const std::vector<const LP*>& lps, const Options& opt) 
     if (lps.empty()) throw std::runtime_error("empty batch");
     if (opt.check_every < 2 || opt.check_every % 2) throw std::runtime_error("batched check_every must be even");
     for (const LP* lp : lps)
-        if (!lp || lp->has_q()) throw std::runtime_error("batched instances must be LPs");
+        if (!lp) throw std::runtime_error("null batch instance");
     chk(ganit_hip_ctx_create(opt.device, &I.ctx));
     char name[256] = {0};
     chk(ganit_hip_ctx_device_info(I.ctx, name, sizeof(name), nullptr));
@@ -115,7 +115,7 @@ void BatchSolver::upload() {
     const int K = size();
     std::vector<ganit_hip_lp_desc> d(K);
     std::vector<const ganit_hip_lp_desc*> dp(K);
-    std::vector<std::array<double, 7>> sc(K);
+    std::vector<std::array<double, 8>> sc(K);
     std::vector<const double*> scp(K);
     for (int k = 0; k < K; ++k) {
         const ScaledLP& s = I.s[k];
@@ -126,8 +126,10 @@ void BatchSolver::upload() {
         e.at_ptr = s.AT.ptr.data(); e.at_idx = s.AT.idx.data(); e.at_val = s.AT.val.data();
         e.c = s.c.data(); e.l = s.l.data(); e.u = s.u.data(); e.lo = s.lo.data(); e.hi = s.hi.data();
         e.rres = s.Rres.data(); e.cres = s.Cres.data();
+        if (s.has_q()) { e.q_ptr = s.Q.ptr.data(); e.q_idx = s.Q.idx.data(); e.q_val = s.Q.val.data(); }
         dp[k] = &d[k];
-        sc[k] = {0.998 / I.norm_a[k], I.norm_a[k], s.cscale * s.bscale, s.obj_const, s.bnorm, s.cnorm, I.w0[k]};
+        sc[k] = {0.998 / I.norm_a[k], I.norm_a[k], s.cscale * s.bscale, s.obj_const, s.bnorm, s.cnorm, I.w0[k],
+                 s.normQ};
         scp[k] = sc[k].data();
     }
     const unsigned mmask = (1u << GANIT_F64) | (I.mixed ? (1u << I.low_mdt) : 0u);

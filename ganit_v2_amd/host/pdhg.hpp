@@ -117,7 +117,7 @@ class GpuSolver {
     std::unique_ptr<Impl> p_;
 };
 
-// extension: many LPs solved together by the block-diagonal batched kernels.  Each instance gets the single
+// extension: many LPs and convex QPs solved together by the block-diagonal batched kernels.  Each instance gets the single
 // solve's scaling, initial point, step sizes and restart rules; a precision level runs for all selected
 // instances at once, every result is certified in FP64, and only the instances the ladder sends on take
 // part in the next level.

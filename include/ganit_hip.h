@@ -185,13 +185,15 @@ int ganit_hip_pdhg_adopt_average(ganit_hip_pdhg* pd, int vdt);
  * any pointer may be NULL.  Synchronises. */
 int ganit_hip_pdhg_store(ganit_hip_pdhg* pd, int vdt, double* x, double* y, double* xr, double* yr);
 
-/* ------------------------------------------------------------------ batched PDHG (many uneven LPs)
+/* ------------------------------------------------------------------ batched PDHG (many uneven LPs / QPs)
  * No reference counterpart: the reference solves one LP per call and its branch-and-cut uses only the
- * dual simplex (src/mip.cpp:116, 312-323).  The instances (scaled LPs, no Q) are packed block-diagonally;
- * each keeps its own restart state, step sizes, primal weight and status on the device, and the check
- * logic of src/pdhg_impl.hpp:144-189 is applied per instance by a device kernel. */
+ * dual simplex (src/mip.cpp:116, 312-323).  The instances (scaled LPs or convex QPs) are packed
+ * block-diagonally (A, A^T and, when any instance has one, Q); each keeps its own restart state, step sizes
+ * (with the QP cap of src/pdhg_impl.hpp:87-91), primal weight and status on the device, and the check logic
+ * of src/pdhg_impl.hpp:144-189 is applied per instance by a device kernel. */
 typedef struct ganit_hip_batch ganit_hip_batch;
-/* inst_scalars[k] = {eta, ||A||, obj_scale, obj_const, bnorm, cnorm, w0} of instance k. */
+/* inst_scalars[k] = {eta, ||A||, obj_scale, obj_const, bnorm, cnorm, w0} of instance k, followed by ||Q'||
+ * when instance k has a Q (lps[k]->q_ptr non-NULL). */
 int ganit_hip_batch_create(ganit_hip_ctx* ctx, int ninst, const ganit_hip_lp_desc* const* lps,
                            const double* const* inst_scalars, unsigned mdt_mask, unsigned vdt_mask,
                            ganit_hip_batch** out);

@@ -92,8 +92,9 @@ int ganit_solver_adjoint(ganit_solver* s, const double* gx, const double* aopts,
                          double* info);
 void ganit_solver_free(ganit_solver* s);
 
-/* Many LPs solved together by the block-diagonal batched kernels (no reference counterpart: the reference
- * solves one problem per call; this serves branch-and-bound node relaxations and strong-branching probes).
+/* Many LPs and convex QPs solved together by the block-diagonal batched kernels (no reference counterpart:
+ * the reference solves one problem per call; this serves branch-and-bound node relaxations, strong-branching
+ * probes and batches of QP layers).
  * Each instance gets the single solve's scaling, initial point, step sizes, restart rules and FP64
  * certification; opts as ganit_solve_gpu (check_every must be even; eval_fp64 is ignored).
  * run: out[18*k..] = instance k's results in ganit_solve_gpu's layout (solve_s is the whole batch's);

@@ -1,4 +1,4 @@
-"""Batched PDHG (many uneven LPs in one block-diagonal launch sequence) against the oracle, per instance.
+"""Batched PDHG (many uneven LPs and QPs in one block-diagonal launch sequence) against the oracle, per instance.
 
 The batched kernels sum every row in the same order as the single-instance row-block kernels and the
 oracle's CPU row loop (rows here stay under 256 nonzeros), and each instance's decisions are the single
@@ -82,6 +82,28 @@ def test_batch_matches_sequential_gpu_solves():
         assert r["status"] == s["status"], k
         if s["status"] == "OPTIMAL":
             assert abs(r["pobj"] - s["pobj"]) <= 5e-4 * (1 + abs(s["pobj"])), k
+
+
+def _qp_mix():
+    """QPs (a differentiable-layer QP, two small portfolios) between LP node relaxations."""
+    qps = [g.Problem.generate("diff_qp", seed=1, n=150, m_eq=20, m_le=15, density=0.05, qdensity=0.02),
+           g.Problem.generate("portfolio", seed=0, assets=200, factors=4),
+           g.Problem.generate("portfolio", seed=1, assets=200, factors=4)]
+    lps = _problems(3, seed0=500)
+    return [qps[0], lps[0], qps[1], lps[1], qps[2], lps[2]]
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_batch_with_qps_matches_oracle(oracle, precision):
+    """QPs in the batch: the block-diagonal Q, the owning thread's sequential Q-row sum in the primal kernel and
+    in the check, the QP step-size cap and the x'Qx objective terms of each instance's decisions -- each instance
+    follows the oracle's own QP trajectory (same status; same iteration count for most; certified tolerance)."""
+    probs = _qp_mix()
+    res = g.Batch(probs, precision=precision, tol=1e-4).run()
+    oracle.set_threads(4)
+    refs = [oracle.OracleLP.from_export(p.export()).solve(precision=precision, tol=1e-4, time_limit=120)
+            for p in probs]
+    _compare(res, refs, 1e-4, 0.8)
 
 
 def test_batch_is_deterministic():
