@@ -197,11 +197,13 @@ def batched_lps(count=512):
     return out
 
 
-def diff_qps(count, threads=32):
-    """Differentiable QPs (n=2000, 300 equality + 200 <= rows), the instances run concurrently from host
-    threads (one solver, HIP context and stream each; these solves are launch-bound, so the GPU overlaps them):
-    the FP32 path -- forward (mixed ladder, tol 1e-6) + FP32 adjoint -- timed as one batch, then the FP64
-    check -- forward (tol 1e-9) + FP64 adjoint -- and each instance's FP32 gradient error against it."""
+def diff_qps(count, threads=32, batched=True):
+    """Differentiable QPs (n=2000, 300 equality + 200 <= rows): the FP32 path -- forward (mixed ladder, tol 1e-6)
+    + FP32 adjoint -- timed as one batch, then the FP64 check -- forward (tol 1e-9) + FP64 adjoint -- and each
+    instance's FP32 gradient error against it.  batched: the forwards of a phase run as one batched solve
+    (ganit_v2_amd.Batch) and each instance's solver takes its solution over for the adjoint; else every
+    instance runs forward and adjoint in its own host thread (one solver, HIP context and stream each; these
+    solves are launch-bound, so the GPU overlaps them).  The adjoints run concurrently either way."""
     from concurrent.futures import ThreadPoolExecutor
     import numpy as np
     import ganit_v2_amd as g
@@ -218,14 +220,40 @@ def diff_qps(count, threads=32):
               + (f", adjoint {a['seconds']:.2f} s" if a else ""), file=sys.stderr, flush=True)
         return r, a
 
+    def batch_path(precision, tol, adj_precision, adj_tol):
+        res = g.Batch(probs, precision=precision, tol=tol, time_limit=300).run(return_solution=True)
+        n_opt = sum(r["status"] == "OPTIMAL" for r in res)
+        print(f"[bench] diff_qp {precision} batch: {n_opt}/{count} OPTIMAL, max {max(r['iters'] for r in res)} "
+              f"iterations, forward {res[0]['solve_s']:.2f} s", file=sys.stderr, flush=True)
+
+        def adjoint(k):
+            r = res[k]
+            if r["status"] != "OPTIMAL":
+                return r, None
+            s = probs[k].solver(precision=precision, tol=tol)
+            s.upload()
+            s.set_solution(r["x"], r["y"], fp32_iterate=precision != "double" and r["fp32_iters"] == r["iters"])
+            a = s.adjoint(ws[k], precision=adj_precision, tol=adj_tol)
+            print(f"[bench] diff_qp {precision} adjoint {k}: {a['seconds']:.2f} s", file=sys.stderr, flush=True)
+            return r, a
+
+        with ThreadPoolExecutor(nt) as ex:
+            return list(ex.map(adjoint, range(count)))
+
     nt = max(1, min(threads, count))
     t = time.perf_counter()
-    with ThreadPoolExecutor(nt) as ex:
-        f32 = list(ex.map(lambda a: path(a, "mixed", 1e-6, "single", 1e-5), zip(probs, ws)))
+    if batched:
+        f32 = batch_path("mixed", 1e-6, "single", 1e-5)
+    else:
+        with ThreadPoolExecutor(nt) as ex:
+            f32 = list(ex.map(lambda a: path(a, "mixed", 1e-6, "single", 1e-5), zip(probs, ws)))
     wall32 = time.perf_counter() - t
     t = time.perf_counter()
-    with ThreadPoolExecutor(nt) as ex:
-        f64 = list(ex.map(lambda a: path(a, "double", 1e-9, "double", 1e-10), zip(probs, ws)))
+    if batched:
+        f64 = batch_path("double", 1e-9, "double", 1e-10)
+    else:
+        with ThreadPoolExecutor(nt) as ex:
+            f64 = list(ex.map(lambda a: path(a, "double", 1e-9, "double", 1e-10), zip(probs, ws)))
     wall64 = time.perf_counter() - t
     rel = lambda u, v: float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300))
     rows = []
@@ -239,6 +267,7 @@ def diff_qps(count, threads=32):
         print(f"[bench] diff_qp {row}", file=sys.stderr, flush=True)
     errs = [max(r["grad_err_fp32"].values()) for r in rows if "grad_err_fp32" in r]
     return dict(workload=f"{count} differentiable QPs n=2000, 300 E + 200 L rows, seeds 0..{count - 1}",
+                forward="one batched solve per phase" if batched else "concurrent single solves",
                 threads=nt, fp32_batch_wall_s=round(wall32, 2), fp64_check_wall_s=round(wall64, 2),
                 # PDHG iterations of all instances per second of the FP32 phase (the GPU overlaps the solves;
                 # each instance's own solve_s and time limit run on the wall clock while it shares the GPU)
@@ -291,6 +320,8 @@ def main():
                     help="skip the portfolio-QP ladder and the batched-LP comparison reported beside the headline")
     ap.add_argument("--c4", type=int, default=0,
                     help="also run N differentiable QPs (forward + adjoint) and report their times and errors")
+    ap.add_argument("--c4-concurrent", action="store_true",
+                    help="with --c4: forwards as concurrent single solves instead of one batched solve per phase")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the fused iteration kernels (the launches the roofline figure uses), plus one "
                          "plain 1 GiB device copy as the counter calibration; for profiler runs")
@@ -382,7 +413,7 @@ def main():
         extras["portfolio_qp"] = portfolio_ladder()
         extras["batched_lp"] = batched_lps()
     if a.c4 > 0:
-        extras["diff_qp"] = diff_qps(a.c4)
+        extras["diff_qp"] = diff_qps(a.c4, batched=not a.c4_concurrent)
 
     cpu = None
     if not a.no_cpu:

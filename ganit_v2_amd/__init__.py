@@ -73,6 +73,7 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_solver_free.argtypes = [_vp]
     lib.ganit_solver_index_bytes.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.ganit_solver_adjoint.argtypes = [_vp, _f64p, _f64p, _vp, _vp, _vp, _vp]
+    lib.ganit_solver_set_solution.argtypes = [_vp, _f64p, _f64p, ctypes.c_int]
     lib.ganit_batch_create.restype = _vp
     lib.ganit_batch_create.argtypes = [_vp, ctypes.c_int, _f64p]
     lib.ganit_batch_upload.argtypes = [_vp]
@@ -306,6 +307,16 @@ class Solver:
                     "solver_adjoint")
         return dict(dc=dc, db=db, dq=dq, iters=int(info[0]), rel_change=float(info[1]), free_vars=int(info[2]),
                     active_rows=int(info[3]), seconds=float(info[4]))
+
+    def set_solution(self, x, y, fp32_iterate: bool = False) -> None:
+        """Make an original-space solution (x, y) found elsewhere (a Batch run) this solver's solution, as if
+        run() had ended there, so that adjoint() differentiates at it."""
+        d = self._dims
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if x.shape != (d["n"],) or y.shape != (d["m"],):
+            raise ValueError(f"x, y must have shapes ({d['n']},), ({d['m']},)")
+        _check_host(host_lib().ganit_solver_set_solution(self._h, x, y, int(bool(fp32_iterate))), "set_solution")
 
     def index_bytes(self, which: int) -> int:
         """Bytes per column index the fused kernels read for A (0), A^T (1), Q (2): 2 (compact) or 4."""

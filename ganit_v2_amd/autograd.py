@@ -26,7 +26,7 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 import torch
 
-from . import Problem
+from . import Batch, Problem
 
 ROW_EQ, ROW_LE, ROW_GE = 0, 1, 2
 
@@ -37,7 +37,7 @@ class QPLayer:
 
     def __init__(self, n, A_rows, A_cols, A_vals, senses, l, u, Q_rows=(), Q_cols=(), Q_vals=(),
                  precision="mixed", tol=1e-6, adjoint_precision="double", adjoint_tol=1e-8,
-                 adjoint_max_iter=200_000, **solve_opts):
+                 adjoint_max_iter=200_000, batch_forward=True, **solve_opts):
         self.n = int(n)
         self.senses = np.asarray(senses, dtype=np.int8)
         self.m = len(self.senses)
@@ -49,6 +49,8 @@ class QPLayer:
         self.u = np.asarray(u, np.float64)
         self.opts = dict(precision=precision, tol=tol, **solve_opts)
         self.adjoint_opts = dict(precision=adjoint_precision, tol=adjoint_tol, max_iter=adjoint_max_iter)
+        # a (B, n) input: its B forward solves as one batched solve (else as concurrent single solves)
+        self.batch_forward = bool(batch_forward)
         self.last = None  # (result dict, adjoint dict) of the latest forward / backward, for inspection
 
     @classmethod
@@ -116,6 +118,28 @@ class QPLayer:
         return solver, p, r
 
 
+    def _forward_batch(self, C: np.ndarray, B: np.ndarray, Qd: np.ndarray):
+        """The instances' forward solves as one batched solve (each follows its single solve's trajectory); each
+        instance's solver then takes its solution over, so that its adjoint differentiates there."""
+        problems = [self.problem(c, b, q) for c, b, q in zip(C, B, Qd)]
+        res = Batch(problems, **self.opts).run(return_solution=True)
+        for k, r in enumerate(res):
+            if r["status"] != "OPTIMAL":
+                raise RuntimeError(f"QP layer forward solve {k} ended {r['status']}")
+        low = self.opts.get("precision", "mixed") != "double"
+
+        def adopt(k):
+            solver = problems[k].solver(**self.opts)
+            solver.upload()
+            # an instance certified in the FP32 level never ran an FP64 iteration
+            solver.set_solution(res[k]["x"], res[k]["y"], fp32_iterate=low and res[k]["fp32_iters"] == res[k]["iters"])
+            return solver
+
+        with ThreadPoolExecutor(self.threads(len(problems))) as ex:
+            solvers = list(ex.map(adopt, range(len(problems))))
+        return list(zip(solvers, problems, res))
+
+
 def _host(t: torch.Tensor) -> np.ndarray:
     return np.ascontiguousarray(t.detach().to("cpu", torch.float64).numpy())
 
@@ -144,13 +168,17 @@ __all__ = ["QPLayer", "QPFunction", "ROW_EQ", "ROW_LE", "ROW_GE"]
 
 
 class QPBatchFunction(torch.autograd.Function):
-    """Row k of x* = QP(c[k], b[k], diag q[k]); the B forward solves and the B adjoints run concurrently."""
+    """Row k of x* = QP(c[k], b[k], diag q[k]); the B forward solves run as one batched solve (or concurrently),
+    the B adjoints concurrently."""
 
     @staticmethod
     def forward(ctx, c, b, q, layer: QPLayer):
         C, B, Qd = _host(c), _host(b), _host(q)
-        with ThreadPoolExecutor(layer.threads(len(C))) as ex:
-            outs = list(ex.map(layer._forward_one, C, B, Qd))
+        if layer.batch_forward:
+            outs = layer._forward_batch(C, B, Qd)
+        else:
+            with ThreadPoolExecutor(layer.threads(len(C))) as ex:
+                outs = list(ex.map(layer._forward_one, C, B, Qd))
         ctx.solvers = [o[0] for o in outs]
         ctx.problems = [o[1] for o in outs]
         ctx.layer = layer

@@ -210,6 +210,15 @@ int ganit_solver_kernel_times(ganit_solver* h, int level, int reps, double* ms) 
     return guard([&] { reinterpret_cast<ganit::GpuSolver*>(h)->kernel_times(level, reps, ms); });
 }
 
+int ganit_solver_set_solution(ganit_solver* h, const double* x, const double* y, int fp32_iterate) {
+    return guard([&] {
+        auto* sv = reinterpret_cast<ganit::GpuSolver*>(h);
+        if (!sv || !x || !y) throw std::runtime_error("null argument");
+        const int n = sv->scaled().n, m = sv->scaled().m;
+        sv->set_solution(std::vector<double>(x, x + n), std::vector<double>(y, y + m), fp32_iterate != 0);
+    });
+}
+
 int ganit_solver_adjoint(ganit_solver* h, const double* gx, const double* ao, double* dc, double* db, double* dq,
                          double* info) {
     return guard([&] {

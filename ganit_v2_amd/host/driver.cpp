@@ -453,6 +453,25 @@ AdjointResult GpuSolver::adjoint(const std::vector<double>& gx, const AdjointOpt
     return r;
 }
 
+void GpuSolver::set_solution(const std::vector<double>& x, const std::vector<double>& y, bool fp32_iterate) {
+    Impl& I = *p_;
+    const ScaledLP& s = I.s;
+    if ((int)x.size() != s.n || (int)y.size() != s.m) throw std::runtime_error("set_solution: wrong lengths");
+    if (fp32_iterate && !I.mixed) throw std::runtime_error("an FP32 iterate needs a mixed or dynamic solver");
+    upload();
+    // the inverse of run()'s unscaling: x = x' C bscale, y = sign y' R cscale
+    const double sign = s.maximize ? -1.0 : 1.0;
+    std::vector<double> xs(s.n), ys(s.m);
+    for (int j = 0; j < s.n; ++j) xs[j] = x[j] / (s.C[j] * s.bscale);
+    for (int i = 0; i < s.m; ++i) ys[i] = sign * y[i] / (s.R[i] * s.cscale);
+    chk(ganit_hip_pdhg_load(I.pd, GANIT_F64, xs.data(), ys.data(), nullptr, nullptr));
+    if (I.mixed) chk(ganit_hip_pdhg_load(I.pd, GANIT_F32, xs.data(), ys.data(), nullptr, nullptr));
+    I.solved = true;
+    I.final_fp32 = fp32_iterate;
+    I.w_final = I.w0;  // the gradient at a fixed point does not depend on the step sizes; any weight serves
+    I.xs_final = xs;
+}
+
 Result solve_pdhg_gpu(const LP& lp, const Options& opt) {
     GpuSolver solver(lp, opt);
     return solver.run();

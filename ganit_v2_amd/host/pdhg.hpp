@@ -109,6 +109,10 @@ class GpuSolver {
     // Gradient of L(x*) with respect to c, the active row bounds and diag Q, given gx = dL/dx* at the last
     // run()'s solution: the adjoint PDHG iteration of libganit_hip.so at that solution.
     AdjointResult adjoint(const std::vector<double>& gx, const AdjointOptions& opt);
+    // Make (x, y) -- an original-space solution found elsewhere, e.g. by the batched solver -- this solver's
+    // solution, as if run() had ended there: adjoint() then differentiates at it.  fp32_iterate: the point
+    // came out of an FP32-vector level (its bound activity is judged to FP32 rounding).
+    void set_solution(const std::vector<double>& x, const std::vector<double>& y, bool fp32_iterate);
     const ScaledLP& scaled() const;
     double norm_a() const;
 

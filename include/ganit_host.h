@@ -90,6 +90,10 @@ int ganit_solver_index_bytes(ganit_solver* s, int which, int* bytes);
  * seconds. */
 int ganit_solver_adjoint(ganit_solver* s, const double* gx, const double* aopts, double* dc, double* db, double* dq,
                          double* info);
+/* Make (x (n), y (m)), an original-space solution found elsewhere (ganit_batch_run), the solver's solution as if
+ * ganit_solver_run had ended there, so that ganit_solver_adjoint differentiates at it; fp32_iterate = 1 when
+ * the point came out of an FP32-vector level (needs a mixed or dynamic solver).  No reference counterpart. */
+int ganit_solver_set_solution(ganit_solver* s, const double* x, const double* y, int fp32_iterate);
 void ganit_solver_free(ganit_solver* s);
 
 /* Many LPs and convex QPs solved together by the block-diagonal batched kernels (no reference counterpart:

@@ -128,11 +128,14 @@ def test_autograd_function_matches_adjoint():
     assert np.allclose(x.detach().numpy(), r["x"])
 
 
-def test_batched_layer_matches_single_instances():
-    """A batch (B x n data) runs its instances concurrently; each row's solution and gradients agree with the
-    same instance through the single-instance layer."""
+@pytest.mark.parametrize("precision,tol,batch_forward", [("double", 1e-9, True), ("double", 1e-9, False),
+                                                         ("mixed", 1e-6, True)])
+def test_batched_layer_matches_single_instances(precision, tol, batch_forward):
+    """A batch (B x n data) runs its forward solves as one batched solve (or concurrently) and its adjoints
+    concurrently; each row's solution and gradients agree with the same instance through the single-instance
+    layer."""
     p = g.Problem.generate("diff_qp", seed=4, n=500, m_eq=75, m_le=50, density=0.02, qdensity=0.002)
-    layer = QPLayer.from_problem(p, precision="double", tol=1e-9, adjoint_tol=1e-10)
+    layer = QPLayer.from_problem(p, precision=precision, tol=tol, adjoint_tol=1e-10, batch_forward=batch_forward)
     c0, b0, q0 = layer.data_of(p)
     rng = np.random.default_rng(3)
     k = 3
