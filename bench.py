@@ -221,7 +221,8 @@ def diff_qps(count, threads=32, batched=True):
         return r, a
 
     def batch_path(precision, tol, adj_precision, adj_tol):
-        res = g.Batch(probs, precision=precision, tol=tol, time_limit=300).run(return_solution=True)
+        # the batch's clock is shared by its instances; 150 s keeps each phase's silent stretch short
+        res = g.Batch(probs, precision=precision, tol=tol, time_limit=150).run(return_solution=True)
         n_opt = sum(r["status"] == "OPTIMAL" for r in res)
         print(f"[bench] diff_qp {precision} batch: {n_opt}/{count} OPTIMAL, max {max(r['iters'] for r in res)} "
               f"iterations, forward {res[0]['solve_s']:.2f} s", file=sys.stderr, flush=True)
