@@ -54,9 +54,11 @@ void column_view(const Csr& a, std::vector<int>& cptr, std::vector<int>& perm, s
     cptr.assign((size_t)n + 1, 0);
     perm.resize((size_t)nnz);
     trow.resize((size_t)nnz);
-    // threads, bounded so that the per-thread column counters stay within about 64 MB
+    // threads, bounded so that the per-thread column counters stay within about 64 MB; one inside an enclosing
+    // parallel region (the batched set-up scales its instances concurrently), where nested regions run serially
     const int64_t cap = std::max<int64_t>(1, (int64_t)16 * 1024 * 1024 / std::max(n, 1));
-    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({(int64_t)omp_get_max_threads(), cap, (int64_t)m}));
+    const int64_t avail = omp_in_parallel() ? 1 : (int64_t)omp_get_max_threads();
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({avail, cap, (int64_t)m}));
     // row ranges of about nnz / nt entries each
     std::vector<int> rb((size_t)nt + 1, m);
     rb[0] = 0;
