@@ -218,6 +218,37 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
+        if constexpr (WIN && NX == 1) {
+            if (wl > 0 && cnt <= kTile) {
+                // a windowed long row of at most kTile entries: its window (coalesced loads), offsets and
+                // values are all loaded in one phase, the window goes to LDS (past the reduction slots) and the
+                // products read it there -- the same products, added in the same per-thread order, as the
+                // gather loop below
+                constexpr int kPer = kTile / kBlock, kWPer = kWin / kBlock;
+                V* win = lds + kTile;
+                const int wlast = wl - 1, last = cnt - 1;
+                V wv[kWPer];
+                M mv[kPer];
+                int off[kPer];
+#pragma unroll
+                for (int k = 0; k < kWPer; ++k) wv[k] = gather(0, cb + min((int)threadIdx.x + k * kBlock, wlast));
+#pragma unroll
+                for (int k = 0; k < kPer; ++k) {
+                    const int q = min((int)threadIdx.x + k * kBlock, last);
+                    mv[k] = a.val[base + q];
+                    off[k] = (int)i16[base + q];
+                }
+#pragma unroll
+                for (int k = 0; k < kWPer; ++k) win[(int)threadIdx.x + k * kBlock] = wv[k];
+                __syncthreads();
+#pragma unroll
+                for (int k = 0; k < kPer; ++k)
+                    if ((int)threadIdx.x + k * kBlock < cnt) s[0] += widen<V>(mv[k]) * win[off[k]];
+                block_sum_v<V, NX>(s, lds);
+                if (threadIdx.x == 0) row_fn(r0, s, pv);
+                return;
+            }
+        }
         if (i16 && wraw >= 0) {
 #pragma unroll 8
             for (int p = base + threadIdx.x; p < base + cnt; p += kBlock) {

@@ -284,12 +284,16 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
         const int span = span_of(p0, p1, mn);
         if (r1 - r0 == 1 && p1 - p0 > kLong) {
             if (p1 - p0 > kSplit) continue;  // a split row: its chunks carry their own bases (below)
-            // a long row: compact when its span fits 16 bits, else 32-bit indices for this row only
+            // a long row: compact when its span fits 16 bits, else 32-bit indices for this row only; windowed
+            // on the same terms as a staged block when it has at most kTile entries
             if (span > 65535) {
                 bc[2 * q + 1] = -1;
                 n32 += p1 - p0;
             } else {
                 bc[2 * q] = mn;
+                bc[2 * q + 1] = (windows && p1 - p0 <= kTile && span + 1 <= kWin && span + 1 <= 2 * (p1 - p0))
+                                    ? span + 1
+                                    : 0;
                 fill(p0, p1, mn);
             }
             continue;
