@@ -173,7 +173,8 @@ int ganit_hip_pdhg_iterate(ganit_hip_pdhg* pd, int mdt, int vdt, int iters, doub
  * out[19]      primal ray    : sum_i ((A d)_i's distance to its row's recession cone / row scale)^2
  * out[20..21]  primal ray    : c'd, sum_j (d_j's distance to its bounds' recession cone * column scale)^2
  * Entries whose flag is off are returned as 0.  Replaces the 8-12 separate reductions and host syncs of
- * one check (src/pdhg_impl.hpp:92-118, 144-147, 179) with three kernels and one synchronisation. */
+ * one check (src/pdhg_impl.hpp:92-118, 144-147, 179) with two kernels (three with split rows, plus an averaging
+ * kernel when a matrix runs a lanes-per-row variant) and one synchronisation. */
 int ganit_hip_pdhg_check(ganit_hip_pdhg* pd, int mdt, int vdt, int flags, double inv_k, double* out);
 /* Restart (src/pdhg_impl.hpp:177-188): optionally adopt the average, then (xr, yr) = (x, y) and clear the
  * running sums. */
