@@ -191,6 +191,30 @@ def test_cli_json_line(tmp_path):
     assert p.returncode == 2  # not optimal
 
 
+def test_cli_detect_unbounded(tmp_path):
+    """min -x0 + x2 with x0 - x1 = 0, x1 + x2 >= 1, x >= 0 is unbounded along d = (1, 1, 0): with
+    --detect-unbounded the CLI reports DUAL_INFEASIBLE (exit code 2, not optimal); without it the solve runs to
+    its iteration limit as the reference's would."""
+    import json
+    import os
+    import subprocess
+    inf = np.inf
+    p = g.Problem.from_triplets(2, 3, np.array([0, 0, 1, 1], np.int32), np.array([0, 1, 1, 2], np.int32),
+                                np.array([1.0, -1.0, 1.0, 1.0]), np.array([-1.0, 0.0, 1.0]), np.zeros(3),
+                                np.full(3, inf), np.array([0.0, 1.0]), np.array([0.0, inf]))
+    f = str(tmp_path / "unbounded.mps")
+    p.write_mps(f)
+    exe = os.path.join(g.BIN_DIR, "ganit")
+    run = lambda *a: subprocess.run([exe, f, "--json", "--quiet", *a], stdout=subprocess.PIPE,
+                                    stderr=subprocess.PIPE, text=True, timeout=120)
+    r = run("--detect-unbounded")
+    assert r.returncode == 2, r.stderr
+    assert json.loads(r.stdout.strip().splitlines()[-1])["status"] == "DUAL_INFEASIBLE"
+    r = run("--iters", "20000")
+    assert r.returncode == 2, r.stderr
+    assert json.loads(r.stdout.strip().splitlines()[-1])["status"] != "DUAL_INFEASIBLE"
+
+
 @pytest.mark.parametrize("name", ["afiro", "adlittle", "25fv47", "random_lp"])
 def test_kahan_ladder_matches_oracle_kahan(oracle, name):
     """The Kahan-compensated FP32 check reductions in L1 (--kahan): same status as the oracle's kahan mode and
