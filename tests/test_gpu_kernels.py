@@ -159,7 +159,8 @@ def test_windowed_rowblock_spmv_bitwise(hipctx, oracle, monkeypatch, mdt, vdt, w
 def test_compact_indices_match_plain(hipctx, oracle, monkeypatch, mdt, vdt):
     """16-bit column offsets (per staged block, long row and split-row chunk, each falling back to 32-bit
     indices where its span does not fit) give the same bits as the plain 32-bit indices, on one matrix with
-    banded short rows, long rows (compact and too wide) and split rows (compact and too-wide chunks)."""
+    banded short rows, long rows (compact and gathered through an LDS window, and too wide) and split rows
+    (compact and too-wide chunks)."""
     from ganit_v2_amd import hip
     rng = np.random.default_rng(60 + mdt)
     n = 300000
@@ -168,7 +169,7 @@ def test_compact_indices_match_plain(hipctx, oracle, monkeypatch, mdt, vdt):
         k = int(rng.integers(0, 60))
         s0 = min(i * 150, n - 1000)
         rows.append(np.sort(s0 + rng.choice(1000, k, replace=False)))
-    rows.append(np.sort(rng.choice(2000, 1000, replace=False)))         # long row, compact
+    rows.append(np.sort(rng.choice(2000, 1000, replace=False)))         # long row, compact and windowed
     rows.append(np.sort(rng.choice(n, 1000, replace=False)))            # long row, span > 65535
     rows.append(np.sort(rng.choice(60000, 20000, replace=False)))       # split row, compact chunks
     rows.append(np.sort(rng.choice(n, 9000, replace=False)))            # split row: two too-wide chunks, one compact
