@@ -369,7 +369,9 @@ def main():
     # other ladder configurations, one timed solve each after a warm-up (reported, not the headline)
     ladder = []
     for cfg in (dict(precision="mixed"), dict(precision="dynamic"), dict(precision="mixed", kahan=True),
-                dict(precision="mixed", values="bf16", stall_checks=a.stall_checks, eval_fp64=True)):
+                dict(precision="mixed", values="bf16", stall_checks=a.stall_checks, eval_fp64=True),
+                dict(precision="mixed", values="bf16f32", stall_checks=12, stall_checks_l2=a.stall_checks,
+                     eval_fp64=True)):
         sv = staged(**cfg)
         sv.run()
         r = sv.run()

@@ -23,7 +23,7 @@ BIN_DIR = os.path.join(PKG_DIR, "bin")
 
 STATUS = ["OPTIMAL", "PRIMAL_INFEASIBLE", "ITERATION_LIMIT", "TIME_LIMIT", "NUMERICAL_ERROR", "DUAL_INFEASIBLE"]
 PRECISION = {"double": 0, "mixed": 1, "dynamic": 2}
-VALUES = {"f32": 0, "bf16": 1}
+VALUES = {"f32": 0, "bf16": 1, "bf16f32": 2}
 
 
 class NativeLibraryMissing(RuntimeError):

@@ -163,7 +163,8 @@ ganit::Options options_from(const double* o) {
     const int pr = (int)o[8];
     opt.precision = pr == 1 ? ganit::Precision::Mixed : pr == 2 ? ganit::Precision::Dynamic : ganit::Precision::Double;
     opt.stall_checks = (int)o[9]; opt.stall_checks_l2 = (int)o[10];
-    opt.values = (int)o[11] == 1 ? ganit::Values::BF16 : ganit::Values::F32;
+    const int vals = (int)o[11];
+    opt.values = vals == 1 ? ganit::Values::BF16 : vals == 2 ? ganit::Values::BF16_F32 : ganit::Values::F32;
     opt.device = (int)o[12];
     opt.eval_fp64 = o[13] != 0;
     const int bits = (int)o[14];  // bit 0: kahan, bit 1: detect_unbounded

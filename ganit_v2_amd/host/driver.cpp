@@ -272,7 +272,8 @@ void GpuSolver::upload() {
     d.c = s.c.data(); d.l = s.l.data(); d.u = s.u.data(); d.lo = s.lo.data(); d.hi = s.hi.data();
     d.rres = s.Rres.data(); d.cres = s.Cres.data();
     // every precision the ladder will use is resident at once, plus FP64 for certification
-    const unsigned mmask = (1u << GANIT_F64) | (I.mixed ? (1u << I.low_mdt) : 0u);
+    const unsigned mmask = (1u << GANIT_F64) | (I.mixed ? (1u << I.low_mdt) : 0u) |
+                           (I.mixed && I.opt.values == Values::BF16_F32 ? (1u << GANIT_BF16) : 0u);
     const unsigned vmask = (1u << GANIT_F64) | (I.mixed ? (1u << GANIT_F32) : 0u);
     chk(ganit_hip_pdhg_create(I.ctx, &d, mmask, vmask, &I.pd));
 }
@@ -312,7 +313,10 @@ Result GpuSolver::run() {
     if (opt.verbose) {
         std::printf("GANIT PDHG | device %s | %s | precision %s%s | rows %d cols %d nnz %lld | ||A||~%.3e | tol %.1e\n",
                     res.device.c_str(), s.has_q() ? "QP" : "LP", pname,
-                    mixed ? (opt.values == Values::BF16 ? (opt.eval_fp64 ? " (BF16 values, FP64 checks)" : " (BF16 values)")
+                    mixed ? (opt.values != Values::F32
+                                 ? (opt.values == Values::BF16 ? (opt.eval_fp64 ? " (BF16 values, FP64 checks)" : " (BF16 values)")
+                                                               : (opt.eval_fp64 ? " (BF16 then FP32 values, FP64 checks)"
+                                                                                : " (BF16 then FP32 values)"))
                                                         : (opt.eval_fp64 ? " (FP64 checks)" : ""))
                           : "",
                     s.m, s.n, (long long)s.A.nnz(),
@@ -359,7 +363,18 @@ Result GpuSolver::run() {
 
     Exit ex;
     long fp32_iters = 0;
-    if (mixed) {
+    if (mixed && opt.values == Values::BF16_F32) {
+        // BF16 values until they stall, then FP32 values on the same FP32 vectors, then FP64
+        const Level L0{GANIT_BF16, GANIT_F32, "L0"};
+        ex = timed_phase(0, L0, true, opt.stall_checks);
+        bool done = finished(ex, "L0 (BF16 matrix)", "switching to L1");
+        if (!done) {
+            ex = timed_phase(0, L1, true, opt.stall_checks_l2);
+            done = finished(ex, "L1 (FP32)", "switching to L3");
+        }
+        fp32_iters = st.it;
+        if (!done) ex = timed_phase(2, L3, false, 0);
+    } else if (mixed) {
         ex = timed_phase(0, L1, true, opt.stall_checks);
         fp32_iters = st.it;
         bool done = finished(ex, "L1 (FP32)",

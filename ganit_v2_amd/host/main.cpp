@@ -2,7 +2,7 @@
 // summary line and the exit codes follow the reference's src/main.cpp:41-211 for the PDHG path; keys after
 // "nnz" in the JSON line are additions.
 //   ganit model.mps [--device gpu] [--tol 1e-4] [--precision double|mixed|dynamic] [--time S] [--iters N]
-//                   [--values f32|bf16] [--eval level|fp64] [--kahan] [--detect-unbounded] [--gpu K] [--sol FILE] [--json] [--quiet]
+//                   [--values f32|bf16|bf16f32] [--eval level|fp64] [--kahan] [--detect-unbounded] [--gpu K] [--sol FILE] [--json] [--quiet]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -23,7 +23,8 @@ static void usage() {
         "  --tol X            relative KKT tolerance (default 1e-4)\n"
         "  --precision double|mixed|dynamic  PDHG arithmetic; mixed: FP32->FP64,\n"
         "                     dynamic: FP32 -> FP32 matrix/FP64 vectors -> FP64 (always FP64-certified)\n"
-        "  --values f32|bf16  matrix values of the reduced-precision levels (default f32)\n"
+        "  --values f32|bf16|bf16f32  matrix values of the reduced-precision levels (default f32; bf16f32:\n"
+        "                     BF16 until --stall-checks checks without progress, then FP32 until --stall-checks-l2)\n"
         "  --eval level|fp64  KKT checks of the reduced-precision levels at their own precision (default)\n"
         "                     or in FP64 with the FP64 matrix\n"
         "  --kahan            L1 checks: Kahan-compensated FP32 sums (instead of FP64 sums)\n"
@@ -66,7 +67,8 @@ int main(int argc, char** argv) {
             const std::string v = next();
             if (v == "f32") opt.values = Values::F32;
             else if (v == "bf16") opt.values = Values::BF16;
-            else { std::fprintf(stderr, "--values must be f32 or bf16\n"); return 1; }
+            else if (v == "bf16f32") opt.values = Values::BF16_F32;
+            else { std::fprintf(stderr, "--values must be f32, bf16 or bf16f32\n"); return 1; }
         }
         else if (s == "--eval") {
             const std::string v = next();

@@ -14,7 +14,10 @@ namespace ganit {
 // Double = L3 only; Mixed = L1 -> L3; Dynamic = L1 -> L2 -> L3 (reference pdhg.hpp:10, pdhg_impl.hpp:13-22).
 enum class Precision { Double, Mixed, Dynamic };
 // extension: element type of the matrix values in the reduced-precision levels L1 and L2.
-enum class Values { F32, BF16 };
+// Matrix values of the reduced-precision levels: FP32, BF16, or BF16 first and FP32 once BF16 stalls
+// (extension: the L1 phase of a mixed or dynamic ladder runs as a BF16 phase, left after stall_checks checks
+// without progress, then an FP32 phase, left after stall_checks_l2; both on the FP32 vectors).
+enum class Values { F32, BF16, BF16_F32 };
 
 struct Options {
     double tol = 1e-4;           // relative KKT tolerance (primal, dual, gap)

@@ -56,7 +56,9 @@ int ganit_lp_scaled(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, doub
 /* solve_pdhg_gpu (reference include/ganit/pdhg.hpp:45-47, src/backend_cuda.cu:382-393).
  * opts[0..14] = tol, max_iter, time_limit, check_every, ruiz_iters, pock_chambolle, verbose, infeas_tol,
  *               precision (0 double, 1 mixed, 2 dynamic), stall_checks, stall_checks_l2,
- *               values (0 FP32, 1 BF16 matrix values in L1/L2), device,
+ *               values (0 FP32, 1 BF16 matrix values in L1/L2, 2 BF16 then FP32: the L1 phase runs on BF16
+ *               values until stall_checks checks without progress, then on FP32 values until stall_checks_l2),
+ *               device,
  *               eval_fp64 (1: L1/L2 checks evaluated in FP64 with the FP64 matrix),
  *               flags: bit 0 kahan (L1's FP32 checks sum FP32-rounded terms with Kahan compensation),
  *                      bit 1 detect_unbounded (LPs: test the primal ray x - xr as a dual-infeasibility

@@ -72,7 +72,7 @@ def test_random_lp_fp64_vs_reference(seed):
 
 
 @pytest.mark.parametrize("precision", ["mixed", "dynamic"])
-@pytest.mark.parametrize("values", ["f32", "bf16"])
+@pytest.mark.parametrize("values", ["f32", "bf16", "bf16f32"])
 @pytest.mark.parametrize("name", ["afiro", "adlittle", "25fv47", "port_small", "random_lp"])
 def test_ladder_is_fp64_certified(name, precision, values):
     p = g.Problem.generate("random_lp", seed=0) if name == "random_lp" else g.Problem.read_mps(instance_path(name))
@@ -88,7 +88,7 @@ def test_ladder_is_fp64_certified(name, precision, values):
 def test_ladder_with_fp64_checks(name):
     p = g.Problem.generate("random_lp", seed=1) if name == "random_lp" else g.Problem.read_mps(instance_path(name))
     d64 = p.solve_gpu(time_limit=120)
-    for values in ("f32", "bf16"):
+    for values in ("f32", "bf16", "bf16f32"):
         r = p.solve_gpu(precision="mixed", values=values, eval_fp64=True, time_limit=120)
         assert r["status"] == "OPTIMAL" and _certified(r, 1e-4)
         assert abs(r["pobj"] - d64["pobj"]) <= 5e-4 * (1 + abs(d64["pobj"]))
