@@ -37,16 +37,16 @@ constexpr int kBN = 12, kBM = 7;
 // ------------------------------------------------------------------ iteration kernels
 template <class M, class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* __restrict__ blocks,
-                                                      const int* __restrict__ list,
-                                                      const int* __restrict__ blk_inst, CsrView<M> Q,
+                                                      const int2* __restrict__ list, CsrView<M> Q,
                                                       const BatchState* __restrict__ st, const V* __restrict__ y,
                                                       const V* __restrict__ x, const V* __restrict__ c,
                                                       const V* __restrict__ l, const V* __restrict__ u,
                                                       V* __restrict__ xn, V* __restrict__ xbar,
                                                       V* __restrict__ sumx) {
     __shared__ V lds[kTile + kWin];
-    const int b = list[xcd_block_id()];  // a workgroup of a running instance
-    const BatchState& s = st[blk_inst[b]];
+    const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
+    const int b = e.x;
+    const BatchState& s = st[e.y];
     if (!s.active) return;
     const V tau = V(s.tau);
     const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
@@ -74,14 +74,14 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
 
 template <class M, class V>
 __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __restrict__ blocks,
-                                                    const int* __restrict__ list,
-                                                    const int* __restrict__ blk_inst, const BatchState* __restrict__ st,
+                                                    const int2* __restrict__ list, const BatchState* __restrict__ st,
                                                     const V* __restrict__ xbar, const V* __restrict__ y,
                                                     const V* __restrict__ lo, const V* __restrict__ hi,
                                                     V* __restrict__ yn, V* __restrict__ sumy) {
     __shared__ V lds[kTile + kWin];
-    const int b = list[xcd_block_id()];  // a workgroup of a running instance
-    const BatchState& s = st[blk_inst[b]];
+    const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
+    const int b = e.x;
+    const BatchState& s = st[e.y];
     if (!s.active) return;
     const V sigma = V(s.sigma);
     const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
@@ -136,8 +136,7 @@ __device__ __forceinline__ double bt_row_obj(double yi, double lw, double hg) {
 // Same sums as check_n_block of the single-instance path, one partial row per workgroup.
 template <class M, class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int* __restrict__ blocks,
-                                                       const int* __restrict__ list,
-                                                       const int* __restrict__ blk_inst, CsrView<M> Q,
+                                                       const int2* __restrict__ list, CsrView<M> Q,
                                                        const BatchState* __restrict__ st, const V* __restrict__ x,
                                                        const V* __restrict__ y, const V* __restrict__ xa,
                                                        const V* __restrict__ ya, const V* __restrict__ xr,
@@ -147,8 +146,9 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
                                                        double* __restrict__ partials) {
     __shared__ V lds[kTile];
     __shared__ double red[(kBlock / kWave) * kBN];
-    const int b = list[xcd_block_id()];  // a workgroup of a running instance
-    if (!st[blk_inst[b]].active) return;
+    const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
+    const int b = e.x;
+    if (!st[e.y].active) return;
     double acc[kBN];
 #pragma unroll
     for (int k = 0; k < kBN; ++k) acc[k] = 0.0;
@@ -204,8 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
 
 template <class M, class V>
 __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* __restrict__ blocks,
-                                                       const int* __restrict__ list,
-                                                       const int* __restrict__ blk_inst,
+                                                       const int2* __restrict__ list,
                                                        const BatchState* __restrict__ st, const V* __restrict__ x,
                                                        const V* __restrict__ xa, const V* __restrict__ y,
                                                        const V* __restrict__ ya, const V* __restrict__ yr,
@@ -214,8 +213,9 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* 
                                                        double* __restrict__ partials) {
     __shared__ V lds[kTile];
     __shared__ double red[(kBlock / kWave) * kBM];
-    const int b = list[xcd_block_id()];  // a workgroup of a running instance
-    if (!st[blk_inst[b]].active) return;
+    const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
+    const int b = e.x;
+    if (!st[e.y].active) return;
     double acc[kBM];
 #pragma unroll
     for (int k = 0; k < kBM; ++k) acc[k] = 0.0;
@@ -384,7 +384,7 @@ struct Batch {
     int ninst = 0, N = 0, Mr = 0;
     Csr *A = nullptr, *AT = nullptr;
     Csr* Q = nullptr;  // block-diagonal Q when an instance is a QP
-    int *blk_inst_a = nullptr, *blk_inst_at = nullptr, *first_a = nullptr, *first_at = nullptr;
+    int *first_a = nullptr, *first_at = nullptr;  // per instance, its first row block (A, A^T)
     int *col_inst = nullptr, *row_inst = nullptr;
     BatchInst* inst = nullptr;
     BatchState* st = nullptr;
@@ -398,7 +398,7 @@ struct Batch {
     size_t part_n_cap = 0, part_m_cap = 0;
     // launch lists: the row blocks of the instances taking part (kernels map blockIdx through them)
     std::vector<int> first_a_h, first_at_h;
-    int *list_a = nullptr, *list_at = nullptr;
+    int2 *list_a = nullptr, *list_at = nullptr;  // {row block, instance} of each launched workgroup
     int nact_a = 0, nact_at = 0;
     std::vector<char> listed;  // instances in the current lists
     BatchVecs vs[2];
@@ -421,8 +421,7 @@ struct Batch {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         drop_graphs();
         for (void* p : owned) (void)hipFree(p);
-        for (void* p : {(void*)blk_inst_a, (void*)blk_inst_at, (void*)first_a, (void*)first_at, (void*)part_n,
-                        (void*)part_m, (void*)list_a, (void*)list_at})
+        for (void* p : {(void*)first_a, (void*)first_at, (void*)part_n, (void*)part_m, (void*)list_a, (void*)list_at})
             if (p) (void)hipFree(p);
         if (host_running) (void)hipHostFree(host_running);
         csr_destroy(A);
@@ -438,14 +437,14 @@ unsigned bt_grid(int64_t n) { return std::min<unsigned>(grid_for(n), 8192u); }
 // Changing the lists changes grid sizes, so captured graphs are dropped.
 void bt_list(Batch* bt, const std::vector<char>& on) {
     if (on == bt->listed) return;
-    auto fill = [&](const std::vector<int>& first, int* dev, int& count) {
-        std::vector<int> l;
+    auto fill = [&](const std::vector<int>& first, int2* dev, int& count) {
+        std::vector<int2> l;
         for (int k = 0; k < bt->ninst; ++k)
             if (on[k])
-                for (int b = first[k]; b < first[k + 1]; ++b) l.push_back(b);
+                for (int b = first[k]; b < first[k + 1]; ++b) l.push_back(make_int2(b, k));
         count = (int)l.size();
         if (count > 0)
-            GH_CHECK(hipMemcpyAsync(dev, l.data(), sizeof(int) * l.size(), hipMemcpyHostToDevice, bt->ctx->stream));
+            GH_CHECK(hipMemcpyAsync(dev, l.data(), sizeof(int2) * l.size(), hipMemcpyHostToDevice, bt->ctx->stream));
         GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
     };
     fill(bt->first_a_h, bt->list_a, bt->nact_a);
@@ -471,7 +470,7 @@ void bt_partition(Batch* bt, int rows, int nnz) {
     Ctx* ctx = bt->ctx;
     const int ninst = bt->ninst;
     auto build = [&](Csr* a, const std::vector<int>& ptr, const std::vector<int>& idx, bool rows_of_a,
-                     int** blk_inst, int** first, std::vector<int>& first_h, int** list) {
+                     int** first, std::vector<int>& first_h, int2** list) {
         std::vector<int> blocks{0}, bi, fi{0};
         std::vector<int> lp;
         for (int k = 0; k < ninst; ++k) {
@@ -495,27 +494,20 @@ void bt_partition(Batch* bt, int rows, int nnz) {
         a->lanes = 0;
         a->rb_rows = rows;
         a->rb_nnz = nnz;
-        if (*blk_inst) GH_CHECK(hipFree(*blk_inst));
         if (*first) GH_CHECK(hipFree(*first));
-        *blk_inst = nullptr;
         *first = nullptr;
-        GH_CHECK(hipMalloc(blk_inst, sizeof(int) * std::max<size_t>(bi.size(), 1)));
         GH_CHECK(hipMalloc(first, sizeof(int) * fi.size()));
-        if (!bi.empty())
-            GH_CHECK(hipMemcpyAsync(*blk_inst, bi.data(), sizeof(int) * bi.size(), hipMemcpyHostToDevice,
-                                    ctx->stream));
         GH_CHECK(hipMemcpyAsync(*first, fi.data(), sizeof(int) * fi.size(), hipMemcpyHostToDevice, ctx->stream));
         GH_CHECK(hipStreamSynchronize(ctx->stream));
         first_h = fi;
         if (*list) GH_CHECK(hipFree(*list));
         *list = nullptr;
-        GH_CHECK(hipMalloc(list, sizeof(int) * std::max<size_t>(bi.size(), 1)));
+        GH_CHECK(hipMalloc(list, sizeof(int2) * std::max<size_t>(bi.size(), 1)));
     };
     GH_CHECK(hipStreamSynchronize(ctx->stream));
     bt->drop_graphs();
-    build(bt->A, bt->aptr_h, bt->aidx_h, true, &bt->blk_inst_a, &bt->first_a, bt->first_a_h, &bt->list_a);
-    build(bt->AT, bt->atptr_h, bt->atidx_h, false, &bt->blk_inst_at, &bt->first_at, bt->first_at_h,
-          &bt->list_at);
+    build(bt->A, bt->aptr_h, bt->aidx_h, true, &bt->first_a, bt->first_a_h, &bt->list_a);
+    build(bt->AT, bt->atptr_h, bt->atidx_h, false, &bt->first_at, bt->first_at_h, &bt->list_at);
     bt->listed.assign(ninst, 0);
     bt_list(bt, std::vector<char>(ninst, 1));
     const size_t need_n = (size_t)kBN * std::max(bt->AT->nblocks, 1), need_m = (size_t)kBM * std::max(bt->A->nblocks, 1);
@@ -546,18 +538,18 @@ void bt_iteration(Batch* bt, int mdt_unused) {
     if (bt->nact_at > 0) {
         if (bt->Q)
             hipLaunchKernelGGL((k_bt_primal<M, V, true>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->Q->view<M>(), bt->st,
+                               bt->AT->blocks, bt->list_at, bt->Q->view<M>(), bt->st,
                                (const V*)v.Y[c], (const V*)v.X[c], (const V*)v.c, (const V*)v.l, (const V*)v.u,
                                (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
         else
             hipLaunchKernelGGL((k_bt_primal<M, V, false>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, bt->AT->view<M>(), bt->st,
+                               bt->AT->blocks, bt->list_at, bt->AT->view<M>(), bt->st,
                                (const V*)v.Y[c], (const V*)v.X[c], (const V*)v.c, (const V*)v.l, (const V*)v.u,
                                (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
     }
     if (bt->nact_a > 0)
         hipLaunchKernelGGL((k_bt_dual<M, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<M>(), bt->A->blocks,
-                           bt->list_a, bt->blk_inst_a, bt->st, (const V*)v.xbar, (const V*)v.Y[c], (const V*)v.lo,
+                           bt->list_a, bt->st, (const V*)v.xbar, (const V*)v.Y[c], (const V*)v.lo,
                            (const V*)v.hi, (V*)v.Y[1 - c], (V*)v.sumy);
     bt->cur ^= 1;
 }
@@ -575,18 +567,18 @@ void bt_check(Batch* bt, int avg, int mode) {
         const CsrView<M> q = bt->Q ? bt->Q->view<M>() : bt->AT->view<M>();
         if (bt->Q)
             hipLaunchKernelGGL((k_bt_check_n<M, V, true>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, q, bt->st, (const V*)v.X[c],
+                               bt->AT->blocks, bt->list_at, q, bt->st, (const V*)v.X[c],
                                (const V*)v.Y[c], (const V*)v.xa, (const V*)v.ya, (const V*)v.xr, (const V*)v.yr,
                                (const V*)v.c, (const V*)v.l, (const V*)v.u, (const V*)v.cres, avg, bt->part_n);
         else
             hipLaunchKernelGGL((k_bt_check_n<M, V, false>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                               bt->AT->blocks, bt->list_at, bt->blk_inst_at, q, bt->st, (const V*)v.X[c],
+                               bt->AT->blocks, bt->list_at, q, bt->st, (const V*)v.X[c],
                                (const V*)v.Y[c], (const V*)v.xa, (const V*)v.ya, (const V*)v.xr, (const V*)v.yr,
                                (const V*)v.c, (const V*)v.l, (const V*)v.u, (const V*)v.cres, avg, bt->part_n);
     }
     if (bt->nact_a > 0)
     hipLaunchKernelGGL((k_bt_check_m<M, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<M>(),
-                       bt->A->blocks, bt->list_a, bt->blk_inst_a, bt->st, (const V*)v.X[c], (const V*)v.xa, (const V*)v.Y[c],
+                       bt->A->blocks, bt->list_a, bt->st, (const V*)v.X[c], (const V*)v.xa, (const V*)v.Y[c],
                        (const V*)v.ya, (const V*)v.yr, (const V*)v.lo, (const V*)v.hi, (const V*)v.rres, avg,
                        bt->part_m);
     hipLaunchKernelGGL(k_bt_reduce, dim3(bt->ninst), dim3(kWave), 0, s, bt->ninst, bt->first_at, bt->first_a, bt->st,
