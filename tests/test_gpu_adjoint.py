@@ -128,6 +128,24 @@ def test_autograd_function_matches_adjoint():
     assert np.allclose(x.detach().numpy(), r["x"])
 
 
+def test_set_solution_then_adjoint_matches_run_then_adjoint():
+    """A solver that takes over another solver's original-space solution (Solver.set_solution) differentiates at
+    the same point: its gradients equal those after its own run() to the adjoint's tolerance, and it refuses
+    vectors of the wrong length."""
+    p = g.Problem.generate("diff_qp", seed=5, n=400, m_eq=60, m_le=40, density=0.02, qdensity=0.002)
+    s, r = _solve(p, precision="double", tol=1e-9)
+    w = np.random.default_rng(12).normal(size=p.dims()["n"])
+    a = s.adjoint(w, tol=1e-11)
+    t = p.solver(precision="double", tol=1e-9)
+    t.upload()
+    t.set_solution(r["x"], r["y"])
+    b = t.adjoint(w, tol=1e-11)
+    for k in ("dc", "db", "dq"):
+        assert _rel(b[k], a[k]) < 1e-7, k
+    with pytest.raises(ValueError):
+        t.set_solution(r["x"][:-1], r["y"])
+
+
 @pytest.mark.parametrize("precision,tol,batch_forward", [("double", 1e-9, True), ("double", 1e-9, False),
                                                          ("mixed", 1e-6, True)])
 def test_batched_layer_matches_single_instances(precision, tol, batch_forward):
