@@ -512,9 +512,30 @@ inline float widen<float, bf16>(bf16 v) { return (float)v; }
 template <>
 inline double widen<double, bf16>(bf16 v) { return (double)v; }
 
+// Kahan-compensated FP32 sum of f(i) over [0, k) (extension, no reference code): every term is rounded to
+// float, each OpenMP thread accumulates its share in float with a running compensation (Kahan 1965; the error
+// of one thread's sum is at most (2u + O(k u^2)) sum |term|, u = 2^-24), and the per-thread float partials are
+// added in double.
+template <class F>
+double kahan_fp32_sum(int k, F f) {
+    double total = 0.0;
+#pragma omp parallel reduction(+ : total)
+    {
+        float sacc = 0.0f, comp = 0.0f;
+#pragma omp for
+        for (int i = 0; i < k; ++i) {
+            const float yv = (float)f(i) - comp;
+            const volatile float t = sacc + yv;
+            comp = (t - sacc) - yv;
+            sacc = t;
+        }
+        total += (double)sacc;
+    }
+    return total;
+}
+
 // Reductions accumulate in double as the reference does (backend_cpu.cpp:101-155).  With
-// kahan = true they instead accumulate in float with Kahan compensation per thread, and the per-thread
-// float partials are combined in double (extension, no reference code).
+// kahan = true they instead use kahan_fp32_sum.
 template <class M, class R>
 struct CpuBackend {
     using Vec = std::vector<R>;
@@ -592,20 +613,7 @@ struct CpuBackend {
             for (int i = 0; i < k; ++i) acc += f(i);
             return acc;
         }
-        double total = 0.0;
-#pragma omp parallel reduction(+ : total)
-        {
-            float sacc = 0.0f, comp = 0.0f;
-#pragma omp for
-            for (int i = 0; i < k; ++i) {
-                const float yv = (float)f(i) - comp;
-                const volatile float t = sacc + yv;
-                comp = (t - sacc) - yv;
-                sacc = t;
-            }
-            total += (double)sacc;
-        }
-        return total;
+        return kahan_fp32_sum(k, f);
     }
 
     double primal_res2(const Vec& ax) {  // backend_cpu.cpp:101-112
@@ -1121,6 +1129,12 @@ int oracle_spmv(int mdt, int vdt, int rows, int cols, const int* ptr, const int*
     else if (mdt == 2 && vdt == 1) spmv_t<bf16, float>(rows, ptr, idx, val, x, cols, y);
     else return 1;
     return 0;
+}
+
+// The reductions' Kahan-compensated FP32 sum (kahan_fp32_sum) of a double array, on the current OpenMP
+// thread count.
+double oracle_kahan_sum(long long k, const double* v) {
+    return kahan_fp32_sum((int)k, [&](int i) { return v[i]; });
 }
 
 // BF16 rounding of a double array (float first, then round-to-nearest-even), widened back to double.

@@ -46,6 +46,8 @@ def lib() -> ctypes.CDLL:
         L.oracle_solve_traced.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp]
         L.oracle_spmv.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
         L.oracle_round_bf16.argtypes = [ctypes.c_longlong, _vp, _vp]
+        L.oracle_kahan_sum.argtypes = [ctypes.c_longlong, _vp]
+        L.oracle_kahan_sum.restype = ctypes.c_double
         L.oracle_primal_step.argtypes = [ctypes.c_int, ctypes.c_int] + [_vp] * 6 + [ctypes.c_double] + [_vp] * 3
         L.oracle_dual_step.argtypes = [ctypes.c_int, ctypes.c_int] + [_vp] * 4 + [ctypes.c_double] + [_vp] * 2
         L.oracle_kkt.argtypes = [_vp, _vp, _vp, _vp]
@@ -188,6 +190,12 @@ def round_bf16(a):
     out = np.zeros_like(a)
     lib().oracle_round_bf16(len(a), _p(a), _p(out))
     return out
+
+
+def kahan_sum(a) -> float:
+    """The oracle's Kahan-compensated FP32 sum (the reductions of its kahan option) of an array."""
+    a = np.ascontiguousarray(a, np.float64)
+    return float(lib().oracle_kahan_sum(len(a), _p(a)))
 
 
 def primal_step(vdt, x, aty, qx, c, l, u, tau, sumx):

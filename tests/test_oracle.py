@@ -228,3 +228,50 @@ def test_oracle_kahan_mode_reaches_the_same_answer(oracle, name):
     b = lp.solve(precision="mixed", kahan=1, time_limit=60)
     assert a["status"] == b["status"] == "OPTIMAL"
     assert abs(a["pobj"] - b["pobj"]) <= 5e-4 * (1 + abs(a["pobj"]))
+
+
+def _kahan_bound(x32: np.ndarray) -> float:
+    """Kahan's error bound for compensated summation of k terms, |S_hat - S| <= (2u + k u^2) sum |x_i|,
+    u = 2^-24 (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., Thm 4.8 / eq. 4.9)."""
+    u = 2.0 ** -24
+    return (2 * u + len(x32) * u * u) * float(np.abs(x32).sum())
+
+
+def _kahan_cases():
+    rng = np.random.default_rng(5)
+    k = 1 << 20
+    # positive terms near 1: the running float sum reaches 2^20, where every plain addition rounds to 1/8
+    near_one = rng.uniform(0.5, 1.5, k)
+    # ill-conditioned: large terms that cancel pairwise in random order, plus small terms carrying the sum
+    big = rng.uniform(1e3, 1e4, k // 4)
+    ill = np.concatenate([big, -big, rng.uniform(0.0, 1e-2, k // 2)])
+    rng.shuffle(ill)
+    return {"near_one": near_one, "ill_conditioned": ill}
+
+
+@pytest.mark.parametrize("case", ["near_one", "ill_conditioned"])
+@pytest.mark.parametrize("threads", [1, 4])
+def test_oracle_kahan_sum_within_kahan_bound(oracle, case, threads):
+    """The oracle's Kahan-compensated FP32 sum (the reductions of its kahan option) against math.fsum of the same
+    FP32-rounded terms: within Kahan's bound, on sums where plain sequential FP32 accumulation is not -- so a lost
+    or sign-flipped compensation fails here."""
+    x32 = _kahan_cases()[case].astype(np.float32)
+    exact = math.fsum(x32.astype(np.float64))
+    bound = _kahan_bound(x32)
+    plain = float(np.cumsum(x32, dtype=np.float32)[-1])  # sequential FP32 accumulation, no compensation
+    assert abs(plain - exact) > bound  # the case separates compensated from plain summation
+    oracle.set_threads(threads)
+    try:
+        got = oracle.kahan_sum(x32.astype(np.float64))
+    finally:
+        oracle.set_threads(1)
+    assert abs(got - exact) <= bound, (got, exact, bound, plain)
+
+
+def test_oracle_kahan_sum_exact_cases(oracle):
+    """Sums that FP32 holds exactly come back exact; the empty sum is 0."""
+    oracle.set_threads(1)
+    assert oracle.kahan_sum(np.zeros(0)) == 0.0
+    assert oracle.kahan_sum(np.arange(1000, dtype=np.float64)) == 499500.0
+    # 2^24 + 1 + 1: plain FP32 loses both ones, the compensation keeps them
+    assert oracle.kahan_sum(np.array([2.0 ** 24, 1.0, 1.0])) == 2.0 ** 24 + 2
