@@ -67,8 +67,41 @@ def main():
                 record(out, inst, path, ["--device", "cpu", "--precision", "mixed"], 1)
                 record(out, inst, path, ["--device", "cpu", "--tol", "1e-8", "--time", "300"], 8)
                 record(out, inst, path, ["--method", "simplex", "--time", "600"], 1)
+        blending_runs(out, os.path.join(td_root(), "blending"))
     print("wrote", dst)
 
 
+BLENDING = dict(crudes=60, products=16, specs=5, periods=365)
+
+
+def td_root() -> str:
+    return tempfile.gettempdir()
+
+
+def blending_runs(out, workdir):
+    """The 2.9M-nonzero blending LP (60 crudes x 16 products x 5 specs x 365 periods, seed 0), written as MPS
+    (107 MB) in `workdir`: the reference CLI at tol 1e-4 in every precision at 1 and 8 threads, and in FP64 at
+    tol 1e-8 (the accurate objective that 1e-4 solves are measured against)."""
+    os.makedirs(workdir, exist_ok=True)
+    p = g.Problem.generate("blending", seed=0, **BLENDING)
+    path = os.path.join(workdir, "blending_s0.mps")
+    p.write_mps(path)
+    inst = "blending_s0"
+    d = p.dims()
+    meta = {"instance": inst, "generator": "blending", "seed": 0, **BLENDING, "m": d["m"], "n": d["n"],
+            "nnz": d["nnz"], "mps_sha256_16": sha(path)}
+    out.write(json.dumps({"meta": meta}) + "\n")
+    for threads in (1, 8):
+        for prec in ("double", "mixed", "dynamic"):
+            record(out, inst, path, ["--device", "cpu", "--precision", prec, "--time", "600"], threads, timeout=900)
+    record(out, inst, path, ["--device", "cpu", "--tol", "1e-8", "--time", "1200"], 8, timeout=1500)
+    os.remove(path)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["--append-blending"]:
+        # add only the blending LP's runs to an existing file (the other runs take much longer to redo)
+        with open(os.path.join(HERE, "reference_runs.jsonl"), "a") as f:
+            blending_runs(f, os.path.join(td_root(), "blending"))
+    else:
+        main()

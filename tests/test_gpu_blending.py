@@ -1,0 +1,117 @@
+"""The headline workload on the GPU against the reference CLI: the 2.9M-nonzero blending LP (60 crudes x 16
+products x 5 specs x 365 periods, seed 0) solved at full size in every precision configuration, each result
+checked against the reference's own runs of the same LP (tests/golden/reference_runs.jsonl, written by
+tests/golden/make_reference_runs.py from the LP this package's generator produces; test_host.py checks that the
+generator still writes that file byte for byte).
+
+* FP64 (the reference's L3 path): same status and iteration count as the reference CLI, objective equal to its
+  printed digits -- the row-block kernels reproduce the CPU row loop's sums, so the trajectories coincide.
+* Mixed and dynamic ladders at the reference's defaults (stall 12, level-precision checks), and the extensions
+  (FP64 checks with stall 60 -- the benchmark's configuration --, BF16 values, BF16 then FP32 values): the
+  reference's status, the FP64-certified tolerance (pdhg_impl.hpp:300-303), and an objective within a
+  tolerance-sized bound, 1e-4 (1 + |p*|), of the reference's FP64 objective at tol 1e-8, p*.  The error of each
+  against p* is printed; BF16's is the largest (BF16-rounded values are a perturbed problem until the FP64 checks
+  pull the iterate back).
+* FP64 at tol 1e-8 against the reference's tol-1e-8 run: objectives within 1e-7 relative.
+"""
+import pytest
+
+import ganit_v2_amd as g
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+INSTANCE = "blending_s0"
+SPEC = dict(crudes=60, products=16, specs=5, periods=365)
+TOL = 1e-4
+
+
+def _ref(precision, threads=1):
+    return golden(INSTANCE, ["--device", "cpu", "--precision", precision, "--time", "600"], threads)
+
+
+def _accurate():
+    return golden(INSTANCE, ["--device", "cpu", "--tol", "1e-8", "--time", "1200"], 8)
+
+
+@pytest.fixture(scope="module")
+def blending():
+    meta = next(r["meta"] for r in _golden_meta() if r["meta"]["instance"] == INSTANCE)
+    p = g.Problem.generate("blending", seed=0, **SPEC)
+    d = p.dims()
+    assert (d["m"], d["n"], d["nnz"]) == (meta["m"], meta["n"], meta["nnz"])
+    return p
+
+
+def _golden_meta():
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "reference_runs.jsonl")) as f:
+        return [json.loads(l) for l in f if '"meta"' in l]
+
+
+def _solve(p, **opts):
+    s = p.solver(tol=TOL, **opts)
+    s.upload()
+    return s.run()
+
+
+def _certified(r, tol=TOL):
+    return max(r["rel_p"], r["rel_d"], r["rel_gap"]) <= tol * (1 + 1e-7)
+
+
+def test_blending_fp64_follows_the_reference_cli(blending):
+    ref = _ref("double")
+    r = _solve(blending, precision="double")
+    print(f"gpu FP64 {r['status']} it {r['iters']} restarts {r['restarts']} obj {r['pobj']!r} | reference "
+          f"{ref['status']} it {ref['iters']} restarts {ref['restarts']} obj {ref['pobj']!r}", flush=True)
+    assert r["status"] == ref["status"] == "OPTIMAL"
+    assert _certified(r)
+    assert r["iters"] == ref["iters"] and r["restarts"] == ref["restarts"]
+    # the reference prints 13 significant digits
+    assert abs(r["pobj"] - ref["pobj"]) <= 5e-7 + 1e-12 * abs(ref["pobj"])
+    assert abs(r["dobj"] - ref["dobj"]) <= 5e-7 + 1e-12 * abs(ref["dobj"])
+
+
+CONFIGS = {
+    "mixed": (dict(precision="mixed"), "mixed"),
+    "dynamic": (dict(precision="dynamic"), "dynamic"),
+    "mixed_fp64_checks_stall60": (dict(precision="mixed", stall_checks=60, eval_fp64=True), "mixed"),
+    "bf16_fp64_checks_stall60": (dict(precision="mixed", values="bf16", stall_checks=60, eval_fp64=True), "mixed"),
+    "bf16f32_fp64_checks": (dict(precision="mixed", values="bf16f32", stall_checks=12, stall_checks_l2=60,
+                                 eval_fp64=True), "mixed"),
+}
+
+
+@pytest.mark.parametrize("name", list(CONFIGS))
+def test_blending_ladders_match_the_reference(blending, name):
+    opts, ref_prec = CONFIGS[name]
+    ref = _ref(ref_prec)
+    pstar = _accurate()["pobj"]
+    r = _solve(blending, **opts)
+    err = abs(r["pobj"] - pstar) / (1 + abs(pstar))
+    ref_err = abs(ref["pobj"] - pstar) / (1 + abs(pstar))
+    print(f"gpu {name}: {r['status']} it {r['iters']} (L1 {r['fp32_iters']}, L2 {r['l2_iters']}) obj {r['pobj']!r} "
+          f"rel. err. vs FP64 tol-1e-8 objective {err:.2e} | reference {ref_prec}: it {ref['iters']} "
+          f"(L1 {ref['fp32_iters']}, L2 {ref['l2_iters']}) obj {ref['pobj']!r} rel. err. {ref_err:.2e}", flush=True)
+    assert r["status"] == ref["status"] == "OPTIMAL"
+    assert _certified(r)
+    assert err <= TOL
+    if name in ("mixed", "dynamic"):
+        # the reference's own ladder: same level split, objective within the same tolerance-sized bound
+        assert abs(r["pobj"] - ref["pobj"]) <= TOL * (1 + abs(ref["pobj"]))
+        assert r["fp32_iters"] > 0 and (r["l2_iters"] > 0) == (ref["l2_iters"] > 0)
+
+
+def test_blending_fp64_accurate_matches_reference(blending):
+    ref = _accurate()
+    s = blending.solver(precision="double", tol=1e-8)
+    s.upload()
+    r = s.run()
+    print(f"gpu FP64 tol 1e-8: {r['status']} it {r['iters']} obj {r['pobj']!r} | reference it {ref['iters']} "
+          f"obj {ref['pobj']!r}", flush=True)
+    assert r["status"] == ref["status"] == "OPTIMAL"
+    assert _certified(r, 1e-8)
+    assert abs(r["pobj"] - ref["pobj"]) <= 1e-7 * (1 + abs(ref["pobj"]))
+

@@ -332,3 +332,24 @@ def test_batch_workload_generator_sizes():
     a = g.Problem.generate("batch_lp", seed=7).export()
     b = g.Problem.generate("batch_lp", seed=7).export()
     assert np.array_equal(a["A"].val, b["A"].val) and np.array_equal(a["u"], b["u"])
+
+
+def test_blending_generator_writes_the_golden_lp(tmp_path):
+    """The blending LP the reference CLI's golden runs were made on (tests/golden/reference_runs.jsonl, meta line
+    of blending_s0) is what the generator still produces: same dimensions and the same MPS bytes (sha256)."""
+    import hashlib
+    import json
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "reference_runs.jsonl")) as f:
+        meta = next(json.loads(l)["meta"] for l in f if '"blending_s0"' in l and '"meta"' in l)
+    p = g.Problem.generate("blending", seed=0, crudes=meta["crudes"], products=meta["products"], specs=meta["specs"],
+                           periods=meta["periods"])
+    d = p.dims()
+    assert (d["m"], d["n"], d["nnz"]) == (meta["m"], meta["n"], meta["nnz"])
+    path = tmp_path / "blending.mps"
+    p.write_mps(str(path))
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    assert h.hexdigest()[:16] == meta["mps_sha256_16"]
