@@ -2,6 +2,7 @@
 // backend-concept kernels (SpMV, primal / dual step, the FP64 reductions) behind the C ABI.
 #include <algorithm>
 #include <exception>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce2(int64_t n, F f, ReduceSlot s
     double acc[2] = {0.0, 0.0};
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         double a0, a1;
-        f((int)i, a0, a1);
+        f(i, a0, a1);
         acc[0] += a0;
         acc[1] += a1;
     }
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void k_finish2(const double* __restrict__ p
 template <class V>
 struct PrimalResTerm {
     const V *ax, *lo, *hi, *rr;
-    __device__ void operator()(int i, double& a0, double& a1) const {
+    __device__ void operator()(int64_t i, double& a0, double& a1) const {
         const double a = double(ax[i]);
         const double p = fmin(fmax(a, double(lo[i])), double(hi[i]));
         const double r = (a - p) / double(rr[i]);
@@ -126,7 +127,7 @@ template <class V>
 struct DualResTerm {
     const V *aty, *qx, *c, *l, *u, *cr;
     double cw;
-    __device__ void operator()(int j, double& a0, double& a1) const {
+    __device__ void operator()(int64_t j, double& a0, double& a1) const {
         const double z = cw * double(c[j]) - double(aty[j]) + (qx ? double(qx[j]) : 0.0);
         const double lj = double(l[j]), uj = double(u[j]);
         double r = 0.0, ob = 0.0;
@@ -140,7 +141,7 @@ struct DualResTerm {
 template <class V>
 struct RowDualObjTerm {
     const V *y, *lo, *hi;
-    __device__ void operator()(int i, double& a0, double& a1) const {
+    __device__ void operator()(int64_t i, double& a0, double& a1) const {
         const double yi = double(y[i]), lw = double(lo[i]), hg = double(hi[i]);
         a0 = 0.0;
         if (yi > 0 && isfinite(lw)) a0 = yi * lw;
@@ -151,7 +152,7 @@ struct RowDualObjTerm {
 template <class V>
 struct DotTerm {
     const V *a, *b;
-    __device__ void operator()(int i, double& a0, double& a1) const {
+    __device__ void operator()(int64_t i, double& a0, double& a1) const {
         a0 = double(a[i]) * double(b[i]);
         a1 = 0.0;
     }
@@ -159,16 +160,54 @@ struct DotTerm {
 template <class V>
 struct Diff2Term {
     const V *a, *b;
-    __device__ void operator()(int i, double& a0, double& a1) const {
+    __device__ void operator()(int64_t i, double& a0, double& a1) const {
         const double d = double(a[i]) - double(b[i]);
         a0 = d * d;
         a1 = 0.0;
     }
 };
 
+// Grid of the fine-grained reductions: one workgroup per 256 entries, at most kMaxReduceGrid.
+unsigned reduce_grid(int64_t n) {
+    return std::min<unsigned>(grid_for(std::max<int64_t>(n, 1)), (unsigned)kMaxReduceGrid);
+}
+
+// ================================================================ allocation-free dot product
+// The scratch of ganit_hip_dot_noalloc: the reduction grid's block partials (two doubles per workgroup, as
+// k_reduce2 writes them), 16 KiB.  Every GPU the library runs on holds its own copy at its own address until
+// the process ends; host code reaches it only through the address hipGetSymbolAddress returns for the
+// current device (noalloc_scratch), never through this name.
+constexpr int kNoAllocDoubles = kMaxReduceGrid * 2;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+// One output of k_reduce2's partials (the first of each pair), folded in the order k_finish2 folds it, so
+// the value is bitwise the one ganit_hip_dot returns.
+__global__ __launch_bounds__(kBlock) void k_finish_first(const double* __restrict__ partials, int nparts,
+                                                         double* __restrict__ out) {
+    __shared__ double lds[kBlock / kWave];
+    sum_partials_one(partials, 2, 0, nparts, out, lds);
+}
+
+constexpr int kMaxNoAllocDevices = 64;
+
+// Device address of `scratch` on `device` (which must be current), looked up once per device.
+double* noalloc_scratch(int device) {
+    static std::mutex mu;
+    static double* addr[kMaxNoAllocDevices] = {};
+    GH_REQUIRE(device >= 0 && device < kMaxNoAllocDevices, "device ordinal beyond the allocation-free entry's table");
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void* p = nullptr;
+        GH_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        GH_REQUIRE(p, "hipGetSymbolAddress returned no address for the reduction scratch");
+        addr[device] = static_cast<double*>(p);
+    }
+    return addr[device];
+}
+
 template <class F>
 void reduce2(Ctx* ctx, int64_t n, F f, double* o0, double* o1) {
-    const unsigned g = std::min<unsigned>(grid_for(std::max<int64_t>(n, 1)), (unsigned)kMaxReduceGrid);
+    const unsigned g = reduce_grid(n);
     ReduceSlot slot{ctx->red_partials, ctx->red_out};
     hipLaunchKernelGGL(k_reduce2<F>, dim3(g), dim3(kBlock), 0, ctx->stream, n, f, slot);
     hipLaunchKernelGGL(k_finish2, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->red_partials, (int)g, ctx->red_out);
@@ -980,6 +1019,32 @@ int ganit_hip_dot(ganit_hip_ctx* h, int vdt, int64_t n, const void* a, const voi
             using V = typename decltype(T)::type;
             reduce2(c, n, DotTerm<V>{(const V*)a, (const V*)b}, out, nullptr);
         });
+    });
+}
+
+int ganit_hip_dot_noalloc(void* hip_stream, int vdt, int64_t n, const void* a, const void* b, double* out) {
+    return guarded([&] {
+        GH_REQUIRE(out && n >= 0 && (n == 0 || (a && b)), "bad dot_noalloc arguments");
+        const unsigned g = reduce_grid(n);
+        GH_REQUIRE(2 * (int64_t)g <= kNoAllocDoubles,
+                   "dot_noalloc: the reduction grid needs more partials than its 16 KiB scratch holds");
+        int device = -1;
+        GH_CHECK(hipGetDevice(&device));
+        auto stream = static_cast<hipStream_t>(hip_stream);
+        if (stream) {
+            hipDevice_t sdev = -1;
+            GH_CHECK(hipStreamGetDevice(stream, &sdev));
+            GH_REQUIRE(sdev == device, "dot_noalloc: the stream belongs to another device than the current one");
+        }
+        double* partials = noalloc_scratch(device);
+        with_vdt(vdt, [&](auto T) {
+            using V = typename decltype(T)::type;
+            using F = DotTerm<V>;
+            ReduceSlot slot{partials, nullptr};
+            hipLaunchKernelGGL(k_reduce2<F>, dim3(g), dim3(kBlock), 0, stream, n, F{(const V*)a, (const V*)b}, slot);
+        });
+        hipLaunchKernelGGL(k_finish_first, dim3(1), dim3(kBlock), 0, stream, (const double*)partials, (int)g, out);
+        GH_CHECK(hipGetLastError());
     });
 }
 

@@ -57,6 +57,7 @@ def lib() -> ctypes.CDLL:
         "ganit_hip_row_dual_obj": [_vp, _i, _i, _vp, _vp, _vp, ctypes.POINTER(_d)],
         "ganit_hip_dot": [_vp, _i, _i64, _vp, _vp, ctypes.POINTER(_d)],
         "ganit_hip_diff2": [_vp, _i, _i64, _vp, _vp, ctypes.POINTER(_d)],
+        "ganit_hip_dot_noalloc": [_vp, _i, _i64, _vp, _vp, _vp],
         "ganit_hip_pdhg_create": [_vp, _vp, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(_vp)],
         "ganit_hip_pdhg_destroy": [_vp],
         "ganit_hip_pdhg_plan": [_vp, _i, ctypes.POINTER(_i), _vp],

@@ -108,6 +108,16 @@ int ganit_hip_row_dual_obj(ganit_hip_ctx* ctx, int vdt, int m, const void* y, co
                            double* out);
 int ganit_hip_dot(ganit_hip_ctx* ctx, int vdt, int64_t n, const void* a, const void* b, double* out);
 int ganit_hip_diff2(ganit_hip_ctx* ctx, int vdt, int64_t n, const void* a, const void* b, double* out);
+/* ganit_hip_dot without a context and without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The block partials go to a fixed 16 KiB device array inside the library (one
+ * copy per GPU, held until the process ends; the reduction grid never exceeds 1024 workgroups, so every n
+ * fits); the result, bitwise equal to ganit_hip_dot's, is written ASYNCHRONOUSLY on `hip_stream` (a
+ * hipStream_t of the current device, NULL = its null stream) to `out`, a DEVICE pointer to one double.
+ * Nothing synchronises.  Calls that overlap on one GPU (other streams, other host threads) share that
+ * array and race: order them on one stream.  No hipMalloc, hipFree or allocating library call is made;
+ * the first call on each device looks up the array's address there (hipGetSymbolAddress). */
+int ganit_hip_dot_noalloc(void* hip_stream, int vdt, int64_t n, const void* a, const void* b, double* out);
 
 /* ------------------------------------------------------------------ device-resident PDHG
  * The fused path behind this library's own driver.  One ganit_hip_pdhg holds the scaled problem
