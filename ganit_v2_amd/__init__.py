@@ -80,6 +80,10 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_batch_run.argtypes = [_vp, _f64p, _vp, _vp]
     lib.ganit_batch_stats.argtypes = [_vp, _f64p]
     lib.ganit_batch_free.argtypes = [_vp]
+    lib.ganit_batch_set_bounds.argtypes = [_vp, ctypes.c_int, _f64p, _f64p]
+    lib.ganit_batch_set_start.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_double]
+    lib.ganit_batch_set_options.argtypes = [_vp, _f64p]
+    lib.ganit_batch_weights.argtypes = [_vp, _f64p]
     _host = lib
     return lib
 
@@ -325,9 +329,11 @@ class Solver:
         return int(b.value)
 
     def kernel_times(self, level: int, reps: int = 50):
-        ms = np.zeros(2)
+        """Device ms of the primal and dual fused kernels (each launched back to back) and of one whole iteration
+        (both in solve order) at level 0 (L1), 1 (L2) or 2 (L3)."""
+        ms = np.zeros(3)
         _check_host(host_lib().ganit_solver_kernel_times(self._h, int(level), int(reps), ms), "kernel_times")
-        return float(ms[0]), float(ms[1])
+        return float(ms[0]), float(ms[1]), float(ms[2])
 
 
 def solve_many(problems, threads: int = 8, **opts) -> list:
@@ -399,6 +405,41 @@ class Batch:
             xo += d["n"]
             yo += d["m"]
         return res
+
+    def set_bounds(self, k: int, l, u) -> None:
+        """Replace instance k's variable bounds (original space) in place: a branch-and-bound child of the same
+        LP without rebuilding or re-uploading the batch."""
+        n = self._dims[k]["n"]
+        l = np.ascontiguousarray(l, dtype=np.float64)
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        if l.shape != (n,) or u.shape != (n,):
+            raise ValueError(f"bounds of instance {k} must have shape ({n},)")
+        _check_host(host_lib().ganit_batch_set_bounds(self._h, int(k), l, u), "batch_set_bounds")
+
+    def set_start(self, k: int, x=None, y=None, w: float = 0.0) -> None:
+        """Start instance k's next runs from (x, y) in original space (e.g. its parent's solution) with primal
+        weight w (<= 0: its own initial weight); x=None returns it to the cold start."""
+        d = self._dims[k]
+        if x is None:
+            _check_host(host_lib().ganit_batch_set_start(self._h, int(k), None, None, 0.0), "batch_set_start")
+            return
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        if x.shape != (d["n"],) or y.shape != (d["m"],):
+            raise ValueError(f"start of instance {k} must have shapes ({d['n']},), ({d['m']},)")
+        _check_host(host_lib().ganit_batch_set_start(self._h, int(k), _ptr(x), _ptr(y), float(w)),
+                    "batch_set_start")
+
+    def set_options(self, **opts) -> None:
+        """New tolerances, limits, stall rules or check interval for the next runs (precision, values, scaling
+        and device must stay as built)."""
+        _check_host(host_lib().ganit_batch_set_options(self._h, _options(**opts)), "batch_set_options")
+
+    def weights(self) -> np.ndarray:
+        """The primal weight each instance ended the last run with."""
+        w = np.zeros(len(self._problems))
+        _check_host(host_lib().ganit_batch_weights(self._h, w), "batch_weights")
+        return w
 
     def stats(self) -> dict:
         st = np.zeros(24)

@@ -406,7 +406,11 @@ struct Batch {
     int cur = 0;
     BatchParams P{};
     std::vector<void*> owned;
-    std::map<std::tuple<int, int, int>, hipGraphExec_t> graphs;
+    // A captured block bakes in its kernels' arguments: the dtypes, the parameters passed by value to
+    // k_bt_decide and the iterate buffers X[cur] / Y[cur].  All of them are in the key, so a phase with other
+    // tolerances, limits or stall rules, or an iterate reloaded into the other buffer, captures its own block.
+    using GraphKey = std::tuple<int, int, int, int, double, double, int, int, int>;
+    std::map<GraphKey, hipGraphExec_t> graphs;
     void* alloc(size_t bytes) {
         void* p = nullptr;
         GH_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16)));
@@ -783,6 +787,33 @@ int ganit_hip_batch_load(ganit_hip_batch* h, int vdt, const double* x, const dou
     });
 }
 
+int ganit_hip_batch_set_bounds(ganit_hip_batch* h, int k, const double* l, const double* u) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && l && u, "null argument");
+        GH_REQUIRE(k >= 0 && k < bt->ninst, "instance index out of range");
+        const BatchInst& in = bt->inst_h[k];
+        for (int j = 0; j < in.n; ++j)
+            GH_REQUIRE(!(l[j] > u[j]) && !std::isnan(l[j]) && !std::isnan(u[j]),
+                       "variable bounds must satisfy l <= u and not be NaN");
+        hipStream_t st = bt->ctx->stream;
+        for (int slot = 0; slot < 2; ++slot) {
+            if (!bt->on[slot] || in.n == 0) continue;
+            BatchVecs& v = bt->vs[slot];
+            if (slot == 0) {
+                GH_CHECK(hipMemcpyAsync((double*)v.l + in.col0, l, 8 * (size_t)in.n, hipMemcpyHostToDevice, st));
+                GH_CHECK(hipMemcpyAsync((double*)v.u + in.col0, u, 8 * (size_t)in.n, hipMemcpyHostToDevice, st));
+                GH_CHECK(hipStreamSynchronize(st));
+            } else {
+                std::vector<float> fl(l, l + in.n), fu(u, u + in.n);
+                GH_CHECK(hipMemcpyAsync((float*)v.l + in.col0, fl.data(), 4 * (size_t)in.n, hipMemcpyHostToDevice, st));
+                GH_CHECK(hipMemcpyAsync((float*)v.u + in.col0, fu.data(), 4 * (size_t)in.n, hipMemcpyHostToDevice, st));
+                GH_CHECK(hipStreamSynchronize(st));
+            }
+        }
+    });
+}
+
 int ganit_hip_batch_handover(ganit_hip_batch* h, int from, int to) {
     return guarded([&] {
         auto* bt = reinterpret_cast<Batch*>(h);
@@ -848,6 +879,7 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
         bt->P.stall_exit = params[5] != 0 ? 1 : 0;
         const double time_limit = params[6];
         GH_REQUIRE(bt->P.check_every >= 2 && bt->P.check_every % 2 == 0, "batched check interval must be even");
+        GH_REQUIRE(std::isfinite(bt->P.tol) && std::isfinite(bt->P.infeas_tol), "batched tolerances must be finite");
         hipStream_t st = bt->ctx->stream;
         const auto t0 = std::chrono::steady_clock::now();
         bt_types(mdt, vdt, [&](auto MT, auto VT) {
@@ -855,7 +887,8 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
             using V = typename decltype(VT)::type;
             bt_list(bt, bt_active(bt));  // launch only the selected instances' workgroups
             bt_check<M, V>(bt, 0, kInit);  // k_restart of every running instance
-            const auto key = std::make_tuple(mdt, vdt, bt->P.check_every);
+            const Batch::GraphKey key{mdt, vdt, bt->cur, bt->P.check_every, bt->P.tol, bt->P.infeas_tol,
+                                      bt->P.max_iter, bt->P.stall_checks, bt->P.stall_exit};
             auto block = [&]() {
                 for (int k = 0; k < bt->P.check_every; ++k) bt_iteration<M, V>(bt, mdt);
                 bt_check<M, V>(bt, 1, kCheck);

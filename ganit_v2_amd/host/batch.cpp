@@ -45,7 +45,7 @@ bool converged6(const double* k, double tol) { return k[2] <= tol && k[3] <= tol
 }  // namespace
 
 struct BatchSolver::Impl {
-    const Options opt;
+    Options opt;
     ganit_hip_ctx* ctx = nullptr;
     ganit_hip_batch* bt = nullptr;
     std::string device;
@@ -58,6 +58,11 @@ struct BatchSolver::Impl {
     bool mixed = false;
     int low_mdt = GANIT_F32;
     BatchStats stats;
+    // per-instance start of the next runs in scaled space (warm[k] = 0: the cold start proj(0), y = 0)
+    std::vector<char> warm;
+    std::vector<std::vector<double>> xw, yw;
+    std::vector<double> ww;
+    std::vector<double> w_final;  // primal weights at the end of the last run
     explicit Impl(const Options& o) : opt(o) {}
     ~Impl() {
         if (bt) ganit_hip_batch_destroy(bt);
@@ -97,6 +102,10 @@ BatchSolver::BatchSolver(const std::vector<const LP*>& lps, const Options& opt) 
     }
     I.rows = I.row0[K];
     I.cols = I.col0[K];
+    I.warm.assign(K, 0);
+    I.xw.resize(K);
+    I.yw.resize(K);
+    I.ww.assign(K, 0.0);
     I.mixed = opt.precision != Precision::Double;
     I.low_mdt = opt.values == Values::BF16 ? GANIT_BF16 : GANIT_F32;
     I.setup_seconds = since(t0);
@@ -147,6 +156,56 @@ void BatchSolver::upload() {
     }
 }
 
+void BatchSolver::set_bounds(int k, const std::vector<double>& l, const std::vector<double>& u) {
+    Impl& I = *p_;
+    if (k < 0 || k >= size()) throw std::runtime_error("batch instance index out of range");
+    ScaledLP& s = I.s[k];
+    if ((int)l.size() != s.n || (int)u.size() != s.n) throw std::runtime_error("bounds must have n entries");
+    for (int j = 0; j < s.n; ++j)
+        if (std::isnan(l[j]) || std::isnan(u[j]) || l[j] > u[j])
+            throw std::runtime_error("variable bounds must satisfy l <= u and not be NaN");
+    // scale_lp's arithmetic for the variable bounds (divide by the column factor, then by bscale); nothing else in
+    // the scaled problem depends on them (bscale and the initial weight come from the row bounds)
+    for (int j = 0; j < s.n; ++j) {
+        s.l[j] = l[j] / s.C[j] / s.bscale;
+        s.u[j] = u[j] / s.C[j] / s.bscale;
+    }
+    if (I.bt) chk(ganit_hip_batch_set_bounds(I.bt, k, s.l.data(), s.u.data()));
+}
+
+void BatchSolver::set_start(int k, const std::vector<double>& x, const std::vector<double>& y, double w) {
+    Impl& I = *p_;
+    if (k < 0 || k >= size()) throw std::runtime_error("batch instance index out of range");
+    const ScaledLP& s = I.s[k];
+    if ((int)x.size() != s.n || (int)y.size() != s.m) throw std::runtime_error("start point has the wrong size");
+    const double sign = s.maximize ? -1.0 : 1.0;
+    // the inverse of the solution's unscaling x = x' C bscale, y = sign y' R cscale (pdhg_impl.hpp:313-314)
+    I.xw[k].resize(s.n);
+    I.yw[k].resize(s.m);
+    for (int j = 0; j < s.n; ++j) I.xw[k][j] = x[j] / s.C[j] / s.bscale;
+    for (int i = 0; i < s.m; ++i) I.yw[k][i] = sign * y[i] / s.R[i] / s.cscale;
+    I.ww[k] = w;
+    I.warm[k] = 1;
+}
+
+void BatchSolver::clear_start(int k) {
+    Impl& I = *p_;
+    if (k < 0 || k >= size()) throw std::runtime_error("batch instance index out of range");
+    I.warm[k] = 0;
+}
+
+void BatchSolver::set_options(const Options& o) {
+    Impl& I = *p_;
+    const Options& c = I.opt;
+    if (o.precision != c.precision || o.values != c.values || o.device != c.device || o.ruiz_iters != c.ruiz_iters ||
+        o.pock_chambolle != c.pock_chambolle)
+        throw std::runtime_error("a batch keeps the precision, matrix values, scaling and device it was built with");
+    if (o.check_every < 2 || o.check_every % 2) throw std::runtime_error("batched check_every must be even");
+    I.opt = o;
+}
+
+std::vector<double> BatchSolver::final_weights() const { return p_->w_final; }
+
 std::vector<Result> BatchSolver::run() {
     Impl& I = *p_;
     const Options& opt = I.opt;
@@ -162,11 +221,20 @@ std::vector<Result> BatchSolver::run() {
     chk(ganit_hip_batch_dims(bt, dims));
     S.rows = dims[1]; S.cols = dims[2]; S.nnz = dims[3]; S.blocks_a = dims[4]; S.blocks_at = dims[5];
 
-    std::vector<double> x0(I.cols), y0(I.rows, 0.0);
-    for (int k = 0; k < K; ++k)
-        for (int j = 0; j < I.s[k].n; ++j) x0[I.col0[k] + j] = std::min(std::max(0.0, I.s[k].l[j]), I.s[k].u[j]);
+    // each instance's start: proj(0), y = 0 and its initial weight (pdhg_impl.hpp:253-262), or its warm start
+    // projected onto its current bounds
+    std::vector<double> x0(I.cols), y0(I.rows, 0.0), w(I.w0);
+    for (int k = 0; k < K; ++k) {
+        const ScaledLP& s = I.s[k];
+        for (int j = 0; j < s.n; ++j)
+            x0[I.col0[k] + j] = std::min(std::max(I.warm[k] ? I.xw[k][j] : 0.0, s.l[j]), s.u[j]);
+        if (I.warm[k]) {
+            std::copy(I.yw[k].begin(), I.yw[k].end(), y0.begin() + I.row0[k]);
+            if (I.ww[k] > 0 && std::isfinite(I.ww[k])) w[k] = I.ww[k];
+        }
+    }
     const int first_vdt = I.mixed ? GANIT_F32 : GANIT_F64;
-    chk(ganit_hip_batch_load(bt, first_vdt, x0.data(), y0.data(), I.w0.data()));
+    chk(ganit_hip_batch_load(bt, first_vdt, x0.data(), y0.data(), w.data()));
     S.upload_seconds = since(t_solve);
 
     std::vector<int> ints(5 * K);
@@ -241,6 +309,8 @@ std::vector<Result> BatchSolver::run() {
     const double solve_s = since(t_solve);
     S.solve_seconds = solve_s;
     std::vector<Result> out(K);
+    I.w_final.resize(K);
+    for (int k = 0; k < K; ++k) I.w_final[k] = dbl[13 * k + 12];
     for (int k = 0; k < K; ++k) {
         const ScaledLP& s = I.s[k];
         Result& r = out[k];

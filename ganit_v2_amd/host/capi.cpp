@@ -303,6 +303,47 @@ int ganit_batch_stats(const ganit_batch* h, double* st) {
     });
 }
 
+int ganit_batch_set_bounds(ganit_batch* h, int k, const double* l, const double* u) {
+    return guard([&] {
+        auto* b = reinterpret_cast<ganit::BatchSolver*>(h);
+        if (!b || !l || !u) throw std::runtime_error("null argument");
+        if (k < 0 || k >= b->size()) throw std::runtime_error("batch instance index out of range");
+        const int n = b->scaled(k).n;
+        b->set_bounds(k, std::vector<double>(l, l + n), std::vector<double>(u, u + n));
+    });
+}
+
+int ganit_batch_set_start(ganit_batch* h, int k, const double* x, const double* y, double w) {
+    return guard([&] {
+        auto* b = reinterpret_cast<ganit::BatchSolver*>(h);
+        if (!b) throw std::runtime_error("null batch");
+        if (k < 0 || k >= b->size()) throw std::runtime_error("batch instance index out of range");
+        if (!x) {
+            b->clear_start(k);
+            return;
+        }
+        if (!y) throw std::runtime_error("null dual start");
+        const ganit::ScaledLP& s = b->scaled(k);
+        b->set_start(k, std::vector<double>(x, x + s.n), std::vector<double>(y, y + s.m), w);
+    });
+}
+
+int ganit_batch_set_options(ganit_batch* h, const double* o) {
+    return guard([&] {
+        if (!h || !o) throw std::runtime_error("null argument");
+        reinterpret_cast<ganit::BatchSolver*>(h)->set_options(options_from(o));
+    });
+}
+
+int ganit_batch_weights(const ganit_batch* h, double* w) {
+    return guard([&] {
+        if (!h || !w) throw std::runtime_error("null argument");
+        const std::vector<double> v = reinterpret_cast<const ganit::BatchSolver*>(h)->final_weights();
+        if (v.empty()) throw std::runtime_error("the batch has not run yet");
+        std::copy(v.begin(), v.end(), w);
+    });
+}
+
 void ganit_batch_free(ganit_batch* h) { delete reinterpret_cast<ganit::BatchSolver*>(h); }
 
 }  // extern "C"

@@ -104,8 +104,8 @@ class GpuSolver {
     GpuSolver& operator=(const GpuSolver&) = delete;
     void upload();   // device-resident problem in every precision the ladder needs; idempotent
     Result run();    // one full solve from the initial point (uploads first if needed)
-    // Average device milliseconds of the two fused iteration kernels at a precision level
-    // (0 = L1, 1 = L2, 2 = L3): ms[0] primal kernel, ms[1] dual kernel.
+    // Device milliseconds of the fused iteration kernels at a precision level (0 = L1, 1 = L2, 2 = L3):
+    // ms[0] primal kernel, ms[1] dual kernel (each launched back to back), ms[2] one whole iteration.
     void kernel_times(int level, int reps, double* ms);
     // Bytes per column index the fused kernels read for A (0), A^T (1) or Q (2): 2 with compact indices, else 4.
     int index_bytes(int which);
@@ -146,6 +146,20 @@ class BatchSolver {
     BatchSolver& operator=(const BatchSolver&) = delete;
     void upload();              // pack + upload every instance, autotune the partition; idempotent
     std::vector<Result> run();  // one solve of every instance from its initial point
+    // Branch and bound on a resident batch (extension; the reference re-solves nodes with its dual simplex,
+    // src/mip.cpp:306-323, warm-started from the parent's basis, tests/test_warm.cpp:8-38):
+    // replace instance k's variable bounds (original space, n_k entries each) in place -- the scaled copy and
+    // the device copy follow, nothing is rebuilt or re-uploaded besides those two vectors ...
+    void set_bounds(int k, const std::vector<double>& l, const std::vector<double>& u);
+    // ... start instance k's next runs from an original-space point (x: n_k, y: m_k; e.g. its parent's solution,
+    // projected onto the instance's bounds) with primal weight w (w <= 0: the instance's own initial weight) ...
+    void set_start(int k, const std::vector<double>& x, const std::vector<double>& y, double w);
+    void clear_start(int k);  // ... or from the cold start again (x = proj(0), y = 0)
+    // Change tolerances, limits, stall rules, check interval or verbosity between runs; precision, matrix
+    // values, scaling and device stay as constructed (extension).
+    void set_options(const Options& opt);
+    // The primal weight each instance ended the last run with (what a child's warm start takes over).
+    std::vector<double> final_weights() const;
     int size() const;
     double setup_seconds() const;
     const ScaledLP& scaled(int k) const;

@@ -213,6 +213,11 @@ int ganit_hip_batch_destroy(ganit_hip_batch* bt);
 int ganit_hip_batch_dims(const ganit_hip_batch* bt, int64_t* dims);
 /* Initial iterate (concatenated x, y) and primal weights; every instance starts running. */
 int ganit_hip_batch_load(ganit_hip_batch* bt, int vdt, const double* x, const double* y, const double* w);
+/* Replace instance k's variable bounds in place, in scaled space (l, u: that instance's n entries), in every
+ * resident vector dtype; the next load and phase run on them.  Nothing is rebuilt: a branch-and-bound child
+ * differs from its parent only here.  No reference counterpart (its branch-and-cut changes bounds in the dual
+ * simplex, src/simplex.hpp:64-73 set_col_bounds and warm start, src/mip.cpp:306-307). */
+int ganit_hip_batch_set_bounds(ganit_hip_batch* bt, int k, const double* l, const double* u);
 /* Convert every instance's (x, y, xr, yr) between working sets and clear the target's running sums. */
 int ganit_hip_batch_handover(ganit_hip_batch* bt, int from_vdt, int to_vdt);
 /* The instances with sel[k] != 0 run in the next phase; their iteration count, restarts, weight and
@@ -250,10 +255,12 @@ int ganit_hip_batch_store(ganit_hip_batch* bt, int vdt, double* x, double* y);
 int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* pd, int mdt, int vdt, const double* gx, double tau, double sigma,
                            const double* params, double* p, double* mu, double* info);
 
-/* Average device time in milliseconds of each fused iteration kernel on the current iterate, measured with
- * HIP events on the context stream over `reps` back-to-back launches: ms[0] = primal kernel
- * (A^T y [+ Q x] + primal prox), ms[1] = dual kernel (A xbar + dual prox).  Advances the iterate; no
- * reference counterpart (the reference has no per-kernel timers). */
+/* Device time in milliseconds of the fused iteration kernels, measured with HIP events on the context stream from
+ * the current iterate: ms[0] = primal kernel (A^T y [+ Q x] + primal prox) and ms[1] = dual kernel (A xbar +
+ * dual prox), each averaged over `reps` back-to-back launches of that kernel alone (its duration plus the
+ * dispatch gap to the next launch); ms[2] = one whole iteration (both kernels in the order a solve launches
+ * them), averaged over `reps` iterations with events only at the ends.  Advances the iterate; no reference
+ * counterpart (the reference has no per-kernel timers). */
 int ganit_hip_pdhg_time_kernels(ganit_hip_pdhg* pd, int mdt, int vdt, int reps, double tau, double sigma,
                                 double* ms);
 

@@ -73,8 +73,9 @@ int ganit_solve_gpu(const ganit_lp* lp, const double* opts, double* out, double*
 /* The same solve split into stages (no reference counterpart: the reference always re-uploads per solve).
  * create = HIP context + scaling + norm estimate (opts as above); upload = device-resident problem in every
  * precision the ladder needs; run = one full solve from the initial point with the problem already in HBM
- * (out / x / y as ganit_solve_gpu); kernel_times = average device ms of the two fused iteration kernels
- * at level 0 (L1), 1 (L2) or 2 (L3): ms[0] primal, ms[1] dual. */
+ * (out / x / y as ganit_solve_gpu); kernel_times = device ms of the fused iteration kernels at level 0 (L1),
+ * 1 (L2) or 2 (L3): ms[0] primal and ms[1] dual kernel (each launched back to back), ms[2] one whole iteration
+ * (both kernels in solve order) -- three entries. */
 typedef struct ganit_solver ganit_solver;
 ganit_solver* ganit_solver_create(const ganit_lp* lp, const double* opts);
 int ganit_solver_upload(ganit_solver* s);
@@ -115,6 +116,20 @@ ganit_batch* ganit_batch_create(const ganit_lp* const* lps, int ninst, const dou
 int ganit_batch_upload(ganit_batch* b);
 int ganit_batch_run(ganit_batch* b, double* out, double* x, double* y);
 int ganit_batch_stats(const ganit_batch* b, double* stats);
+/* Branch and bound on a resident batch (no reference counterpart; the reference's branch-and-cut re-solves a
+ * node with its dual simplex after set_col_bounds, warm-started from the parent's basis: src/mip.cpp:306-323,
+ * src/simplex.hpp:64-73, tests/test_warm.cpp:8-38).
+ * set_bounds: replace instance k's variable bounds (original space, n_k entries each) in place; the scaled
+ *   and device copies follow, nothing else is rebuilt.
+ * set_start: start instance k's next runs from the original-space point x (n_k), y (m_k) -- projected onto its
+ *   bounds -- with primal weight w (w <= 0: its own initial weight); x = NULL returns it to the cold start.
+ * set_options: new tolerances, limits, stall rules, check interval or verbosity (opts as ganit_solve_gpu);
+ *   precision, values, scaling and device must stay as created.
+ * weights: the primal weight each instance ended the last run with (ninst entries). */
+int ganit_batch_set_bounds(ganit_batch* b, int k, const double* l, const double* u);
+int ganit_batch_set_start(ganit_batch* b, int k, const double* x, const double* y, double w);
+int ganit_batch_set_options(ganit_batch* b, const double* opts);
+int ganit_batch_weights(const ganit_batch* b, double* w);
 void ganit_batch_free(ganit_batch* b);
 
 #ifdef __cplusplus
