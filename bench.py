@@ -4,7 +4,8 @@ mixed precision (L1 FP32 -> L3 FP64) vs plain FP64, on one MI355X; plus the fuse
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--cpu-sample-s S]
 
 A step is one full solve from the initial point with the scaled problem already resident in HBM (upload,
-scaling and norm estimation happen before the timed region).  Prints ONE JSON line.
+scaling and norm estimation happen before the timed region); "with_upload" reports a fresh solver's first solve,
+whose solve_s includes the upload as the reference's does.  Prints ONE JSON line.
 """
 from __future__ import annotations
 
@@ -45,14 +46,24 @@ def load_traffic():
 
 
 def cpu_model():
+    """The host CPU as the OS names it (/proc/cpuinfo's model name, with family / model / stepping, and the
+    logical CPUs this process may use)."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if ":" in line:
+                    k, v = (t.strip() for t in line.split(":", 1))
+                    info.setdefault(k, v)
     except OSError:
         pass
-    return "unknown"
+    name = info.get("model name", "unknown")
+    fam = ", ".join(f"{k} {info[k]}" for k in ("cpu family", "model", "stepping") if k in info)
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return f"{name} ({fam}; {usable} logical CPUs usable)" if fam else f"{name} ({usable} logical CPUs usable)"
 
 
 def reference_cpu_baseline(problem, threads, time_limit):
@@ -92,7 +103,7 @@ def reference_cpu_baseline(problem, threads, time_limit):
                    f"12, level-precision checks), full solve of the same LP written as MPS: {r['status']} in "
                    f"{r['iters']} iterations ({r['fp32_iters']} L1), solve_s as it reports it (read "
                    f"{r['read_s']:.2f} s and setup {r['setup_s']:.2f} s excluded)"),
-        "status": r["status"], "iters": r["iters"],
+        "status": r["status"], "iters": r["iters"], "pobj": r["pobj"], "dobj": r["dobj"],
     }
 
 
@@ -279,6 +290,75 @@ def diff_qps(count, threads=32, batched=True):
                 grad_err_fp32_median=float(np.median(errs)) if errs else None, rows=rows)
 
 
+def reference_cli(path, threads, *args, timeout=600):
+    """One run of the reference CLI built from its sources (oracle/_ref/ganit) on an MPS file; its --json line,
+    or None where that binary is not built."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "ganit")
+    if not os.access(exe, os.X_OK):
+        return None
+    out = subprocess.run([exe, path, "--device", "cpu", "--json", "--quiet", *args], capture_output=True, text=True,
+                         env=dict(os.environ, OMP_NUM_THREADS=str(threads)), timeout=timeout)
+    lines = [l for l in out.stdout.splitlines() if l.startswith("{")]
+    return json.loads(lines[-1]) if out.returncode in (0, 2) and lines else None
+
+
+def random_lp_c0(threads):
+    """The seeded random LP (m=2000 rows, half equalities, n=5000 columns, 10 nonzeros per column, seed 0) in
+    FP64 to 1e-4: the GPU solve (problem resident; and a fresh solver's first run, whose solve_s includes its
+    upload) against the reference CLI on the same LP, on this host's cores and on one core."""
+    import statistics
+    import tempfile
+    import ganit_v2_amd as g
+    p = g.Problem.generate("random_lp", seed=0)
+    d = p.dims()
+    s = p.solver(precision="double", tol=1e-4)
+    first = s.run()
+    runs = [s.run() for _ in range(5)]
+    r = runs[-1]
+    gpu = dict(status=r["status"], iters=r["iters"], pobj=r["pobj"],
+               solve_s=round(statistics.median(x["solve_s"] for x in runs), 6),
+               solve_s_with_upload=round(first["solve_s"], 6), setup_s=round(first["setup_s"], 6))
+    out = dict(workload=f"random LP m={d['m']} n={d['n']} nnz={d['nnz']}, seed 0, FP64, tol 1e-4", gpu=gpu)
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "random_lp_s0.mps")
+        p.write_mps(path)
+        for t in sorted({threads, 1}):
+            ref = reference_cli(path, t, "--precision", "double", "--tol", "1e-4")
+            if ref is None:
+                continue
+            out[f"reference_cpu_{t}_cores"] = dict(status=ref["status"], iters=ref["iters"], pobj=ref["pobj"],
+                                                   solve_s=ref["solve_s"], setup_s=ref["setup_s"])
+            out[f"gpu_speedup_vs_{t}_cores"] = round(ref["solve_s"] / gpu["solve_s"], 2)
+            out[f"obj_rel_err_vs_reference_{t}_cores"] = abs(gpu["pobj"] - ref["pobj"]) / (1 + abs(ref["pobj"]))
+    out["cpu_model"] = cpu_model()
+    print(f"[bench] random LP (C0) {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
+
+
+def large_blending_kernels(periods, reps):
+    """The fused iteration kernels on a blending LP large enough that an iteration's streams cannot stay in the
+    256 MB Infinity Cache (60 crudes x 16 products x 5 specs x `periods` periods): HIP-event times per launch in
+    the order a solve runs them, at L1 and L3, and their algorithmic GB/s."""
+    import ganit_v2_amd as g
+    prob = g.Problem.generate("blending", seed=0, crudes=60, products=16, specs=5, periods=periods)
+    d = prob.dims()
+    m, n, nnz = d["m"], d["n"], d["nnz"]
+    out = dict(workload=f"blending LP 60 x 16 x 5 x {periods} periods, seed 0", m=m, n=n, nnz=nnz)
+    for level, (prec, sm, sv) in {"L1": ("mixed", 4, 4), "L3": ("double", 8, 8)}.items():
+        s = prob.solver(precision=prec, tol=1e-4)
+        s.upload()
+        ms_p, ms_d, ms_it = s.kernel_times(0 if level == "L1" else 2, reps)
+        bp, bd = kernel_bytes(m, n, nnz, sm, sv, s.index_bytes(1), s.index_bytes(0))
+        out[level] = dict(primal_us=round(ms_p * 1e3, 3), dual_us=round(ms_d * 1e3, 3),
+                          iteration_us=round(ms_it * 1e3, 3), primal_bytes=bp, dual_bytes=bd,
+                          iteration_bytes=bp + bd, primal_gbs=round(bp / (ms_p * 1e6), 1),
+                          dual_gbs=round(bd / (ms_d * 1e6), 1), iteration_gbs=round((bp + bd) / (ms_it * 1e6), 1))
+        del s
+    print(f"[bench] large blending kernels {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
+
+
 def kernels_only(prob, staged, a, m, n, nnz):
     """Back-to-back launches of the fused kernels at L1 and L3 (HIP events on the solver's stream), then one
     torch copy of 1 GiB: a plain copy kernel of known size whose FETCH_SIZE / WRITE_SIZE calibrate the
@@ -288,10 +368,11 @@ def kernels_only(prob, staged, a, m, n, nnz):
     out = {}
     for level, (sm, sv, sv_obj) in {"L1": (s_m_low, 4, staged("mixed", values=a.values)),
                                     "L3": (8, 8, staged("double"))}.items():
-        ms_p, ms_d = sv_obj.kernel_times(0 if level == "L1" else 2, a.reps)
+        ms_p, ms_d, ms_it = sv_obj.kernel_times(0 if level == "L1" else 2, a.reps)
         bp, bd = kernel_bytes(m, n, nnz, sm, sv, sv_obj.index_bytes(1), sv_obj.index_bytes(0))
-        out[level] = dict(primal_us=ms_p * 1e3, dual_us=ms_d * 1e3, primal_gbs=bp / (ms_p * 1e6),
-                          dual_gbs=bd / (ms_d * 1e6), primal_bytes=bp, dual_bytes=bd)
+        out[level] = dict(primal_us=ms_p * 1e3, dual_us=ms_d * 1e3, iteration_us=ms_it * 1e3,
+                          primal_gbs=bp / (ms_p * 1e6), dual_gbs=bd / (ms_d * 1e6),
+                          iteration_gbs=(bp + bd) / (ms_it * 1e6), primal_bytes=bp, dual_bytes=bd)
         del sv_obj
     src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
     src.fill_(1.0)
@@ -323,6 +404,11 @@ def main():
                     help="also run N differentiable QPs (forward + adjoint) and report their times and errors")
     ap.add_argument("--c4-concurrent", action="store_true",
                     help="with --c4: forwards as concurrent single solves instead of one batched solve per phase")
+    ap.add_argument("--periods", type=int, default=WORKLOAD["periods"],
+                    help="periods of the blending LP (365: the 2.9M-nonzero headline workload); other values are "
+                         "for --kernels-only profiler runs")
+    ap.add_argument("--large-periods", type=int, default=1460,
+                    help="periods of the blending LP whose kernels are timed in extras.large_blending (0: skip)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the fused iteration kernels (the launches the roofline figure uses), plus one "
                          "plain 1 GiB device copy as the counter calibration; for profiler runs")
@@ -335,7 +421,10 @@ def main():
     import ganit_v2_amd as g
 
     t0 = time.time()
-    prob = g.Problem.generate("blending", seed=0, **WORKLOAD)
+    if a.periods != WORKLOAD["periods"] and not a.kernels_only:
+        print("--periods other than 365 is for --kernels-only runs", file=sys.stderr)
+        sys.exit(2)
+    prob = g.Problem.generate("blending", seed=0, **dict(WORKLOAD, periods=a.periods))
     d = prob.dims()
     m, n, nnz = d["m"], d["n"], d["nnz"]
     print(f"[bench] blending LP m={m} n={n} nnz={nnz} generated in {time.time() - t0:.1f}s", file=sys.stderr,
@@ -359,6 +448,16 @@ def main():
     last = runs[-1]
     print(f"[bench] {a.precision}: {[round(r['solve_s'], 4) for r in runs]} s, status {last['status']}, "
           f"iters {last['iters']} (L1 {last['fp32_iters']})", file=sys.stderr, flush=True)
+
+    # the reference's solve_s includes the device upload (pdhg_impl.hpp:273-274, 305): a fresh solver's first run,
+    # whose solve_s also holds the upload of every precision and the autotuner's trials
+    cold = prob.solver(precision=a.precision, tol=1e-4, verbose=0, values=a.values, stall_checks=a.stall_checks,
+                       eval_fp64=(a.eval == "fp64"))
+    rc = cold.run()
+    cold_run = dict(solve_s_with_upload=round(rc["solve_s"], 5), upload_s=round(rc["upload_s"], 5),
+                    setup_s=round(rc["setup_s"], 5), status=rc["status"], iters=rc["iters"])
+    del cold
+    print(f"[bench] {a.precision} with upload: {cold_run}", file=sys.stderr, flush=True)
 
     fp64 = staged("double")
     fp64.run()
@@ -389,18 +488,32 @@ def main():
     for level, (sm, sv, solver) in {"L1": (s_m_low, 4, mixed), "L3": (8, 8, fp64)}.items():
         if level == "L1" and a.precision == "double":
             continue
-        ms_p, ms_d = solver.kernel_times(0 if level == "L1" else 2, a.reps)
+        ms_p, ms_d, ms_it = solver.kernel_times(0 if level == "L1" else 2, a.reps)
         ib_at, ib_a = solver.index_bytes(1), solver.index_bytes(0)
         bp, bd = kernel_bytes(m, n, nnz, sm, sv, ib_at, ib_a)
         kern[level] = dict(primal_ms=ms_p, dual_ms=ms_d, primal_gbs=bp / (ms_p * 1e6), dual_gbs=bd / (ms_d * 1e6),
                            primal_bytes=bp, dual_bytes=bd, index_bytes_at=ib_at, index_bytes_a=ib_a,
-                           iteration_us=(ms_p + ms_d) * 1e3,
-                           ref_sequence_gbs=reference_sequence_bytes(m, n, nnz, sm, sv) / ((ms_p + ms_d) * 1e6))
+                           iteration_us=ms_it * 1e3, iteration_gbs=(bp + bd) / (ms_it * 1e6),
+                           ref_sequence_gbs=reference_sequence_bytes(m, n, nnz, sm, sv) / (ms_it * 1e6))
     print(f"[bench] kernels {json.dumps(kern)}", file=sys.stderr, flush=True)
+    # the roofline entry is the kernel with the most device time per iteration of the headline level (both
+    # kernels launch once per iteration; each is timed over back-to-back launches of itself); the other fused
+    # kernel's figures, and each level's whole-iteration rate, ride along under "kernels"
     lead = kern.get("L1", kern["L3"])
     dom = "primal" if lead["primal_ms"] >= lead["dual_ms"] else "dual"
     level_dom = "L1" if "L1" in kern else "L3"
-    traffic = load_traffic()
+    traffic = load_traffic() or {}
+    per_kernel = {}
+    for lv, kv in kern.items():
+        for which in ("primal", "dual"):
+            per_kernel[f"k_iter_{which} ({lv})"] = dict(
+                achieved=round(kv[f"{which}_gbs"], 1), frac=round(kv[f"{which}_gbs"] / PEAK_HBM_GBS, 4),
+                us=round(kv[f"{which}_ms"] * 1e3, 3), algorithmic_bytes=kv[f"{which}_bytes"],
+                traffic=traffic.get(f"k_iter_{which}_{lv}"))
+        per_kernel[f"iteration ({lv})"] = dict(achieved=round(kv["iteration_gbs"], 1),
+                                               frac=round(kv["iteration_gbs"] / PEAK_HBM_GBS, 4),
+                                               us=round(kv["iteration_us"], 3),
+                                               algorithmic_bytes=kv["primal_bytes"] + kv["dual_bytes"])
     roof = {
         "bound": "hbm",
         "kernel": f"k_iter_{dom} ({level_dom})",
@@ -408,11 +521,15 @@ def main():
         "peak": PEAK_HBM_GBS,
         "unit": "GB/s",
         "frac": round(lead[f"{dom}_gbs"] / PEAK_HBM_GBS, 4),
-        "traffic": (traffic or {}).get(f"k_iter_{dom}_{level_dom}"),
+        "traffic": traffic.get(f"k_iter_{dom}_{level_dom}"),
+        "kernels": per_kernel,
     }
 
     extras = {}
     if not a.no_extras:
+        extras["random_lp_c0"] = random_lp_c0(os.cpu_count() or 1)
+        if a.large_periods > 0:
+            extras["large_blending"] = large_blending_kernels(a.large_periods, a.reps)
         extras["portfolio_qp"] = portfolio_ladder()
         extras["batched_lp"] = batched_lps()
     if a.c4 > 0:
@@ -421,6 +538,8 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(prob, last, a.cpu_sample_s)
+        if cpu and "pobj" in cpu:
+            cpu["gpu_obj_rel_err_vs_reference"] = abs(last["pobj"] - cpu["pobj"]) / (1 + abs(cpu["pobj"]))
         print(f"[bench] cpu baseline {json.dumps(cpu)}", file=sys.stderr, flush=True)
 
     line = {
@@ -443,6 +562,7 @@ def main():
         "iters": last["iters"], "l1_iters": last["fp32_iters"], "restarts": last["restarts"],
         "fp64_s": round(t_fp64, 5), "fp64_iters": f_last["iters"], "fp64_status": f_last["status"],
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+        "with_upload": cold_run,
         "obj": last["pobj"], "obj_fp64": f_last["pobj"], "obj_rel_err_vs_fp64": obj_err,
         "certified_rel": [last["rel_p"], last["rel_d"], last["rel_gap"]],
         "ladder": ladder,

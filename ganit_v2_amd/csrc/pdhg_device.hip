@@ -1066,7 +1066,7 @@ int ganit_hip_pdhg_time_kernels(ganit_hip_pdhg* h, int mdt, int vdt, int reps, d
         GH_REQUIRE(pd && ms && reps > 0, "bad timing arguments");
         VecSet& s = pd->set(vdt);
         hipStream_t st = pd->ctx->stream;
-        hipEvent_t ev[3];
+        hipEvent_t ev[4];
         for (auto& e : ev) GH_CHECK(hipEventCreate(&e));
         try {
             hipLaunchKernelGGL(k_set_steps, dim3(1), dim3(64), 0, st, pd->prm, tau, sigma);
@@ -1075,19 +1075,23 @@ int ganit_hip_pdhg_time_kernels(ganit_hip_pdhg* h, int mdt, int vdt, int reps, d
                     using M = typename decltype(MT)::type;
                     using V = typename decltype(VT)::type;
                     launch_iteration<M, V>(pd, s);  // warm-up, and leaves xbar defined
+                    // each kernel back to back (its duration plus the dispatch gap to the next launch)
                     GH_CHECK(hipEventRecord(ev[0], st));
                     for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 0);
                     GH_CHECK(hipEventRecord(ev[1], st));
                     for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 1);
                     GH_CHECK(hipEventRecord(ev[2], st));
+                    // whole iterations in the order a solve launches them, events only at the ends
+                    for (int r = 0; r < reps; ++r) launch_iteration<M, V>(pd, s);
+                    GH_CHECK(hipEventRecord(ev[3], st));
                 });
             });
-            GH_CHECK(hipEventSynchronize(ev[2]));
-            float a = 0, b = 0;
-            GH_CHECK(hipEventElapsedTime(&a, ev[0], ev[1]));
-            GH_CHECK(hipEventElapsedTime(&b, ev[1], ev[2]));
-            ms[0] = a / reps;
-            ms[1] = b / reps;
+            GH_CHECK(hipEventSynchronize(ev[3]));
+            for (int k = 0; k < 3; ++k) {
+                float t = 0;
+                GH_CHECK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
+                ms[k] = t / reps;
+            }
         } catch (...) {
             for (auto& e : ev) (void)hipEventDestroy(e);
             throw;
