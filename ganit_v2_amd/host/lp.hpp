@@ -1,8 +1,9 @@
-// Core LP/QP data types of the ganit host library (mirror of the reference's include/ganit/lp.hpp:14-47).
+// Core LP/QP data types of the ganit host library (same fields as the reference's include/ganit/lp.hpp:14-47,
+// which its driver and reader fill and read).  The problem they hold:
 //
 //   minimize    c'x + 0.5 x'Qx + obj_const
-//   subject to  lo <= A x <= hi      (row bounds, may be +-inf)
-//               l  <=  x  <= u       (column bounds, may be +-inf)
+//   subject to  lo <= A x <= hi      (an infinite entry leaves that side of the row open)
+//               l  <=  x  <= u       (likewise for the variables)
 #pragma once
 #include <cstdint>
 #include <limits>
@@ -16,9 +17,9 @@ constexpr double kInf = std::numeric_limits<double>::infinity();
 // Compressed sparse row matrix with 32-bit indices (reference lp.hpp:19-25).
 struct Csr {
     int rows = 0, cols = 0;
-    std::vector<int> ptr;     // rows + 1
-    std::vector<int> idx;     // nnz column indices, ascending within a row
-    std::vector<double> val;  // nnz values
+    std::vector<int> ptr;     // row r's entries are [ptr[r], ptr[r + 1])
+    std::vector<int> idx;     // column of each entry, ascending within a row
+    std::vector<double> val;  // value of each entry
     int64_t nnz() const { return static_cast<int64_t>(val.size()); }
 };
 
@@ -38,8 +39,8 @@ struct LP {
     double obj_const = 0.0;
     bool maximize = false;
     std::vector<std::string> row_names, col_names;
-    std::vector<char> is_int;
-    Csr Q;  // n x n symmetric, full storage, or empty
+    std::vector<char> is_int;  // 1 for a column inside an integer MARKER block of the MPS file
+    Csr Q;  // the quadratic term: both triangles stored (n x n), no entries for an LP
     bool has_q() const { return Q.nnz() > 0; }
     int num_int() const;
 };

@@ -9,17 +9,18 @@
 
 namespace ganit {
 
+// The problem PDHG iterates on.  Field names and layout are the reference's ScaledLP, which its driver reads.
 struct ScaledLP {
     int m = 0, n = 0;
-    Csr A, AT;
-    std::vector<double> c, l, u, lo, hi;  // scaled data
-    std::vector<double> R, C;             // row / column scaling factors
-    double bscale = 1.0, cscale = 1.0;    // uniform bound / objective rescaling
-    std::vector<double> Rres, Cres;       // residual unscaling: R / bscale, C / cscale
+    Csr A, AT;                            // A' and its stored transpose
+    std::vector<double> c, l, u, lo, hi;  // objective, variable bounds and row bounds after scaling
+    std::vector<double> R, C;             // accumulated Ruiz (and Pock-Chambolle) factors of rows / columns
+    double bscale = 1.0, cscale = 1.0;    // one factor for all bounds, one for the objective
+    std::vector<double> Rres, Cres;       // R / bscale and C / cscale: residuals back in original units
     Csr Q;                                // sign * C Q C * bscale / cscale
     double normQ = 0.0;                   // ||Q'||_2 estimate (60 power steps, x1.01)
     double obj_const = 0.0;
-    double bnorm = 0.0, cnorm = 0.0;      // norms of the ORIGINAL rhs and objective
+    double bnorm = 0.0, cnorm = 0.0;      // ||row bounds||, ||c|| of the problem as given (before scaling)
     bool maximize = false;
     bool has_q() const { return Q.nnz() > 0; }
 };
