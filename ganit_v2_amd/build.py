@@ -27,8 +27,6 @@ OBJ = os.path.join(ROOT, "_build")
 
 HIP_SOURCES = ["runtime.hip", "pdhg_device.hip", "batch.hip", "adjoint.hip"]
 HOST_SOURCES = ["lp.cpp", "scaling.cpp", "generators.cpp", "driver.cpp", "batch.cpp", "capi.cpp"]
-MICRO = os.path.join(PKG, "microbench")
-MICROBENCH_SOURCES = ["launch_floor.hip", "grid_barrier.hip"]
 
 ARCH = os.environ.get("GANIT_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -62,12 +60,6 @@ def build(verbose: bool = True) -> dict:
     for s in HOST_SOURCES + ["main.cpp"]:
         o = os.path.join(OBJ, "host_" + s.replace(".cpp", ".o"))
         jobs.append([CXX, *CXX_FLAGS, "-c", os.path.join(HOST, s), "-o", o])
-    # stand-alone hardware microbenchmarks (launch floor, streaming rate, grid-barrier cost)
-    micro = {}
-    for s in MICROBENCH_SOURCES:
-        exe_m = os.path.join(BIN, "ganit_" + s.replace(".hip", ""))
-        micro[s] = exe_m
-        jobs.append([HIPCC, *HIP_FLAGS, os.path.join(MICRO, s), "-o", exe_m])
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 4)) as ex:
         list(ex.map(_run, jobs))
     hip_so = os.path.join(LIB, "libganit_hip.so")
@@ -82,7 +74,7 @@ def build(verbose: bool = True) -> dict:
           f"-L{LIB}", "-lganit", "-lganit_hip", "-Wl,-rpath,$ORIGIN/../lib"])
     if verbose:
         print(f"built {hip_so}\nbuilt {host_so}\nbuilt {exe}")
-    return {"hip": hip_so, "host": host_so, "exe": exe, "microbench": micro}
+    return {"hip": hip_so, "host": host_so, "exe": exe}
 
 
 if __name__ == "__main__":
