@@ -199,6 +199,14 @@ def batched_lps(count=512):
         rs = res.pop(name + "_results")
         errs = [abs(a["pobj"] - b["pobj"]) / (1 + abs(b["pobj"])) for a, b in zip(rs, seq)
                 if a["status"] == b["status"] == "OPTIMAL"]
+        both = [k for k, (a, b) in enumerate(zip(rs, seq)) if a["status"] == b["status"] == "OPTIMAL"]
+        worst = both[int(np.argmax(errs))]
+        res[name]["largest_obj_difference"] = dict(
+            seed=worst, obj=rs[worst]["pobj"], obj_sequential_fp64=seq[worst]["pobj"], rel_err=max(errs),
+            certified_rel=[rs[worst]["rel_p"], rs[worst]["rel_d"], rs[worst]["rel_gap"]],
+            iters=rs[worst]["iters"], l1_iters=rs[worst]["fp32_iters"])
+        res[name]["certified_all"] = all(max(r["rel_p"], r["rel_d"], r["rel_gap"]) <= 1e-4 * (1 + 1e-7)
+                                         for r in rs if r["status"] == "OPTIMAL")
         res[name]["status_mismatches"] = sum(1 for a, b in zip(rs, seq) if a["status"] != b["status"])
         res[name]["same_iterations"] = sum(1 for a, b in zip(rs, seq) if a["iters"] == b["iters"])
         res[name]["obj_rel_err_vs_sequential_fp64"] = dict(max=max(errs), median=float(np.median(errs)))
