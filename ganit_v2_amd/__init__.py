@@ -64,6 +64,7 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_lp_planted.argtypes = [_vp, _vp, _vp]
     lib.ganit_lp_write_mps.argtypes = [_vp, ctypes.c_char_p]
     lib.ganit_lp_scaled.argtypes = [_vp, ctypes.c_int, ctypes.c_int] + [_vp] * 19
+    lib.ganit_lp_scaled_gpu.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_vp] * 19
     lib.ganit_solve_gpu.argtypes = [_vp, _f64p, _f64p, _vp, _vp]
     lib.ganit_solver_create.restype = _vp
     lib.ganit_solver_create.argtypes = [_vp, _f64p]
@@ -71,6 +72,7 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_solver_run.argtypes = [_vp, _f64p, _vp, _vp]
     lib.ganit_solver_kernel_times.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _f64p]
     lib.ganit_solver_free.argtypes = [_vp]
+    lib.ganit_solver_info.argtypes = [_vp, _f64p]
     lib.ganit_solver_index_bytes.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.ganit_solver_adjoint.argtypes = [_vp, _f64p, _f64p, _vp, _vp, _vp, _vp]
     lib.ganit_solver_set_solution.argtypes = [_vp, _f64p, _f64p, ctypes.c_int]
@@ -210,8 +212,9 @@ class Problem:
     def write_mps(self, path: str) -> None:
         _check_host(host_lib().ganit_lp_write_mps(self._h, os.fsencode(path)), "write_mps")
 
-    def scaled(self, ruiz_iters: int = 10, pock_chambolle: bool = True) -> dict:
-        """The scaled problem exactly as the solver uploads it (scale_lp + estimate_norm)."""
+    def scaled(self, ruiz_iters: int = 10, pock_chambolle: bool = True, on_gpu: bool = False, device: int = 0) -> dict:
+        """The scaled problem exactly as the solver uploads it (scale_lp + estimate_norm); on_gpu: with the matrix
+        work of both on GPU `device` (the same bits)."""
         d = self.dims()
         m, n = d["m"], d["n"]
         scal = np.zeros(9)
@@ -221,8 +224,10 @@ class Problem:
         Q = CSR(n, n, np.zeros(n + 1, np.int32), np.zeros(d["qnnz"], np.int32), np.zeros(d["qnnz"]))
         v = {k: np.zeros(n) for k in ("c", "l", "u", "C", "Cres")}
         v.update({k: np.zeros(m) for k in ("lo", "hi", "R", "Rres")})
-        _check_host(host_lib().ganit_lp_scaled(
-            self._h, int(ruiz_iters), int(bool(pock_chambolle)), _ptr(scal), _ptr(A.ptr), _ptr(A.idx), _ptr(A.val),
+        head = ((host_lib().ganit_lp_scaled_gpu, int(device)) if on_gpu else (host_lib().ganit_lp_scaled,))
+        _check_host(head[0](
+            self._h, int(ruiz_iters), int(bool(pock_chambolle)), *head[1:], _ptr(scal), _ptr(A.ptr), _ptr(A.idx),
+            _ptr(A.val),
             _ptr(AT.ptr), _ptr(AT.idx), _ptr(AT.val), _ptr(Q.ptr), _ptr(Q.idx), _ptr(Q.val), _ptr(v["c"]),
             _ptr(v["l"]), _ptr(v["u"]), _ptr(v["lo"]), _ptr(v["hi"]), _ptr(v["R"]), _ptr(v["C"]), _ptr(v["Rres"]),
             _ptr(v["Cres"])), "scaled")
@@ -250,12 +255,16 @@ _RESULT_KEYS = ["status", "pobj", "dobj", "rel_p", "rel_d", "rel_gap", "iters", 
                 "restarts", "setup_s", "solve_s", "norm_a", "upload_s", "l1_s", "l2_s", "l3_s", "checks"]
 
 
+SETUP = {"auto": 0, "host": 1, "gpu": 2}
+
+
 def _options(tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10, pock_chambolle=True,
              verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12, stall_checks_l2=24, values="f32",
-             device=0, eval_fp64=False, kahan=False, detect_unbounded=False) -> np.ndarray:
+             device=0, eval_fp64=False, kahan=False, detect_unbounded=False, setup="auto") -> np.ndarray:
+    flags = int(bool(kahan)) | (2 if detect_unbounded else 0) | (SETUP[setup] << 2)
     return np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
                      infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device,
-                     int(bool(eval_fp64)), int(bool(kahan)) | (2 if detect_unbounded else 0)], dtype=np.float64)
+                     int(bool(eval_fp64)), flags], dtype=np.float64)
 
 
 def _result(out, x, y) -> dict:
@@ -321,6 +330,19 @@ class Solver:
         if x.shape != (d["n"],) or y.shape != (d["m"],):
             raise ValueError(f"x, y must have shapes ({d['n']},), ({d['m']},)")
         _check_host(host_lib().ganit_solver_set_solution(self._h, x, y, int(bool(fp32_iterate))), "set_solution")
+
+    def _info(self):
+        info = np.zeros(3)
+        _check_host(host_lib().ganit_solver_info(self._h, info), "solver_info")
+        return info
+
+    def setup_seconds(self) -> float:
+        """Seconds of this solver's set-up (scaling + norm estimate), outside solve_s as in the reference."""
+        return float(self._info()[0])
+
+    def setup_on_device(self) -> bool:
+        """Whether the set-up's matrix work ran on the GPU."""
+        return bool(self._info()[2])
 
     def index_bytes(self, which: int) -> int:
         """Bytes per column index the fused kernels read for A (0), A^T (1), Q (2): 2 (compact) or 4."""

@@ -3,6 +3,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "../../include/ganit_hip.h"
 #include "../../include/ganit_host.h"
 #include "generators.hpp"
 #include "pdhg.hpp"
@@ -136,21 +137,52 @@ int ganit_lp_write_mps(const ganit_lp* hh, const char* path) {
     return guard([&] { ganit::write_mps(reinterpret_cast<const Handle*>(hh)->lp, path); });
 }
 
+namespace {
+void scaled_out(const ganit::ScaledLP& s, double norm_a, double* scal, int* aptr, int* aidx, double* aval, int* tptr,
+                int* tidx, double* tval, int* qptr, int* qidx, double* qval, double* c, double* l, double* u,
+                double* lo, double* hi, double* R, double* C, double* Rres, double* Cres) {
+    if (scal) {
+        const double v[9] = {s.bscale, s.cscale, s.normQ, s.obj_const, s.bnorm, s.cnorm, norm_a, (double)s.AT.nnz(),
+                             s.maximize ? 1.0 : 0.0};
+        std::copy(v, v + 9, scal);
+    }
+    put(aptr, s.A.ptr); put(aidx, s.A.idx); put(aval, s.A.val);
+    put(tptr, s.AT.ptr); put(tidx, s.AT.idx); put(tval, s.AT.val);
+    put(qptr, s.Q.ptr); put(qidx, s.Q.idx); put(qval, s.Q.val);
+    put(c, s.c); put(l, s.l); put(u, s.u); put(lo, s.lo); put(hi, s.hi);
+    put(R, s.R); put(C, s.C); put(Rres, s.Rres); put(Cres, s.Cres);
+}
+}  // namespace
+
 int ganit_lp_scaled(const ganit_lp* hh, int ruiz, int pc, double* scal, int* aptr, int* aidx, double* aval, int* tptr,
                     int* tidx, double* tval, int* qptr, int* qidx, double* qval, double* c, double* l, double* u,
                     double* lo, double* hi, double* R, double* C, double* Rres, double* Cres) {
     return guard([&] {
         const ganit::ScaledLP s = ganit::scale_lp(reinterpret_cast<const Handle*>(hh)->lp, ruiz, pc != 0);
-        if (scal) {
-            const double v[9] = {s.bscale, s.cscale, s.normQ, s.obj_const, s.bnorm, s.cnorm, ganit::estimate_norm(s),
-                                 (double)s.AT.nnz(), s.maximize ? 1.0 : 0.0};
-            std::copy(v, v + 9, scal);
+        scaled_out(s, scal ? ganit::estimate_norm(s) : 0.0, scal, aptr, aidx, aval, tptr, tidx, tval, qptr, qidx,
+                   qval, c, l, u, lo, hi, R, C, Rres, Cres);
+    });
+}
+
+int ganit_lp_scaled_gpu(const ganit_lp* hh, int ruiz, int pc, int device, double* scal, int* aptr, int* aidx,
+                        double* aval, int* tptr, int* tidx, double* tval, int* qptr, int* qidx, double* qval, double* c,
+                        double* l, double* u, double* lo, double* hi, double* R, double* C, double* Rres,
+                        double* Cres) {
+    return guard([&] {
+        ganit_hip_ctx* ctx = nullptr;
+        if (ganit_hip_ctx_create(device, &ctx) != GANIT_OK)
+            throw std::runtime_error(std::string("HIP backend: ") + ganit_hip_last_error());
+        try {
+            double norm_a = 0;
+            const ganit::ScaledLP s =
+                ganit::scale_lp_device(reinterpret_cast<const Handle*>(hh)->lp, ruiz, pc != 0, ctx, &norm_a);
+            scaled_out(s, norm_a, scal, aptr, aidx, aval, tptr, tidx, tval, qptr, qidx, qval, c, l, u, lo, hi, R, C,
+                       Rres, Cres);
+        } catch (...) {
+            ganit_hip_ctx_destroy(ctx);
+            throw;
         }
-        put(aptr, s.A.ptr); put(aidx, s.A.idx); put(aval, s.A.val);
-        put(tptr, s.AT.ptr); put(tidx, s.AT.idx); put(tval, s.AT.val);
-        put(qptr, s.Q.ptr); put(qidx, s.Q.idx); put(qval, s.Q.val);
-        put(c, s.c); put(l, s.l); put(u, s.u); put(lo, s.lo); put(hi, s.hi);
-        put(R, s.R); put(C, s.C); put(Rres, s.Rres); put(Cres, s.Cres);
+        ganit_hip_ctx_destroy(ctx);
     });
 }
 
@@ -167,9 +199,12 @@ ganit::Options options_from(const double* o) {
     opt.values = vals == 1 ? ganit::Values::BF16 : vals == 2 ? ganit::Values::BF16_F32 : ganit::Values::F32;
     opt.device = (int)o[12];
     opt.eval_fp64 = o[13] != 0;
-    const int bits = (int)o[14];  // bit 0: kahan, bit 1: detect_unbounded
+    const int bits = (int)o[14];  // bit 0: kahan, bit 1: detect_unbounded, bits 2-3: set-up (0 auto, 1 host, 2 GPU)
     opt.kahan = (bits & 1) != 0;
     opt.detect_unbounded = (bits & 2) != 0;
+    const int where = (bits >> 2) & 3;
+    if (where == 3) throw std::runtime_error("set-up placement must be 0 (auto), 1 (host) or 2 (GPU)");
+    opt.setup = where == 1 ? ganit::Setup::Host : where == 2 ? ganit::Setup::Device : ganit::Setup::Auto;
     return opt;
 }
 void result_out(const ganit::Result& r, double* out, double* x, double* y) {
@@ -250,6 +285,16 @@ int ganit_solver_adjoint(ganit_solver* h, const double* gx, const double* ao, do
 
 int ganit_solver_index_bytes(ganit_solver* h, int which, int* bytes) {
     return guard([&] { *bytes = reinterpret_cast<ganit::GpuSolver*>(h)->index_bytes(which); });
+}
+
+int ganit_solver_info(const ganit_solver* h, double* info) {
+    return guard([&] {
+        if (!h || !info) throw std::runtime_error("null argument");
+        auto* s = reinterpret_cast<const ganit::GpuSolver*>(h);
+        info[0] = s->setup_seconds();
+        info[1] = s->norm_a();
+        info[2] = s->setup_on_device() ? 1.0 : 0.0;
+    });
 }
 
 void ganit_solver_free(ganit_solver* h) { delete reinterpret_cast<ganit::GpuSolver*>(h); }

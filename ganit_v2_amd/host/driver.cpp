@@ -38,6 +38,22 @@ void chk(int rc) {
     if (rc != GANIT_OK) throw std::runtime_error(std::string("HIP backend: ") + ganit_hip_last_error());
 }
 
+// Setup::Auto: the GPU set-up pays for its upload from about a million nonzeros on; its kernels give each row
+// and each column to one thread, so a matrix with a row or column longer than kSetupRow stays on the host.
+constexpr int64_t kSetupNnz = 1000000;
+constexpr int kSetupRow = 4096;
+
+bool place_setup_on_device(const LP& lp, Setup where) {
+    if (where != Setup::Auto) return where == Setup::Device;
+    if (lp.A.nnz() < kSetupNnz) return false;
+    for (int i = 0; i < lp.m; ++i)
+        if (lp.A.ptr[i + 1] - lp.A.ptr[i] > kSetupRow) return false;
+    std::vector<int> cols(lp.n, 0);
+    for (const int j : lp.A.idx)
+        if (++cols[j] > kSetupRow) return false;
+    return true;
+}
+
 struct KKT {  // reference pdhg_impl.hpp:36-42
     double pobj = 0, dobj = 0, rel_p = 0, rel_d = 0, rel_gap = 0, err = 0;
     bool finite() const {
@@ -218,6 +234,7 @@ struct GpuSolver::Impl {
     std::string device;
     ScaledLP s;
     double normA = 0, setup_seconds = 0;
+    bool setup_on_device = false;
     bool mixed = false;
     int low_mdt = GANIT_F32;
     std::vector<double> x0, y0;
@@ -243,8 +260,13 @@ GpuSolver::GpuSolver(const LP& lp, const Options& opt) : p_(new Impl(opt)) {
     chk(ganit_hip_ctx_device_info(I.ctx, name, sizeof(name), nullptr));
     I.device = std::string("gpu:") + name;
     const auto t0 = Clock::now();
-    I.s = scale_lp(lp, opt.ruiz_iters, opt.pock_chambolle);
-    I.normA = estimate_norm(I.s);
+    I.setup_on_device = place_setup_on_device(lp, opt.setup);
+    if (I.setup_on_device) {
+        I.s = scale_lp_device(lp, opt.ruiz_iters, opt.pock_chambolle, I.ctx, &I.normA);
+    } else {
+        I.s = scale_lp(lp, opt.ruiz_iters, opt.pock_chambolle);
+        I.normA = estimate_norm(I.s);
+    }
     I.x0.resize(I.s.n);
     for (int j = 0; j < I.s.n; ++j) I.x0[j] = std::min(std::max(0.0, I.s.l[j]), I.s.u[j]);
     I.y0.assign(I.s.m, 0.0);
@@ -259,6 +281,8 @@ GpuSolver::~GpuSolver() = default;
 
 const ScaledLP& GpuSolver::scaled() const { return p_->s; }
 double GpuSolver::norm_a() const { return p_->normA; }
+double GpuSolver::setup_seconds() const { return p_->setup_seconds; }
+bool GpuSolver::setup_on_device() const { return p_->setup_on_device; }
 
 void GpuSolver::upload() {
     Impl& I = *p_;
@@ -303,6 +327,7 @@ Result GpuSolver::run() {
     res.device = I.device;
     res.norm_a = I.normA;
     res.setup_seconds = I.setup_seconds;
+    res.setup_on_device = I.setup_on_device;
     Shared sh{s, opt, 0.998 / I.normA, I.normA, s.cscale * s.bscale, Clock::now()};
     State st;
     st.w = I.w0;

@@ -2,7 +2,8 @@
 // summary line and the exit codes follow the reference's src/main.cpp:41-211 for the PDHG path; keys after
 // "nnz" in the JSON line are additions.
 //   ganit model.mps [--device gpu] [--tol 1e-4] [--precision double|mixed|dynamic] [--time S] [--iters N]
-//                   [--values f32|bf16|bf16f32] [--eval level|fp64] [--kahan] [--detect-unbounded] [--gpu K] [--sol FILE] [--json] [--quiet]
+//                   [--values f32|bf16|bf16f32] [--eval level|fp64] [--kahan] [--detect-unbounded]
+//                   [--setup auto|host|gpu] [--gpu K] [--sol FILE] [--json] [--quiet]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -29,6 +30,8 @@ static void usage() {
         "                     or in FP64 with the FP64 matrix\n"
         "  --kahan            L1 checks: Kahan-compensated FP32 sums (instead of FP64 sums)\n"
         "  --detect-unbounded LPs: also test for a dual-infeasibility (unboundedness) ray\n"
+        "  --setup auto|host|gpu  where scaling and the norm estimate run (same results; auto: the GPU\n"
+        "                     from a million nonzeros)\n"
         "  --time S           time limit in seconds (default 3600)\n"
         "  --iters N          iteration limit\n"
         "  --stall-checks K   checks without progress before leaving L1 (default 12)\n"
@@ -78,6 +81,13 @@ int main(int argc, char** argv) {
         }
         else if (s == "--kahan") opt.kahan = true;
         else if (s == "--detect-unbounded") opt.detect_unbounded = true;
+        else if (s == "--setup") {
+            const std::string v = next();
+            if (v == "auto") opt.setup = Setup::Auto;
+            else if (v == "host") opt.setup = Setup::Host;
+            else if (v == "gpu") opt.setup = Setup::Device;
+            else { std::fprintf(stderr, "--setup must be auto, host or gpu\n"); return 1; }
+        }
         else if (s == "--time") opt.time_limit = std::stod(next());
         else if (s == "--iters") opt.max_iter = std::stol(next());
         else if (s == "--gpu") opt.device = std::stoi(next());

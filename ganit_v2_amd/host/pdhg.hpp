@@ -18,6 +18,10 @@ enum class Precision { Double, Mixed, Dynamic };
 // (extension: the L1 phase of a mixed or dynamic ladder runs as a BF16 phase, left after stall_checks checks
 // without progress, then an FP32 phase, left after stall_checks_l2; both on the FP32 vectors).
 enum class Values { F32, BF16, BF16_F32 };
+// extension: where the matrix half of the set-up runs (scale_lp's Ruiz / Pock-Chambolle passes and the transpose,
+// estimate_norm's SpMVs): on the host, on the GPU (bitwise the same results), or Auto = the GPU for problems of at
+// least a million nonzeros whose rows and columns all have at most 4096 entries (each is swept by one thread).
+enum class Setup { Auto, Host, Device };
 
 struct Options {
     double tol = 1e-4;           // relative KKT tolerance (primal, dual, gap)
@@ -41,6 +45,7 @@ struct Options {
                                   // thread, FP64 across threads (instead of FP64 sums)
     bool detect_unbounded = false;  // LPs: also test the primal ray x - xr as a dual-infeasibility
                                     // (unboundedness) certificate, reported as Status::DualInfeasible
+    Setup setup = Setup::Auto;      // where the set-up's matrix work runs
 };
 
 // DualInfeasible (an extension, only with Options::detect_unbounded) comes last so the reference's codes keep
@@ -63,6 +68,7 @@ struct Result {
     // extensions
     double norm_a = 0;
     double upload_seconds = 0;        // device upload of the scaled problem (inside solve_seconds)
+    bool setup_on_device = false;     // the set-up's matrix work ran on the GPU
     double iterate_seconds[3] = {0};  // wall time in L1 / L2 / L3 iteration blocks + checks
     long checks = 0;
 };
@@ -118,6 +124,8 @@ class GpuSolver {
     void set_solution(const std::vector<double>& x, const std::vector<double>& y, bool fp32_iterate);
     const ScaledLP& scaled() const;
     double norm_a() const;
+    double setup_seconds() const;  // scaling + norm estimate, as Result::setup_seconds reports it
+    bool setup_on_device() const;  // the set-up's matrix work ran on the GPU
 
    private:
     struct Impl;

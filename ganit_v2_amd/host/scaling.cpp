@@ -7,6 +7,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/ganit_hip.h"
 
 namespace ganit {
 
@@ -115,7 +119,6 @@ ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_
     s.R.assign(lp.m, 1.0);
     s.C.assign(lp.n, 1.0);
     s.maximize = lp.maximize;
-    const double sign = lp.maximize ? -1.0 : 1.0;
 
     // Column view of A's pattern: entry q of column j is A entry perm[q] (row trow[q]); rows ascending within
     // a column.  The scaled A^T is read off it at the end.
@@ -165,6 +168,14 @@ ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_
 #pragma omp parallel for schedule(static)
     for (long q = 0; q < (long)perm.size(); ++q) s.AT.val[q] = s.A.val[perm[q]];
 
+    scale_vectors(lp, bound_obj_rescale, s);
+    return s;
+}
+
+// The part of scale_lp after the matrix (reference src/scaling.cpp:72-128): objective, bounds, the uniform
+// bound / objective rescaling, the residual weights, Q and its norm, the norms of the data as given.
+void scale_vectors(const LP& lp, bool bound_obj_rescale, ScaledLP& s) {
+    const double sign = lp.maximize ? -1.0 : 1.0;
     s.c.resize(lp.n);
     s.l.resize(lp.n);
     s.u.resize(lp.n);
@@ -214,6 +225,39 @@ ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_
     }
     s.bnorm = finite_bound_norm(lp.lo, lp.hi);
     s.cnorm = norm2(lp.c);
+}
+
+namespace {
+void hip_chk(int rc) {
+    if (rc != GANIT_OK) throw std::runtime_error(std::string("HIP set-up: ") + ganit_hip_last_error());
+}
+}  // namespace
+
+ScaledLP scale_lp_device(const LP& lp, int ruiz_iters, bool pock_chambolle, ganit_hip_ctx* ctx, double* norm_a,
+                         int power_iters, bool bound_obj_rescale) {
+    ScaledLP s;
+    s.m = lp.m;
+    s.n = lp.n;
+    s.A = lp.A;
+    s.maximize = lp.maximize;
+    s.R.resize(lp.m);
+    s.C.resize(lp.n);
+    s.AT = empty_csr(lp.n, lp.m);
+    s.AT.idx.resize(s.A.val.size());
+    s.AT.val.resize(s.A.val.size());
+    ganit_hip_setup* h = nullptr;
+    hip_chk(ganit_hip_setup_create(ctx, lp.m, lp.n, lp.A.ptr.data(), lp.A.idx.data(), lp.A.val.data(), ruiz_iters,
+                                   pock_chambolle ? 1 : 0, &h));
+    try {
+        hip_chk(ganit_hip_setup_result(h, s.R.data(), s.C.data(), s.A.val.data(), s.AT.ptr.data(), s.AT.idx.data(),
+                                       s.AT.val.data()));
+        if (norm_a) hip_chk(ganit_hip_setup_norm(h, power_iters, norm_a));
+    } catch (...) {
+        ganit_hip_setup_destroy(h);
+        throw;
+    }
+    ganit_hip_setup_destroy(h);
+    scale_vectors(lp, bound_obj_rescale, s);
     return s;
 }
 

@@ -7,6 +7,8 @@
 #pragma once
 #include "lp.hpp"
 
+struct ganit_hip_ctx;  // include/ganit_hip.h
+
 namespace ganit {
 
 // The problem PDHG iterates on.  Field names and layout are the reference's ScaledLP, which its driver reads.
@@ -26,6 +28,13 @@ struct ScaledLP {
 };
 
 ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_obj_rescale = true);
+// scale_lp's second half (everything after the matrix and its factors R, C): c, l, u, lo, hi, the bound and
+// objective rescaling, Rres, Cres, Q and ||Q'||, bnorm, cnorm.
+void scale_vectors(const LP& lp, bool bound_obj_rescale, ScaledLP& s);
+// scale_lp with the matrix half on a GPU (Ruiz / Pock-Chambolle passes and the transpose); with norm_a, also
+// estimate_norm(result, power_iters) with the power iteration's SpMVs on the GPU.  Bitwise the same results.
+ScaledLP scale_lp_device(const LP& lp, int ruiz_iters, bool pock_chambolle, ::ganit_hip_ctx* ctx,
+                         double* norm_a, int power_iters = 50, bool bound_obj_rescale = true);
 // ||A'||_2 by power iteration on A'^T A' (50 steps, x1.01).
 double estimate_norm(const ScaledLP& s, int iters = 50);
 double norm2(const std::vector<double>& v);

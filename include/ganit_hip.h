@@ -196,6 +196,25 @@ int ganit_hip_pdhg_adopt_average(ganit_hip_pdhg* pd, int vdt);
  * any pointer may be NULL.  Synchronises. */
 int ganit_hip_pdhg_store(ganit_hip_pdhg* pd, int vdt, double* x, double* y, double* xr, double* yr);
 
+/* ------------------------------------------------------------------ problem set-up on the device
+ * The matrix half of the reference's scale_lp (src/scaling.cpp:36-70: ruiz_iters inf-norm Ruiz passes, then the
+ * Pock-Chambolle 1-norm pass when pock_chambolle != 0), the stable transpose of the pattern (src/lp.cpp:55-72)
+ * and the SpMVs of estimate_norm's power iteration (src/scaling.cpp:131-155), on A (CSR, FP64, unscaled)
+ * uploaded once.  Every result is bitwise the reference's host arithmetic (maxima exact; row and column 1-norms
+ * summed by one thread in the reference's order; the same factor products; sequential row sums without FMA). */
+typedef struct ganit_hip_setup ganit_hip_setup;
+int ganit_hip_setup_create(ganit_hip_ctx* ctx, int m, int n, const int* a_ptr, const int* a_idx, const double* a_val,
+                           int ruiz_iters, int pock_chambolle, ganit_hip_setup** out);
+/* R (m) and C (n): the accumulated row / column factors; a_val (nnz): the scaled values of A in A's order;
+ * at_ptr (n + 1), at_idx, at_val (nnz): the scaled transpose.  Any pointer may be NULL.  Synchronises. */
+int ganit_hip_setup_result(const ganit_hip_setup* s, double* R, double* C, double* a_val, int* at_ptr, int* at_idx,
+                           double* at_val);
+/* estimate_norm of the scaled matrix (src/scaling.cpp:131-155): `iters` power-iteration steps on A'^T A' from the
+ * normalised all-ones vector, the SpMVs on the device and each step's ||A'^T A' v|| summed on the host in the
+ * reference's serial order; *norm = sqrt(lambda) * 1.01 (1 for an empty matrix).  Synchronises. */
+int ganit_hip_setup_norm(ganit_hip_setup* s, int iters, double* norm);
+int ganit_hip_setup_destroy(ganit_hip_setup* s);
+
 /* ------------------------------------------------------------------ batched PDHG (many uneven LPs / QPs)
  * No reference counterpart: the reference solves one LP per call and its branch-and-cut uses only the
  * dual simplex (src/mip.cpp:116, 312-323).  The instances (scaled LPs or convex QPs) are packed

@@ -52,6 +52,12 @@ int ganit_lp_write_mps(const ganit_lp* lp, const char* path);
 int ganit_lp_scaled(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, double* scal, int* aptr, int* aidx,
                     double* aval, int* tptr, int* tidx, double* tval, int* qptr, int* qidx, double* qval, double* c,
                     double* l, double* u, double* lo, double* hi, double* R, double* C, double* Rres, double* Cres);
+/* The same with the matrix half on GPU `device` (Ruiz / Pock-Chambolle passes, transpose, the power iteration's
+ * SpMVs; no reference counterpart -- the reference's set-up is host code).  Bitwise the same outputs. */
+int ganit_lp_scaled_gpu(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, int device, double* scal, int* aptr,
+                        int* aidx, double* aval, int* tptr, int* tidx, double* tval, int* qptr, int* qidx,
+                        double* qval, double* c, double* l, double* u, double* lo, double* hi, double* R, double* C,
+                        double* Rres, double* Cres);
 
 /* solve_pdhg_gpu (reference include/ganit/pdhg.hpp:45-47, src/backend_cuda.cu:382-393).
  * opts[0..14] = tol, max_iter, time_limit, check_every, ruiz_iters, pock_chambolle, verbose, infeas_tol,
@@ -62,7 +68,10 @@ int ganit_lp_scaled(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, doub
  *               eval_fp64 (1: L1/L2 checks evaluated in FP64 with the FP64 matrix),
  *               flags: bit 0 kahan (L1's FP32 checks sum FP32-rounded terms with Kahan compensation),
  *                      bit 1 detect_unbounded (LPs: test the primal ray x - xr as a dual-infeasibility
- *                      certificate; ignored by ganit_batch_*)
+ *                      certificate; ignored by ganit_batch_*),
+ *                      bits 2-3 where the set-up's matrix work runs: 0 automatic (the GPU from a million nonzeros,
+ *                      when no row or column has more than 4096 entries), 1 host, 2 GPU -- bitwise the same
+ *                      scaled problem and norm estimate either way (ignored by ganit_batch_*)
  * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical,
  *                       5 dual infeasible -- only with detect_unbounded),
  *               pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,
@@ -97,6 +106,9 @@ int ganit_solver_adjoint(ganit_solver* s, const double* gx, const double* aopts,
  * ganit_solver_run had ended there, so that ganit_solver_adjoint differentiates at it; fp32_iterate = 1 when
  * the point came out of an FP32-vector level (needs a mixed or dynamic solver).  No reference counterpart. */
 int ganit_solver_set_solution(ganit_solver* s, const double* x, const double* y, int fp32_iterate);
+/* info[0..2] = set-up seconds (scaling + norm estimate), ||A'|| estimate, 1 if the set-up's matrix work ran on the
+ * GPU (no reference counterpart). */
+int ganit_solver_info(const ganit_solver* s, double* info);
 void ganit_solver_free(ganit_solver* s);
 
 /* Many LPs and convex QPs solved together by the block-diagonal batched kernels (no reference counterpart:
