@@ -260,8 +260,9 @@ SETUP = {"auto": 0, "host": 1, "gpu": 2}
 
 def _options(tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10, pock_chambolle=True,
              verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12, stall_checks_l2=24, values="f32",
-             device=0, eval_fp64=False, kahan=False, detect_unbounded=False, setup="auto") -> np.ndarray:
-    flags = int(bool(kahan)) | (2 if detect_unbounded else 0) | (SETUP[setup] << 2)
+             device=0, eval_fp64=False, kahan=False, detect_unbounded=False, setup="auto",
+             polish=False) -> np.ndarray:
+    flags = int(bool(kahan)) | (2 if detect_unbounded else 0) | (SETUP[setup] << 2) | (16 if polish else 0)
     return np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
                      infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device,
                      int(bool(eval_fp64)), flags], dtype=np.float64)
@@ -332,7 +333,7 @@ class Solver:
         _check_host(host_lib().ganit_solver_set_solution(self._h, x, y, int(bool(fp32_iterate))), "set_solution")
 
     def _info(self):
-        info = np.zeros(3)
+        info = np.zeros(4)
         _check_host(host_lib().ganit_solver_info(self._h, info), "solver_info")
         return info
 
@@ -343,6 +344,10 @@ class Solver:
     def setup_on_device(self) -> bool:
         """Whether the set-up's matrix work ran on the GPU."""
         return bool(self._info()[2])
+
+    def polished(self) -> bool:
+        """Whether the last run() kept a polished point (option polish=True)."""
+        return bool(self._info()[3])
 
     def index_bytes(self, which: int) -> int:
         """Bytes per column index the fused kernels read for A (0), A^T (1), Q (2): 2 (compact) or 4."""

@@ -25,7 +25,7 @@ LIB = os.path.join(PKG, "lib")
 BIN = os.path.join(PKG, "bin")
 OBJ = os.path.join(ROOT, "_build")
 
-HIP_SOURCES = ["runtime.hip", "pdhg_device.hip", "batch.hip", "adjoint.hip", "setup.hip"]
+HIP_SOURCES = ["runtime.hip", "pdhg_device.hip", "batch.hip", "adjoint.hip", "setup.hip", "polish.hip"]
 HOST_SOURCES = ["lp.cpp", "scaling.cpp", "generators.cpp", "driver.cpp", "batch.cpp", "capi.cpp"]
 
 ARCH = os.environ.get("GANIT_OFFLOAD_ARCH", "gfx950")

@@ -104,9 +104,53 @@ KKT kkt_from(const Shared& sh, double rd2, double vobj, double cx, double xqx, d
 KKT kkt_current(const Shared& sh, const double* o) { return kkt_from(sh, o[0], o[1], o[2], o[3], o[12], o[13]); }
 KKT kkt_average(const Shared& sh, const double* o) { return kkt_from(sh, o[4], o[5], o[6], o[7], o[14], o[15]); }
 
+KKT evaluate(Shared& sh, ganit_hip_pdhg* pd, int mdt, int vdt, int flags = 0);
+
+// Options::polish (extension; the reference lists crossover / polish as planned, README.md:195): project the FP64
+// iterate of an LP onto the face named by its active set (ganit_hip_pdhg_polish), for a few activity tolerances
+// from loose to tight, and keep the first result whose FP64 KKT errors are all <= polish_tol and whose variables
+// lie within their bounds to that tolerance; otherwise the PDHG point stays.  (xs, ys, k) are the point in and
+// out; returns whether a polished point was kept.
+bool polish(Shared& sh, ganit_hip_pdhg* pd, std::vector<double>& xs, std::vector<double>& ys, KKT& k) {
+    const ScaledLP& s = sh.s;
+    const double ptol = sh.opt.polish_tol;
+    std::vector<double> xp(s.n), yp(s.m);
+    for (const double act_tol : {1e-6, 1e-5, 1e-7, 1e-4, 1e-8}) {
+        // up to three rounds: each re-reads the active set from the previous round's point while that improves
+        double last_err = k.err;
+        for (int round = 0; round < 3; ++round) {
+            const double params[3] = {act_tol, 1e-14, 20000};
+            double info[6];
+            chk(ganit_hip_pdhg_polish(pd, params, info));
+            const KKT kp = evaluate(sh, pd, GANIT_F64, GANIT_F64, 0);
+            chk(ganit_hip_pdhg_store(pd, GANIT_F64, xp.data(), yp.data(), nullptr, nullptr));
+            double viol = 0;  // bound violation of the polished x, relative like the activity test
+            for (int j = 0; j < s.n; ++j) {
+                if (std::isfinite(s.l[j])) viol = std::max(viol, (s.l[j] - xp[j]) / (1.0 + std::fabs(s.l[j])));
+                if (std::isfinite(s.u[j])) viol = std::max(viol, (xp[j] - s.u[j]) / (1.0 + std::fabs(s.u[j])));
+            }
+            if (sh.opt.verbose)
+                std::printf("  polish at activity tolerance %.0e, round %d: %d free, %d active rows, LSQR %d / %d "
+                            "products, rel p %.2e d %.2e gap %.2e, bound violation %.2e\n",
+                            act_tol, round, (int)info[0], (int)info[1], (int)info[2], (int)info[4], kp.rel_p,
+                            kp.rel_d, kp.rel_gap, viol);
+            if (kp.finite() && kp.converged(ptol) && viol <= ptol) {
+                xs = xp;
+                ys = yp;
+                k = kp;
+                return true;
+            }
+            if (!kp.finite() || viol > ptol || !(kp.err < last_err)) break;
+            last_err = kp.err;
+        }
+        chk(ganit_hip_pdhg_load(pd, GANIT_F64, xs.data(), ys.data(), nullptr, nullptr));  // back to the PDHG point
+    }
+    return false;
+}
+
 // KKT of the current iterate of working set `vdt` evaluated with matrix values `mdt` (extra check flags,
 // e.g. GANIT_CHECK_FP64, in `flags`).
-KKT evaluate(Shared& sh, ganit_hip_pdhg* pd, int mdt, int vdt, int flags = 0) {
+KKT evaluate(Shared& sh, ganit_hip_pdhg* pd, int mdt, int vdt, int flags) {
     double o[GANIT_CHECK_NOUT];
     chk(ganit_hip_pdhg_check(pd, mdt, vdt, flags, 0.0, o));
     return kkt_current(sh, o);
@@ -235,6 +279,7 @@ struct GpuSolver::Impl {
     ScaledLP s;
     double normA = 0, setup_seconds = 0;
     bool setup_on_device = false;
+    bool polished = false;
     bool mixed = false;
     int low_mdt = GANIT_F32;
     std::vector<double> x0, y0;
@@ -283,6 +328,7 @@ const ScaledLP& GpuSolver::scaled() const { return p_->s; }
 double GpuSolver::norm_a() const { return p_->normA; }
 double GpuSolver::setup_seconds() const { return p_->setup_seconds; }
 bool GpuSolver::setup_on_device() const { return p_->setup_on_device; }
+bool GpuSolver::last_polished() const { return p_->polished; }
 
 void GpuSolver::upload() {
     Impl& I = *p_;
@@ -413,14 +459,23 @@ Result GpuSolver::run() {
     } else {
         ex = timed_phase(2, L3, false, 0);
     }
-    const KKT fin = fp64_eval();  // final FP64 certification (pdhg_impl.hpp:300-303)
+    KKT fin = fp64_eval();  // final FP64 certification (pdhg_impl.hpp:300-303)
     Status status = status_from(ex);
     if (status == Status::Optimal && !fin.converged(opt.tol * 1.0000001)) status = Status::NumericalError;
 
     std::vector<double> xs(s.n), ys(s.m);
     chk(ganit_hip_pdhg_store(pd, GANIT_F64, xs.data(), ys.data(), nullptr, nullptr));
+    if (opt.polish && status == Status::Optimal && !s.has_q()) {
+        const double t0 = sh.elapsed();
+        res.polished = polish(sh, pd, xs, ys, fin);
+        res.polish_seconds = sh.elapsed() - t0;
+        if (opt.verbose)
+            std::printf("  polish: %s, FP64-checked error %.2e (%.3fs)\n", res.polished ? "vertex kept" : "no vertex",
+                        fin.err, res.polish_seconds);
+    }
     res.solve_seconds = sh.elapsed();
     I.solved = true;
+    I.polished = res.polished;
     I.final_fp32 = solution_vdt == GANIT_F32;
     I.w_final = st.w;
     I.xs_final = xs;

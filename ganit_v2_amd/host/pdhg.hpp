@@ -46,6 +46,10 @@ struct Options {
     bool detect_unbounded = false;  // LPs: also test the primal ray x - xr as a dual-infeasibility
                                     // (unboundedness) certificate, reported as Status::DualInfeasible
     Setup setup = Setup::Auto;      // where the set-up's matrix work runs
+    bool polish = false;            // LPs: after an optimal solve, project the point onto the face its active set
+                                    // names (crossover-style polish, on the GPU) and keep it when its FP64 KKT
+                                    // errors are all <= polish_tol
+    double polish_tol = 1e-9;
 };
 
 // DualInfeasible (an extension, only with Options::detect_unbounded) comes last so the reference's codes keep
@@ -69,6 +73,8 @@ struct Result {
     double norm_a = 0;
     double upload_seconds = 0;        // device upload of the scaled problem (inside solve_seconds)
     bool setup_on_device = false;     // the set-up's matrix work ran on the GPU
+    bool polished = false;            // Options::polish replaced the PDHG point by its polished vertex
+    double polish_seconds = 0;        // inside solve_seconds
     double iterate_seconds[3] = {0};  // wall time in L1 / L2 / L3 iteration blocks + checks
     long checks = 0;
 };
@@ -126,6 +132,7 @@ class GpuSolver {
     double norm_a() const;
     double setup_seconds() const;  // scaling + norm estimate, as Result::setup_seconds reports it
     bool setup_on_device() const;  // the set-up's matrix work ran on the GPU
+    bool last_polished() const;    // the last run kept a polished point (Options::polish)
 
    private:
     struct Impl;

@@ -274,6 +274,17 @@ int ganit_hip_batch_store(ganit_hip_batch* bt, int vdt, double* x, double* y);
 int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* pd, int mdt, int vdt, const double* gx, double tau, double sigma,
                            const double* params, double* p, double* mu, double* info);
 
+/* Polish an LP's FP64 iterate (no reference counterpart; the reference lists crossover / polish as planned,
+ * README.md:195).  The active set is read off the iterate with tolerance params[0] (a variable within it of a
+ * bound is fixed there; a row is active when it is an equality, its dual is not negligible, or its activity is
+ * that close to a bound), and two least-squares projections are solved by conjugate gradients (relative residual
+ * params[1], at most params[2] products each): x onto the face {A_T x = b_T, fixed variables at their bounds},
+ * y onto {free variables' reduced costs = 0, y = 0 off the active rows}.  The result replaces the FP64 iterate
+ * (and restart point); judge it with a check (its FP64 KKT errors) and reload the old point to reject it.
+ * info[0..5] = free variables, active rows, primal CG products, primal CG relative residual, dual CG products,
+ * dual CG relative residual.  LPs only. */
+int ganit_hip_pdhg_polish(ganit_hip_pdhg* pd, const double* params, double* info);
+
 /* Device time in milliseconds of the fused iteration kernels, measured with HIP events on the context stream from
  * the current iterate: ms[0] = primal kernel (A^T y [+ Q x] + primal prox) and ms[1] = dual kernel (A xbar +
  * dual prox), each averaged over `reps` back-to-back launches of that kernel alone (its duration plus the

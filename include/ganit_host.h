@@ -71,7 +71,9 @@ int ganit_lp_scaled_gpu(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, 
  *                      certificate; ignored by ganit_batch_*),
  *                      bits 2-3 where the set-up's matrix work runs: 0 automatic (the GPU from a million nonzeros,
  *                      when no row or column has more than 4096 entries), 1 host, 2 GPU -- bitwise the same
- *                      scaled problem and norm estimate either way (ignored by ganit_batch_*)
+ *                      scaled problem and norm estimate either way (ignored by ganit_batch_*),
+ *                      bit 4 polish (LPs: project an optimal point onto its active face on the GPU and keep it
+ *                      when its FP64 KKT errors are all <= 1e-9; ignored by ganit_batch_*)
  * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical,
  *                       5 dual infeasible -- only with detect_unbounded),
  *               pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,
@@ -106,8 +108,8 @@ int ganit_solver_adjoint(ganit_solver* s, const double* gx, const double* aopts,
  * ganit_solver_run had ended there, so that ganit_solver_adjoint differentiates at it; fp32_iterate = 1 when
  * the point came out of an FP32-vector level (needs a mixed or dynamic solver).  No reference counterpart. */
 int ganit_solver_set_solution(ganit_solver* s, const double* x, const double* y, int fp32_iterate);
-/* info[0..2] = set-up seconds (scaling + norm estimate), ||A'|| estimate, 1 if the set-up's matrix work ran on the
- * GPU (no reference counterpart). */
+/* info[0..3] = set-up seconds (scaling + norm estimate), ||A'|| estimate, 1 if the set-up's matrix work ran on the
+ * GPU, 1 if the last run kept a polished point (no reference counterpart). */
 int ganit_solver_info(const ganit_solver* s, double* info);
 void ganit_solver_free(ganit_solver* s);
 
