@@ -216,32 +216,32 @@ def batched_lps(count=512):
     return out
 
 
-def diff_qps(count, threads=32, batched=True):
+def diff_qps(count, threads=32, batched=True, scale_q=True, cd_h=1e-4):
     """Differentiable QPs (n=2000, 300 equality + 200 <= rows): the FP32 path -- forward (mixed ladder, tol 1e-6)
-    + FP32 adjoint -- timed as one batch, then the FP64 check -- forward (tol 1e-9) + FP64 adjoint -- and each
-    instance's FP32 gradient error against it.  batched: the forwards of a phase run as one batched solve
-    (ganit_v2_amd.Batch) and each instance's solver takes its solution over for the adjoint; else every
-    instance runs forward and adjoint in its own host thread (one solver, HIP context and stream each; these
-    solves are launch-bound, so the GPU overlaps them).  The adjoints run concurrently either way."""
+    + FP32 adjoint -- timed as one batch, then the FP64 path -- forward (tol 1e-9) + FP64 adjoint -- and each
+    instance's FP32 gradient error against it; then each instance's FP64 gradient against FP64 central differences
+    of the loss along the gradient's own unit direction in c, in b and in diag Q (step cd_h, the 6 perturbed QPs of
+    every instance solved as one batch at tol 1e-11).  batched: the forwards of a phase run as one batched solve (ganit_v2_amd.Batch)
+    and each instance's solver takes its solution over for the adjoint; else every instance runs forward and adjoint
+    in its own host thread.  scale_q: the column scaling also equilibrates Q (the QP layer's default)."""
     from concurrent.futures import ThreadPoolExecutor
     import numpy as np
     import ganit_v2_amd as g
+    from ganit_v2_amd.autograd import QPLayer
     probs = [g.Problem.generate("diff_qp", seed=seed) for seed in range(count)]
     ws = [np.random.default_rng(1000 + seed).normal(size=p.dims()["n"]) for seed, p in enumerate(probs)]
 
     def path(args, precision, tol, adj_precision, adj_tol):
         p, w = args
-        s = p.solver(precision=precision, tol=tol, time_limit=300)
+        s = p.solver(precision=precision, tol=tol, time_limit=300, scale_q=scale_q)
         s.upload()
         r = s.run(return_solution=True)
         a = s.adjoint(w, precision=adj_precision, tol=adj_tol) if r["status"] == "OPTIMAL" else None
-        print(f"[bench] diff_qp {precision}: {r['status']} {r['iters']} iterations, forward {r['solve_s']:.2f} s"
-              + (f", adjoint {a['seconds']:.2f} s" if a else ""), file=sys.stderr, flush=True)
         return r, a
 
     def batch_path(precision, tol, adj_precision, adj_tol):
         # the batch's clock is shared by its instances; 150 s keeps each phase's silent stretch short
-        res = g.Batch(probs, precision=precision, tol=tol, time_limit=150).run(return_solution=True)
+        res = g.Batch(probs, precision=precision, tol=tol, time_limit=150, scale_q=scale_q).run(return_solution=True)
         n_opt = sum(r["status"] == "OPTIMAL" for r in res)
         print(f"[bench] diff_qp {precision} batch: {n_opt}/{count} OPTIMAL, max {max(r['iters'] for r in res)} "
               f"iterations, forward {res[0]['solve_s']:.2f} s", file=sys.stderr, flush=True)
@@ -250,12 +250,10 @@ def diff_qps(count, threads=32, batched=True):
             r = res[k]
             if r["status"] != "OPTIMAL":
                 return r, None
-            s = probs[k].solver(precision=precision, tol=tol)
+            s = probs[k].solver(precision=precision, tol=tol, scale_q=scale_q)
             s.upload()
             s.set_solution(r["x"], r["y"], fp32_iterate=precision != "double" and r["fp32_iters"] == r["iters"])
-            a = s.adjoint(ws[k], precision=adj_precision, tol=adj_tol)
-            print(f"[bench] diff_qp {precision} adjoint {k}: {a['seconds']:.2f} s", file=sys.stderr, flush=True)
-            return r, a
+            return r, s.adjoint(ws[k], precision=adj_precision, tol=adj_tol)
 
         with ThreadPoolExecutor(nt) as ex:
             return list(ex.map(adjoint, range(count)))
@@ -275,27 +273,64 @@ def diff_qps(count, threads=32, batched=True):
         with ThreadPoolExecutor(nt) as ex:
             f64 = list(ex.map(lambda a: path(a, "double", 1e-9, "double", 1e-10), zip(probs, ws)))
     wall64 = time.perf_counter() - t
+    # central differences of L = w'x* along the FP64 gradient's own direction in c, in b and in diag Q (unit
+    # vectors; the largest change of L, so the difference quotient is not lost in the solves' tolerance), all
+    # 6 count perturbed QPs solved at once
+    t = time.perf_counter()
+    layers = [QPLayer.from_problem(p) for p in probs]
+    cd_probs, dirs = [], []
+    for k, (p, lay) in enumerate(zip(probs, layers)):
+        c, b, q = (v.numpy() for v in lay.data_of(p))
+        a64 = f64[k][1]
+        rng = np.random.default_rng(2000 + k)
+        unit = lambda v: v / max(np.linalg.norm(v), 1e-300)
+        dc, db, dq = ((unit(a64[key]) if a64 is not None and np.linalg.norm(a64[key]) > 0 else unit(rng.normal(size=n)))
+                      for key, n in (("dc", c.size), ("db", b.size), ("dq", q.size)))
+        dirs.append((dc, db, dq))
+        for sgn in (1.0, -1.0):
+            cd_probs += [lay.problem(c + sgn * cd_h * dc, b, q), lay.problem(c, b + sgn * cd_h * db, q),
+                         lay.problem(c, b, q + sgn * cd_h * dq)]
+    cd_res = g.Batch(cd_probs, precision="double", tol=1e-11, time_limit=300, scale_q=scale_q).run(
+        return_solution=True)
+    wall_cd = time.perf_counter() - t
     rel = lambda u, v: float(np.linalg.norm(u - v) / max(np.linalg.norm(v), 1e-300))
     rows = []
     for seed, ((r32, a32), (r64, a64)) in enumerate(zip(f32, f64)):
-        row = dict(seed=seed, fwd32_status=r32["status"], fwd32_s=round(r32["solve_s"], 3), fwd32_iters=r32["iters"],
-                   fwd64_status=r64["status"], fwd64_s=round(r64["solve_s"], 3))
+        row = dict(seed=seed, fwd32_status=r32["status"], fwd32_iters=r32["iters"], fwd64_status=r64["status"],
+                   fwd64_iters=r64["iters"])
         if a32 is not None and a64 is not None:
             row.update(adj32_s=round(a32["seconds"], 3), adj64_s=round(a64["seconds"], 3), adj64_iters=a64["iters"],
                        grad_err_fp32={k: rel(a32[k], a64[k]) for k in ("dc", "db", "dq")})
+        if a64 is not None:
+            plus, minus = cd_res[6 * seed: 6 * seed + 3], cd_res[6 * seed + 3: 6 * seed + 6]
+            cd = {}
+            for j, key in enumerate(("dc", "db", "dq")):
+                if plus[j]["status"] == minus[j]["status"] == "OPTIMAL":
+                    fd = (ws[seed] @ plus[j]["x"] - ws[seed] @ minus[j]["x"]) / (2 * cd_h)
+                    ad = float(a64[key] @ dirs[seed][j])
+                    cd[key] = float(abs(ad - fd) / max(abs(fd), 1e-12))
+            row["grad_err_fp64_vs_central_differences"] = cd
         rows.append(row)
-        print(f"[bench] diff_qp {row}", file=sys.stderr, flush=True)
     errs = [max(r["grad_err_fp32"].values()) for r in rows if "grad_err_fp32" in r]
-    return dict(workload=f"{count} differentiable QPs n=2000, 300 E + 200 L rows, seeds 0..{count - 1}",
-                forward="one batched solve per phase" if batched else "concurrent single solves",
-                threads=nt, fp32_batch_wall_s=round(wall32, 2), fp64_check_wall_s=round(wall64, 2),
-                # PDHG iterations of all instances per second of the FP32 phase (the GPU overlaps the solves;
-                # each instance's own solve_s and time limit run on the wall clock while it shares the GPU)
-                fp32_aggregate_iters_per_s=round(sum(r32["iters"] for r32, _ in f32) / wall32),
-                optimal_fp32=sum(r["fwd32_status"] == "OPTIMAL" for r in rows),
-                optimal_fp64=sum(r["fwd64_status"] == "OPTIMAL" for r in rows),
-                grad_err_fp32_max=max(errs) if errs else None,
-                grad_err_fp32_median=float(np.median(errs)) if errs else None, rows=rows)
+    cds = [max(r["grad_err_fp64_vs_central_differences"].values()) for r in rows
+           if r.get("grad_err_fp64_vs_central_differences")]
+    out = dict(workload=f"{count} differentiable QPs n=2000, 300 E + 200 L rows, seeds 0..{count - 1}",
+               forward="one batched solve per phase" if batched else "concurrent single solves",
+               scale_q=scale_q, threads=nt, fp32_batch_wall_s=round(wall32, 3), fp64_check_wall_s=round(wall64, 3),
+               central_difference_wall_s=round(wall_cd, 3), cd_step=cd_h,
+               optimal_fp32=sum(r["fwd32_status"] == "OPTIMAL" for r in rows),
+               optimal_fp64=sum(r["fwd64_status"] == "OPTIMAL" for r in rows),
+               fwd32_iters_max=max(r["fwd32_iters"] for r in rows), fwd64_iters_max=max(r["fwd64_iters"] for r in rows),
+               grad_err_fp32_vs_fp64_max=max(errs) if errs else None,
+               grad_err_fp32_vs_fp64_median=float(np.median(errs)) if errs else None,
+               grad_err_fp64_vs_cd_max=max(cds) if cds else None,
+               grad_err_fp64_vs_cd_median=float(np.median(cds)) if cds else None,
+               cd_within_1e6=int(sum(c <= 1e-6 for c in cds)),
+               cd_worst=(max(((v, r["seed"], k) for r in rows
+                              for k, v in r.get("grad_err_fp64_vs_central_differences", {}).items()), default=None)),
+               rows=rows)
+    print(f"[bench] diff_qp {json.dumps({k: v for k, v in out.items() if k != 'rows'})}", file=sys.stderr, flush=True)
+    return out
 
 
 def reference_cli(path, threads, *args, timeout=600):
@@ -346,8 +381,8 @@ def random_lp_c0(threads):
 
 def large_blending_kernels(periods, reps):
     """The fused iteration kernels on a blending LP large enough that an iteration's streams cannot stay in the
-    256 MB Infinity Cache (60 crudes x 16 products x 5 specs x `periods` periods): HIP-event times per launch in
-    the order a solve runs them, at L1 and L3, and their algorithmic GB/s."""
+    256 MB Infinity Cache (60 crudes x 16 products x 5 specs x `periods` periods): HIP-event times of each fused
+    kernel (back-to-back launches) and of whole iterations, at L1 and L3, and their algorithmic GB/s."""
     import ganit_v2_amd as g
     prob = g.Problem.generate("blending", seed=0, crudes=60, products=16, specs=5, periods=periods)
     d = prob.dims()
@@ -413,8 +448,8 @@ def main():
                     help="evaluate L1 restart candidates in FP64 (default) or at L1 precision")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the portfolio-QP ladder and the batched-LP comparison reported beside the headline")
-    ap.add_argument("--c4", type=int, default=0,
-                    help="also run N differentiable QPs (forward + adjoint) and report their times and errors")
+    ap.add_argument("--c4", type=int, default=64,
+                    help="differentiable QPs (forward + adjoint) reported in extras.diff_qp (0: skip)")
     ap.add_argument("--c4-concurrent", action="store_true",
                     help="with --c4: forwards as concurrent single solves instead of one batched solve per phase")
     ap.add_argument("--periods", type=int, default=WORKLOAD["periods"],

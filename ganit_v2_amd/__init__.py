@@ -261,8 +261,9 @@ SETUP = {"auto": 0, "host": 1, "gpu": 2}
 def _options(tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10, pock_chambolle=True,
              verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12, stall_checks_l2=24, values="f32",
              device=0, eval_fp64=False, kahan=False, detect_unbounded=False, setup="auto",
-             polish=False) -> np.ndarray:
-    flags = int(bool(kahan)) | (2 if detect_unbounded else 0) | (SETUP[setup] << 2) | (16 if polish else 0)
+             polish=False, scale_q=False) -> np.ndarray:
+    flags = (int(bool(kahan)) | (2 if detect_unbounded else 0) | (SETUP[setup] << 2) | (16 if polish else 0)
+             | (32 if scale_q else 0))
     return np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
                      infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device,
                      int(bool(eval_fp64)), flags], dtype=np.float64)

@@ -37,7 +37,7 @@ class QPLayer:
 
     def __init__(self, n, A_rows, A_cols, A_vals, senses, l, u, Q_rows=(), Q_cols=(), Q_vals=(),
                  precision="mixed", tol=1e-6, adjoint_precision="double", adjoint_tol=1e-8,
-                 adjoint_max_iter=200_000, batch_forward=True, **solve_opts):
+                 adjoint_max_iter=200_000, batch_forward=True, scale_q=True, **solve_opts):
         self.n = int(n)
         self.senses = np.asarray(senses, dtype=np.int8)
         self.m = len(self.senses)
@@ -47,7 +47,9 @@ class QPLayer:
         self.Q_off = (qr[off], qc[off], qv[off])
         self.l = np.asarray(l, np.float64)
         self.u = np.asarray(u, np.float64)
-        self.opts = dict(precision=precision, tol=tol, **solve_opts)
+        # scale_q: the column scaling also equilibrates Q ([Q A'; A 0]); on these QPs it takes the forward solve from
+        # 1e5-1e7 PDHG iterations to a few hundred (the same optimum; the gradient is the solution map's either way)
+        self.opts = dict(precision=precision, tol=tol, scale_q=scale_q, **solve_opts)
         self.adjoint_opts = dict(precision=adjoint_precision, tol=adjoint_tol, max_iter=adjoint_max_iter)
         # a (B, n) input: its B forward solves as one batched solve (else as concurrent single solves)
         self.batch_forward = bool(batch_forward)

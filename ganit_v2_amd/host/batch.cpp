@@ -88,7 +88,7 @@ BatchSolver::BatchSolver(const std::vector<const LP*>& lps, const Options& opt) 
     // independent set-ups, one instance per thread (the per-instance arithmetic is the single solve's)
 #pragma omp parallel for schedule(dynamic, 1)
     for (int k = 0; k < K; ++k) {
-        I.s[k] = scale_lp(*lps[k], opt.ruiz_iters, opt.pock_chambolle);
+        I.s[k] = scale_lp(*lps[k], opt.ruiz_iters, opt.pock_chambolle, true, opt.scale_q);
         I.norm_a[k] = estimate_norm(I.s[k]);
         const double cn = norm2(I.s[k].c), bn = finite_bound_norm(I.s[k].lo, I.s[k].hi);
         I.w0[k] = (cn > 1e-10 && bn > 1e-10) ? cn / bn : 1.0;

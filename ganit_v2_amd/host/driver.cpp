@@ -305,11 +305,11 @@ GpuSolver::GpuSolver(const LP& lp, const Options& opt) : p_(new Impl(opt)) {
     chk(ganit_hip_ctx_device_info(I.ctx, name, sizeof(name), nullptr));
     I.device = std::string("gpu:") + name;
     const auto t0 = Clock::now();
-    I.setup_on_device = place_setup_on_device(lp, opt.setup);
+    I.setup_on_device = !(opt.scale_q && lp.has_q()) && place_setup_on_device(lp, opt.setup);
     if (I.setup_on_device) {
         I.s = scale_lp_device(lp, opt.ruiz_iters, opt.pock_chambolle, I.ctx, &I.normA);
     } else {
-        I.s = scale_lp(lp, opt.ruiz_iters, opt.pock_chambolle);
+        I.s = scale_lp(lp, opt.ruiz_iters, opt.pock_chambolle, true, opt.scale_q);
         I.normA = estimate_norm(I.s);
     }
     I.x0.resize(I.s.n);

@@ -3,7 +3,7 @@
 // "nnz" in the JSON line are additions.
 //   ganit model.mps [--device gpu] [--tol 1e-4] [--precision double|mixed|dynamic] [--time S] [--iters N]
 //                   [--values f32|bf16|bf16f32] [--eval level|fp64] [--kahan] [--detect-unbounded]
-//                   [--setup auto|host|gpu] [--polish] [--gpu K] [--sol FILE] [--json] [--quiet]
+//                   [--setup auto|host|gpu] [--polish] [--scale-q] [--gpu K] [--sol FILE] [--json] [--quiet]
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -32,6 +32,7 @@ static void usage() {
         "  --detect-unbounded LPs: also test for a dual-infeasibility (unboundedness) ray\n"
         "  --polish           LPs: project an optimal PDHG point onto its active face and keep it when its\n"
         "                     FP64 KKT errors are all <= 1e-9 (a vertex-accurate objective)\n"
+        "  --scale-q          QPs: equilibrate Q together with A in the column scaling\n"
         "  --setup auto|host|gpu  where scaling and the norm estimate run (same results; auto: the GPU\n"
         "                     from a million nonzeros)\n"
         "  --time S           time limit in seconds (default 3600)\n"
@@ -84,6 +85,7 @@ int main(int argc, char** argv) {
         else if (s == "--kahan") opt.kahan = true;
         else if (s == "--detect-unbounded") opt.detect_unbounded = true;
         else if (s == "--polish") opt.polish = true;
+        else if (s == "--scale-q") opt.scale_q = true;
         else if (s == "--setup") {
             const std::string v = next();
             if (v == "auto") opt.setup = Setup::Auto;

@@ -50,6 +50,8 @@ struct Options {
                                     // names (crossover-style polish, on the GPU) and keep it when its FP64 KKT
                                     // errors are all <= polish_tol
     double polish_tol = 1e-9;
+    bool scale_q = false;           // QPs: column scaling also equilibrates Q (scale_lp's q_columns); changes the
+                                    // trajectory, so it is off unless asked for
 };
 
 // DualInfeasible (an extension, only with Options::detect_unbounded) comes last so the reference's codes keep

@@ -99,6 +99,20 @@ void column_view(const Csr& a, std::vector<int>& cptr, std::vector<int>& perm, s
     }
 }
 
+// Fold the columns of the current C Q C into the column norms cs: the max (sum = false) or the sum (sum = true)
+// of |C_j Q_jk C_k| over row j of the symmetric Q, combined with A's column norm by max, resp. addition.
+void q_column_norms(const Csr& Q, const std::vector<double>& C, bool sum, std::vector<double>& cs) {
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int j = 0; j < Q.rows; ++j) {
+        double t = 0.0;
+        for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
+            const double v = std::fabs(C[j] * Q.val[p] * C[Q.idx[p]]);
+            t = sum ? t + v : std::max(t, v);
+        }
+        cs[j] = sum ? cs[j] + t : std::max(cs[j], t);
+    }
+}
+
 // CSR SpMV with each row summed sequentially (the reference's host loops).
 void spmv_seq_rows(const Csr& a, const std::vector<double>& x, std::vector<double>& y) {
 #pragma omp parallel for schedule(dynamic, 1024)
@@ -111,7 +125,7 @@ void spmv_seq_rows(const Csr& a, const std::vector<double>& x, std::vector<doubl
 
 }  // namespace
 
-ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_obj_rescale) {
+ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_obj_rescale, bool q_columns) {
     ScaledLP s;
     s.m = lp.m;
     s.n = lp.n;
@@ -139,6 +153,7 @@ ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_
             for (int q = cptr[j]; q < cptr[j + 1]; ++q) mx = std::max(mx, std::fabs(s.A.val[perm[q]]));
             cs[j] = mx;
         }
+        if (q_columns && lp.has_q()) q_column_norms(lp.Q, s.C, false, cs);
         inv_sqrt_or_one(rs);
         inv_sqrt_or_one(cs);
         apply(s.A, rs, cs, s.R, s.C);
@@ -156,6 +171,7 @@ ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_
             for (int q = cptr[j]; q < cptr[j + 1]; ++q) t += std::fabs(s.A.val[perm[q]]);
             cs[j] = t;
         }
+        if (q_columns && lp.has_q()) q_column_norms(lp.Q, s.C, true, cs);
         inv_sqrt_or_one(rs);
         inv_sqrt_or_one(cs);
         apply(s.A, rs, cs, s.R, s.C);

@@ -27,7 +27,12 @@ struct ScaledLP {
     bool has_q() const { return Q.nnz() > 0; }
 };
 
-ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_obj_rescale = true);
+ScaledLP scale_lp(const LP& lp, int ruiz_iters, bool pock_chambolle, bool bound_obj_rescale = true,
+                  bool q_columns = false);
+// q_columns (extension, QPs; the reference scales columns by A alone): each Ruiz and Pock-Chambolle column factor
+// also takes the column of the current C Q C into account (the max, resp. the 1-norm, over A's and Q's column), the
+// equilibration of the block matrix [Q A'; A 0] -- it keeps C Q C's diagonal from spreading over orders of
+// magnitude when A's column scales are uneven.
 // scale_lp's second half (everything after the matrix and its factors R, C): c, l, u, lo, hi, the bound and
 // objective rescaling, Rres, Cres, Q and ||Q'||, bnorm, cnorm.
 void scale_vectors(const LP& lp, bool bound_obj_rescale, ScaledLP& s);

@@ -73,7 +73,8 @@ int ganit_lp_scaled_gpu(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, 
  *                      when no row or column has more than 4096 entries), 1 host, 2 GPU -- bitwise the same
  *                      scaled problem and norm estimate either way (ignored by ganit_batch_*),
  *                      bit 4 polish (LPs: project an optimal point onto its active face on the GPU and keep it
- *                      when its FP64 KKT errors are all <= 1e-9; ignored by ganit_batch_*)
+ *                      when its FP64 KKT errors are all <= 1e-9; ignored by ganit_batch_*),
+ *                      bit 5 scale_q (QPs: the column scaling also equilibrates Q, the block matrix [Q A'; A 0])
  * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical,
  *                       5 dual infeasible -- only with detect_unbounded),
  *               pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,
