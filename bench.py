@@ -146,7 +146,10 @@ def portfolio_ladder():
     out = dict(workload="portfolio QP 100000 assets x 50 factors, seed 0, tol 1e-6", m=d["m"], n=d["n"], nnz=d["nnz"])
     runs = {}
     for name, cfg in (("fp64", dict(precision="double")), ("mixed", dict(precision="mixed")),
-                      ("dynamic", dict(precision="dynamic"))):
+                      ("dynamic", dict(precision="dynamic")),
+                      # extension: the column scaling also equilibrates Q (a different trajectory, same optimum)
+                      ("fp64_scale_q", dict(precision="double", scale_q=True)),
+                      ("mixed_scale_q", dict(precision="mixed", scale_q=True))):
         s = p.solver(tol=1e-6, time_limit=120, **cfg)
         s.upload()
         s.run()
@@ -156,7 +159,7 @@ def portfolio_ladder():
                           certified_rel=[r["rel_p"], r["rel_d"], r["rel_gap"]])
         print(f"[bench] portfolio {name}: {runs[name]}", file=sys.stderr, flush=True)
     f = runs["fp64"]
-    for name in ("mixed", "dynamic"):
+    for name in ("mixed", "dynamic", "fp64_scale_q", "mixed_scale_q"):
         runs[name]["speedup_vs_fp64"] = round(f["solve_s"] / runs[name]["solve_s"], 3)
         runs[name]["obj_rel_err_vs_fp64"] = abs(runs[name]["obj"] - f["obj"]) / (1 + abs(f["obj"]))
     out["runs"] = runs
