@@ -88,3 +88,14 @@ def test_cli_flags_and_exit_codes(tmp_path):
     assert r.returncode == 1 and "GPU" in r.stderr
     r = _cli(str(tmp_path / "missing.mps"), "--quiet")
     assert r.returncode == 1 and "cannot open" in r.stderr
+
+
+def test_cli_new_flags_parse(tmp_path):
+    afiro = os.path.join(ROOT, "tests", "golden", "instances", "afiro.mps.gz")
+    r = _cli(afiro, "--setup", "sideways")
+    assert r.returncode == 1 and "--setup" in r.stderr
+    # accepted flags reach the device check (this build runs on the GPU only)
+    for flags in (["--polish"], ["--scale-q"], ["--setup", "gpu"], ["--setup", "host"], ["--setup", "auto"]):
+        r = _cli(afiro, *flags, "--device", "cpu")
+        assert r.returncode == 1 and "GPU" in r.stderr, flags
+    assert "--polish" in _cli("--help").stdout and "--setup" in _cli("--help").stdout
