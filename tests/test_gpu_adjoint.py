@@ -165,11 +165,15 @@ def test_batched_layer_matches_single_instances(precision, tol, batch_forward):
     X = layer(Cb, Bb, Qb)
     assert X.shape == (k, layer.n)
     (W * X).sum().backward()
+    # A mixed solve of these QPs ends on its FP32 iterate (a few hundred iterations with the Q-aware scaling): the
+    # batch and the single solve take the same iterations and restarts, but the single solver's autotuned kernel
+    # variant may sum a row in another order, so the two points agree to a few FP32 ulps, not to FP64 rounding.
+    x_tol = 1e-7 if precision == "double" else 1e-6
     for i in range(k):
         c, b, q = (t[i].clone().requires_grad_(True) for t in (C, B, Q))
         x = layer(c, b, q)
         (W[i] * x).sum().backward()
-        assert _rel(X[i].detach().numpy(), x.detach().numpy()) < 1e-7, i
+        assert _rel(X[i].detach().numpy(), x.detach().numpy()) < x_tol, i
         for bt, st in ((Cb.grad[i], c.grad), (Bb.grad[i], b.grad), (Qb.grad[i], q.grad)):
             assert _rel(bt.numpy(), st.numpy()) < 1e-5, i
 
