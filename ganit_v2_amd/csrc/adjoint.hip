@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_at(CsrView<M> AT, const int* __r
                                                    const V* __restrict__ ly, const uint8_t* __restrict__ F,
                                                    const V* __restrict__ lx, const uint8_t* __restrict__ D,
                                                    V two_sigma, V* __restrict__ r, V* __restrict__ p) {
-    __shared__ V lds[kTile];
+    __shared__ __align__(16) V lds[kTile];
     stream_block<M, V, 1>(
         AT, blocks, xcd_block_id(), [&](int, int i) { return F[i] ? ly[i] : V(0); },
         [&](int j, V (&t)[1]) {
@@ -81,7 +81,7 @@ template <class M, class V, bool MEASURE, bool RES>
 __global__ __launch_bounds__(kBlock) void k_adj_a(CsrView<M> A, const int* __restrict__ blocks, SplitPlan sp,
                                                   const V* __restrict__ p, const uint8_t* __restrict__ F, V tau,
                                                   const V* ly, V* out, double* __restrict__ partials) {
-    __shared__ V lds[kTile];
+    __shared__ __align__(16) V lds[kTile];
     __shared__ double red[(kBlock / kWave) * 2];
     double acc[2] = {0.0, 0.0};
     stream_block<M, V, 1>(

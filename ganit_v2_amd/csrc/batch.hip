@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
                                                       const V* __restrict__ l, const V* __restrict__ u,
                                                       V* __restrict__ xn, V* __restrict__ xbar,
                                                       V* __restrict__ sumx) {
-    __shared__ V lds[kTile + kWin];
+    __shared__ __align__(16) V lds[kTile + kWin];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
     const int b = e.x;
     const BatchState& s = st[e.y];
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
                                                     const V* __restrict__ xbar, const V* __restrict__ y,
                                                     const V* __restrict__ lo, const V* __restrict__ hi,
                                                     V* __restrict__ yn, V* __restrict__ sumy) {
-    __shared__ V lds[kTile + kWin];
+    __shared__ __align__(16) V lds[kTile + kWin];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
     const int b = e.x;
     const BatchState& s = st[e.y];
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
                                                        const V* __restrict__ l, const V* __restrict__ u,
                                                        const V* __restrict__ cres, int avg,
                                                        double* __restrict__ partials) {
-    __shared__ V lds[kTile];
+    __shared__ __align__(16) V lds[kTile];
     __shared__ double red[(kBlock / kWave) * kBN];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
     const int b = e.x;
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* 
                                                        const V* __restrict__ lo, const V* __restrict__ hi,
                                                        const V* __restrict__ rres, int avg,
                                                        double* __restrict__ partials) {
-    __shared__ V lds[kTile];
+    __shared__ __align__(16) V lds[kTile];
     __shared__ double red[(kBlock / kWave) * kBM];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
     const int b = e.x;

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace ganit_hip {
 
@@ -30,6 +31,21 @@ constexpr int kChunk = 4096;
 // of the vector in LDS with coalesced loads and gather from LDS instead of global memory (windowed blocks).
 constexpr int kWin = 2048;
 static_assert(kTile % kBlock == 0 && kWin % kBlock == 0 && kLong <= kTile, "staging is whole slots per thread");
+static_assert((kWin & (kWin - 1)) == 0 && kWin >= 16, "the window swizzle permutes [0, kWin)");
+
+// Window layout.  Gathers from a window hit LDS banks at the stride between a row's columns, and rows whose
+// columns lie a fixed stride apart (a blending LP's quality rows: one column per crude, 16 apart) make 10-20
+// lanes of a wave wait on one bank.  A swizzled window keeps entry w at slot w ^ (w >> 4) -- a permutation of
+// [0, kWin) that moves stride-2, -8 or -16 runs onto distinct banks and maps each aligned run of 16 entries
+// onto an aligned run of 16 slots (the four runs a wave fills landing on distinct banks), so the coalesced
+// fill stays conflict-free.  Each matrix says which layout
+// its windows use (CsrView::wswz, chosen at upload from the gathers' bank pattern); the values read are the
+// same either way.
+template <bool SW>
+__device__ __forceinline__ int win_slot(int w) {
+    if constexpr (SW) return w ^ (w >> 4);
+    else return w;
+}
 
 // ---------------------------------------------------------------- BF16 storage
 struct bf16_t {
@@ -58,6 +74,7 @@ struct CsrView {
     // span needs 32-bit indices (split-row chunks carry their own bases, SplitPlan::cbase)
     const uint16_t* __restrict__ idx16 = nullptr;
     const int* __restrict__ bcol = nullptr;
+    int wswz = 0;  // 1: windows are staged swizzled (win_slot<true>)
 };
 
 // ---------------------------------------------------------------- block numbering
@@ -86,6 +103,36 @@ __device__ __forceinline__ V row_dot(const CsrView<M>& a, int row, int lane, con
     V t = V(0);
     for (int p = beg + lane; p < end; p += LPR) t += widen<V>(a.val[p]) * x[a.idx[p]];
     return group_sum<LPR>(t);
+}
+
+// ---------------------------------------------------------------- row sums in row order
+// s[k] += lds[k * kTile + q] for q = beg .. end-1, in that order (the scalar CPU row loop's additions), with
+// the aligned middle read 16 bytes at a time: a quarter of the LDS reads, and rows that start a fixed stride
+// apart (8 entries for most columns of A^T) no longer meet on one bank.  lds must be 16-byte aligned.
+template <class V, int NX>
+__device__ __forceinline__ void ordered_row_sums(const V* lds, int beg, int end, V (&s)[NX]) {
+    constexpr int W = 16 / (int)sizeof(V);
+    typedef V Wide __attribute__((ext_vector_type(W)));
+    static_assert(kTile % W == 0, "tiles start 16-byte aligned");
+    int q = beg;
+    const int head = min(end, (beg + W - 1) & ~(W - 1));
+    for (; q < head; ++q) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) s[k] += lds[k * kTile + q];
+    }
+#pragma unroll 8
+    for (; q + W <= end; q += W) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) {
+            const Wide w = *reinterpret_cast<const Wide*>(lds + k * kTile + q);
+#pragma unroll
+            for (int e = 0; e < W; ++e) s[k] += w[e];
+        }
+    }
+    for (; q < end; ++q) {
+#pragma unroll
+        for (int k = 0; k < NX; ++k) s[k] += lds[k * kTile + q];
+    }
 }
 
 // ---------------------------------------------------------------- block sums in the vector type
@@ -238,12 +285,17 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
                     mv[k] = a.val[base + q];
                     off[k] = (int)i16[base + q];
                 }
+                auto products = [&](auto swz) {
+                    constexpr bool SW = decltype(swz)::value;
 #pragma unroll
-                for (int k = 0; k < kWPer; ++k) win[(int)threadIdx.x + k * kBlock] = wv[k];
-                __syncthreads();
+                    for (int k = 0; k < kWPer; ++k) win[win_slot<SW>((int)threadIdx.x + k * kBlock)] = wv[k];
+                    __syncthreads();
 #pragma unroll
-                for (int k = 0; k < kPer; ++k)
-                    if ((int)threadIdx.x + k * kBlock < cnt) s[0] += widen<V>(mv[k]) * win[off[k]];
+                    for (int k = 0; k < kPer; ++k)
+                        if ((int)threadIdx.x + k * kBlock < cnt) s[0] += widen<V>(mv[k]) * win[win_slot<SW>(off[k])];
+                };
+                if (a.wswz) products(std::true_type{});
+                else products(std::false_type{});
                 block_sum_v<V, NX>(s, lds);
                 if (threadIdx.x == 0) row_fn(r0, s, pv);
                 return;
@@ -304,11 +356,17 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
             mv[k] = a.val[base + q];
             off[k] = (int)i16[base + q];
         }
+        auto products = [&](auto swz) {
+            constexpr bool SW = decltype(swz)::value;
 #pragma unroll
-        for (int k = 0; k < kWPer; ++k) win[(int)threadIdx.x + k * kBlock] = wv[k];
-        __syncthreads();
+            for (int k = 0; k < kWPer; ++k) win[win_slot<SW>((int)threadIdx.x + k * kBlock)] = wv[k];
+            __syncthreads();
 #pragma unroll
-        for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * win[off[k]];
+            for (int k = 0; k < kPer; ++k)
+                lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * win[win_slot<SW>(off[k])];
+        };
+        if (a.wswz) products(std::true_type{});
+        else products(std::false_type{});
     } else if (cnt > 0) {
         const int last = cnt - 1;
         M mv[kPer];
@@ -342,11 +400,7 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
         V s[NX];
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
-#pragma unroll 8
-        for (int q = beg; q < end; ++q) {
-#pragma unroll
-            for (int k = 0; k < NX; ++k) s[k] += lds[k * kTile + q];
-        }
+        ordered_row_sums<V, NX>(lds, beg, end, s);
         row_fn(r, s, pv);
     }
 }
@@ -442,18 +496,21 @@ __device__ __forceinline__ void stream_block_seq_pf(const CsrView<M>& a, const i
                 acc[v] = V(0);
                 continue;
             }
+            auto products = [&](auto swz) {
+                constexpr bool SW = decltype(swz)::value;
 #pragma unroll
-            for (int k = 0; k < kWPer; ++k) win[(int)threadIdx.x + k * kBlock] = wv[v][k];
-            __syncthreads();
+                for (int k = 0; k < kWPer; ++k) win[win_slot<SW>((int)threadIdx.x + k * kBlock)] = wv[v][k];
+                __syncthreads();
 #pragma unroll
-            for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * win[col[k]];
+                for (int k = 0; k < kPer; ++k)
+                    lds[(int)threadIdx.x + k * kBlock] = widen<V>(mv[k]) * win[win_slot<SW>(col[k])];
+            };
+            if (a.wswz) products(std::true_type{});
+            else products(std::false_type{});
             __syncthreads();
-            V t = V(0);
-            if (own) {
-#pragma unroll 8
-                for (int q = pbeg - base; q < pend - base; ++q) t += lds[q];
-            }
-            acc[v] = t;
+            V t[1] = {V(0)};
+            if (own) ordered_row_sums<V, 1>(lds, pbeg - base, pend - base, t);
+            acc[v] = t[0];
             __syncthreads();  // the next vector reuses the window and the tile
         }
     } else {
@@ -477,12 +534,9 @@ __device__ __forceinline__ void stream_block_seq_pf(const CsrView<M>& a, const i
                 for (int k = 0; k < kPer; ++k) lds[(int)threadIdx.x + k * kBlock] = pr[v][k];
             }
             __syncthreads();
-            V t = V(0);
-            if (own) {
-#pragma unroll 8
-                for (int q = pbeg - base; q < pend - base; ++q) t += lds[q];
-            }
-            acc[v] = t;
+            V t[1] = {V(0)};
+            if (own) ordered_row_sums<V, 1>(lds, pbeg - base, pend - base, t);
+            acc[v] = t[0];
             __syncthreads();  // the next vector reuses the tile
         }
     }

@@ -84,6 +84,7 @@ struct Csr {
     int* bcol = nullptr;        // block b's smallest column, bcol[2b + 1] = its window length (-1: long row, 32-bit)
     int* chunk_base = nullptr;  // per split-row chunk: its smallest column, or -1 (32-bit indices)
     int64_t idx32_entries = 0;  // entries still read through 32-bit indices when idx16 is present
+    int wswz = 0;               // 1: windows staged swizzled (chosen from the gathers' LDS bank pattern)
     int* blocks = nullptr;   // row-block descriptors of the LDS-staged variant: 4 ints per block (first row,
                              // end row, first nonzero, end nonzero), device; see upload_row_blocks
     int nblocks = 0;
@@ -104,17 +105,17 @@ struct Csr {
 template <>
 inline CsrView<double> Csr::view<double>() const {
     GH_REQUIRE(rows == 0 || v64, "matrix has no FP64 values resident");
-    return {rows, ptr, idx, v64, idx16, bcol};
+    return {rows, ptr, idx, v64, idx16, bcol, wswz};
 }
 template <>
 inline CsrView<float> Csr::view<float>() const {
     GH_REQUIRE(rows == 0 || v32, "matrix has no FP32 values resident");
-    return {rows, ptr, idx, v32, idx16, bcol};
+    return {rows, ptr, idx, v32, idx16, bcol, wswz};
 }
 template <>
 inline CsrView<bf16_t> Csr::view<bf16_t>() const {
     GH_REQUIRE(rows == 0 || v16, "matrix has no BF16 values resident");
-    return {rows, ptr, idx, v16, idx16, bcol};
+    return {rows, ptr, idx, v16, idx16, bcol, wswz};
 }
 
 Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, const double* val, unsigned mask);

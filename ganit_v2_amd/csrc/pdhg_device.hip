@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const 
                                                            const V* __restrict__ l, const V* __restrict__ u,
                                                            const double* __restrict__ prm, V* __restrict__ xn,
                                                            V* __restrict__ xbar, V* __restrict__ sumx) {
-    __shared__ V lds[kTile + kWin];
+    __shared__ __align__(16) V lds[kTile + kWin];
     // the row's operands, and the step size with them: prm is written by the previous launch, so its load is
     // issued once the block is known, in flight together with the block's index / value loads (and it is
     // read in V already, so nothing waits for it before the row update)
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(kBlock) void k_iter_dual_rb(CsrView<M> A, const int
                                                          const V* __restrict__ lo, const V* __restrict__ hi,
                                                          const double* __restrict__ prm, V* __restrict__ yn,
                                                          V* __restrict__ sumy) {
-    __shared__ V lds[kTile + kWin];
+    __shared__ __align__(16) V lds[kTile + kWin];
     struct Ops {
         V y, lo, hi, s, sigma;
     };
@@ -879,6 +879,16 @@ int ganit_hip_pdhg_index_bytes(const ganit_hip_pdhg* h, int which, int* bytes) {
         *bytes = (a->lanes == 0 && a->idx16 && a->nnz > 0)
                      ? (int)((2 * (a->nnz - a->idx32_entries) + 4 * a->idx32_entries) / a->nnz)
                      : 4;
+    });
+}
+
+int ganit_hip_pdhg_window_layout(const ganit_hip_pdhg* h, int which, int* swizzled) {
+    return guarded([&] {
+        auto* pd = reinterpret_cast<const Pdhg*>(h);
+        GH_REQUIRE(pd && swizzled && which >= 0 && which <= 2, "bad window-layout query");
+        const Csr* a = which == 0 ? pd->A : which == 1 ? pd->AT : pd->Q;
+        GH_REQUIRE(a, "problem has no Q");
+        *swizzled = (a->idx16 && a->bcol) ? a->wswz : 0;
     });
 }
 

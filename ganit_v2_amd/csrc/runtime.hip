@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<M> a, const V* __restri
 template <class M, class V>
 __global__ __launch_bounds__(kBlock) void k_spmv_rb(CsrView<M> a, const int* __restrict__ blocks, SplitPlan sp,
                                                     const V* __restrict__ x, V* __restrict__ y) {
-    __shared__ V lds[kTile + kWin];
+    __shared__ __align__(16) V lds[kTile + kWin];
     stream_block<M, V, 1, true>(
         a, blocks, xcd_block_id(), [&](int, int j) { return x[j]; }, [&](int r, V (&t)[1]) { y[r] = t[0]; }, lds,
         sp);
@@ -288,6 +288,41 @@ int* upload_row_blocks(Ctx* ctx, const int* ptr, const std::vector<int>& rb) {
     return dev;
 }
 
+// Window layout of a matrix (win_slot in device_common.hpp): 1 = swizzled.  A wave's gather from an LDS window
+// takes as many cycles as the most distinct addresses that meet on one of the LDS's 64 four-byte banks; this
+// sums that count over the gathers of up to 256 windowed blocks spread over the matrix, for entry w at slot w
+// and at slot w ^ (w >> 4), and keeps the swizzled layout when it costs at most 3/4 of the plain one (the
+// swizzle's index arithmetic is not free, so a matrix whose gathers rarely conflict stays plain).
+int window_layout(const std::vector<int>& rb, const int* ptr, const std::vector<int>& bc,
+                  const std::vector<uint16_t>& c16) {
+    constexpr int kBanks = 64;
+    const size_t nb = rb.size() - 1;
+    const size_t step = std::max<size_t>(1, nb / 256);
+    std::vector<int> stamp(kWin, -1);
+    int64_t cost[2] = {0, 0};
+    int gid = 0;
+    for (size_t q = 0; q < nb; q += step) {
+        if (bc[2 * q + 1] <= 0) continue;
+        const int p0 = ptr[rb[q]], p1 = ptr[rb[q + 1]];
+        for (int g = p0; g < p1; g += kWave) {  // one wave's lanes read 64 consecutive entries of the block
+            const int ge = std::min(g + kWave, p1);
+            for (int sw = 0; sw < 2; ++sw, ++gid) {
+                int per_bank[kBanks] = {0};
+                int worst = 0;
+                for (int p = g; p < ge; ++p) {
+                    const int w = c16[p];
+                    const int slot = sw ? (w ^ (w >> 4)) : w;
+                    if (stamp[slot] == gid) continue;  // lanes reading one address share the read
+                    stamp[slot] = gid;
+                    worst = std::max(worst, ++per_bank[slot % kBanks]);
+                }
+                cost[sw] += worst;
+            }
+        }
+    }
+    return (cost[0] > 0 && 4 * cost[1] <= 3 * cost[0]) ? 1 : 0;
+}
+
 void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, const std::vector<int>& rb) {
     if (a->idx16) GH_CHECK(hipFree(a->idx16));
     if (a->bcol) GH_CHECK(hipFree(a->bcol));
@@ -296,6 +331,7 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
     a->bcol = nullptr;
     a->chunk_base = nullptr;
     a->idx32_entries = 0;
+    a->wswz = 0;
     const char* env = std::getenv("GANIT_HIP_COMPACT_IDX");  // 0: keep the 32-bit indices only
     if ((env && env[0] == '0') || a->nnz == 0 || rb.size() < 2) return;
     const char* wenv = std::getenv("GANIT_HIP_WINDOW");  // 0: no windowed blocks (every gather from HBM / L2)
@@ -369,6 +405,8 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
                                 ctx->stream));
     }
     a->idx32_entries = n32;
+    const char* senv = std::getenv("GANIT_HIP_WINDOW_SWIZZLE");  // 0 / 1: force the plain / swizzled layout
+    a->wswz = (senv && (senv[0] == '0' || senv[0] == '1')) ? senv[0] - '0' : window_layout(rb, ptr, bc, c16);
     GH_CHECK(hipStreamSynchronize(ctx->stream));
 }
 
@@ -539,10 +577,12 @@ struct PartitionPlan {
     int* bcol = nullptr;
     int* chunk_base = nullptr;
     int64_t idx32_entries = 0;
+    int wswz = 0;
 };
 
 PartitionPlan plan_of(const Csr* a) {
-    return {true, a->rb_rows, a->rb_nnz, a->nblocks, a->blocks, a->idx16, a->bcol, a->chunk_base, a->idx32_entries};
+    return {true,   a->rb_rows,   a->rb_nnz,     a->nblocks,         a->blocks,
+            a->idx16, a->bcol, a->chunk_base, a->idx32_entries, a->wswz};
 }
 
 void attach_plan(Csr* a, const PartitionPlan& p) {
@@ -554,6 +594,7 @@ void attach_plan(Csr* a, const PartitionPlan& p) {
     a->bcol = p.bcol;
     a->chunk_base = p.chunk_base;
     a->idx32_entries = p.idx32_entries;
+    a->wswz = p.wswz;
 }
 
 // `a` lets go of its plan (its owner keeps the buffers); the next csr_set_partition builds a fresh one

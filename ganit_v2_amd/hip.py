@@ -62,6 +62,7 @@ def lib() -> ctypes.CDLL:
         "ganit_hip_pdhg_destroy": [_vp],
         "ganit_hip_pdhg_plan": [_vp, _i, ctypes.POINTER(_i), _vp],
         "ganit_hip_pdhg_set_lanes": [_vp, _i, _i],
+        "ganit_hip_pdhg_window_layout": [_vp, _i, ctypes.POINTER(_i)],
         "ganit_hip_pdhg_load": [_vp, _i, _vp, _vp, _vp, _vp],
         "ganit_hip_pdhg_handover": [_vp, _i, _i],
         "ganit_hip_pdhg_iterate": [_vp, _i, _i, _i, _d, _d],
@@ -264,6 +265,12 @@ class DevicePdhg:
 
     def set_lanes(self, which: int, lanes: int):
         check(lib().ganit_hip_pdhg_set_lanes(self.h, which, lanes), "pdhg_set_lanes")
+
+    def window_layout(self, which: int) -> int:
+        """1 when matrix `which` (0 A, 1 A^T, 2 Q) stages its LDS windows swizzled, else 0."""
+        sw = _i()
+        check(lib().ganit_hip_pdhg_window_layout(self.h, which, ctypes.byref(sw)), "pdhg_window_layout")
+        return sw.value
 
     def load(self, vdt, x, y, xr=None, yr=None):
         x = np.ascontiguousarray(x, np.float64)

@@ -139,6 +139,10 @@ int ganit_hip_pdhg_destroy(ganit_hip_pdhg* pd);
  * on LDS-staged row blocks with compact indices (each block's columns = a base column + 16-bit offsets), else 4.
  * No reference counterpart (the reference always reads 32-bit indices). */
 int ganit_hip_pdhg_index_bytes(const ganit_hip_pdhg* pd, int which, int* bytes);
+/* Whether matrix `which` (0 A, 1 A^T, 2 Q) stages its LDS windows swizzled (entry w at slot w ^ (w >> 4)):
+ * chosen at upload from the bank pattern of its gathers (GANIT_HIP_WINDOW_SWIZZLE=0/1 forces it); the row sums
+ * are the same bits either way.  No reference counterpart (the reference has no LDS windows). */
+int ganit_hip_pdhg_window_layout(const ganit_hip_pdhg* pd, int which, int* swizzled);
 /* The matrices' variant plans (A, AT, Q); the variant may be overridden per matrix (which: 0 = A, 1 = AT,
  * 2 = Q), with the lanes_per_row values of ganit_hip_csr_set_lanes.  ganit_hip_pdhg_create picks each
  * matrix's variant by timing them (ganit_hip_csr_autotune) unless GANIT_HIP_AUTOTUNE=0. */

@@ -128,14 +128,16 @@ def _windowed_blocks(ptr, idx, max_rows=256, max_nnz=2048, klong=256, kwin=2048)
     return hit, staged
 
 
-@pytest.mark.parametrize("window", [True, False])
+@pytest.mark.parametrize("window", ["plain", "swizzled", "off"])
 @pytest.mark.parametrize("mdt,vdt", PAIRS)
 def test_windowed_rowblock_spmv_bitwise(hipctx, oracle, monkeypatch, mdt, vdt, window):
-    """Banded rows: most blocks gather from a copy of their window of x in LDS; with windows on (and with
-    them switched off) the row sums are the oracle's bits exactly."""
+    """Banded rows: most blocks gather from a copy of their window of x in LDS; with windows on (in either
+    layout) and with them switched off, the row sums are the oracle's bits exactly."""
     from ganit_v2_amd import hip
-    if not window:
+    if window == "off":
         monkeypatch.setenv("GANIT_HIP_WINDOW", "0")
+    else:
+        monkeypatch.setenv("GANIT_HIP_WINDOW_SWIZZLE", "1" if window == "swizzled" else "0")
     rng = np.random.default_rng(40 + 3 * mdt + vdt)
     m, n = 4000, 9000
     lens = rng.integers(0, 120, m)
@@ -449,12 +451,16 @@ def test_pdhg_iteration_matches_oracle_steps(hipctx, oracle):
     assert np.max(np.abs(y1 - yn)) <= 1e-11 * (1 + np.max(np.abs(yn)))
 
 
+@pytest.mark.parametrize("swizzle", ["auto", "0", "1"])
 @pytest.mark.parametrize("kind", ["random_lp", "blending"])
-def test_rowblock_iteration_is_bit_exact(hipctx, oracle, kind):
+def test_rowblock_iteration_is_bit_exact(hipctx, oracle, monkeypatch, kind, swizzle):
     """With A and A^T on row blocks, one fused iteration reproduces the oracle's spmv_AT -> primal_step ->
-    spmv_A -> dual_step bit for bit (FP64 and FP32); the blending LP's blocks gather through LDS windows."""
+    spmv_A -> dual_step bit for bit (FP64 and FP32); the blending LP's blocks gather through LDS windows, in
+    the layout chosen at upload and in each layout forced."""
     import ganit_v2_amd as g
     from ganit_v2_amd import hip
+    if swizzle != "auto":
+        monkeypatch.setenv("GANIT_HIP_WINDOW_SWIZZLE", swizzle)
     if kind == "random_lp":
         s = g.Problem.generate("random_lp", seed=4).scaled()
     else:
@@ -466,6 +472,8 @@ def test_rowblock_iteration_is_bit_exact(hipctx, oracle, kind):
     pd = hip.DevicePdhg(hipctx, s, mdt_mask=3, vdt_mask=3)
     pd.set_lanes(0, 0)
     pd.set_lanes(1, 0)
+    if swizzle != "auto":
+        assert pd.window_layout(0) == pd.window_layout(1) == int(swizzle)
     rng = np.random.default_rng(6)
     for vdt, mdt in ((F64, F64), (F32, F32)):
         cast = (lambda a: a) if vdt == F64 else (lambda a: a.astype(np.float32).astype(np.float64))
@@ -481,6 +489,16 @@ def test_rowblock_iteration_is_bit_exact(hipctx, oracle, kind):
         axb = oracle.spmv(mdt, vdt, m, n, A.ptr, A.idx, A.val, xbar)
         yn, _ = oracle.dual_step(vdt, y0, axb, s["lo"], s["hi"], sigma, np.zeros(m))
         assert np.array_equal(x1, xn) and np.array_equal(y1, yn)
+
+
+def test_window_layout_follows_the_bank_pattern(hipctx):
+    """The blending LP's quality rows gather columns 16 apart (one per crude): its A stages swizzled windows,
+    while A^T's gathers (a column's few rows) rarely meet on one bank and keep the plain layout."""
+    import ganit_v2_amd as g
+    from ganit_v2_amd import hip
+    s = g.Problem.generate("blending", crudes=20, products=16, specs=5, periods=30, seed=0).scaled()
+    pd = hip.DevicePdhg(hipctx, s, mdt_mask=3, vdt_mask=3)
+    assert pd.window_layout(0) == 1 and pd.window_layout(1) == 0
 
 
 def test_graph_replay_matches_eager(hipctx, monkeypatch):
