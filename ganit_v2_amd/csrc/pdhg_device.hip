@@ -11,6 +11,7 @@
 // all 22 sums deterministically; the host then reads them with a single copy.
 #include <algorithm>
 #include <cstdlib>
+#include <cstring>
 #include <limits>
 #include <map>
 #include <tuple>
@@ -82,18 +83,20 @@ __global__ __launch_bounds__(kBlock) void k_iter_primal_rb(CsrView<M> AT, const 
                                                            SplitPlan sp, CsrView<M> Q, const V* __restrict__ y,
                                                            const V* __restrict__ x, const V* __restrict__ c,
                                                            const V* __restrict__ l, const V* __restrict__ u,
-                                                           const double* __restrict__ prm, V* __restrict__ xn,
-                                                           V* __restrict__ xbar, V* __restrict__ sumx) {
+                                                           V l0, V u0, const double* __restrict__ prm,
+                                                           V* __restrict__ xn, V* __restrict__ xbar,
+                                                           V* __restrict__ sumx) {
     __shared__ __align__(16) V lds[kTile + kWin];
     // the row's operands, and the step size with them: prm is written by the previous launch, so its load is
     // issued once the block is known, in flight together with the block's index / value loads (and it is
-    // read in V already, so nothing waits for it before the row update)
+    // read in V already, so nothing waits for it before the row update).  l / u null: every column has the
+    // bounds l0 / u0 (x >= 0 with no upper bounds, say), and the kernel streams neither vector.
     struct Ops {
         V x, c, l, u, s, tau;
     };
     stream_block_pf<M, V, 1, true>(
         AT, blocks, xcd_block_id(), [&](int, int i) { return y[i]; },
-        [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j], step_as<V>(prm, 0)}; },
+        [&](int j) { return Ops{x[j], c[j], l ? l[j] : l0, u ? u[j] : u0, sumx[j], step_as<V>(prm, 0)}; },
         [&](int j, V (&aty)[1], Ops& o) {
 #pragma clang fp contract(off)
             V qx = V(0);
@@ -629,14 +632,16 @@ void launch_one(Pdhg* pd, VecSet& s, int which) {
         for (int pass = 0; pass < (pd->AT->nsplit > 0 ? 2 : 1); ++pass) {
             const dim3 g(pass ? (unsigned)pd->AT->nsplit : pd->AT->rb_grid());
             const SplitPlan sp = pass ? pd->AT->split_finish() : pd->AT->split();
+            const V* l = pd->l_uniform ? nullptr : (const V*)s.l;
+            const V* u = pd->u_uniform ? nullptr : (const V*)s.u;
+            const V l0 = V(pd->l_value), u0 = V(pd->u_value);
             if (pd->hasq)
                 hipLaunchKernelGGL((k_iter_primal_rb<M, V, true>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp,
-                                   pd->Q->view<M>(), y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn,
-                                   (V*)s.xbar, (V*)s.sumx);
+                                   pd->Q->view<M>(), y, x, (const V*)s.c, l, u, l0, u0, pd->prm, xn, (V*)s.xbar,
+                                   (V*)s.sumx);
             else
                 hipLaunchKernelGGL((k_iter_primal_rb<M, V, false>), g, dim3(kBlock), 0, st, at, pd->AT->blocks, sp, at,
-                                   y, x, (const V*)s.c, (const V*)s.l, (const V*)s.u, pd->prm, xn, (V*)s.xbar,
-                                   (V*)s.sumx);
+                                   y, x, (const V*)s.c, l, u, l0, u0, pd->prm, xn, (V*)s.xbar, (V*)s.sumx);
         }
     } else if (which == 0 && pd->n > 0) {
         const CsrView<M> at = pd->AT->view<M>();
@@ -778,6 +783,15 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
             pd->hasq = lp->q_ptr && lp->q_ptr[lp->n] > 0;
             if (pd->hasq) pd->Q = csr_create(ctx, lp->n, lp->n, lp->q_ptr, lp->q_idx, lp->q_val, mdt_mask);
             if (pd->hasq) pd->Q->lanes = pd->AT->lanes;
+            // column bounds that are the same bits in every column go to the primal kernel as two scalars
+            auto uniform = [&](const double* v, bool& flag, double& value) {
+                flag = lp->n > 0 && std::all_of(v, v + lp->n, [&](double t) {
+                    return std::memcmp(&t, v, sizeof(double)) == 0;
+                });
+                value = lp->n > 0 ? v[0] : 0.0;
+            };
+            uniform(lp->l, pd->l_uniform, pd->l_value);
+            uniform(lp->u, pd->u_uniform, pd->u_value);
             for (int vdt = 0; vdt < 2; ++vdt) {
                 if (!(vdt_mask & (1u << vdt))) continue;
                 VecSet& s = pd->vs[vdt];

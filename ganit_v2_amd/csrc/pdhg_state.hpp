@@ -23,6 +23,10 @@ struct Pdhg {
     Ctx* ctx = nullptr;
     int m = 0, n = 0;
     bool hasq = false;
+    // every column's lower (upper) bound is the same bits: the primal iteration kernel takes the value, not
+    // the vector
+    bool l_uniform = false, u_uniform = false;
+    double l_value = 0.0, u_value = 0.0;
     Csr *A = nullptr, *AT = nullptr, *Q = nullptr;
     VecSet vs[2];
     double* prm = nullptr;
