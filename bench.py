@@ -22,14 +22,39 @@ PEAK_HBM_GBS = 8000.0
 WORKLOAD = dict(crudes=60, products=16, specs=5, periods=365)
 
 
-def kernel_bytes(m, n, nnz, s_m, s_v, ib_at=4, ib_a=4):
+def kernel_bytes(m, n, nnz, s_m, s_v, ib_at=4, ib_a=4, bound_vectors=2):
     """Algorithmic bytes per launch of the two fused iteration kernels, with the column-index width each
     matrix is read with (2 with compact indices, 4 otherwise).
-    primal: A^T (row pointers + nnz index/value) + gathered y + x, c, l, u, sumx read + xn, xbar, sumx written
+    primal: A^T (row pointers + nnz index/value) + gathered y + x, c, [l, u,] sumx read + xn, xbar, sumx
+            written (bound_vectors = how many of l, u differ between columns and are streamed: a bound that
+            is the same in every column is a kernel argument)
     dual:   A   (row pointers + nnz index/value) + gathered xbar + y, lo, hi, sumy read + yn, sumy written"""
-    primal = 4 * (n + 1) + nnz * (ib_at + s_m) + m * s_v + 8 * n * s_v
+    primal = 4 * (n + 1) + nnz * (ib_at + s_m) + m * s_v + (6 + bound_vectors) * n * s_v
     dual = 4 * (m + 1) + nnz * (ib_a + s_m) + n * s_v + 6 * m * s_v
     return primal, dual
+
+
+def streamed_bound_vectors(prob):
+    """How many of the scaled column-bound vectors l, u the primal kernel streams (a vector whose entries are
+    all the same bits is passed as a scalar instead)."""
+    import numpy as np
+    sc = prob.scaled()
+    return sum(int(len(v) > 0 and not np.all(v.view(np.uint64) == v.view(np.uint64)[0])) for v in (sc["l"], sc["u"]))
+
+
+def timed_kernels(solver, level, reps, m, n, nnz, s_m, s_v, bound_vectors):
+    """HIP-event times of the fused kernels at `level` (0 L1, 2 L3) and their algorithmic GB/s: each kernel with
+    none of its data in L2 (as in a solve, where the two alternate), the whole iteration in solve order, and
+    each kernel back to back (part of its data then stays in L2 between launches)."""
+    t = solver.kernel_times(level, reps)
+    bp, bd = kernel_bytes(m, n, nnz, s_m, s_v, solver.index_bytes(1), solver.index_bytes(0), bound_vectors)
+    return dict(primal_us=t["primal"] * 1e3, dual_us=t["dual"] * 1e3, iteration_us=t["iteration"] * 1e3,
+                primal_gbs=bp / (t["primal"] * 1e6), dual_gbs=bd / (t["dual"] * 1e6),
+                iteration_gbs=(bp + bd) / (t["iteration"] * 1e6), primal_bytes=bp, dual_bytes=bd,
+                primal_b2b_us=t["primal_b2b"] * 1e3, dual_b2b_us=t["dual_b2b"] * 1e3,
+                primal_b2b_gbs=bp / (t["primal_b2b"] * 1e6), dual_b2b_gbs=bd / (t["dual_b2b"] * 1e6),
+                sweep_us=t["sweep"] * 1e3, sweep_gbs=(64 << 20) / (t["sweep"] * 1e6),
+                index_bytes_at=solver.index_bytes(1), index_bytes_a=solver.index_bytes(0))
 
 
 def reference_sequence_bytes(m, n, nnz, s_m, s_v):
@@ -391,15 +416,13 @@ def large_blending_kernels(periods, reps):
     d = prob.dims()
     m, n, nnz = d["m"], d["n"], d["nnz"]
     out = dict(workload=f"blending LP 60 x 16 x 5 x {periods} periods, seed 0", m=m, n=n, nnz=nnz)
+    nbv = streamed_bound_vectors(prob)
     for level, (prec, sm, sv) in {"L1": ("mixed", 4, 4), "L3": ("double", 8, 8)}.items():
         s = prob.solver(precision=prec, tol=1e-4)
         s.upload()
-        ms_p, ms_d, ms_it = s.kernel_times(0 if level == "L1" else 2, reps)
-        bp, bd = kernel_bytes(m, n, nnz, sm, sv, s.index_bytes(1), s.index_bytes(0))
-        out[level] = dict(primal_us=round(ms_p * 1e3, 3), dual_us=round(ms_d * 1e3, 3),
-                          iteration_us=round(ms_it * 1e3, 3), primal_bytes=bp, dual_bytes=bd,
-                          iteration_bytes=bp + bd, primal_gbs=round(bp / (ms_p * 1e6), 1),
-                          dual_gbs=round(bd / (ms_d * 1e6), 1), iteration_gbs=round((bp + bd) / (ms_it * 1e6), 1))
+        kv = timed_kernels(s, 0 if level == "L1" else 2, reps, m, n, nnz, sm, sv, nbv)
+        out[level] = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in kv.items()}
+        out[level]["iteration_bytes"] = kv["primal_bytes"] + kv["dual_bytes"]
         del s
     print(f"[bench] large blending kernels {json.dumps(out)}", file=sys.stderr, flush=True)
     return out
@@ -417,13 +440,10 @@ def kernels_only(prob, staged, a, m, n, nnz):
     import torch
     s_m_low = 2 if a.values == "bf16" else 4
     out = {}
+    nbv = streamed_bound_vectors(prob)
     for level, (sm, sv, sv_obj) in {"L1": (s_m_low, 4, staged("mixed", values=a.values)),
                                     "L3": (8, 8, staged("double"))}.items():
-        ms_p, ms_d, ms_it = sv_obj.kernel_times(0 if level == "L1" else 2, a.reps)
-        bp, bd = kernel_bytes(m, n, nnz, sm, sv, sv_obj.index_bytes(1), sv_obj.index_bytes(0))
-        out[level] = dict(primal_us=ms_p * 1e3, dual_us=ms_d * 1e3, iteration_us=ms_it * 1e3,
-                          primal_gbs=bp / (ms_p * 1e6), dual_gbs=bd / (ms_d * 1e6),
-                          iteration_gbs=(bp + bd) / (ms_it * 1e6), primal_bytes=bp, dual_bytes=bd)
+        out[level] = timed_kernels(sv_obj, 0 if level == "L1" else 2, a.reps, m, n, nnz, sm, sv, nbv)
         del sv_obj
     src = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
     src.fill_(1.0)
@@ -542,23 +562,21 @@ def main():
 
     # per-kernel device times (HIP events on the solver's stream) at L1 and L3
     s_m_low = 2 if a.values == "bf16" else 4
+    nbv = streamed_bound_vectors(prob)
     kern = {}
     for level, (sm, sv, solver) in {"L1": (s_m_low, 4, mixed), "L3": (8, 8, fp64)}.items():
         if level == "L1" and a.precision == "double":
             continue
-        ms_p, ms_d, ms_it = solver.kernel_times(0 if level == "L1" else 2, a.reps)
-        ib_at, ib_a = solver.index_bytes(1), solver.index_bytes(0)
-        bp, bd = kernel_bytes(m, n, nnz, sm, sv, ib_at, ib_a)
-        kern[level] = dict(primal_ms=ms_p, dual_ms=ms_d, primal_gbs=bp / (ms_p * 1e6), dual_gbs=bd / (ms_d * 1e6),
-                           primal_bytes=bp, dual_bytes=bd, index_bytes_at=ib_at, index_bytes_a=ib_a,
-                           iteration_us=ms_it * 1e3, iteration_gbs=(bp + bd) / (ms_it * 1e6),
-                           ref_sequence_gbs=reference_sequence_bytes(m, n, nnz, sm, sv) / (ms_it * 1e6))
+        kern[level] = timed_kernels(solver, 0 if level == "L1" else 2, a.reps, m, n, nnz, sm, sv, nbv)
+        kern[level]["ref_sequence_gbs"] = (reference_sequence_bytes(m, n, nnz, sm, sv)
+                                           / (kern[level]["iteration_us"] * 1e3))
     print(f"[bench] kernels {json.dumps(kern)}", file=sys.stderr, flush=True)
-    # the roofline entry is the kernel with the most device time per iteration of the headline level (both
-    # kernels launch once per iteration; each is timed over back-to-back launches of itself); the other fused
-    # kernel's figures, and each level's whole-iteration rate, ride along under "kernels"
+    # the roofline entry is the kernel with the most device time per iteration of the headline level, timed as
+    # it runs in a solve: launched with none of its own data in L2 (the two kernels alternate, and each one's
+    # share of the matrix evicts the other's); the other fused kernel's figures, both kernels back to back,
+    # and each level's whole-iteration rate ride along under "kernels"
     lead = kern.get("L1", kern["L3"])
-    dom = "primal" if lead["primal_ms"] >= lead["dual_ms"] else "dual"
+    dom = "primal" if lead["primal_us"] >= lead["dual_us"] else "dual"
     level_dom = "L1" if "L1" in kern else "L3"
     traffic = load_traffic() or {}
     per_kernel = {}
@@ -566,8 +584,14 @@ def main():
         for which in ("primal", "dual"):
             per_kernel[f"k_iter_{which} ({lv})"] = dict(
                 achieved=round(kv[f"{which}_gbs"], 1), frac=round(kv[f"{which}_gbs"] / PEAK_HBM_GBS, 4),
-                us=round(kv[f"{which}_ms"] * 1e3, 3), algorithmic_bytes=kv[f"{which}_bytes"],
-                traffic=traffic.get(f"k_iter_{which}_{lv}"))
+                us=round(kv[f"{which}_us"], 3), algorithmic_bytes=kv[f"{which}_bytes"],
+                traffic=traffic.get(f"k_iter_{which}_{lv}"),
+                back_to_back=dict(us=round(kv[f"{which}_b2b_us"], 3),
+                                  achieved=round(kv[f"{which}_b2b_gbs"], 1),
+                                  frac=round(kv[f"{which}_b2b_gbs"] / PEAK_HBM_GBS, 4)))
+        per_kernel[f"stream read 64 MiB ({lv} run)"] = dict(
+            achieved=round(kv["sweep_gbs"], 1), frac=round(kv["sweep_gbs"] / PEAK_HBM_GBS, 4),
+            us=round(kv["sweep_us"], 3), algorithmic_bytes=64 << 20)
         per_kernel[f"iteration ({lv})"] = dict(achieved=round(kv["iteration_gbs"], 1),
                                                frac=round(kv["iteration_gbs"] / PEAK_HBM_GBS, 4),
                                                us=round(kv["iteration_us"], 3),

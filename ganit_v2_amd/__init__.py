@@ -356,12 +356,14 @@ class Solver:
         _check_host(host_lib().ganit_solver_index_bytes(self._h, int(which), ctypes.byref(b)), "index_bytes")
         return int(b.value)
 
-    def kernel_times(self, level: int, reps: int = 50):
-        """Device ms of the primal and dual fused kernels (each launched back to back) and of one whole iteration
-        (both in solve order) at level 0 (L1), 1 (L2) or 2 (L3)."""
-        ms = np.zeros(3)
+    def kernel_times(self, level: int, reps: int = 50) -> dict:
+        """Device ms of the fused iteration kernels at level 0 (L1), 1 (L2) or 2 (L3): primal / dual = each kernel
+        with none of its data in L2 (as in a solve, where they alternate), iteration = both in solve order,
+        primal_b2b / dual_b2b = each launched back to back (part of its data stays in L2), sweep = the 64 MiB
+        streaming read the first two are measured against."""
+        ms = np.zeros(6)
         _check_host(host_lib().ganit_solver_kernel_times(self._h, int(level), int(reps), ms), "kernel_times")
-        return float(ms[0]), float(ms[1]), float(ms[2])
+        return dict(zip(("primal", "dual", "iteration", "primal_b2b", "dual_b2b", "sweep"), map(float, ms)))
 
 
 def solve_many(problems, threads: int = 8, **opts) -> list:

@@ -134,6 +134,17 @@ __global__ __launch_bounds__(kBlock) void k_iter_dual_rb(CsrView<M> A, const int
         lds, sp);
 }
 
+// Streams a buffer once (grid-stride float4 loads); the store is never taken but keeps the loads.
+__global__ __launch_bounds__(kBlock) void k_stream_read(const float4* __restrict__ a, int64_t n4,
+                                                        float* __restrict__ sink) {
+    float acc = 0.f;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += (int64_t)gridDim.x * kBlock) {
+        const float4 v = a[i];
+        acc += v.x + v.y + v.z + v.w;
+    }
+    if (acc == 1234.5f) sink[blockIdx.x] = acc;
+}
+
 __global__ void k_set_steps(double* prm, double tau, double sigma) {
     if (threadIdx.x == 0) {
         prm[0] = tau;
@@ -1090,37 +1101,76 @@ int ganit_hip_pdhg_time_kernels(ganit_hip_pdhg* h, int mdt, int vdt, int reps, d
         GH_REQUIRE(pd && ms && reps > 0, "bad timing arguments");
         VecSet& s = pd->set(vdt);
         hipStream_t st = pd->ctx->stream;
-        hipEvent_t ev[4];
-        for (auto& e : ev) GH_CHECK(hipEventCreate(&e));
+        constexpr int kEv = 7;
+        hipEvent_t ev[kEv] = {};
+        float4* sweep = nullptr;
+        float* sink = nullptr;
+        auto release = [&] {
+            for (auto& e : ev)
+                if (e) (void)hipEventDestroy(e);
+            (void)hipFree(sweep);
+            (void)hipFree(sink);
+        };
         try {
+            for (auto& e : ev) GH_CHECK(hipEventCreate(&e));
+            // a read of this many bytes passes every XCD's L2 several times over: after it, a kernel finds
+            // none of its own data there, as in a solve, where the two kernels alternate
+            const int64_t sweep_bytes = int64_t(64) << 20;
+            const unsigned sweep_grid = 2048;
+            GH_CHECK(hipMalloc(&sweep, sweep_bytes));
+            GH_CHECK(hipMalloc(&sink, sizeof(float) * sweep_grid));
+            GH_CHECK(hipMemsetAsync(sweep, 0, sweep_bytes, st));
+            auto sweep_l2 = [&] {
+                hipLaunchKernelGGL(k_stream_read, dim3(sweep_grid), dim3(kBlock), 0, st, (const float4*)sweep,
+                                   sweep_bytes / 16, sink);
+            };
             hipLaunchKernelGGL(k_set_steps, dim3(1), dim3(64), 0, st, pd->prm, tau, sigma);
             with_mdt(mdt, [&](auto MT) {
                 with_vdt(vdt, [&](auto VT) {
                     using M = typename decltype(MT)::type;
                     using V = typename decltype(VT)::type;
                     launch_iteration<M, V>(pd, s);  // warm-up, and leaves xbar defined
-                    // each kernel back to back (its duration plus the dispatch gap to the next launch)
+                    sweep_l2();
                     GH_CHECK(hipEventRecord(ev[0], st));
-                    for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 0);
+                    for (int r = 0; r < reps; ++r) sweep_l2();  // the sweep alone
                     GH_CHECK(hipEventRecord(ev[1], st));
-                    for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 1);
+                    for (int r = 0; r < reps; ++r) {  // each kernel after a sweep: L2 holds none of its data
+                        launch_one<M, V>(pd, s, 0);
+                        sweep_l2();
+                    }
                     GH_CHECK(hipEventRecord(ev[2], st));
+                    for (int r = 0; r < reps; ++r) {
+                        launch_one<M, V>(pd, s, 1);
+                        sweep_l2();
+                    }
+                    GH_CHECK(hipEventRecord(ev[3], st));
                     // whole iterations in the order a solve launches them, events only at the ends
                     for (int r = 0; r < reps; ++r) launch_iteration<M, V>(pd, s);
-                    GH_CHECK(hipEventRecord(ev[3], st));
+                    GH_CHECK(hipEventRecord(ev[4], st));
+                    // each kernel back to back: its own data partly stays in L2 from one launch to the next
+                    for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 0);
+                    GH_CHECK(hipEventRecord(ev[5], st));
+                    for (int r = 0; r < reps; ++r) launch_one<M, V>(pd, s, 1);
+                    GH_CHECK(hipEventRecord(ev[6], st));
                 });
             });
-            GH_CHECK(hipEventSynchronize(ev[3]));
-            for (int k = 0; k < 3; ++k) {
-                float t = 0;
-                GH_CHECK(hipEventElapsedTime(&t, ev[k], ev[k + 1]));
-                ms[k] = t / reps;
+            GH_CHECK(hipEventSynchronize(ev[6]));
+            float t[kEv - 1];
+            for (int k = 0; k + 1 < kEv; ++k) {
+                GH_CHECK(hipEventElapsedTime(&t[k], ev[k], ev[k + 1]));
+                t[k] /= reps;
             }
+            ms[5] = t[0];
+            ms[0] = t[1] - t[0];
+            ms[1] = t[2] - t[0];
+            ms[2] = t[3];
+            ms[3] = t[4];
+            ms[4] = t[5];
         } catch (...) {
-            for (auto& e : ev) (void)hipEventDestroy(e);
+            release();
             throw;
         }
-        for (auto& e : ev) GH_CHECK(hipEventDestroy(e));
+        release();
     });
 }
 

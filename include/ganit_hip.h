@@ -290,11 +290,17 @@ int ganit_hip_pdhg_adjoint(ganit_hip_pdhg* pd, int mdt, int vdt, const double* g
 int ganit_hip_pdhg_polish(ganit_hip_pdhg* pd, const double* params, double* info);
 
 /* Device time in milliseconds of the fused iteration kernels, measured with HIP events on the context stream from
- * the current iterate: ms[0] = primal kernel (A^T y [+ Q x] + primal prox) and ms[1] = dual kernel (A xbar +
- * dual prox), each averaged over `reps` back-to-back launches of that kernel alone (its duration plus the
- * dispatch gap to the next launch); ms[2] = one whole iteration (both kernels in the order a solve launches
- * them), averaged over `reps` iterations with events only at the ends.  Advances the iterate; no reference
- * counterpart (the reference has no per-kernel timers). */
+ * the current iterate, each figure averaged over `reps` launches (a launch's duration plus the dispatch gap to
+ * the next one).  Six entries:
+ *   ms[0] primal kernel (A^T y [+ Q x] + primal prox) and ms[1] dual kernel (A xbar + dual prox), each launched
+ *         after a 64 MiB streaming read that leaves none of its data in L2 -- as in a solve, where the two
+ *         kernels alternate -- less the read's own time (ms[5]);
+ *   ms[2] one whole iteration, both kernels in the order a solve launches them;
+ *   ms[3] primal and ms[4] dual kernel launched back to back, where part of a kernel's own data stays in L2
+ *         from one launch to the next;
+ *   ms[5] the 64 MiB streaming read alone.
+ * Allocates the 64 MiB for the duration of the call.  Advances the iterate; no reference counterpart (the
+ * reference has no per-kernel timers). */
 int ganit_hip_pdhg_time_kernels(ganit_hip_pdhg* pd, int mdt, int vdt, int reps, double tau, double sigma,
                                 double* ms);
 

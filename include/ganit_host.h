@@ -86,8 +86,10 @@ int ganit_solve_gpu(const ganit_lp* lp, const double* opts, double* out, double*
  * create = HIP context + scaling + norm estimate (opts as above); upload = device-resident problem in every
  * precision the ladder needs; run = one full solve from the initial point with the problem already in HBM
  * (out / x / y as ganit_solve_gpu); kernel_times = device ms of the fused iteration kernels at level 0 (L1),
- * 1 (L2) or 2 (L3): ms[0] primal and ms[1] dual kernel (each launched back to back), ms[2] one whole iteration
- * (both kernels in solve order) -- three entries. */
+ * 1 (L2) or 2 (L3), six entries as ganit_hip_pdhg_time_kernels: ms[0] primal and ms[1] dual kernel with
+ * nothing of theirs in L2 (as in a solve), ms[2] one whole iteration (both kernels in solve order), ms[3] primal
+ * and ms[4] dual kernel launched back to back, ms[5] the 64 MiB streaming read the cold figures are taken
+ * against. */
 typedef struct ganit_solver ganit_solver;
 ganit_solver* ganit_solver_create(const ganit_lp* lp, const double* opts);
 int ganit_solver_upload(ganit_solver* s);
