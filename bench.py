@@ -249,9 +249,10 @@ def diff_qps(count, threads=32, batched=True, scale_q=True, cd_h=1e-4):
     + FP32 adjoint -- timed as one batch, then the FP64 path -- forward (tol 1e-9) + FP64 adjoint -- and each
     instance's FP32 gradient error against it; then each instance's FP64 gradient against FP64 central differences
     of the loss along the gradient's own unit direction in c, in b and in diag Q (step cd_h, the 6 perturbed QPs of
-    every instance solved as one batch at tol 1e-11).  batched: the forwards of a phase run as one batched solve (ganit_v2_amd.Batch)
-    and each instance's solver takes its solution over for the adjoint; else every instance runs forward and adjoint
-    in its own host thread.  scale_q: the column scaling also equilibrates Q (the QP layer's default)."""
+    every instance solved as one batch at tol 1e-11).  batched: the forwards of a phase run as one batched solve
+    (ganit_v2_amd.Batch) and the adjoints as one batched GMRES on the same batch (Batch.adjoint); else every
+    instance runs forward and adjoint in its own host thread.  scale_q: the column scaling also equilibrates Q (the
+    QP layer's default)."""
     from concurrent.futures import ThreadPoolExecutor
     import numpy as np
     import ganit_v2_amd as g
@@ -267,24 +268,20 @@ def diff_qps(count, threads=32, batched=True, scale_q=True, cd_h=1e-4):
         a = s.adjoint(w, precision=adj_precision, tol=adj_tol) if r["status"] == "OPTIMAL" else None
         return r, a
 
+    adj_walls = {}
+
     def batch_path(precision, tol, adj_precision, adj_tol):
         # the batch's clock is shared by its instances; 150 s keeps each phase's silent stretch short
-        res = g.Batch(probs, precision=precision, tol=tol, time_limit=150, scale_q=scale_q).run(return_solution=True)
+        bt = g.Batch(probs, precision=precision, tol=tol, time_limit=150, scale_q=scale_q)
+        res = bt.run(return_solution=True)
         n_opt = sum(r["status"] == "OPTIMAL" for r in res)
+        t0 = time.perf_counter()
+        adj = bt.adjoint(ws, precision=adj_precision, tol=adj_tol)
+        adj_walls[precision] = time.perf_counter() - t0
         print(f"[bench] diff_qp {precision} batch: {n_opt}/{count} OPTIMAL, max {max(r['iters'] for r in res)} "
-              f"iterations, forward {res[0]['solve_s']:.2f} s", file=sys.stderr, flush=True)
-
-        def adjoint(k):
-            r = res[k]
-            if r["status"] != "OPTIMAL":
-                return r, None
-            s = probs[k].solver(precision=precision, tol=tol, scale_q=scale_q)
-            s.upload()
-            s.set_solution(r["x"], r["y"], fp32_iterate=precision != "double" and r["fp32_iters"] == r["iters"])
-            return r, s.adjoint(ws[k], precision=adj_precision, tol=adj_tol)
-
-        with ThreadPoolExecutor(nt) as ex:
-            return list(ex.map(adjoint, range(count)))
+              f"iterations, forward {res[0]['solve_s']:.2f} s; batched adjoint {adj_walls[precision]:.3f} s, "
+              f"max {max(a['iters'] for a in adj)} GMRES products", file=sys.stderr, flush=True)
+        return [(r, a if r["status"] == "OPTIMAL" else None) for r, a in zip(res, adj)]
 
     nt = max(1, min(threads, count))
     t = time.perf_counter()
@@ -346,6 +343,7 @@ def diff_qps(count, threads=32, batched=True, scale_q=True, cd_h=1e-4):
                forward="one batched solve per phase" if batched else "concurrent single solves",
                scale_q=scale_q, threads=nt, fp32_batch_wall_s=round(wall32, 3), fp64_check_wall_s=round(wall64, 3),
                central_difference_wall_s=round(wall_cd, 3), cd_step=cd_h,
+               adjoint_wall_s={k: round(v, 4) for k, v in adj_walls.items()} if batched else None,
                optimal_fp32=sum(r["fwd32_status"] == "OPTIMAL" for r in rows),
                optimal_fp64=sum(r["fwd64_status"] == "OPTIMAL" for r in rows),
                fwd32_iters_max=max(r["fwd32_iters"] for r in rows), fwd64_iters_max=max(r["fwd64_iters"] for r in rows),

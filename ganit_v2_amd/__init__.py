@@ -86,6 +86,7 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_batch_set_start.argtypes = [_vp, ctypes.c_int, _vp, _vp, ctypes.c_double]
     lib.ganit_batch_set_options.argtypes = [_vp, _f64p]
     lib.ganit_batch_weights.argtypes = [_vp, _f64p]
+    lib.ganit_batch_adjoint.argtypes = [_vp, _f64p, _f64p, _vp, _vp, _vp, _vp]
     _host = lib
     return lib
 
@@ -470,6 +471,32 @@ class Batch:
         w = np.zeros(len(self._problems))
         _check_host(host_lib().ganit_batch_weights(self._h, w), "batch_weights")
         return w
+
+    def adjoint(self, gx, precision: str = "double", tol: float = 1e-8, max_iter: int = 20_000, every: int = 0,
+                mask_tol: float = -1.0) -> list:
+        """Solver.adjoint (GMRES form) for every instance at its solution of the last run(), all instances in
+        lockstep on the block-diagonal matrices: gx = a list of the instances' dL/dx* (or one (K, n) array when
+        all n agree); returns one dict per instance, as Solver.adjoint (seconds: the whole call)."""
+        ns = [d["n"] for d in self._dims]
+        parts = [np.ascontiguousarray(g, dtype=np.float64).reshape(-1) for g in gx]
+        if len(parts) != len(ns) or any(len(g) != n for g, n in zip(parts, ns)):
+            raise ValueError("gx must hold one dL/dx* of length n_k per instance")
+        g = np.concatenate(parts)
+        ao = np.array([1.0 if precision == "single" else 0.0, tol, max_iter, 0.0, every, mask_tol], dtype=np.float64)
+        N, M = sum(ns), sum(d["m"] for d in self._dims)
+        dc, dq, db, info = np.zeros(N), np.zeros(N), np.zeros(M), np.zeros(5 * len(ns))
+        _check_host(host_lib().ganit_batch_adjoint(self._h, g, ao, _ptr(dc), _ptr(db), _ptr(dq), _ptr(info)),
+                    "batch_adjoint")
+        out, xo, yo = [], 0, 0
+        for k, d in enumerate(self._dims):
+            n, m = d["n"], d["m"]
+            i = info[5 * k:5 * k + 5]
+            out.append(dict(dc=dc[xo:xo + n].copy(), db=db[yo:yo + m].copy(), dq=dq[xo:xo + n].copy(),
+                            iters=int(i[0]), rel_change=float(i[1]), free_vars=int(i[2]), active_rows=int(i[3]),
+                            seconds=float(i[4])))
+            xo += n
+            yo += m
+        return out
 
     def stats(self) -> dict:
         st = np.zeros(24)

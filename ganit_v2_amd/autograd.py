@@ -37,7 +37,7 @@ class QPLayer:
 
     def __init__(self, n, A_rows, A_cols, A_vals, senses, l, u, Q_rows=(), Q_cols=(), Q_vals=(),
                  precision="mixed", tol=1e-6, adjoint_precision="double", adjoint_tol=1e-8,
-                 adjoint_max_iter=200_000, batch_forward=True, scale_q=True, **solve_opts):
+                 adjoint_max_iter=200_000, batch_forward=True, batch_adjoint=True, scale_q=True, **solve_opts):
         self.n = int(n)
         self.senses = np.asarray(senses, dtype=np.int8)
         self.m = len(self.senses)
@@ -51,8 +51,11 @@ class QPLayer:
         # 1e5-1e7 PDHG iterations to a few hundred (the same optimum; the gradient is the solution map's either way)
         self.opts = dict(precision=precision, tol=tol, scale_q=scale_q, **solve_opts)
         self.adjoint_opts = dict(precision=adjoint_precision, tol=adjoint_tol, max_iter=adjoint_max_iter)
-        # a (B, n) input: its B forward solves as one batched solve (else as concurrent single solves)
+        # a (B, n) input: its B forward solves as one batched solve (else as concurrent single solves) ...
         self.batch_forward = bool(batch_forward)
+        # ... and, after a batched forward, its B adjoints as one batched GMRES (else each instance's solver takes
+        # its solution over and runs its own adjoint, concurrently)
+        self.batch_adjoint = bool(batch_adjoint)
         self.last = None  # (result dict, adjoint dict) of the latest forward / backward, for inspection
 
     @classmethod
@@ -121,13 +124,19 @@ class QPLayer:
 
 
     def _forward_batch(self, C: np.ndarray, B: np.ndarray, Qd: np.ndarray):
-        """The instances' forward solves as one batched solve (each follows its single solve's trajectory); each
-        instance's solver then takes its solution over, so that its adjoint differentiates there."""
+        """The instances' forward solves as one batched solve (each follows its single solve's trajectory).
+        Returns the batch, the problems and the results."""
         problems = [self.problem(c, b, q) for c, b, q in zip(C, B, Qd)]
-        res = Batch(problems, **self.opts).run(return_solution=True)
+        batch = Batch(problems, **self.opts)
+        res = batch.run(return_solution=True)
         for k, r in enumerate(res):
             if r["status"] != "OPTIMAL":
                 raise RuntimeError(f"QP layer forward solve {k} ended {r['status']}")
+        return batch, problems, res
+
+    def _adopt(self, problems, res):
+        """A solver per instance that takes over that instance's batched solution, so that its own adjoint
+        differentiates there."""
         low = self.opts.get("precision", "mixed") != "double"
 
         def adopt(k):
@@ -138,8 +147,7 @@ class QPLayer:
             return solver
 
         with ThreadPoolExecutor(self.threads(len(problems))) as ex:
-            solvers = list(ex.map(adopt, range(len(problems))))
-        return list(zip(solvers, problems, res))
+            return list(ex.map(adopt, range(len(problems))))
 
 
 def _host(t: torch.Tensor) -> np.ndarray:
@@ -171,29 +179,38 @@ __all__ = ["QPLayer", "QPFunction", "ROW_EQ", "ROW_LE", "ROW_GE"]
 
 class QPBatchFunction(torch.autograd.Function):
     """Row k of x* = QP(c[k], b[k], diag q[k]); the B forward solves run as one batched solve (or concurrently),
-    the B adjoints concurrently."""
+    the B adjoints as one batched GMRES after a batched forward (or concurrently)."""
 
     @staticmethod
     def forward(ctx, c, b, q, layer: QPLayer):
         C, B, Qd = _host(c), _host(b), _host(q)
+        ctx.batch, ctx.solvers = None, None
         if layer.batch_forward:
-            outs = layer._forward_batch(C, B, Qd)
+            batch, problems, res = layer._forward_batch(C, B, Qd)
+            if layer.batch_adjoint:
+                ctx.batch = batch
+            else:
+                ctx.solvers = layer._adopt(problems, res)
         else:
             with ThreadPoolExecutor(layer.threads(len(C))) as ex:
                 outs = list(ex.map(layer._forward_one, C, B, Qd))
-        ctx.solvers = [o[0] for o in outs]
-        ctx.problems = [o[1] for o in outs]
+            ctx.solvers = [o[0] for o in outs]
+            problems, res = [o[1] for o in outs], [o[2] for o in outs]
+        ctx.problems = problems
         ctx.layer = layer
-        layer.last = ([o[2] for o in outs], None)
-        x = np.stack([o[2]["x"] for o in outs])
+        layer.last = (res, None)
+        x = np.stack([r["x"] for r in res])
         return torch.from_numpy(x).to(device=c.device, dtype=c.dtype)
 
     @staticmethod
     def backward(ctx, gx):
         layer: QPLayer = ctx.layer
         G = _host(gx)
-        with ThreadPoolExecutor(layer.threads(len(G))) as ex:
-            adj = list(ex.map(lambda sg: sg[0].adjoint(sg[1], **layer.adjoint_opts), zip(ctx.solvers, G)))
+        if ctx.batch is not None:
+            adj = ctx.batch.adjoint(list(G), **layer.adjoint_opts)
+        else:
+            with ThreadPoolExecutor(layer.threads(len(G))) as ex:
+                adj = list(ex.map(lambda sg: sg[0].adjoint(sg[1], **layer.adjoint_opts), zip(ctx.solvers, G)))
         layer.last = (layer.last[0] if layer.last else None, adj)
         conv = lambda key: torch.from_numpy(np.stack([a[key] for a in adj])).to(device=gx.device, dtype=gx.dtype)
         return conv("dc"), conv("db"), conv("dq"), None

@@ -596,6 +596,185 @@ void bt_check(Batch* bt, int avg, int mode) {
     GH_CHECK(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ adjoint of every instance at once
+// The adjoint of the single-instance solve (adjoint.hip: restarted GMRES on (I - J^T) lambda = (g, 0), J^T one
+// transposed PDHG iteration with the projections frozen at the solution) for every instance of the batch in
+// lockstep.  The block-diagonal A, A^T and Q apply all instances' J^T in one set of launches, each row and
+// column with its own instance's tau and sigma; the Krylov basis holds concatenated vectors; dot products are
+// per instance (one workgroup per instance and basis vector, fixed order); each instance's Hessenberg system is
+// solved on the host.  An instance that meets its tolerance stops extending its system, and its coefficients
+// stay zero while the others go on.
+
+// D_j = 1 unless x_j is within mtol (1 + |bound|) of a finite l_j or u_j (mtol of column j's instance);
+// F_i = (lo_i == hi_i) || y_i != 0.
+__global__ __launch_bounds__(kBlock) void k_ba_masks(int N, int M_, const int* __restrict__ col_inst,
+                                                     const double* __restrict__ mtol, const double* __restrict__ x,
+                                                     const double* __restrict__ l, const double* __restrict__ u,
+                                                     const double* __restrict__ y, const double* __restrict__ lo,
+                                                     const double* __restrict__ hi, uint8_t* __restrict__ D,
+                                                     uint8_t* __restrict__ F) {
+    const int64_t total = (int64_t)N + M_;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        if (t < N) {
+            const double v = x[t], lj = l[t], uj = u[t], tl = mtol[col_inst[t]];
+            const bool at_l = isfinite(lj) && v - lj <= tl * (1.0 + fabs(lj));
+            const bool at_u = isfinite(uj) && uj - v <= tl * (1.0 + fabs(uj));
+            D[t] = (at_l || at_u) ? 0 : 1;
+        } else {
+            const int64_t i = t - N;
+            F[i] = (lo[i] == hi[i] || y[i] != 0.0) ? 1 : 0;
+        }
+    }
+}
+
+// g = V(g64), lambda = (g, 0)
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_ba_init(int N, int M_, const double* __restrict__ g64,
+                                                    V* __restrict__ g, V* __restrict__ lam) {
+    const int64_t total = (int64_t)N + M_;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        if (t < N) {
+            g[t] = V(g64[t]);
+            lam[t] = V(g64[t]);
+        } else {
+            lam[t] = V(0);
+        }
+    }
+}
+
+// r = A^T (F o ly), p = D o (lx - 2 sigma_k r) over every row block of A^T (one instance per block)
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_ba_at(CsrView<M> AT, const int* __restrict__ blocks,
+                                                  const int* __restrict__ col_inst, const V* __restrict__ two_sigma,
+                                                  const V* __restrict__ ly, const uint8_t* __restrict__ F,
+                                                  const V* __restrict__ lx, const uint8_t* __restrict__ D,
+                                                  V* __restrict__ r, V* __restrict__ p) {
+    __shared__ __align__(16) V lds[kTile];
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
+    stream_block<M, V, 1>(
+        AT, blocks, xcd_block_id(), [&](int, int i) { return F[i] ? ly[i] : V(0); },
+        [&](int j, V (&t)[1]) {
+            r[j] = t[0];
+            p[j] = D[j] ? V(lx[j] - two_sigma[col_inst[j]] * t[0]) : V(0);
+        },
+        lds, none);
+}
+
+// y part of (I - J^T) lambda: out = ly - (F o ly + tau_k A p)
+template <class M, class V>
+__global__ __launch_bounds__(kBlock) void k_ba_a(CsrView<M> A, const int* __restrict__ blocks,
+                                                 const int* __restrict__ row_inst, const V* __restrict__ tau,
+                                                 const V* __restrict__ p, const uint8_t* __restrict__ F,
+                                                 const V* __restrict__ ly, V* __restrict__ out) {
+    __shared__ __align__(16) V lds[kTile];
+    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
+    stream_block<M, V, 1>(
+        A, blocks, xcd_block_id(), [&](int, int j) { return p[j]; },
+        [&](int i, V (&t)[1]) {
+            const V in = ly[i];
+            const V old = F[i] ? in : V(0);
+            const V nv = old + tau[row_inst[i]] * t[0];
+            out[i] = V(in - nv);
+        },
+        lds, none);
+}
+
+// x part of (I - J^T) lambda: out = lx - (p - tau_k Q p + sigma_k r), the Q row summed by the owning thread
+template <class M, class V, bool HASQ>
+__global__ __launch_bounds__(kBlock) void k_ba_x(int N, CsrView<M> Q, const int* __restrict__ col_inst,
+                                                 const V* __restrict__ tau, const V* __restrict__ sigma,
+                                                 const V* __restrict__ p, const V* __restrict__ r,
+                                                 const V* __restrict__ lx, V* __restrict__ out) {
+    for (int j = blockIdx.x * kBlock + threadIdx.x; j < N; j += gridDim.x * kBlock) {
+        const int k = col_inst[j];
+        V qp = V(0);
+        if (HASQ)
+            for (int q = Q.ptr[j]; q < Q.ptr[j + 1]; ++q) qp += widen<V>(Q.val[q]) * p[Q.idx[q]];
+        const V jt = p[j] - tau[k] * qp + sigma[k] * r[j];
+        out[j] = V(lx[j] - jt);
+    }
+}
+
+// h[c * K + k] = <basis_c, w> over instance k's entries (its columns, then its rows unless xonly), FP64, one
+// workgroup per (instance, basis vector): grid (K, ncols)
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_bgm_dots(int N, int64_t L, const BatchInst* __restrict__ inst, int K,
+                                                     const V* __restrict__ basis, const V* __restrict__ w, int xonly,
+                                                     double* __restrict__ h) {
+    __shared__ double red[kBlock / kWave];
+    const int k = blockIdx.x, c = blockIdx.y;
+    const V* __restrict__ v = basis + (size_t)c * L;
+    const int c0 = inst[k].col0, n = inst[k].n;
+    const int64_t r0 = (int64_t)N + inst[k].row0;
+    const int m = xonly ? 0 : inst[k].m;
+    double s[1] = {0.0};
+    for (int q = threadIdx.x; q < n; q += kBlock) s[0] += double(v[c0 + q]) * double(w[c0 + q]);
+    for (int q = threadIdx.x; q < m; q += kBlock) s[0] += double(v[r0 + q]) * double(w[r0 + q]);
+    block_sum<1>(s, red);
+    if (threadIdx.x == 0) h[(size_t)c * K + k] = s[0];
+}
+
+// w += sgn * sum_{c < ncols} h[c * K + k(t)] basis_c, k(t) the instance of entry t
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_bgm_update(int N, int64_t L, const int* __restrict__ col_inst,
+                                                       const int* __restrict__ row_inst, int K,
+                                                       const V* __restrict__ basis, int ncols,
+                                                       const double* __restrict__ h, double sgn, V* __restrict__ w) {
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < L; t += (int64_t)gridDim.x * kBlock) {
+        const int k = t < N ? col_inst[t] : row_inst[t - N];
+        double a = double(w[t]);
+        for (int c = 0; c < ncols; ++c) a += sgn * h[(size_t)c * K + k] * double(basis[(size_t)c * L + t]);
+        w[t] = V(a);
+    }
+}
+
+// out = w / sqrt(nrm2[k(t)]), 0 where that norm is 0 (out may alias w)
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_bgm_scale(int N, int64_t L, const int* __restrict__ col_inst,
+                                                      const int* __restrict__ row_inst, const V* w,
+                                                      const double* __restrict__ nrm2, V* out) {
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < L; t += (int64_t)gridDim.x * kBlock) {
+        const double q = nrm2[t < N ? col_inst[t] : row_inst[t - N]];
+        const double s = q > 0.0 ? 1.0 / sqrt(q) : 0.0;
+        out[t] = V(double(w[t]) * s);
+    }
+}
+
+// out = (g, 0) - a
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_bgm_rhs_minus(int N, int64_t L, const V* __restrict__ g,
+                                                          const V* __restrict__ a, V* __restrict__ out) {
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < L; t += (int64_t)gridDim.x * kBlock)
+        out[t] = V((t < N ? double(g[t]) : 0.0) - double(a[t]));
+}
+
+// p64 = p, mu64 = F o ly
+template <class V>
+__global__ __launch_bounds__(kBlock) void k_ba_out(int N, int M_, const V* __restrict__ p, const V* __restrict__ ly,
+                                                   const uint8_t* __restrict__ F, double* __restrict__ p64,
+                                                   double* __restrict__ mu64) {
+    const int64_t total = (int64_t)N + M_;
+    for (int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x; t < total; t += (int64_t)gridDim.x * kBlock) {
+        if (t < N) p64[t] = double(p[t]);
+        else mu64[t - N] = F[t - N] ? double(ly[t - N]) : 0.0;
+    }
+}
+
+// A device buffer owned by one call.
+struct BtBuf {
+    void* p = nullptr;
+    explicit BtBuf(size_t bytes) { GH_CHECK(hipMalloc(&p, std::max<size_t>(bytes, 16))); }
+    ~BtBuf() {
+        if (p) (void)hipFree(p);
+    }
+    BtBuf(const BtBuf&) = delete;
+    BtBuf& operator=(const BtBuf&) = delete;
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
 template <class F>
 void bt_types(int mdt, int vdt, F&& f) {
     with_mdt(mdt, [&](auto MT) {
@@ -1089,6 +1268,227 @@ int ganit_hip_batch_store(ganit_hip_batch* h, int vdt, double* x, double* y) {
         };
         down(v.X[bt->cur], x, bt->N);
         down(v.Y[bt->cur], y, bt->Mr);
+    });
+}
+
+// params = {max products, tol, restart length}; tau, sigma, mask_tol: one per instance; gx (N) in the scaled space.
+int ganit_hip_batch_adjoint(ganit_hip_batch* h, int mdt, int vdt, const double* gx, const double* tau,
+                            const double* sigma, const double* mask_tol, const double* params, double* p_out,
+                            double* mu_out, double* info) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && gx && tau && sigma && mask_tol && params && p_out && mu_out && info, "null argument");
+        const int max_iter = (int)std::min(params[0], 2.0e9);
+        const double tol = params[1];
+        const int kk = (int)params[2];
+        GH_REQUIRE(max_iter > 0 && kk > 0 && kk <= 1024, "bad adjoint parameters (restart length 1..1024)");
+        const int slot = vdt == GANIT_F64 ? 0 : 1;
+        GH_REQUIRE(bt->on[0] && bt->on[slot], "vector dtype not resident");
+        const int K = bt->ninst, N = bt->N, Mr = bt->Mr;
+        for (int k = 0; k < K; ++k)
+            GH_REQUIRE(tau[k] > 0 && sigma[k] > 0 && mask_tol[k] >= 0, "bad step sizes or mask tolerance");
+        const int64_t L = (int64_t)N + Mr;
+        const size_t e = dtype_size(vdt);
+        hipStream_t st = bt->ctx->stream;
+        const BatchVecs& v64 = bt->vs[0];
+        BtBuf D(N), F(Mr), g(e * N), r(e * N), p(e * N), g64(8 * (size_t)N), lam(e * L), w(e * L),
+            basis(e * (size_t)L * (kk + 1)), steps(e * 3 * (size_t)K), mt(8 * (size_t)K),
+            hbuf(sizeof(double) * ((size_t)(2 * (kk + 1) + 1) * K)), p64(8 * (size_t)N), mu64(8 * (size_t)Mr);
+        GH_CHECK(hipMemcpyAsync(g64.p, gx, 8 * (size_t)N, hipMemcpyHostToDevice, st));
+        GH_CHECK(hipMemcpyAsync(mt.p, mask_tol, 8 * (size_t)K, hipMemcpyHostToDevice, st));
+        const unsigned gl = bt_grid(L), gn = bt_grid(std::max(N, 1));
+        hipLaunchKernelGGL(k_ba_masks, dim3(gl), dim3(kBlock), 0, st, N, Mr, bt->col_inst, mt.as<double>(),
+                           (const double*)v64.X[bt->cur], (const double*)v64.l, (const double*)v64.u,
+                           (const double*)v64.Y[bt->cur], (const double*)v64.lo, (const double*)v64.hi,
+                           D.as<uint8_t>(), F.as<uint8_t>());
+        std::vector<int> prods(K, 0);
+        std::vector<double> rel(K, 0.0);
+        bt_types(mdt, vdt, [&](auto MT, auto VT) {
+            using M = typename decltype(MT)::type;
+            using V = typename decltype(VT)::type;
+            // per-instance step sizes in V: tau, sigma, 2 sigma
+            std::vector<V> sv(3 * (size_t)K);
+            for (int k = 0; k < K; ++k) {
+                sv[k] = V(tau[k]);
+                sv[K + k] = V(sigma[k]);
+                sv[2 * K + k] = V(2.0 * sigma[k]);
+            }
+            GH_CHECK(hipMemcpyAsync(steps.p, sv.data(), e * sv.size(), hipMemcpyHostToDevice, st));
+            GH_CHECK(hipStreamSynchronize(st));
+            const V* tv = steps.as<V>();
+            const V* sg = tv + K;
+            const V* s2 = tv + 2 * K;
+            V* lx = lam.as<V>();
+            V* W = w.as<V>();
+            V* Vb = basis.as<V>();
+            hipLaunchKernelGGL(k_ba_init<V>, dim3(gl), dim3(kBlock), 0, st, N, Mr, g64.as<double>(), g.as<V>(), lx);
+            const bool hasq = bt->Q != nullptr;
+            const CsrView<M> qv = hasq ? bt->Q->view<M>() : CsrView<M>{0, nullptr, nullptr, nullptr};
+            // out = (I - J^T) in, for every instance
+            auto apply = [&](const V* in, V* out) {
+                if (N > 0 && bt->AT->nblocks > 0)
+                    hipLaunchKernelGGL((k_ba_at<M, V>), dim3(bt->AT->nblocks), dim3(kBlock), 0, st,
+                                       bt->AT->view<M>(), bt->AT->blocks, bt->col_inst, s2, in + N, F.as<uint8_t>(),
+                                       in, D.as<uint8_t>(), r.as<V>(), p.as<V>());
+                if (Mr > 0 && bt->A->nblocks > 0)
+                    hipLaunchKernelGGL((k_ba_a<M, V>), dim3(bt->A->nblocks), dim3(kBlock), 0, st, bt->A->view<M>(),
+                                       bt->A->blocks, bt->row_inst, tv, p.as<V>(), F.as<uint8_t>(), in + N, out + N);
+                if (N > 0) {
+                    if (hasq)
+                        hipLaunchKernelGGL((k_ba_x<M, V, true>), dim3(gn), dim3(kBlock), 0, st, N, qv, bt->col_inst,
+                                           tv, sg, p.as<V>(), r.as<V>(), in, out);
+                    else
+                        hipLaunchKernelGGL((k_ba_x<M, V, false>), dim3(gn), dim3(kBlock), 0, st, N, qv, bt->col_inst,
+                                           tv, sg, p.as<V>(), r.as<V>(), in, out);
+                }
+            };
+            double* h1 = hbuf.as<double>();            // [(kk + 1) K] first Gram-Schmidt pass
+            double* h2 = h1 + (size_t)(kk + 1) * K;    // [(kk + 1) K] second pass
+            double* nr2 = h2 + (size_t)(kk + 1) * K;   // [K] squared norms
+            auto dots = [&](const V* vecs, int ncols, const V* x, int xonly, double* dst) {
+                hipLaunchKernelGGL(k_bgm_dots<V>, dim3(K, ncols), dim3(kBlock), 0, st, N, L, bt->inst, K, vecs, x,
+                                   xonly, dst);
+            };
+            auto update = [&](const V* vecs, int ncols, const double* hc, double sgn, V* dst) {
+                hipLaunchKernelGGL(k_bgm_update<V>, dim3(gl), dim3(kBlock), 0, st, N, L, bt->col_inst, bt->row_inst, K,
+                                   vecs, ncols, hc, sgn, dst);
+            };
+            auto scale = [&](const V* src, V* dst) {
+                hipLaunchKernelGGL(k_bgm_scale<V>, dim3(gl), dim3(kBlock), 0, st, N, L, bt->col_inst, bt->row_inst,
+                                   src, nr2, dst);
+            };
+            std::vector<double> hh1((size_t)(kk + 1) * K), hh2((size_t)(kk + 1) * K), hn(K);
+            auto fetch = [&](int ncols) {
+                if (ncols > 0) {
+                    GH_CHECK(hipMemcpyAsync(hh1.data(), h1, sizeof(double) * ncols * K, hipMemcpyDeviceToHost, st));
+                    GH_CHECK(hipMemcpyAsync(hh2.data(), h2, sizeof(double) * ncols * K, hipMemcpyDeviceToHost, st));
+                }
+                GH_CHECK(hipMemcpyAsync(hn.data(), nr2, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+                GH_CHECK(hipStreamSynchronize(st));
+            };
+            // ||(g_k, 0)|| per instance
+            dots(g.as<V>(), 1, g.as<V>(), 1, nr2);
+            fetch(0);
+            std::vector<double> bnorm(K), prev_beta(K, INFINITY);
+            std::vector<char> done(K, 0), act(K, 0);
+            for (int k = 0; k < K; ++k) {
+                bnorm[k] = std::sqrt(hn[k]);
+                rel[k] = bnorm[k] > 0 ? INFINITY : 0.0;
+                done[k] = !(bnorm[k] > 0);
+            }
+            // per instance: Hessenberg (kk x kk, column-major), Givens rotations, rotated right-hand side
+            std::vector<double> H((size_t)K * kk * kk), cs((size_t)K * kk), sn((size_t)K * kk),
+                sv2((size_t)K * (kk + 1)), yc;
+            std::vector<int> cols(K, 0);
+            int iters = 0;
+            while (iters < max_iter && std::count(done.begin(), done.end(), (char)0) > 0) {
+                // r0 = b - (I - J^T) lambda into basis column 0; beta_k = ||r0_k||
+                apply(lx, W);
+                hipLaunchKernelGGL(k_bgm_rhs_minus<V>, dim3(gl), dim3(kBlock), 0, st, N, L, g.as<V>(), W, Vb);
+                dots(Vb, 1, Vb, 0, nr2);
+                fetch(0);
+                int running = 0;
+                for (int k = 0; k < K; ++k) {
+                    act[k] = 0;
+                    cols[k] = 0;
+                    if (done[k]) continue;
+                    const double beta = std::sqrt(hn[k]);
+                    rel[k] = beta / bnorm[k];
+                    // converged; or a whole restart cycle gained less than 1%: the working precision's floor
+                    if (!std::isfinite(rel[k]) || rel[k] <= tol || beta > 0.99 * prev_beta[k]) {
+                        done[k] = 1;
+                        continue;
+                    }
+                    prev_beta[k] = beta;
+                    act[k] = 1;
+                    ++running;
+                    double* s = &sv2[(size_t)k * (kk + 1)];
+                    std::fill(s, s + kk + 1, 0.0);
+                    s[0] = beta;
+                }
+                if (running == 0) break;
+                scale(Vb, Vb);
+                for (int j = 0; j < kk && iters < max_iter && running > 0; ++j) {
+                    V* vj1 = Vb + (size_t)(j + 1) * L;
+                    apply(Vb + (size_t)j * L, W);
+                    dots(Vb, j + 1, W, 0, h1);
+                    update(Vb, j + 1, h1, -1.0, W);
+                    dots(Vb, j + 1, W, 0, h2);
+                    update(Vb, j + 1, h2, -1.0, W);
+                    dots(W, 1, W, 0, nr2);
+                    scale(W, vj1);
+                    fetch(j + 1);
+                    ++iters;
+                    for (int k = 0; k < K; ++k) {
+                        if (!act[k]) continue;
+                        ++prods[k];
+                        double* col = &H[((size_t)k * kk + j) * kk];
+                        double* c = &cs[(size_t)k * kk];
+                        double* sn_k = &sn[(size_t)k * kk];
+                        double* s = &sv2[(size_t)k * (kk + 1)];
+                        for (int i = 0; i <= j; ++i) col[i] = hh1[(size_t)i * K + k] + hh2[(size_t)i * K + k];
+                        const double sub = std::sqrt(std::max(hn[k], 0.0));
+                        for (int i = 0; i < j; ++i) {
+                            const double t = c[i] * col[i] + sn_k[i] * col[i + 1];
+                            col[i + 1] = -sn_k[i] * col[i] + c[i] * col[i + 1];
+                            col[i] = t;
+                        }
+                        const double den = std::hypot(col[j], sub);
+                        c[j] = den > 0 ? col[j] / den : 1.0;
+                        sn_k[j] = den > 0 ? sub / den : 0.0;
+                        col[j] = den;
+                        s[j + 1] = -sn_k[j] * s[j];
+                        s[j] = c[j] * s[j];
+                        cols[k] = j + 1;
+                        rel[k] = std::fabs(s[j + 1]) / bnorm[k];
+                        if (!std::isfinite(rel[k]) || rel[k] <= tol || sub <= 1e-300 * bnorm[k]) {
+                            act[k] = 0;
+                            --running;
+                        }
+                    }
+                }
+                // y_k = R_k^{-1} s_k, lambda_k += V y_k (coefficient c of instance k at yc[c * K + k])
+                const int cmax = *std::max_element(cols.begin(), cols.end());
+                if (cmax > 0) {
+                    yc.assign((size_t)cmax * K, 0.0);
+                    for (int k = 0; k < K; ++k) {
+                        const double* Hk = &H[(size_t)k * kk * kk];
+                        const double* s = &sv2[(size_t)k * (kk + 1)];
+                        for (int i = cols[k] - 1; i >= 0; --i) {
+                            double t = s[i];
+                            for (int l2 = i + 1; l2 < cols[k]; ++l2) t -= Hk[(size_t)l2 * kk + i] * yc[(size_t)l2 * K + k];
+                            yc[(size_t)i * K + k] = Hk[(size_t)i * kk + i] != 0.0 ? t / Hk[(size_t)i * kk + i] : 0.0;
+                        }
+                    }
+                    GH_CHECK(hipMemcpyAsync(h1, yc.data(), sizeof(double) * yc.size(), hipMemcpyHostToDevice, st));
+                    update(Vb, cmax, h1, 1.0, lx);
+                    GH_CHECK(hipStreamSynchronize(st));
+                }
+                for (int k = 0; k < K; ++k)
+                    if (!std::isfinite(rel[k])) done[k] = 1;
+            }
+            // p and mu at the final lambda
+            if (N > 0 && bt->AT->nblocks > 0)
+                hipLaunchKernelGGL((k_ba_at<M, V>), dim3(bt->AT->nblocks), dim3(kBlock), 0, st, bt->AT->view<M>(),
+                                   bt->AT->blocks, bt->col_inst, s2, lx + N, F.as<uint8_t>(), lx, D.as<uint8_t>(),
+                                   r.as<V>(), p.as<V>());
+            hipLaunchKernelGGL(k_ba_out<V>, dim3(gl), dim3(kBlock), 0, st, N, Mr, p.as<V>(), lx + N, F.as<uint8_t>(),
+                               p64.as<double>(), mu64.as<double>());
+        });
+        GH_CHECK(hipGetLastError());
+        if (N > 0) GH_CHECK(hipMemcpyAsync(p_out, p64.p, 8 * (size_t)N, hipMemcpyDeviceToHost, st));
+        if (Mr > 0) GH_CHECK(hipMemcpyAsync(mu_out, mu64.p, 8 * (size_t)Mr, hipMemcpyDeviceToHost, st));
+        std::vector<uint8_t> dh(N), fh(Mr);
+        if (N > 0) GH_CHECK(hipMemcpyAsync(dh.data(), D.p, N, hipMemcpyDeviceToHost, st));
+        if (Mr > 0) GH_CHECK(hipMemcpyAsync(fh.data(), F.p, Mr, hipMemcpyDeviceToHost, st));
+        GH_CHECK(hipStreamSynchronize(st));
+        for (int k = 0; k < K; ++k) {
+            const BatchInst& in = bt->inst_h[k];
+            info[4 * k + 0] = prods[k];
+            info[4 * k + 1] = rel[k];
+            info[4 * k + 2] = (double)std::count(dh.begin() + in.col0, dh.begin() + in.col0 + in.n, (uint8_t)1);
+            info[4 * k + 3] = (double)std::count(fh.begin() + in.row0, fh.begin() + in.row0 + in.m, (uint8_t)1);
+        }
     });
 }
 

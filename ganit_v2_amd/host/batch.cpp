@@ -63,6 +63,8 @@ struct BatchSolver::Impl {
     std::vector<std::vector<double>> xw, yw;
     std::vector<double> ww;
     std::vector<double> w_final;  // primal weights at the end of the last run
+    std::vector<char> final_fp32;  // per instance: its last run ended in the FP32-vector level
+    bool solved = false;           // a run has completed and nothing has changed the instances since
     explicit Impl(const Options& o) : opt(o) {}
     ~Impl() {
         if (bt) ganit_hip_batch_destroy(bt);
@@ -171,6 +173,7 @@ void BatchSolver::set_bounds(int k, const std::vector<double>& l, const std::vec
         s.u[j] = u[j] / s.C[j] / s.bscale;
     }
     if (I.bt) chk(ganit_hip_batch_set_bounds(I.bt, k, s.l.data(), s.u.data()));
+    I.solved = false;  // the last run's solution is no longer that of these instances
 }
 
 void BatchSolver::set_start(int k, const std::vector<double>& x, const std::vector<double>& y, double w) {
@@ -205,6 +208,59 @@ void BatchSolver::set_options(const Options& o) {
 }
 
 std::vector<double> BatchSolver::final_weights() const { return p_->w_final; }
+
+std::vector<AdjointResult> BatchSolver::adjoint(const std::vector<double>& gx, const AdjointOptions& ao) {
+    Impl& I = *p_;
+    const int K = size();
+    if (!I.solved) throw std::runtime_error("adjoint needs a completed run()");
+    if ((int)gx.size() != I.cols) throw std::runtime_error("adjoint: dL/dx has the wrong length");
+    if (!ao.gmres) throw std::runtime_error("the batched adjoint runs GMRES only");
+    if (ao.fp32 && !I.mixed) throw std::runtime_error("FP32 adjoint needs a mixed or dynamic precision batch");
+    const auto t0 = Clock::now();
+    // per instance, as GpuSolver::adjoint: the steps of its last phase (pdhg_impl.hpp:87-91), dL/dx' = dL/dx C bscale,
+    // and the bound-activity tolerance of where its iterate came from
+    std::vector<double> g(I.cols), tau(K), sigma(K), mtol(K);
+    for (int k = 0; k < K; ++k) {
+        const ScaledLP& s = I.s[k];
+        const double eta = 0.998 / I.norm_a[k];
+        const double w = I.w_final[k] > 0 && std::isfinite(I.w_final[k]) ? I.w_final[k] : I.w0[k];
+        sigma[k] = eta * w;
+        tau[k] = eta / w;
+        if (s.has_q()) tau[k] = std::min(tau[k], 0.998 / (0.5 * s.normQ + sigma[k] * I.norm_a[k] * I.norm_a[k]));
+        for (int j = 0; j < s.n; ++j) g[I.col0[k] + j] = gx[I.col0[k] + j] * s.C[j] * s.bscale;
+        mtol[k] = ao.mask_tol >= 0 ? ao.mask_tol : (I.final_fp32[k] ? 1e-6 : 1e-12);
+    }
+    std::vector<double> p(I.cols), mu(I.rows), info(4 * (size_t)K), xs(I.cols);
+    const int mdt = ao.fp32 ? I.low_mdt : GANIT_F64, vdt = ao.fp32 ? GANIT_F32 : GANIT_F64;
+    const double params[3] = {(double)ao.max_iter, ao.tol, (double)ao.restart};
+    chk(ganit_hip_batch_adjoint(I.bt, mdt, vdt, g.data(), tau.data(), sigma.data(), mtol.data(), params, p.data(),
+                                mu.data(), info.data()));
+    chk(ganit_hip_batch_store(I.bt, GANIT_F64, xs.data(), nullptr));  // the scaled solution (dQ_jj needs x*)
+    const double secs = since(t0);
+    std::vector<AdjointResult> out(K);
+    for (int k = 0; k < K; ++k) {
+        const ScaledLP& s = I.s[k];
+        AdjointResult& r = out[k];
+        const double sign = s.maximize ? -1.0 : 1.0;
+        const int c0 = I.col0[k], r0 = I.row0[k];
+        r.dc.resize(s.n);
+        r.dq.resize(s.n);
+        r.db.resize(s.m);
+        // back to the original data, as GpuSolver::adjoint (scaling.cpp:36-129)
+        for (int j = 0; j < s.n; ++j) {
+            const double dcs = -tau[k] * p[c0 + j];
+            r.dc[j] = sign * dcs * s.C[j] / s.cscale;
+            r.dq[j] = sign * dcs * xs[c0 + j] * s.C[j] * s.C[j] * s.bscale / s.cscale;
+        }
+        for (int i = 0; i < s.m; ++i) r.db[i] = sigma[k] * mu[r0 + i] * s.R[i] / s.bscale;
+        r.iterations = (long)info[4 * k];
+        r.rel_change = info[4 * k + 1];
+        r.free_vars = (int)info[4 * k + 2];
+        r.active_rows = (int)info[4 * k + 3];
+        r.seconds = secs;
+    }
+    return out;
+}
 
 std::vector<Result> BatchSolver::run() {
     Impl& I = *p_;
@@ -311,6 +367,9 @@ std::vector<Result> BatchSolver::run() {
     std::vector<Result> out(K);
     I.w_final.resize(K);
     for (int k = 0; k < K; ++k) I.w_final[k] = dbl[13 * k + 12];
+    I.final_fp32.assign(K, 0);
+    for (int k = 0; k < K; ++k) I.final_fp32[k] = I.mixed && l1_it[k] == ints[5 * k + 1];
+    I.solved = true;
     for (int k = 0; k < K; ++k) {
         const ScaledLP& s = I.s[k];
         Result& r = out[k];

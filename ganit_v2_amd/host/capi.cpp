@@ -394,6 +394,37 @@ int ganit_batch_weights(const ganit_batch* h, double* w) {
     });
 }
 
+int ganit_batch_adjoint(ganit_batch* h, const double* gx, const double* ao, double* dc, double* db, double* dq,
+                        double* info) {
+    return guard([&] {
+        auto* b = reinterpret_cast<ganit::BatchSolver*>(h);
+        if (!b || !gx || !ao) throw std::runtime_error("null argument");
+        if (ao[3] != 0) throw std::runtime_error("the batched adjoint runs GMRES only (method 0)");
+        ganit::AdjointOptions o;
+        o.fp32 = ao[0] != 0;
+        o.tol = ao[1];
+        o.max_iter = (int)ao[2];
+        if (ao[4] > 0) o.restart = (int)ao[4];
+        o.mask_tol = ao[5];
+        size_t cols = 0;
+        for (int k = 0; k < b->size(); ++k) cols += b->scaled(k).n;
+        const std::vector<ganit::AdjointResult> r = b->adjoint(std::vector<double>(gx, gx + cols), o);
+        size_t c0 = 0, r0 = 0;
+        for (size_t k = 0; k < r.size(); ++k) {
+            if (dc) std::copy(r[k].dc.begin(), r[k].dc.end(), dc + c0);
+            if (dq) std::copy(r[k].dq.begin(), r[k].dq.end(), dq + c0);
+            if (db) std::copy(r[k].db.begin(), r[k].db.end(), db + r0);
+            if (info) {
+                const double v[5] = {(double)r[k].iterations, r[k].rel_change, (double)r[k].free_vars,
+                                     (double)r[k].active_rows, r[k].seconds};
+                std::copy(v, v + 5, info + 5 * k);
+            }
+            c0 += r[k].dc.size();
+            r0 += r[k].db.size();
+        }
+    });
+}
+
 void ganit_batch_free(ganit_batch* h) { delete reinterpret_cast<ganit::BatchSolver*>(h); }
 
 }  // extern "C"

@@ -177,6 +177,10 @@ class BatchSolver {
     void set_options(const Options& opt);
     // The primal weight each instance ended the last run with (what a child's warm start takes over).
     std::vector<double> final_weights() const;
+    // Differentiation through the last run (extension): every instance's adjoint at its solution at once --
+    // GpuSolver::adjoint's GMRES form for all instances in lockstep on the block-diagonal matrices.  gx = dL/dx*
+    // of every instance concatenated (sum of n_k entries); one result per instance.  GMRES only.
+    std::vector<AdjointResult> adjoint(const std::vector<double>& gx, const AdjointOptions& ao);
     int size() const;
     double setup_seconds() const;
     const ScaledLP& scaled(int k) const;

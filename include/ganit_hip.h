@@ -261,6 +261,17 @@ int ganit_hip_batch_certify(ganit_hip_batch* bt, int mdt);
  * rel_p, rel_d, rel_gap, err), certified FP64 KKT (same six), primal weight. */
 int ganit_hip_batch_results(ganit_hip_batch* bt, int* ints, double* dbl);
 int ganit_hip_batch_store(ganit_hip_batch* bt, int vdt, double* x, double* y);
+/* The adjoint of ganit_hip_pdhg_adjoint (GMRES form, below) for every instance at once: restarted GMRES on
+ * (I - J_k^T) lambda_k = (gx_k, 0) for all k in lockstep, J_k^T applied for all instances by the block-diagonal
+ * matrices in one set of launches, dot products and Hessenberg systems per instance.  Linearised at the FP64
+ * working set's iterate; gx (N, scaled space), per-instance steps tau[k], sigma[k] and bound-activity tolerance
+ * mask_tol[k]; params = {max operator applications, tol, restart length (1..1024)}.  p_out (N) = p and mu_out
+ * (M) = F o lambda_y at the solution, as ganit_hip_pdhg_adjoint returns them per instance; info[4k..4k+3] =
+ * operator applications, relative residual, free variables and active rows of instance k.  No reference
+ * counterpart (the reference has no differentiation). */
+int ganit_hip_batch_adjoint(ganit_hip_batch* bt, int mdt, int vdt, const double* gx, const double* tau,
+                            const double* sigma, const double* mask_tol, const double* params, double* p_out,
+                            double* mu_out, double* info);
 
 /* Adjoint of the solution map at the FP64 working set's iterate (a converged forward solve; no reference
  * counterpart).  gx = dL/dx in the scaled space (n).  Solves lambda = J^T lambda + (gx, 0), J the PDHG

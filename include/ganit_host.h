@@ -147,6 +147,13 @@ int ganit_batch_set_bounds(ganit_batch* b, int k, const double* l, const double*
 int ganit_batch_set_start(ganit_batch* b, int k, const double* x, const double* y, double w);
 int ganit_batch_set_options(ganit_batch* b, const double* opts);
 int ganit_batch_weights(const ganit_batch* b, double* w);
+/* Differentiation through the last ganit_batch_run, every instance at once (no reference counterpart): the
+ * GMRES adjoint of ganit_solver_adjoint for all instances in lockstep on the block-diagonal matrices.
+ * gx = dL/dx* of the instances concatenated (sum of n_k); aopts as ganit_solver_adjoint (method must be 0);
+ * dc, dq (sum of n_k), db (sum of m_k) concatenated in instance order (any may be NULL); info[5k..5k+4] as
+ * ganit_solver_adjoint's info for instance k (seconds: the whole call). */
+int ganit_batch_adjoint(ganit_batch* b, const double* gx, const double* aopts, double* dc, double* db, double* dq,
+                        double* info);
 void ganit_batch_free(ganit_batch* b);
 
 #ifdef __cplusplus

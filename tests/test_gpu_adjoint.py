@@ -215,3 +215,52 @@ def test_scale_q_adjoint_vs_active_set_gradient_c4(seed):
         plain = p.solver(tol=1e-6, time_limit=60).run()
         assert plain["status"] == "OPTIMAL"
         assert abs(plain["pobj"] - r["pobj"]) <= 1e-5 * (1 + abs(r["pobj"]))
+
+
+def test_batch_adjoint_vs_active_set_gradient_and_single_solves():
+    """Batch.adjoint: every instance's GMRES adjoint in lockstep on the block-diagonal matrices.  Four C4-shape QPs
+    and a smaller one in one batch (uneven instances): FP64 gradients against the active-set KKT reference < 1e-6
+    and against each instance's own single-solver adjoint at the same solution < 1e-7; FP32 (mixed forward to 1e-6 +
+    FP32 batched adjoint) against the reference gradient at the FP64 solution and against the single-solver FP32
+    adjoint at the same FP32 solution < 1e-3."""
+    probs = [g.Problem.generate("diff_qp", seed=s) for s in range(4)]
+    probs.append(g.Problem.generate("diff_qp", seed=4, n=500, m_eq=75, m_le=50, density=0.02, qdensity=0.002))
+    ws = [np.random.default_rng(100 + k).normal(size=p.dims()["n"]) for k, p in enumerate(probs)]
+    bt = g.Batch(probs, precision="double", tol=1e-9, time_limit=120, scale_q=True)
+    res = bt.run(return_solution=True)
+    assert all(r["status"] == "OPTIMAL" for r in res)
+    adj = bt.adjoint(ws, tol=1e-11)
+    bt32 = g.Batch(probs, precision="mixed", tol=1e-6, time_limit=120, scale_q=True)
+    res32 = bt32.run(return_solution=True)
+    assert all(r["status"] == "OPTIMAL" for r in res32)
+    adj32 = bt32.adjoint(ws, precision="single", tol=1e-5)
+    for k, (p, r, a, r32, a32) in enumerate(zip(probs, res, adj, res32, adj32)):
+        d = p.export()
+        ref = adjoint_ref.active_set_gradient(d["A"], d["Q"], r["x"], r["y"], d["l"], d["u"], d["lo"], d["hi"], ws[k])
+        s = p.solver(precision="double", tol=1e-9, scale_q=True)
+        s.upload()
+        s.set_solution(r["x"], r["y"])
+        one = s.adjoint(ws[k], tol=1e-11)
+        errs = {key: _rel(a[key], ref[key]) for key in ("dc", "db", "dq")}
+        errs1 = {key: _rel(a[key], one[key]) for key in ("dc", "db", "dq")}
+        errs32 = {key: _rel(a32[key], ref[key]) for key in ("dc", "db", "dq")}
+        s32 = p.solver(precision="mixed", tol=1e-6, scale_q=True)
+        s32.upload()
+        s32.set_solution(r32["x"], r32["y"], fp32_iterate=r32["fp32_iters"] == r32["iters"])
+        one32 = s32.adjoint(ws[k], precision="single", tol=1e-5)
+        errs32_1 = {key: _rel(a32[key], one32[key]) for key in ("dc", "db", "dq")}
+        print(f"instance {k}: FP32 vs single FP32 {errs32_1}", flush=True)
+        print(f"instance {k}: products {a['iters']} / {a32['iters']} (single {one['iters']}), free {a['free_vars']}/"
+              f"{ref['free']}, active {a['active_rows']}/{ref['active_rows']}; vs reference {errs}, vs single "
+              f"{errs1}, FP32 vs reference {errs32}", flush=True)
+        assert a["free_vars"] == one["free_vars"] and a["active_rows"] == one["active_rows"]
+        for key in errs:
+            assert errs[key] < 1e-6, (k, key, errs[key])
+            assert errs1[key] < 1e-7, (k, key, errs1[key])
+            assert errs32[key] < 1e-3, (k, key, errs32[key])
+            assert errs32_1[key] < 1e-3, (k, key, errs32_1[key])
+    # arguments are checked before anything runs
+    with pytest.raises(ValueError):
+        bt.adjoint(ws[:-1])
+    with pytest.raises(RuntimeError):
+        bt.adjoint(ws, precision="single")  # an FP64 batch has no FP32 working set
