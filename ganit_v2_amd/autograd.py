@@ -13,10 +13,10 @@ C ABI through ctypes; tensors are converted to float64 host arrays at the bounda
     x = layer(c, b, q)              # torch tensors with requires_grad
     (w * x).sum().backward()        # c.grad, b.grad, q.grad
 
-A batch (c, b, q of shape (B, n), (B, m), (B, n)) solves its B instances concurrently from host threads, each
-on its own HIP context and stream: these solves are launch-bound, so the GPU overlaps them and a batch takes
-about as long as its slowest instance.  Each instance runs the single-instance solve (bitwise the same
-whenever the autotuner, which times the kernel variants for every solver, picks the same variants).
+A batch (c, b, q of shape (B, n), (B, m), (B, n)) runs its B forward solves as one batched solve (Batch: the
+instances packed block-diagonally, each following its single solve's trajectory) and its B adjoints as one batched
+GMRES on that batch (Batch.adjoint).  With batch_forward=False the instances are solved concurrently from host
+threads instead, each on its own HIP context and stream, and their adjoints run per instance.
 """
 from __future__ import annotations
 
