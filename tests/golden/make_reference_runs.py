@@ -68,6 +68,7 @@ def main():
                 record(out, inst, path, ["--device", "cpu", "--tol", "1e-8", "--time", "300"], 8)
                 record(out, inst, path, ["--method", "simplex", "--time", "600"], 1)
         blending_runs(out, os.path.join(td_root(), "blending"))
+        portfolio_runs(out, os.path.join(td_root(), "portfolio"))
     print("wrote", dst)
 
 
@@ -98,10 +99,34 @@ def blending_runs(out, workdir):
     os.remove(path)
 
 
+PORTFOLIO = dict(assets=100_000, factors=50)
+
+
+def portfolio_runs(out, workdir):
+    """The 5.1M-nonzero factor-model portfolio QP (100,000 assets x 50 factors, seed 0), written as QPS: the
+    reference CLI at tol 1e-6 in every precision at 8 threads."""
+    os.makedirs(workdir, exist_ok=True)
+    p = g.Problem.generate("portfolio", seed=0, **PORTFOLIO)
+    path = os.path.join(workdir, "portfolio_s0.mps")
+    p.write_mps(path)
+    inst = "portfolio_s0"
+    d = p.dims()
+    meta = {"instance": inst, "generator": "portfolio", "seed": 0, **PORTFOLIO, "m": d["m"], "n": d["n"],
+            "nnz": d["nnz"], "qnnz": d["qnnz"], "mps_sha256_16": sha(path)}
+    out.write(json.dumps({"meta": meta}) + "\n")
+    for prec in ("double", "mixed", "dynamic"):
+        record(out, inst, path, ["--device", "cpu", "--precision", prec, "--tol", "1e-6", "--time", "600"], 8,
+               timeout=900)
+    os.remove(path)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["--append-blending"]:
         # add only the blending LP's runs to an existing file (the other runs take much longer to redo)
         with open(os.path.join(HERE, "reference_runs.jsonl"), "a") as f:
             blending_runs(f, os.path.join(td_root(), "blending"))
+    elif sys.argv[1:] == ["--append-portfolio"]:
+        with open(os.path.join(HERE, "reference_runs.jsonl"), "a") as f:
+            portfolio_runs(f, os.path.join(td_root(), "portfolio"))
     else:
         main()

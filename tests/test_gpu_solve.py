@@ -116,6 +116,35 @@ def test_portfolio_qp_vs_oracle(oracle, assets, factors):
     assert abs(x.sum() - 1.0) < 1e-4 and x.min() > -1e-6 and x.max() < 0.01 + 1e-6
 
 
+def test_portfolio_qp_full_size_ladder():
+    """The full factor-model portfolio QP (100,000 assets x 50 factors, 5.1M nonzeros, seed 0) at 1e-6, against the
+    reference CLI's own runs of the same QP (tests/golden/reference_runs.jsonl, portfolio_s0; test_host.py checks
+    that the generator still writes that QPS file byte for byte): FP64 takes the reference's iterations to its
+    printed objective; the FP32-matrix ladders (mixed L1 -> L3, dynamic L1 -> L2 -> L3) and FP64 with the Q-aware
+    scaling reach the reference's status and the FP64-certified tolerance, with an objective within 1e-6 relative
+    of the reference's FP64 objective (the size of the tolerance itself); every error is printed."""
+    p = g.Problem.generate("portfolio", seed=0, assets=100_000, factors=50)
+    assert p.dims()["nnz"] == 5_100_050
+    ref = {prec: golden("portfolio_s0", ["--device", "cpu", "--precision", prec, "--tol", "1e-6", "--time", "600"], 8)
+           for prec in ("double", "mixed", "dynamic")}
+    p64 = ref["double"]["pobj"]
+    for cfg in (dict(precision="double"), dict(precision="mixed"), dict(precision="dynamic"),
+                dict(precision="double", scale_q=True)):
+        r = p.solve_gpu(tol=1e-6, time_limit=120, **cfg)
+        rr = ref[cfg["precision"]]
+        err = abs(r["pobj"] - p64) / (1 + abs(p64))
+        print(f"{cfg}: {r['status']} iters {r['iters']} (L1 {r['fp32_iters']}, L2 {r['l2_iters']}) obj "
+              f"{r['pobj']:.12e}; reference {rr['status']} iters {rr['iters']} (L1 {rr['fp32_iters']}, L2 "
+              f"{rr['l2_iters']}) obj {rr['pobj']:.12e}; rel err vs reference FP64 {err:.2e}", flush=True)
+        assert r["status"] == rr["status"] == "OPTIMAL" and _certified(r, 1e-6), cfg
+        assert err <= 1e-6, cfg
+        if cfg == dict(precision="double"):
+            assert r["iters"] == rr["iters"]
+            assert abs(r["pobj"] - p64) <= 1e-11 * (1 + abs(p64))  # the reference prints 12 significant digits
+        x = r["x"][:100_000]
+        assert abs(x.sum() - 1.0) < 1e-5 and x.min() > -1e-6 and x.max() < 0.01 + 1e-6
+
+
 def test_limits_and_odd_check_interval():
     p = g.Problem.read_mps(instance_path("25fv47"))
     r = p.solve_gpu(max_iter=640)

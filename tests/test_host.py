@@ -353,3 +353,23 @@ def test_blending_generator_writes_the_golden_lp(tmp_path):
         for chunk in iter(lambda: f.read(1 << 24), b""):
             h.update(chunk)
     assert h.hexdigest()[:16] == meta["mps_sha256_16"]
+
+
+def test_portfolio_generator_writes_the_golden_qp(tmp_path):
+    """The portfolio QP the reference CLI's golden runs were made on (meta line of portfolio_s0) is what the
+    generator still produces: same dimensions and the same QPS bytes (sha256)."""
+    import hashlib
+    import json
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "reference_runs.jsonl")) as f:
+        meta = next(json.loads(l)["meta"] for l in f if '"portfolio_s0"' in l and '"meta"' in l)
+    p = g.Problem.generate("portfolio", seed=0, assets=meta["assets"], factors=meta["factors"])
+    d = p.dims()
+    assert (d["m"], d["n"], d["nnz"], d["qnnz"]) == (meta["m"], meta["n"], meta["nnz"], meta["qnnz"])
+    path = tmp_path / "portfolio.mps"
+    p.write_mps(str(path))
+    h = hashlib.sha256()
+    with open(path, "rb") as f:
+        for chunk in iter(lambda: f.read(1 << 24), b""):
+            h.update(chunk)
+    assert h.hexdigest()[:16] == meta["mps_sha256_16"]
