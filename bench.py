@@ -468,7 +468,9 @@ def main():
     ap.add_argument("--eval", default="fp64", choices=["fp64", "level"],
                     help="evaluate L1 restart candidates in FP64 (default) or at L1 precision")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the portfolio-QP ladder and the batched-LP comparison reported beside the headline")
+                    help="skip the random LP against the reference CLI, the larger blending LP's kernels, the "
+                         "portfolio-QP ladder and the batched-LP comparison reported beside the headline "
+                         "(--c4 0 skips the differentiable QPs)")
     ap.add_argument("--c4", type=int, default=64,
                     help="differentiable QPs (forward + adjoint) reported in extras.diff_qp (0: skip)")
     ap.add_argument("--c4-concurrent", action="store_true",

@@ -832,7 +832,10 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
             const char* at = std::getenv("GANIT_HIP_AUTOTUNE");
             if (!(at && at[0] == '0')) {
                 // Per-matrix variant, timed on the fused iteration kernels that will run, at the lowest resident
-                // precision (the hot level) on a zeroed working set.
+                // precision (the hot level) on a zeroed working set, in solve order: each launch of the candidate
+                // is followed by the other fused kernel, which evicts the candidate's data from L2 as it does in a
+                // solve.  (Timed back to back, a row-group variant re-reads what the previous launch left in L2
+                // and can win the trial yet run at half the row blocks' speed in the solve.)
                 const int mdt = (mdt_mask & 4u) ? GANIT_BF16 : (mdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
                 const int vdt = (vdt_mask & 2u) ? GANIT_F32 : GANIT_F64;
                 VecSet& s = pd->set(vdt);
@@ -849,6 +852,10 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                 });
                 hipLaunchKernelGGL(k_set_steps, dim3(1), dim3(64), 0, ctx->stream, pd->prm, 0.0, 0.0);
                 GH_CHECK(hipGetLastError());
+                // while A^T is tuned, A's kernel (the other one of each pair) runs on its row blocks, the usual
+                // winner, rather than on its lanes-per-row guess from the row lengths
+                for (Csr* a : {pd->AT, pd->A})
+                    if (a->nblocks > 0) a->lanes = 0;
                 for (int which = 0; which < 2; ++which) {
                     Csr* a = which == 0 ? pd->AT : pd->A;
                     if ((which == 0 && pd->n == 0) || (which == 1 && pd->m == 0)) continue;
@@ -858,7 +865,10 @@ int ganit_hip_pdhg_create(ganit_hip_ctx* h, const ganit_hip_lp_desc* lp, unsigne
                             with_vdt(vdt, [&](auto VT) {
                                 using M = typename decltype(MT)::type;
                                 using V = typename decltype(VT)::type;
-                                for (int r = 0; r < k; ++r) launch_one<M, V>(pd, s, which);
+                                for (int r = 0; r < k; ++r) {
+                                    launch_one<M, V>(pd, s, which);
+                                    launch_one<M, V>(pd, s, 1 - which);
+                                }
                             });
                         });
                     });
