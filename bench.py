@@ -192,16 +192,19 @@ def portfolio_ladder():
 
 
 def batched_lps(count=512):
-    """512 uneven random LPs (node-relaxation shapes): the batched kernels (FP64, and FP32 + FP64
-    certification) against sequential single-instance FP64 solves with each problem resident."""
+    """512 uneven random LPs (node-relaxation shapes): the batched kernels (FP64; FP32 + FP64 certification with
+    the reference's ladder rules; and the same with FP64 candidate checks and stall 60, the headline's
+    configuration) against sequential single-instance FP64 solves with each problem resident."""
     import numpy as np
     import ganit_v2_amd as g
     probs = [g.Problem.generate("batch_lp", seed=k) for k in range(count)]
     out = dict(workload=f"{count} random LPs, m log-U[200,4000], n log-U[500,10000], 5-15 nonzeros per column, "
                         f"5% tightened bounds, seeds 0..{count - 1}, tol 1e-4")
     res = {}
-    for name, prec in (("batched_fp64", "double"), ("batched_mixed", "mixed")):
-        b = g.Batch(probs, precision=prec, tol=1e-4)
+    configs = (("batched_fp64", dict(precision="double")), ("batched_mixed", dict(precision="mixed")),
+               ("batched_mixed_fp64_checks", dict(precision="mixed", eval_fp64=True, stall_checks=60)))
+    for name, cfg in configs:
+        b = g.Batch(probs, tol=1e-4, **cfg)
         b.upload()
         b.run()
         t = time.perf_counter()
@@ -223,7 +226,7 @@ def batched_lps(count=512):
         tsum += r["solve_s"]
     wall = time.perf_counter() - t
     res["sequential_fp64"] = dict(solve_s_sum=round(tsum, 4), wall_s=round(wall, 3))
-    for name in ("batched_fp64", "batched_mixed"):
+    for name, _ in configs:
         rs = res.pop(name + "_results")
         errs = [abs(a["pobj"] - b["pobj"]) / (1 + abs(b["pobj"])) for a, b in zip(rs, seq)
                 if a["status"] == b["status"] == "OPTIMAL"]

@@ -134,13 +134,15 @@ __device__ __forceinline__ double bt_row_obj(double yi, double lw, double hg) {
 }
 
 // Same sums as check_n_block of the single-instance path, one partial row per workgroup.
-template <class M, class V, bool HASQ>
+// V: the problem vectors and the arithmetic of the row sums; VI: the iterate's working set (VI = float with
+// V = double is the FP64 evaluation of an FP32 iterate: each entry widened as it is read, exactly).
+template <class M, class V, bool HASQ, class VI = V>
 __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int* __restrict__ blocks,
                                                        const int2* __restrict__ list, CsrView<M> Q,
-                                                       const BatchState* __restrict__ st, const V* __restrict__ x,
-                                                       const V* __restrict__ y, const V* __restrict__ xa,
-                                                       const V* __restrict__ ya, const V* __restrict__ xr,
-                                                       const V* __restrict__ yr, const V* __restrict__ c,
+                                                       const BatchState* __restrict__ st, const VI* __restrict__ x,
+                                                       const VI* __restrict__ y, const VI* __restrict__ xa,
+                                                       const VI* __restrict__ ya, const VI* __restrict__ xr,
+                                                       const VI* __restrict__ yr, const V* __restrict__ c,
                                                        const V* __restrict__ l, const V* __restrict__ u,
                                                        const V* __restrict__ cres, int avg,
                                                        double* __restrict__ partials) {
@@ -155,17 +157,17 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
     const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
     stream_block_seq<M, V, 3>(
         AT, blocks, b,
-        [&](int k, int i) { return k == 0 ? y[i] : k == 1 ? (avg ? ya[i] : V(0)) : V(y[i] - yr[i]); },
+        [&](int k, int i) { return k == 0 ? V(y[i]) : k == 1 ? (avg ? V(ya[i]) : V(0)) : V(V(y[i]) - V(yr[i])); },
         [&](int j, V (&t)[3]) {
 #pragma clang fp contract(off)
             V qc = V(0), qa = V(0);
             if (HASQ) {
                 for (int p = Q.ptr[j]; p < Q.ptr[j + 1]; ++p) {
                     const V a = widen<V>(Q.val[p]);
-                    const V pc = a * x[Q.idx[p]];
+                    const V pc = a * V(x[Q.idx[p]]);
                     qc = qc + pc;
                     if (avg) {
-                        const V pa = a * xa[Q.idx[p]];
+                        const V pa = a * V(xa[Q.idx[p]]);
                         qa = qa + pa;
                     }
                 }
@@ -202,12 +204,12 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
     }
 }
 
-template <class M, class V>
+template <class M, class V, class VI = V>
 __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* __restrict__ blocks,
                                                        const int2* __restrict__ list,
-                                                       const BatchState* __restrict__ st, const V* __restrict__ x,
-                                                       const V* __restrict__ xa, const V* __restrict__ y,
-                                                       const V* __restrict__ ya, const V* __restrict__ yr,
+                                                       const BatchState* __restrict__ st, const VI* __restrict__ x,
+                                                       const VI* __restrict__ xa, const VI* __restrict__ y,
+                                                       const VI* __restrict__ ya, const VI* __restrict__ yr,
                                                        const V* __restrict__ lo, const V* __restrict__ hi,
                                                        const V* __restrict__ rres, int avg,
                                                        double* __restrict__ partials) {
@@ -221,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* 
     for (int k = 0; k < kBM; ++k) acc[k] = 0.0;
     const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
     stream_block_seq<M, V, 2>(
-        A, blocks, b, [&](int k, int j) { return k == 0 ? x[j] : (avg ? xa[j] : V(0)); },
+        A, blocks, b, [&](int k, int j) { return k == 0 ? V(x[j]) : (avg ? V(xa[j]) : V(0)); },
         [&](int i, V (&t)[2]) {
             const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
             auto pres = [&](double a) {
@@ -239,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* 
                 acc[5] += d0 * d0;
                 acc[6] += d1 * d1;
             }
-            acc[4] += bt_row_obj(double(V(y[i] - yr[i])), lw, hg);
+            acc[4] += bt_row_obj(double(V(V(y[i]) - V(yr[i]))), lw, hg);
         },
         lds, none);
     __syncthreads();
@@ -409,7 +411,7 @@ struct Batch {
     // A captured block bakes in its kernels' arguments: the dtypes, the parameters passed by value to
     // k_bt_decide and the iterate buffers X[cur] / Y[cur].  All of them are in the key, so a phase with other
     // tolerances, limits or stall rules, or an iterate reloaded into the other buffer, captures its own block.
-    using GraphKey = std::tuple<int, int, int, int, double, double, int, int, int>;
+    using GraphKey = std::tuple<int, int, int, int, double, double, int, int, int, int>;
     std::map<GraphKey, hipGraphExec_t> graphs;
     void* alloc(size_t bytes) {
         void* p = nullptr;
@@ -558,33 +560,41 @@ void bt_iteration(Batch* bt, int mdt_unused) {
     bt->cur ^= 1;
 }
 
-template <class M, class V>
+// One check of the running instances: the A^T and A passes (current point, and the average with avg), the
+// per-instance fold, the decisions and (mode kCheck) the restarts.  The iterate lives in working set V.  E64: the
+// passes evaluate it in FP64 -- FP64 matrix values and problem vectors, each iterate entry widened as it is read
+// (exact) -- the batched form of the single solve's FP64 candidate evaluation (GANIT_CHECK_FP64); the average is
+// still formed in V, as the iteration would adopt it.
+template <class M, class V, bool E64 = false>
 void bt_check(Batch* bt, int avg, int mode) {
+    using PM = std::conditional_t<E64, double, M>;  // matrix values of the passes
+    using PV = std::conditional_t<E64, double, V>;  // problem vectors and the passes' arithmetic
     hipStream_t s = bt->ctx->stream;
     BatchVecs& v = bt->vs[std::is_same<V, double>::value ? 0 : 1];
+    BatchVecs& pv = bt->vs[std::is_same<PV, double>::value ? 0 : 1];
     const int c = bt->cur;
     if (avg)
         hipLaunchKernelGGL(k_bt_average<V>, dim3(bt_grid((int64_t)bt->N + bt->Mr)), dim3(kBlock), 0, s, bt->N, bt->Mr,
                            bt->col_inst, bt->row_inst, bt->st, (const V*)v.sumx, (const V*)v.sumy, (V*)v.xa,
                            (V*)v.ya);
     if (bt->nact_at > 0) {
-        const CsrView<M> q = bt->Q ? bt->Q->view<M>() : bt->AT->view<M>();
+        const CsrView<PM> q = bt->Q ? bt->Q->view<PM>() : bt->AT->view<PM>();
         if (bt->Q)
-            hipLaunchKernelGGL((k_bt_check_n<M, V, true>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                               bt->AT->blocks, bt->list_at, q, bt->st, (const V*)v.X[c],
+            hipLaunchKernelGGL((k_bt_check_n<PM, PV, true, V>), dim3(bt->nact_at), dim3(kBlock), 0, s,
+                               bt->AT->view<PM>(), bt->AT->blocks, bt->list_at, q, bt->st, (const V*)v.X[c],
                                (const V*)v.Y[c], (const V*)v.xa, (const V*)v.ya, (const V*)v.xr, (const V*)v.yr,
-                               (const V*)v.c, (const V*)v.l, (const V*)v.u, (const V*)v.cres, avg, bt->part_n);
+                               (const PV*)pv.c, (const PV*)pv.l, (const PV*)pv.u, (const PV*)pv.cres, avg, bt->part_n);
         else
-            hipLaunchKernelGGL((k_bt_check_n<M, V, false>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
-                               bt->AT->blocks, bt->list_at, q, bt->st, (const V*)v.X[c],
+            hipLaunchKernelGGL((k_bt_check_n<PM, PV, false, V>), dim3(bt->nact_at), dim3(kBlock), 0, s,
+                               bt->AT->view<PM>(), bt->AT->blocks, bt->list_at, q, bt->st, (const V*)v.X[c],
                                (const V*)v.Y[c], (const V*)v.xa, (const V*)v.ya, (const V*)v.xr, (const V*)v.yr,
-                               (const V*)v.c, (const V*)v.l, (const V*)v.u, (const V*)v.cres, avg, bt->part_n);
+                               (const PV*)pv.c, (const PV*)pv.l, (const PV*)pv.u, (const PV*)pv.cres, avg, bt->part_n);
     }
     if (bt->nact_a > 0)
-    hipLaunchKernelGGL((k_bt_check_m<M, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<M>(),
-                       bt->A->blocks, bt->list_a, bt->st, (const V*)v.X[c], (const V*)v.xa, (const V*)v.Y[c],
-                       (const V*)v.ya, (const V*)v.yr, (const V*)v.lo, (const V*)v.hi, (const V*)v.rres, avg,
-                       bt->part_m);
+        hipLaunchKernelGGL((k_bt_check_m<PM, PV, V>), dim3(bt->nact_a), dim3(kBlock), 0, s, bt->A->view<PM>(),
+                           bt->A->blocks, bt->list_a, bt->st, (const V*)v.X[c], (const V*)v.xa, (const V*)v.Y[c],
+                           (const V*)v.ya, (const V*)v.yr, (const PV*)pv.lo, (const PV*)pv.hi, (const PV*)pv.rres, avg,
+                           bt->part_m);
     hipLaunchKernelGGL(k_bt_reduce, dim3(bt->ninst), dim3(kWave), 0, s, bt->ninst, bt->first_at, bt->first_a, bt->st,
                        bt->part_n, bt->part_m, bt->sums);
     hipLaunchKernelGGL(k_bt_decide, dim3(1), dim3(1024), 0, s, bt->ninst, bt->inst, bt->st, bt->sums, bt->P, mode,
@@ -1057,6 +1067,9 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
         bt->P.stall_checks = (int)params[4];
         bt->P.stall_exit = params[5] != 0 ? 1 : 0;
         const double time_limit = params[6];
+        // FP64 evaluation of a reduced-precision level's checks (the single solve's GANIT_CHECK_FP64)
+        const bool e64 = params[7] != 0 && (mdt != GANIT_F64 || vdt != GANIT_F64);
+        GH_REQUIRE(!e64 || (bt->on[0] && bt->A->v64), "FP64 evaluation needs the FP64 problem resident");
         GH_REQUIRE(bt->P.check_every >= 2 && bt->P.check_every % 2 == 0, "batched check interval must be even");
         GH_REQUIRE(std::isfinite(bt->P.tol) && std::isfinite(bt->P.infeas_tol), "batched tolerances must be finite");
         hipStream_t st = bt->ctx->stream;
@@ -1065,12 +1078,16 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
             using M = typename decltype(MT)::type;
             using V = typename decltype(VT)::type;
             bt_list(bt, bt_active(bt));  // launch only the selected instances' workgroups
-            bt_check<M, V>(bt, 0, kInit);  // k_restart of every running instance
+            auto check = [&](int avg, int mode) {
+                if (e64) bt_check<M, V, true>(bt, avg, mode);
+                else bt_check<M, V>(bt, avg, mode);
+            };
+            check(0, kInit);  // k_restart of every running instance
             const Batch::GraphKey key{mdt, vdt, bt->cur, bt->P.check_every, bt->P.tol, bt->P.infeas_tol,
-                                      bt->P.max_iter, bt->P.stall_checks, bt->P.stall_exit};
+                                      bt->P.max_iter, bt->P.stall_checks, bt->P.stall_exit, e64 ? 1 : 0};
             auto block = [&]() {
                 for (int k = 0; k < bt->P.check_every; ++k) bt_iteration<M, V>(bt, mdt);
-                bt_check<M, V>(bt, 1, kCheck);
+                check(1, kCheck);
                 hipLaunchKernelGGL(k_bt_deactivate, dim3(std::min(1024, (bt->ninst + 255) / 256)), dim3(256), 0, st,
                                    bt->ninst, bt->st);
             };

@@ -303,8 +303,8 @@ std::vector<Result> BatchSolver::run() {
     auto phase = [&](int slot, int mdt, int vdt, bool stall_exit, int stall_checks) {
         const auto t0 = Clock::now();
         const double left = std::max(0.0, opt.time_limit - since(t_solve));
-        const double params[7] = {opt.tol, opt.infeas_tol, (double)opt.max_iter, (double)opt.check_every,
-                                  (double)stall_checks, stall_exit ? 1.0 : 0.0, left};
+        const double params[8] = {opt.tol, opt.infeas_tol, (double)opt.max_iter, (double)opt.check_every,
+                                  (double)stall_checks, stall_exit ? 1.0 : 0.0, left, opt.eval_fp64 ? 1.0 : 0.0};
         if (last_vdt != vdt) chk(ganit_hip_batch_handover(bt, last_vdt, vdt));
         else chk(ganit_hip_batch_handover(bt, vdt, vdt));  // fresh running sums
         last_vdt = vdt;

@@ -248,7 +248,9 @@ int ganit_hip_batch_handover(ganit_hip_batch* bt, int from_vdt, int to_vdt);
 int ganit_hip_batch_select(ganit_hip_batch* bt, const int* sel);
 /* Run one precision phase for the selected instances until each converges, stalls (stall_exit),
  * is certified infeasible or hits the iteration limit; params = {tol, infeas_tol, max_iter,
- * check_every (even), stall_checks, stall_exit, time_limit_s}. */
+ * check_every (even), stall_checks, stall_exit, time_limit_s, eval_fp64}: eval_fp64 != 0 evaluates a
+ * reduced-precision phase's checks in FP64 (FP64 matrix and problem data, the iterate widened as it is read), as
+ * ganit_hip_pdhg_check's GANIT_CHECK_FP64 does for one problem. */
 int ganit_hip_batch_phase(ganit_hip_batch* bt, int mdt, int vdt, const double* params, int* running_left);
 /* Autotuner of the batched kernels: times `reps` iterations at (mdt, vdt) for each row-block partition
  * candidate (rows, nonzeros per workgroup) and keeps the fastest; ms[c] = ms per iteration of candidate c

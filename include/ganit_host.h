@@ -120,7 +120,7 @@ void ganit_solver_free(ganit_solver* s);
  * the reference solves one problem per call; this serves branch-and-bound node relaxations, strong-branching
  * probes and batches of QP layers).
  * Each instance gets the single solve's scaling, initial point, step sizes, restart rules and FP64
- * certification; opts as ganit_solve_gpu (check_every must be even; eval_fp64 is ignored; values 2 runs as
+ * certification; opts as ganit_solve_gpu (check_every must be even; values 2 runs as
  * FP32).
  * run: out[18*k..] = instance k's results in ganit_solve_gpu's layout (solve_s is the whole batch's);
  * x, y = the instances' original-space solutions concatenated in order (either may be NULL).

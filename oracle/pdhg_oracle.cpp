@@ -10,8 +10,9 @@
 //   * scaling + norm estimation ........ src/scaling.cpp:8-155, src/scaling.hpp:8-23
 //   * CPU backend ...................... src/backend_cpu.cpp:25-156
 //   * driver (phase / evaluate / run) .. src/pdhg_impl.hpp:36-328
-// Extensions with no reference code (BF16 matrix values, Kahan-compensated FP32 reductions) are
-// marked where they appear; they are pinned by tests against closed forms and error bounds.
+// Extensions with no reference code (BF16 matrix values, Kahan-compensated FP32 reductions, FP64 evaluation of
+// reduced-precision checks) are marked where they appear; they are pinned by tests against closed forms and error
+// bounds, and the FP64 evaluation by being the reference's own FP64 evaluation applied to exactly widened vectors.
 //
 // How each part is pinned (tests/test_oracle.py): the reference binary's golden JSON lines
 // (tests/golden/reference_runs.jsonl) for end-to-end runs, brute-force / closed-form checks for the
@@ -699,6 +700,8 @@ struct Options {
     int stall_checks_l2 = 24;
     int l1_matrix = 0;   // extension: 0 = FP32 matrix values in L1/L2, 1 = BF16
     int kahan = 0;       // extension: Kahan-compensated FP32 reductions in L1
+    int eval_fp64 = 0;   // extension: L1/L2 checks evaluate the iterate in FP64 (FP64 matrix and problem data,
+                         // each iterate entry widened, exactly)
 };
 
 struct KKT {  // pdhg_impl.hpp:36-42
@@ -755,12 +758,16 @@ static KKT kkt_at(B& be, const Shared& sh, const typename B::Vec& x, const typen
     return k;
 }
 
+// e64 (extension, a reduced-precision level only): every KKT evaluation of the phase -- the initial point, the
+// iterate, the average and the infeasibility ray -- is the FP64 one (kkt_at with the FP64 backend) of the level's
+// vectors widened to double, exactly; the level's own reductions are then used only for the restart distances,
+// which widen each entry too.  Kahan-compensated level reductions do not apply with it.
 template <class B>
 static Exit phase(Shared& sh, State& st, double phase_tol, bool stall_exit, const char* tag, int stall_checks,
-                  bool kahan) {
+                  bool kahan, bool e64 = false) {
     const ScaledLP& s = sh.s;
     const Options& opt = sh.opt;
-    B be(s, kahan);
+    B be(s, kahan && !e64);
     using Vec = typename B::Vec;
     const int m = s.m, n = s.n;
     const bool hasq = s.has_q();
@@ -778,17 +785,36 @@ static Exit phase(Shared& sh, State& st, double phase_tol, bool stall_exit, cons
         tau = sh.eta / w;
         if (hasq) tau = std::min(tau, 0.998 / (0.5 * s.normQ + sigma * sh.normA * sh.normA));
     };
+    // the FP64 evaluation (e64): FP64 backend and work vectors, the level's vectors widened into w1, w2
+    CpuBackend<double, double> be64(s);
+    std::vector<double> w1, w2, ax64(e64 ? m : 0), aty64(e64 ? n : 0), qx64(e64 && hasq ? n : 0);
+    auto widen_into = [](const Vec& v, std::vector<double>& d) { d.assign(v.begin(), v.end()); };
+    auto kkt_of = [&](const Vec& xv, const Vec& yv) {
+        if (!e64) return kkt_at(be, sh, xv, yv, ax_tmp, aty_tmp, qx_tmp);
+        widen_into(xv, w1);
+        widen_into(yv, w2);
+        return kkt_at(be64, sh, w1, w2, ax64, aty64, qx64);
+    };
     auto infeasible = [&]() {  // pdhg_impl.hpp:110-118
-        be.sub(y, yr, ray);
-        be.spmv_AT(ray, aty_tmp);
-        double rd2 = 0, vobj = 0;
-        be.dual_res(aty_tmp, nullptr, rd2, vobj, false);
-        const double dray = (be.row_dual_obj(ray) + vobj) * sh.obj_scale;
+        double rd2 = 0, vobj = 0, robj = 0;
+        if (e64) {  // the ray y - yr formed in FP64 from the widened entries, then the FP64 passes
+            w2.resize(m);
+            for (int i = 0; i < m; ++i) w2[i] = double(y[i]) - double(yr[i]);
+            be64.spmv_AT(w2, aty64);
+            be64.dual_res(aty64, nullptr, rd2, vobj, false);
+            robj = be64.row_dual_obj(w2);
+        } else {
+            be.sub(y, yr, ray);
+            be.spmv_AT(ray, aty_tmp);
+            be.dual_res(aty_tmp, nullptr, rd2, vobj, false);
+            robj = be.row_dual_obj(ray);
+        }
+        const double dray = (robj + vobj) * sh.obj_scale;
         if (!(dray > 0)) return false;
         return std::sqrt(rd2) * (1.0 + s.bnorm) / dray <= opt.infeas_tol;
     };
 
-    KKT k_restart = kkt_at(be, sh, x, y, ax_tmp, aty_tmp, qx_tmp);
+    KKT k_restart = kkt_of(x, y);
     if (st.restart_err > 0) k_restart.err = std::max(k_restart.err, st.restart_err);
     double prev_cand = std::numeric_limits<double>::infinity();
     long it_since = 0;
@@ -813,10 +839,10 @@ static Exit phase(Shared& sh, State& st, double phase_tol, bool stall_exit, cons
         ++it_since;
         if (st.it % opt.check_every != 0) continue;
 
-        KKT kc = kkt_at(be, sh, x, y, ax_tmp, aty_tmp, qx_tmp);
+        KKT kc = kkt_of(x, y);
         be.scale_into(sumx, 1.0 / (double)it_since, xa);
         be.scale_into(sumy, 1.0 / (double)it_since, ya);
-        KKT ka = kkt_at(be, sh, xa, ya, ax_tmp, aty_tmp, qx_tmp);
+        KKT ka = kkt_of(xa, ya);
         const double t = sh.elapsed();
         if (!kc.finite() && !ka.finite()) { why = eNumerical; best = kc; break; }
         const bool avg_better = ka.finite() && (!kc.finite() || ka.err < kc.err);
@@ -892,11 +918,12 @@ static int status_of(Exit e) {  // pdhg_impl.hpp:232-240
 
 template <class ML>
 static Exit run_level1(Shared& sh, State& st, const Options& opt) {
-    return phase<CpuBackend<ML, float>>(sh, st, opt.tol, true, "L1", opt.stall_checks, opt.kahan != 0);
+    return phase<CpuBackend<ML, float>>(sh, st, opt.tol, true, "L1", opt.stall_checks, opt.kahan != 0,
+                                        opt.eval_fp64 != 0);
 }
 template <class ML>
 static Exit run_level2(Shared& sh, State& st, const Options& opt) {
-    return phase<CpuBackend<ML, double>>(sh, st, opt.tol, true, "L2", opt.stall_checks_l2, false);
+    return phase<CpuBackend<ML, double>>(sh, st, opt.tol, true, "L2", opt.stall_checks_l2, false, opt.eval_fp64 != 0);
 }
 
 Result run_pdhg(const LP& lp, const Options& opt) {  // pdhg_impl.hpp:243-328
@@ -1065,7 +1092,7 @@ void oracle_scaled_export(void* h, int* aptr, int* aidx, double* aval, int* tptr
 }
 
 // opts: [tol, max_iter, time_limit, check_every, ruiz_iters, pock_chambolle, verbose, infeas_tol, precision,
-//        stall_checks, stall_checks_l2, l1_matrix(0 f32 / 1 bf16), kahan]
+//        stall_checks, stall_checks_l2, l1_matrix(0 f32 / 1 bf16), kahan, eval_fp64]
 // out:  [status, pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,
 //        setup_s, solve_s, normA]
 int oracle_solve(void* h, const double* o, double* out, double* x, double* y) {
@@ -1075,6 +1102,7 @@ int oracle_solve(void* h, const double* o, double* out, double* x, double* y) {
         opt.ruiz_iters = (int)o[4]; opt.pock_chambolle = o[5] != 0; opt.verbose = (int)o[6];
         opt.infeas_tol = o[7]; opt.precision = (int)o[8]; opt.stall_checks = (int)o[9];
         opt.stall_checks_l2 = (int)o[10]; opt.l1_matrix = (int)o[11]; opt.kahan = (int)o[12];
+        opt.eval_fp64 = (int)o[13];
         Result r = run_pdhg(H(h)->lp, opt);
         const double vals[13] = {(double)r.status, r.pobj, r.dobj, r.rel_p, r.rel_d, r.rel_gap,
                                  (double)r.iterations, (double)r.fp32_iterations, (double)r.l2_iterations,

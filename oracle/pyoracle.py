@@ -139,10 +139,11 @@ class OracleLP:
 
     def solve(self, tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10,
               pock_chambolle=True, verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12,
-              stall_checks_l2=24, l1_matrix=0, kahan=0, trace=False) -> dict:
+              stall_checks_l2=24, l1_matrix=0, kahan=0, eval_fp64=False, trace=False) -> dict:
         d = self.dims()
         opts = np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
-                         infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, l1_matrix, kahan], np.float64)
+                         infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, l1_matrix, kahan,
+                         int(bool(eval_fp64))], np.float64)
         out = np.zeros(13)
         x, y = np.zeros(d["n"]), np.zeros(d["m"])
         if trace:

@@ -174,3 +174,32 @@ def test_batch_mixed_c3_outliers_follow_the_oracle_ladder(oracle):
     err = abs(res[0]["pobj"] - pstar) / (1 + abs(pstar))
     print(f"seed 185: mixed ladder objective error vs the FP64 optimum {err:.2e}", flush=True)
     assert err <= 1e-2
+
+
+def test_batch_mixed_with_fp64_checks_matches_oracle(oracle):
+    """The batched form of the FP64 candidate evaluation (eval_fp64: L1's checks read the FP32 iterate, widen each
+    entry and evaluate it against the FP64 matrix and problem data) against the oracle's own FP64-evaluation ladder
+    per instance (the oracle applies the reference's FP64 evaluation to the widened vectors): same status and, where
+    the FP64 check sums do not flip a restart, the same iterations, L1 split and objective; against each instance's
+    single GPU solve with the same options (whose autotuned kernel variant may sum rows in another order): same
+    status, certified, objective within the tolerance; and with the FP64 checks at least as many instances finish
+    in L1 as with the reference's level-precision checks."""
+    probs = _problems(16, seed0=300)
+    opts = dict(precision="mixed", tol=1e-4, eval_fp64=True, stall_checks=60)
+    res = g.Batch(probs, **opts).run()
+    oracle.set_threads(4)
+    refs = [oracle.OracleLP.from_export(p.export()).solve(**opts) for p in probs]
+    _compare(res, refs, 1e-4, 0.7)
+    for r, o in zip(res, refs):
+        if r["iters"] == o["iters"]:
+            assert r["fp32_iters"] == o["fp32_iters"]
+    for k, (p, r) in enumerate(zip(probs, res)):
+        s = p.solve_gpu(**opts)
+        assert r["status"] == s["status"], k
+        if s["status"] == "OPTIMAL":
+            assert abs(r["pobj"] - s["pobj"]) <= 5e-4 * (1 + abs(s["pobj"])), k
+    plain = g.Batch(probs, precision="mixed", tol=1e-4).run()
+    l1_only = sum(r["fp32_iters"] == r["iters"] for r in res if r["status"] == "OPTIMAL")
+    l1_only_plain = sum(r["fp32_iters"] == r["iters"] for r in plain if r["status"] == "OPTIMAL")
+    print(f"finished in L1: {l1_only} with FP64 checks, {l1_only_plain} with level-precision checks", flush=True)
+    assert l1_only >= l1_only_plain
