@@ -115,3 +115,32 @@ def test_blending_fp64_accurate_matches_reference(blending):
     assert _certified(r, 1e-8)
     assert abs(r["pobj"] - ref["pobj"]) <= 1e-7 * (1 + abs(ref["pobj"]))
 
+
+
+def _oracle_run(config):
+    import json
+    import os
+    from conftest import GOLDEN
+    with open(os.path.join(GOLDEN, "oracle_runs.jsonl")) as f:
+        return next(r for r in map(json.loads, f) if r["instance"] == INSTANCE and r["config"] == config)
+
+
+@pytest.mark.parametrize("config", ["mixed_fp64_checks_stall60", "bf16_fp64_checks_stall60"])
+def test_blending_fp64_check_ladders_follow_the_oracle(blending, config):
+    """The benchmark's configuration (FP32 values and vectors in L1, FP64 candidate checks, stall 60) and its BF16
+    counterpart against the oracle's own runs of the same ladder on the same LP (tests/golden/oracle_runs.jsonl,
+    written by tests/golden/make_oracle_runs.py; the reference CLI has no FP64-evaluation option): same status,
+    iterations, L1 split and restarts, objective within 1e-7 relative.  Not bitwise: A's 365 rows of 960 entries
+    are summed by a block-wide tree on the GPU and sequentially by the oracle, so their FP32 row sums round apart
+    and the trajectories drift by a few ulps (5e-9 relative in the objective)."""
+    o = _oracle_run(config)
+    opts = {"mixed_fp64_checks_stall60": dict(precision="mixed", stall_checks=60, eval_fp64=True),
+            "bf16_fp64_checks_stall60": dict(precision="mixed", values="bf16", stall_checks=60, eval_fp64=True)}[config]
+    r = _solve(blending, **opts)
+    err = abs(r["pobj"] - o["pobj"]) / (1 + abs(o["pobj"]))
+    print(f"gpu {config}: {r['status']} it {r['iters']} (L1 {r['fp32_iters']}) restarts {r['restarts']} obj "
+          f"{r['pobj']!r} | oracle it {o['iters']} (L1 {o['fp32_iters']}) restarts {o['restarts']} obj {o['pobj']!r} "
+          f"| rel. diff {err:.2e}", flush=True)
+    assert r["status"] == o["status"] == "OPTIMAL" and _certified(r)
+    assert (r["iters"], r["fp32_iters"], r["restarts"]) == (o["iters"], o["fp32_iters"], o["restarts"])
+    assert err <= 1e-7
