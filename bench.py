@@ -556,7 +556,7 @@ def main():
         sv.run()
         r = sv.run()
         ladder.append(dict(cfg=cfg, solve_s=round(r["solve_s"], 5), status=r["status"], iters=r["iters"],
-                           l1_iters=r["fp32_iters"], l2_iters=r["l2_iters"], obj=r["pobj"]))
+                           l1_iters=r["fp32_iters"], l2_iters=r["l2_iters"], obj=r["pobj"], variants=sv.variants()))
         print(f"[bench] ladder {ladder[-1]}", file=sys.stderr, flush=True)
         del sv
     t_mixed = sum(r["solve_s"] for r in runs) / len(runs)
@@ -647,6 +647,7 @@ def main():
         "iters": last["iters"], "l1_iters": last["fp32_iters"], "restarts": last["restarts"],
         "fp64_s": round(t_fp64, 5), "fp64_iters": f_last["iters"], "fp64_status": f_last["status"],
         "speedup_vs_fp64": round(t_fp64 / t_mixed, 3),
+        "variants": {"mixed": mixed.variants(), "fp64": fp64.variants()},
         "with_upload": cold_run,
         "obj": last["pobj"], "obj_fp64": f_last["pobj"], "obj_rel_err_vs_fp64": obj_err,
         "certified_rel": [last["rel_p"], last["rel_d"], last["rel_gap"]],

@@ -74,6 +74,7 @@ def host_lib() -> ctypes.CDLL:
     lib.ganit_solver_free.argtypes = [_vp]
     lib.ganit_solver_info.argtypes = [_vp, _f64p]
     lib.ganit_solver_index_bytes.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+    lib.ganit_solver_variant.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
     lib.ganit_solver_adjoint.argtypes = [_vp, _f64p, _f64p, _vp, _vp, _vp, _vp]
     lib.ganit_solver_set_solution.argtypes = [_vp, _f64p, _f64p, ctypes.c_int]
     lib.ganit_batch_create.restype = _vp
@@ -356,6 +357,15 @@ class Solver:
         b = ctypes.c_int(4)
         _check_host(host_lib().ganit_solver_index_bytes(self._h, int(which), ctypes.byref(b)), "index_bytes")
         return int(b.value)
+
+    def variants(self) -> dict:
+        """The kernel variant kept for each matrix: 0 = LDS-staged row blocks, else lanes per row (1..64)."""
+        out = {}
+        for which, name in ((0, "A"), (1, "AT")):
+            v = ctypes.c_int(0)
+            _check_host(host_lib().ganit_solver_variant(self._h, which, ctypes.byref(v)), "variant")
+            out[name] = int(v.value)
+        return out
 
     def kernel_times(self, level: int, reps: int = 50) -> dict:
         """Device ms of the fused iteration kernels at level 0 (L1), 1 (L2) or 2 (L3): primal / dual = each kernel

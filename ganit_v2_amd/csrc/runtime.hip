@@ -658,25 +658,32 @@ void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& l
     try {
         GH_CHECK(hipEventCreate(&e0));
         GH_CHECK(hipEventCreate(&e1));
-        // Warm the clocks and caches first, then time every candidate in two interleaved rounds and keep
-        // each one's best time, so a candidate timed early is not penalised by a cold GPU.  A candidate more
-        // than twice as slow as the best after the first round cannot be chosen and is not timed again.
+        // Warm the clocks and caches first.  A short screening round (a third of the launches) drops every
+        // candidate more than twice as slow as the fastest (a lanes-per-row width far from the rows' lengths can
+        // be 20x slower than the row blocks); the others are timed in two interleaved full rounds and keep their
+        // best, so a candidate timed early is not penalised by a cold GPU.  The choice rests on the full rounds
+        // only: the screening's short windows are too noisy to rank near-equal variants.
         launch(20);
-        for (int round = 0; round < 2; ++round) {
-            const float lead = *std::min_element(best.begin(), best.end());
-            for (size_t c = 0; c < nc; ++c) {
-                if (skip[c] || (round > 0 && best[c] > 2.0f * lead)) continue;
-                use(c);
-                launch(2);
-                GH_CHECK(hipEventRecord(e0, ctx->stream));
-                launch(reps);
-                GH_CHECK(hipEventRecord(e1, ctx->stream));
-                GH_CHECK(hipEventSynchronize(e1));
-                float ms = 0;
-                GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
-                best[c] = std::min(best[c], ms);
-            }
-        }
+        auto time_one = [&](size_t c, int k) {
+            use(c);
+            launch(2);
+            GH_CHECK(hipEventRecord(e0, ctx->stream));
+            launch(k);
+            GH_CHECK(hipEventRecord(e1, ctx->stream));
+            GH_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return ms / (float)k;
+        };
+        const int screen = std::max(1, reps / 3);
+        std::vector<float> first(nc, 1e30f);
+        for (size_t c = 0; c < nc; ++c)
+            if (!skip[c]) first[c] = time_one(c, screen);
+        const float lead = *std::min_element(first.begin(), first.end());
+        for (size_t c = 0; c < nc; ++c) skip[c] = skip[c] || first[c] > 2.0f * lead;
+        for (int round = 0; round < 2; ++round)
+            for (size_t c = 0; c < nc; ++c)
+                if (!skip[c]) best[c] = std::min(best[c], time_one(c, reps));
         // the fastest row-block partition, unless a row-group width beats it by a clear margin: the row
         // blocks' row sums are the CPU row loop's bits, and a short timing window can rank near-equal variants
         // either way

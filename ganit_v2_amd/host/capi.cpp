@@ -291,6 +291,13 @@ int ganit_solver_index_bytes(ganit_solver* h, int which, int* bytes) {
     return guard([&] { *bytes = reinterpret_cast<ganit::GpuSolver*>(h)->index_bytes(which); });
 }
 
+int ganit_solver_variant(ganit_solver* h, int which, int* lanes) {
+    return guard([&] {
+        if (!h || !lanes) throw std::runtime_error("null argument");
+        *lanes = reinterpret_cast<ganit::GpuSolver*>(h)->variant(which);
+    });
+}
+
 int ganit_solver_info(const ganit_solver* h, double* info) {
     return guard([&] {
         if (!h || !info) throw std::runtime_error("null argument");

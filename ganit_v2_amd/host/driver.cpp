@@ -365,6 +365,13 @@ int GpuSolver::index_bytes(int which) {
     return b;
 }
 
+int GpuSolver::variant(int which) {
+    upload();
+    int lanes = 0;
+    chk(ganit_hip_pdhg_plan(p_->pd, which, &lanes, nullptr));
+    return lanes;
+}
+
 Result GpuSolver::run() {
     Impl& I = *p_;
     const Options& opt = I.opt;

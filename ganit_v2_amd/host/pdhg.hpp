@@ -118,11 +118,16 @@ class GpuSolver {
     GpuSolver& operator=(const GpuSolver&) = delete;
     void upload();   // device-resident problem in every precision the ladder needs; idempotent
     Result run();    // one full solve from the initial point (uploads first if needed)
-    // Device milliseconds of the fused iteration kernels at a precision level (0 = L1, 1 = L2, 2 = L3):
-    // ms[0] primal kernel, ms[1] dual kernel (each launched back to back), ms[2] one whole iteration.
+    // Device milliseconds of the fused iteration kernels at a precision level (0 = L1, 1 = L2, 2 = L3): ms[0]
+    // primal kernel, ms[1] dual kernel (each launched after a 64 MiB streaming read, which leaves none of its data
+    // in L2, as in a solve), ms[2] one whole iteration in solve order, ms[3] / ms[4] primal / dual launched back to
+    // back, ms[5] the 64 MiB streaming read alone.
     void kernel_times(int level, int reps, double* ms);
     // Bytes per column index the fused kernels read for A (0), A^T (1) or Q (2): 2 with compact indices, else 4.
     int index_bytes(int which);
+    // The kernel variant the autotuner kept for A (0), A^T (1) or Q (2): 0 = LDS-staged row blocks, else lanes per
+    // row (1..64).
+    int variant(int which);
     // Gradient of L(x*) with respect to c, the active row bounds and diag Q, given gx = dL/dx* at the last
     // run()'s solution: the adjoint PDHG iteration of libganit_hip.so at that solution.
     AdjointResult adjoint(const std::vector<double>& gx, const AdjointOptions& opt);

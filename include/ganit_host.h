@@ -97,6 +97,8 @@ int ganit_solver_run(ganit_solver* s, double* out, double* x, double* y);
 int ganit_solver_kernel_times(ganit_solver* s, int level, int reps, double* ms);
 /* bytes per column index the fused kernels read for A (0), A^T (1), Q (2): 2 (compact) or 4 */
 int ganit_solver_index_bytes(ganit_solver* s, int which, int* bytes);
+/* the kernel variant kept for A (0), A^T (1), Q (2): 0 = LDS-staged row blocks, else lanes per row (1..64) */
+int ganit_solver_variant(ganit_solver* s, int which, int* lanes);
 /* Differentiation through the last ganit_solver_run (no reference counterpart): gx = dL/dx* (n);
  * aopts[0..5] = precision (0 FP64, 1 FP32 matrix + vectors, which needs a mixed/dynamic solver), tol,
  * max operator applications, method (0 GMRES on (I - J^T) lambda = g, tol on the relative residual;
