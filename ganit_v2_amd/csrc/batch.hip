@@ -35,6 +35,16 @@ namespace ganit_hip {
 constexpr int kBN = 12, kBM = 7;
 
 // ------------------------------------------------------------------ iteration kernels
+// A listed workgroup {row block, instance}: its block's head, and whether it works.  The head's loads are issued
+// with the instance state's and the test reads both (a listed block always has a row; `&`, not `&&`), so the two
+// round trips overlap instead of one following the other.
+template <class M>
+__device__ __forceinline__ bool bt_head(const CsrView<M>& a, const int* __restrict__ blocks, int2 e,
+                                        const BatchState* __restrict__ st, BlockHead& h) {
+    h = block_head(a, blocks, e.x);
+    return (st[e.y].active != 0) & (h.d.y > h.d.x);
+}
+
 template <class M, class V, bool HASQ>
 __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* __restrict__ blocks,
                                                       const int2* __restrict__ list, CsrView<M> Q,
@@ -45,16 +55,14 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
                                                       V* __restrict__ sumx) {
     __shared__ __align__(16) V lds[kTile + kWin];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
-    const int b = e.x;
-    const BatchState& s = st[e.y];
-    if (!s.active) return;
-    const V tau = V(s.tau);
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
+    BlockHead h;
+    if (!bt_head(AT, blocks, e, st, h)) return;
+    const V tau = V(st[e.y].tau);
     struct Ops {
         V x, c, l, u, s;
     };
-    stream_block_pf<M, V, 1, true>(
-        AT, blocks, b, [&](int, int i) { return y[i]; },
+    stream_head_pf<M, V, 1, true>(
+        AT, h, [&](int, int i) { return y[i]; },
         [&](int j) { return Ops{x[j], c[j], l[j], u[j], sumx[j]}; },
         [&](int j, V (&aty)[1], Ops& o) {
 #pragma clang fp contract(off)
@@ -69,7 +77,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_primal(CsrView<M> AT, const int* 
             primal_update<V>(o.x, aty[0], qx, o.c, o.l, o.u, tau, xn[j], xbar[j], sm);
             sumx[j] = sm;
         },
-        lds, none);
+        lds);
 }
 
 template <class M, class V>
@@ -80,23 +88,21 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
                                                     V* __restrict__ yn, V* __restrict__ sumy) {
     __shared__ __align__(16) V lds[kTile + kWin];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
-    const int b = e.x;
-    const BatchState& s = st[e.y];
-    if (!s.active) return;
-    const V sigma = V(s.sigma);
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
+    BlockHead h;
+    if (!bt_head(A, blocks, e, st, h)) return;
+    const V sigma = V(st[e.y].sigma);
     struct Ops {
         V y, lo, hi, s;
     };
-    stream_block_pf<M, V, 1, true>(
-        A, blocks, b, [&](int, int j) { return xbar[j]; },
+    stream_head_pf<M, V, 1, true>(
+        A, h, [&](int, int j) { return xbar[j]; },
         [&](int i) { return Ops{y[i], lo[i], hi[i], sumy[i]}; },
         [&](int i, V (&axb)[1], Ops& o) {
             V sm = o.s;
             dual_update<V>(o.y, axb[0], o.lo, o.hi, sigma, yn[i], sm);
             sumy[i] = sm;
         },
-        lds, none);
+        lds);
 }
 
 // ------------------------------------------------------------------ check kernels
@@ -150,15 +156,16 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
     __shared__ double red[(kBlock / kWave) * kBN];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
     const int b = e.x;
-    if (!st[e.y].active) return;
+    BlockHead h;
+    if (!bt_head(AT, blocks, e, st, h)) return;
     double acc[kBN];
 #pragma unroll
     for (int k = 0; k < kBN; ++k) acc[k] = 0.0;
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
-    stream_block_seq<M, V, 3>(
-        AT, blocks, b,
+    stream_head_seq_pf<M, V, 3, false>(
+        AT, h,
         [&](int k, int i) { return k == 0 ? V(y[i]) : k == 1 ? (avg ? V(ya[i]) : V(0)) : V(V(y[i]) - V(yr[i])); },
-        [&](int j, V (&t)[3]) {
+        [](int) { return NoRowOperands{}; },
+        [&](int j, V (&t)[3], NoRowOperands&) {
 #pragma clang fp contract(off)
             V qc = V(0), qa = V(0);
             if (HASQ) {
@@ -195,7 +202,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_n(CsrView<M> AT, const int*
             acc[8] += r2;
             acc[9] += ob;
         },
-        lds, none);
+        lds, ~0u);
     __syncthreads();
     block_sum<kBN>(acc, red);
     if (threadIdx.x == 0) {
@@ -217,14 +224,15 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* 
     __shared__ double red[(kBlock / kWave) * kBM];
     const int2 e = list[xcd_block_id()];  // a workgroup of a running instance: {row block, instance}
     const int b = e.x;
-    if (!st[e.y].active) return;
+    BlockHead h;
+    if (!bt_head(A, blocks, e, st, h)) return;
     double acc[kBM];
 #pragma unroll
     for (int k = 0; k < kBM; ++k) acc[k] = 0.0;
-    const SplitPlan none{0x7fffffff, nullptr, nullptr, nullptr, 0};
-    stream_block_seq<M, V, 2>(
-        A, blocks, b, [&](int k, int j) { return k == 0 ? V(x[j]) : (avg ? V(xa[j]) : V(0)); },
-        [&](int i, V (&t)[2]) {
+    stream_head_seq_pf<M, V, 2, false>(
+        A, h, [&](int k, int j) { return k == 0 ? V(x[j]) : (avg ? V(xa[j]) : V(0)); },
+        [](int) { return NoRowOperands{}; },
+        [&](int i, V (&t)[2], NoRowOperands&) {
             const double lw = double(lo[i]), hg = double(hi[i]), rr = double(rres[i]), yi = double(y[i]);
             auto pres = [&](double a) {
                 const double p = fmin(fmax(a, lw), hg);
@@ -243,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_bt_check_m(CsrView<M> A, const int* 
             }
             acc[4] += bt_row_obj(double(V(V(y[i]) - V(yr[i]))), lw, hg);
         },
-        lds, none);
+        lds, ~0u);
     __syncthreads();
     block_sum<kBM>(acc, red);
     if (threadIdx.x == 0) {

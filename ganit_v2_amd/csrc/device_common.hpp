@@ -179,6 +179,32 @@ struct SplitPlan {
 };
 
 // ---------------------------------------------------------------- LDS-staged row blocks
+// A row block's head, the first thing a kernel reads of it: the descriptor (first row, end row, first nonzero,
+// end nonzero; one 16-byte load) and, with compact indices, the block's base column and window length (an
+// independent load, in flight at the same time).  wraw: window length, 0 none, -1 a long row with 32-bit indices.
+// A batched kernel loads it beside its instance's state, so the two round trips overlap.
+struct BlockHead {
+    int4 d;
+    int cb, wraw;
+};
+template <class M>
+__device__ __forceinline__ BlockHead block_head(const CsrView<M>& a, const int* __restrict__ blocks, int b) {
+    BlockHead h;
+    h.d = reinterpret_cast<const int4*>(blocks)[b];
+    h.cb = 0;
+    h.wraw = -1;
+    if (a.idx16) {
+        const int2 bw = reinterpret_cast<const int2*>(a.bcol)[b];
+        h.cb = bw.x;
+        h.wraw = bw.y;
+    }
+    return h;
+}
+
+template <class M, class V, int NX, bool WIN, class Gather, class Pref, class RowFn>
+__device__ __forceinline__ void stream_head_pf(const CsrView<M>& a, const BlockHead& h, Gather gather, Pref pref,
+                                               RowFn row_fn, V* lds);
+
 // One block of the stream variant: rows [r0, r1) of `a` whose nonzeros fit one LDS tile.  Every product
 // val * x_k[idx] (k < NX gathered vectors, `gather(k, i)`) is formed by a coalesced sweep over the block's
 // nonzeros and stored in LDS (rounded to V there); then thread t adds up row r0 + t sequentially in row
@@ -243,17 +269,18 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
         }
         return;
     }
-    // block descriptor: first row, end row, first nonzero, end nonzero (one 16-byte load); with compact
-    // indices, the block's base column and window length (an independent load, in flight at the same time)
-    const int4 d = reinterpret_cast<const int4*>(blocks)[b];
+    stream_head_pf<M, V, NX, WIN>(a, block_head(a, blocks, b), gather, pref, row_fn, lds);
+}
+
+// The row-block part of stream_block_pf, on a head already loaded (block_head).
+template <class M, class V, int NX, bool WIN, class Gather, class Pref, class RowFn>
+__device__ __forceinline__ void stream_head_pf(const CsrView<M>& a, const BlockHead& h, Gather gather, Pref pref,
+                                               RowFn row_fn, V* lds /* NX * kTile (+ kWin with WIN) */) {
+    static_assert(!WIN || NX == 1, "windowed gathers stage one vector");
+    const int4 d = h.d;
     const uint16_t* __restrict__ i16 = a.idx16;
-    int cb = 0, wraw = -1;  // base column; window length (0: none; -1: a long row with 32-bit indices)
-    if (i16) {
-        const int2 bw = reinterpret_cast<const int2*>(a.bcol)[b];
-        cb = bw.x;
-        wraw = bw.y;
-    }
-    const int wl = WIN ? max(wraw, 0) : 0;
+    const int cb = h.cb;
+    const int wl = WIN ? max(h.wraw, 0) : 0;
     const int r0 = d.x, r1 = d.y;
     const int base = d.z;
     const int cnt = d.w - d.z;
@@ -301,7 +328,7 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
                 return;
             }
         }
-        if (i16 && wraw >= 0) {
+        if (i16 && h.wraw >= 0) {
 #pragma unroll 8
             for (int p = base + threadIdx.x; p < base + cnt; p += kBlock) {
                 const V av = widen<V>(a.val[p]);
@@ -423,6 +450,10 @@ __device__ __forceinline__ void stream_block(const CsrView<M>& a, const int* __r
         [&](int r, V (&t)[NX], NoRowOperands&) { row_fn(r, t); }, lds, sp);
 }
 
+template <class M, class V, int NX, bool WIN, class Gather, class Pref, class RowFn>
+__device__ __forceinline__ void stream_head_seq_pf(const CsrView<M>& a, const BlockHead& h, Gather gather, Pref pref,
+                                                   RowFn row_fn, V* lds, unsigned act);
+
 // act: bit v set = vector v is wanted; an unwanted vector's sums are 0 and, on a staged block, it costs no
 // pass (the long-row and chunk paths still gather it, and the gather returns 0 for it).  pref(r) loads row r's
 // operands before the sweep (as in stream_block_pf), and row_fn(r, sums, operands) runs after it.
@@ -431,15 +462,23 @@ __device__ __forceinline__ void stream_block_seq_pf(const CsrView<M>& a, const i
                                                     Gather gather, Pref pref, RowFn row_fn,
                                                     V* lds /* kTile (+ kWin with WIN) */, const SplitPlan& sp,
                                                     unsigned act = ~0u) {
-    static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
     if (sp.finish || b >= sp.nblocks) {
         stream_block_pf<M, V, NX>(a, blocks, b, gather, pref, row_fn, lds, sp);
         return;
     }
-    const int4 d = reinterpret_cast<const int4*>(blocks)[b];
+    stream_head_seq_pf<M, V, NX, WIN>(a, block_head(a, blocks, b), gather, pref, row_fn, lds, act);
+}
+
+// The row-block part of stream_block_seq_pf, on a head already loaded (block_head).
+template <class M, class V, int NX, bool WIN, class Gather, class Pref, class RowFn>
+__device__ __forceinline__ void stream_head_seq_pf(const CsrView<M>& a, const BlockHead& h, Gather gather, Pref pref,
+                                                   RowFn row_fn, V* lds /* kTile (+ kWin with WIN) */,
+                                                   unsigned act) {
+    static_assert((kBlock / kWave) * NX <= kTile, "reduction slots exceed the tile");
+    const int4 d = h.d;
     const int r0 = d.x, r1 = d.y, base = d.z, cnt = d.w - d.z;
     if (r1 - r0 == 1 && cnt > kLong) {  // long (or split) row: the NX-wide block-reduction path
-        stream_block_pf<M, V, NX>(a, blocks, b, gather, pref, row_fn, lds, sp);
+        stream_head_pf<M, V, NX, false>(a, h, gather, pref, row_fn, lds);
         return;
     }
     // A staged block: its indices and values are loaded once, every gather of every wanted vector is issued
@@ -450,9 +489,8 @@ __device__ __forceinline__ void stream_block_seq_pf(const CsrView<M>& a, const i
     const uint16_t* __restrict__ i16 = a.idx16;
     int cb = 0, wl = 0;
     if (i16) {
-        const int2 bw = reinterpret_cast<const int2*>(a.bcol)[b];
-        cb = bw.x;
-        wl = WIN ? bw.y : 0;
+        cb = h.cb;
+        wl = WIN ? h.wraw : 0;
     }
     const int r = r0 + (int)threadIdx.x;
     const bool own = r < r1;

@@ -264,3 +264,27 @@ def test_batch_adjoint_vs_active_set_gradient_and_single_solves():
         bt.adjoint(ws[:-1])
     with pytest.raises(RuntimeError):
         bt.adjoint(ws, precision="single")  # an FP64 batch has no FP32 working set
+
+
+def test_batch_adjoint_instances_are_independent():
+    """Each instance's GMRES in the batched adjoint runs on its own dot products, Hessenberg system and stopping
+    test, so its result does not depend on the other instances: an instance solved beside others gives the same
+    bits as in a batch of its own (here beside a larger QP and an LP, whose empty Q rows the batch carries)."""
+    small = g.Problem.generate("diff_qp", seed=7, n=400, m_eq=60, m_le=40, density=0.03, qdensity=0.003)
+    big = g.Problem.generate("diff_qp", seed=8, n=900, m_eq=120, m_le=80, density=0.02, qdensity=0.002)
+    lp = g.Problem.generate("random_lp", seed=3, m=120, n=300, m_eq=60)
+    rng = np.random.default_rng(5)
+    ws = [rng.normal(size=p.dims()["n"]) for p in (small, big, lp)]
+    together = g.Batch([small, big, lp], precision="double", tol=1e-8, scale_q=True)
+    r3 = together.run(return_solution=True)
+    alone = g.Batch([small], precision="double", tol=1e-8, scale_q=True)
+    r1 = alone.run(return_solution=True)
+    assert r3[0]["status"] == r1[0]["status"] == "OPTIMAL" and r3[1]["status"] == "OPTIMAL"
+    assert np.array_equal(r3[0]["x"], r1[0]["x"])  # the forward is per instance too
+    a3 = together.adjoint(ws, tol=1e-10)
+    a1 = alone.adjoint(ws[:1], tol=1e-10)
+    print(f"products {a3[0]['iters']} / {a1[0]['iters']}, big {a3[1]['iters']}, lp {a3[2]['iters']}", flush=True)
+    assert a3[0]["iters"] == a1[0]["iters"]
+    for key in ("dc", "db", "dq"):
+        assert np.array_equal(a3[0][key], a1[0][key]), key
+        assert np.all(np.isfinite(a3[1][key])) and np.all(np.isfinite(a3[2][key])), key
