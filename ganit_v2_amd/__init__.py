@@ -263,9 +263,9 @@ SETUP = {"auto": 0, "host": 1, "gpu": 2}
 def _options(tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10, pock_chambolle=True,
              verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12, stall_checks_l2=24, values="f32",
              device=0, eval_fp64=False, kahan=False, detect_unbounded=False, setup="auto",
-             polish=False, scale_q=False) -> np.ndarray:
+             polish=False, scale_q=False, tail_lanes=True) -> np.ndarray:
     flags = (int(bool(kahan)) | (2 if detect_unbounded else 0) | (SETUP[setup] << 2) | (16 if polish else 0)
-             | (32 if scale_q else 0))
+             | (32 if scale_q else 0) | (0 if tail_lanes else 64))
     return np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
                      infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device,
                      int(bool(eval_fp64)), flags], dtype=np.float64)
@@ -397,15 +397,19 @@ def solve_many(problems, threads: int = 8, **opts) -> list:
 
 _BATCH_STATS_KEYS = ["rows", "cols", "nnz", "blocks_a", "blocks_at", "upload_s", "solve_s", "certify_s", "l1_s",
                      "l2_s", "l3_s", "l1_instances", "l2_instances", "l3_instances", "max_iters", "total_iters",
-                     "autotune_pick", "autotune_ms0", "autotune_ms1", "autotune_ms2", "autotune_ms3", "setup_s"]
+                     "autotune_pick", "autotune_ms0", "autotune_ms1", "autotune_ms2", "autotune_ms3", "setup_s",
+                     "tail_iters"]
 
 
 class Batch:
     """Many LPs and convex QPs solved together by the block-diagonal batched kernels (one launch sequence for
     all).
 
-    Options as Problem.solve_gpu (check_every must be even).  run() returns one result dict per instance
-    (solve_s is the whole batch's wall time) and stats() the batch-level figures of that run."""
+    Options as Problem.solve_gpu (check_every must be even), plus tail_lanes (default True): once the running
+    instances are few and small (at most 8 with 2^18 nonzeros: launch-bound), iterate them 8-32 lanes per row
+    instead of on row blocks; tail_lanes=False keeps every row sum in row order (the CPU's bits) to the end.
+    run() returns one result dict per instance (solve_s is the whole batch's wall time) and stats() the
+    batch-level figures of that run (tail_iters: batched iterations run lanes-per-row)."""
 
     def __init__(self, problems, **opts):
         self._problems = list(problems)

@@ -105,6 +105,49 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
         lds);
 }
 
+// The same two kernels for a launch-bound tail: LPR lanes per row (row_dot), a workgroup over kBlock / LPR
+// consecutive rows of one instance, list entry {first row, end row, instance}.  Rows are summed by a fixed
+// lane tree, not in row order (kernels over a few small instances are bound by their launches and dependent
+// loads, which the row-block staging lengthens).  A stopped instance's workgroups load but store nothing, so
+// the state load overlaps the row's loads.
+template <int LPR, class M, class V, bool HASQ>
+__global__ __launch_bounds__(kBlock) void k_bt_primal_lpr(CsrView<M> AT, const int4* __restrict__ list, CsrView<M> Q,
+                                                          const BatchState* __restrict__ st,
+                                                          const V* __restrict__ y, const V* __restrict__ x,
+                                                          const V* __restrict__ c, const V* __restrict__ l,
+                                                          const V* __restrict__ u, V* __restrict__ xn,
+                                                          V* __restrict__ xbar, V* __restrict__ sumx) {
+    const int4 e = list[blockIdx.x];
+    const int j = e.x + (int)threadIdx.x / LPR, lane = (int)threadIdx.x % LPR;
+    if (j >= e.y) return;  // whole row groups leave together
+    const BatchState& s = st[e.z];
+    const V aty = row_dot<LPR>(AT, j, lane, y);
+    const V qx = HASQ ? row_dot<LPR>(Q, j, lane, x) : V(0);
+    if (lane == 0 && s.active) {
+        V sm = sumx[j];
+        primal_update<V>(x[j], aty, qx, c[j], l[j], u[j], V(s.tau), xn[j], xbar[j], sm);
+        sumx[j] = sm;
+    }
+}
+
+template <int LPR, class M, class V>
+__global__ __launch_bounds__(kBlock) void k_bt_dual_lpr(CsrView<M> A, const int4* __restrict__ list,
+                                                        const BatchState* __restrict__ st, const V* __restrict__ xbar,
+                                                        const V* __restrict__ y, const V* __restrict__ lo,
+                                                        const V* __restrict__ hi, V* __restrict__ yn,
+                                                        V* __restrict__ sumy) {
+    const int4 e = list[blockIdx.x];
+    const int i = e.x + (int)threadIdx.x / LPR, lane = (int)threadIdx.x % LPR;
+    if (i >= e.y) return;
+    const BatchState& s = st[e.z];
+    const V axb = row_dot<LPR>(A, i, lane, xbar);
+    if (lane == 0 && s.active) {
+        V sm = sumy[i];
+        dual_update<V>(y[i], axb, lo[i], hi[i], V(s.sigma), yn[i], sm);
+        sumy[i] = sm;
+    }
+}
+
 // ------------------------------------------------------------------ check kernels
 template <class V>
 __global__ __launch_bounds__(kBlock) void k_bt_average(int N, int M_, const int* __restrict__ col_inst,
@@ -410,6 +453,14 @@ struct Batch {
     std::vector<int> first_a_h, first_at_h;
     int2 *list_a = nullptr, *list_at = nullptr;  // {row block, instance} of each launched workgroup
     int nact_a = 0, nact_at = 0;
+    // launch-bound tail: when the listed instances hold at most kTailNnz nonzeros, the iteration kernels run
+    // lanes-per-row over tail lists {first row, end row, instance} (the checks keep the row-block lists)
+    int tail_mode = 1;                 // 0: row blocks to the end (every row sum in row order)
+    int tail_lpr_a = 0, tail_lpr_at = 0;  // lanes per row of the current tail lists; 0: no tail lists
+    int4 *tlist_a = nullptr, *tlist_at = nullptr;
+    size_t tcap_a = 0, tcap_at = 0;
+    int tact_a = 0, tact_at = 0;
+    int64_t tail_iters = 0;  // iterations run on tail lists (all phases)
     std::vector<char> listed;  // instances in the current lists
     BatchVecs vs[2];
     bool on[2] = {false, false};
@@ -435,7 +486,8 @@ struct Batch {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         drop_graphs();
         for (void* p : owned) (void)hipFree(p);
-        for (void* p : {(void*)first_a, (void*)first_at, (void*)part_n, (void*)part_m, (void*)list_a, (void*)list_at})
+        for (void* p : {(void*)first_a, (void*)first_at, (void*)part_n, (void*)part_m, (void*)list_a, (void*)list_at,
+                        (void*)tlist_a, (void*)tlist_at})
             if (p) (void)hipFree(p);
         if (host_running) (void)hipHostFree(host_running);
         csr_destroy(A);
@@ -445,6 +497,60 @@ struct Batch {
 };
 
 unsigned bt_grid(int64_t n) { return std::min<unsigned>(grid_for(n), 8192u); }
+
+// A launch-bound tail: at most kTailInstances instances listed, with at most kTailNnz nonzeros between them.  (Sixteen
+// small LPs with 130K nonzeros ran 7% slower lanes-per-row; a few slow instances left of 512 run 25% faster.)
+constexpr int kTailInstances = 8;
+constexpr int64_t kTailNnz = int64_t(1) << 18;
+
+// Tail lists for the instances flagged in `on` when they are few and small enough (tail_mode, kTailInstances,
+// kTailNnz): per
+// matrix, lanes per row from the listed rows' nnz-weighted median length (8, 16 or 32), and one entry per
+// kBlock / lanes consecutive rows of an instance.  Otherwise none (tail_lpr_* = 0).
+void bt_tail_lists(Batch* bt, const std::vector<char>& on) {
+    bt->tail_lpr_a = bt->tail_lpr_at = 0;
+    if (!bt->tail_mode) return;
+    int64_t nz = 0;
+    int count = 0;
+    for (int k = 0; k < bt->ninst; ++k)
+        if (on[k]) {
+            const BatchInst& in = bt->inst_h[k];
+            nz += bt->aptr_h[in.row0 + in.m] - bt->aptr_h[in.row0];
+            ++count;
+        }
+    if (nz == 0 || nz > kTailNnz || count > kTailInstances) return;
+    auto build = [&](const std::vector<int>& ptr, bool rows_of_a, int4*& dev, size_t& cap, int& count) {
+        std::vector<int> len;
+        for (int k = 0; k < bt->ninst; ++k)
+            if (on[k]) {
+                const BatchInst& in = bt->inst_h[k];
+                const int r0 = rows_of_a ? in.row0 : in.col0, nr = rows_of_a ? in.m : in.n;
+                for (int r = r0; r < r0 + nr; ++r) len.push_back(ptr[r + 1] - ptr[r]);
+            }
+        const int lpr = std::min(32, std::max(8, choose_lanes_from_lengths(len, nz)));
+        const int per = kBlock / lpr;
+        std::vector<int4> l;
+        for (int k = 0; k < bt->ninst; ++k)
+            if (on[k]) {
+                const BatchInst& in = bt->inst_h[k];
+                const int r0 = rows_of_a ? in.row0 : in.col0, r1 = r0 + (rows_of_a ? in.m : in.n);
+                for (int r = r0; r < r1; r += per) l.push_back(make_int4(r, std::min(r + per, r1), k, 0));
+            }
+        count = (int)l.size();
+        if (l.size() > cap) {
+            if (dev) GH_CHECK(hipFree(dev));
+            dev = nullptr;
+            GH_CHECK(hipMalloc(&dev, sizeof(int4) * l.size()));
+            cap = l.size();
+        }
+        if (count > 0)
+            GH_CHECK(hipMemcpyAsync(dev, l.data(), sizeof(int4) * l.size(), hipMemcpyHostToDevice, bt->ctx->stream));
+        GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
+        return lpr;
+    };
+    bt->tail_lpr_a = build(bt->aptr_h, true, bt->tlist_a, bt->tcap_a, bt->tact_a);
+    bt->tail_lpr_at = build(bt->atptr_h, false, bt->tlist_at, bt->tcap_at, bt->tact_at);
+}
 
 // Launch lists for the instances flagged in `on`: the workgroups of instances that have stopped are left out
 // of the grid instead of being launched to return at once, which matters when a few slow instances remain.
@@ -463,6 +569,7 @@ void bt_list(Batch* bt, const std::vector<char>& on) {
     };
     fill(bt->first_a_h, bt->list_a, bt->nact_a);
     fill(bt->first_at_h, bt->list_at, bt->nact_at);
+    bt_tail_lists(bt, on);
     bt->listed = on;
     bt->drop_graphs();
 }
@@ -543,12 +650,49 @@ void bt_partition(Batch* bt, int rows, int nnz) {
     bt->part_nnz = nnz;
 }
 
+// Lanes-per-row iteration over the tail lists (bt_tail_lists).
+template <int LPR, class M, class V>
+void bt_primal_lpr(Batch* bt, BatchVecs& v, int c) {
+    hipStream_t s = bt->ctx->stream;
+    if (bt->tact_at == 0) return;
+    const CsrView<M> q = bt->Q ? bt->Q->view<M>() : bt->AT->view<M>();
+    if (bt->Q)
+        hipLaunchKernelGGL((k_bt_primal_lpr<LPR, M, V, true>), dim3(bt->tact_at), dim3(kBlock), 0, s,
+                           bt->AT->view<M>(), bt->tlist_at, q, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
+                           (const V*)v.c, (const V*)v.l, (const V*)v.u, (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
+    else
+        hipLaunchKernelGGL((k_bt_primal_lpr<LPR, M, V, false>), dim3(bt->tact_at), dim3(kBlock), 0, s,
+                           bt->AT->view<M>(), bt->tlist_at, q, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
+                           (const V*)v.c, (const V*)v.l, (const V*)v.u, (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
+}
+template <int LPR, class M, class V>
+void bt_dual_lpr(Batch* bt, BatchVecs& v, int c) {
+    if (bt->tact_a == 0) return;
+    hipLaunchKernelGGL((k_bt_dual_lpr<LPR, M, V>), dim3(bt->tact_a), dim3(kBlock), 0, bt->ctx->stream,
+                       bt->A->view<M>(), bt->tlist_a, bt->st, (const V*)v.xbar, (const V*)v.Y[c], (const V*)v.lo,
+                       (const V*)v.hi, (V*)v.Y[1 - c], (V*)v.sumy);
+}
+
 template <class M, class V>
 void bt_iteration(Batch* bt, int mdt_unused) {
     (void)mdt_unused;
     hipStream_t s = bt->ctx->stream;
     BatchVecs& v = bt->vs[std::is_same<V, double>::value ? 0 : 1];
     const int c = bt->cur;
+    if (bt->tail_lpr_a > 0) {
+        switch (bt->tail_lpr_at) {
+            case 8: bt_primal_lpr<8, M, V>(bt, v, c); break;
+            case 16: bt_primal_lpr<16, M, V>(bt, v, c); break;
+            default: bt_primal_lpr<32, M, V>(bt, v, c); break;
+        }
+        switch (bt->tail_lpr_a) {
+            case 8: bt_dual_lpr<8, M, V>(bt, v, c); break;
+            case 16: bt_dual_lpr<16, M, V>(bt, v, c); break;
+            default: bt_dual_lpr<32, M, V>(bt, v, c); break;
+        }
+        bt->cur ^= 1;
+        return;
+    }
     if (bt->nact_at > 0) {
         if (bt->Q)
             hipLaunchKernelGGL((k_bt_primal<M, V, true>), dim3(bt->nact_at), dim3(kBlock), 0, s, bt->AT->view<M>(),
@@ -1132,6 +1276,7 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
             while (running > 0) {
                 if (graphs) GH_CHECK(hipGraphLaunch(graph(), st));
                 else block();
+                if (bt->tail_lpr_a > 0) bt->tail_iters += bt->P.check_every;
                 GH_CHECK(hipMemcpyAsync(bt->host_running, bt->running, sizeof(int), hipMemcpyDeviceToHost, st));
                 GH_CHECK(hipStreamSynchronize(st));
                 running = *bt->host_running;
@@ -1221,6 +1366,29 @@ int ganit_hip_batch_autotune(ganit_hip_batch* h, int mdt, int vdt, int reps, dou
         if (ms)
             for (size_t c = 0; c < cand.size(); ++c) ms[c] = best[c];
         if (chosen) *chosen = (int)pick;
+    });
+}
+
+// Launch-bound tail switch: mode 1 (the default) lets the iteration kernels of a small running set run
+// lanes-per-row, 0 keeps the row blocks (row-order sums) to the end, -1 leaves it; tail_iters (may be null) =
+// iterations run on tail lists so far.
+int ganit_hip_batch_tail(ganit_hip_batch* h, int mode, int64_t* tail_iters) {
+    return guarded([&] {
+        auto* bt = reinterpret_cast<Batch*>(h);
+        GH_REQUIRE(bt && mode >= -1 && mode <= 1, "bad tail arguments");
+        if (mode >= 0 && mode != bt->tail_mode) {
+            GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
+            bt->tail_mode = mode;
+            const std::vector<char> on = bt->listed;
+            bt->listed.assign(bt->ninst, 0);  // rebuild the lists (and drop the graphs) under the new mode
+            if (std::count(on.begin(), on.end(), (char)1) > 0) {
+                bt_list(bt, on);
+            } else {
+                bt_tail_lists(bt, on);
+                bt->drop_graphs();
+            }
+        }
+        if (tail_iters) *tail_iters = bt->tail_iters;
     });
 }
 

@@ -146,7 +146,9 @@ void BatchSolver::upload() {
     const unsigned mmask = (1u << GANIT_F64) | (I.mixed ? (1u << I.low_mdt) : 0u);
     const unsigned vmask = (1u << GANIT_F64) | (I.mixed ? (1u << GANIT_F32) : 0u);
     chk(ganit_hip_batch_create(I.ctx, K, dp.data(), scp.data(), mmask, vmask, &I.bt));
-    // pick the row-block partition for the level that runs first (the batch's bulk of iterations)
+    // pick the row-block partition for the level that runs first (the batch's bulk of iterations), timed on the
+    // row blocks whatever the tail mode
+    chk(ganit_hip_batch_tail(I.bt, 0, nullptr));
     const char* at = std::getenv("GANIT_HIP_AUTOTUNE");
     if (!(at && at[0] == '0')) {
         const int vdt = I.mixed ? GANIT_F32 : GANIT_F64, mdt = I.mixed ? I.low_mdt : GANIT_F64;
@@ -291,6 +293,8 @@ std::vector<Result> BatchSolver::run() {
     }
     const int first_vdt = I.mixed ? GANIT_F32 : GANIT_F64;
     chk(ganit_hip_batch_load(bt, first_vdt, x0.data(), y0.data(), w.data()));
+    int64_t tail0 = 0;
+    chk(ganit_hip_batch_tail(bt, opt.tail_lanes ? 1 : 0, &tail0));
     S.upload_seconds = since(t_solve);
 
     std::vector<int> ints(5 * K);
@@ -364,6 +368,9 @@ std::vector<Result> BatchSolver::run() {
     chk(ganit_hip_batch_store(bt, GANIT_F64, xs.data(), ys.data()));
     const double solve_s = since(t_solve);
     S.solve_seconds = solve_s;
+    int64_t tail1 = 0;
+    chk(ganit_hip_batch_tail(bt, -1, &tail1));
+    S.tail_iterations = (long)(tail1 - tail0);
     std::vector<Result> out(K);
     I.w_final.resize(K);
     for (int k = 0; k < K; ++k) I.w_final[k] = dbl[13 * k + 12];

@@ -200,12 +200,13 @@ ganit::Options options_from(const double* o) {
     opt.device = (int)o[12];
     opt.eval_fp64 = o[13] != 0;
     // bit 0: kahan, bit 1: detect_unbounded, bits 2-3: set-up (0 auto, 1 host, 2 GPU), bit 4: polish,
-    // bit 5: scale_q
+    // bit 5: scale_q, bit 6: no lanes-per-row tail in batches
     const int bits = (int)o[14];
     opt.kahan = (bits & 1) != 0;
     opt.detect_unbounded = (bits & 2) != 0;
     opt.polish = (bits & 16) != 0;
     opt.scale_q = (bits & 32) != 0;
+    opt.tail_lanes = (bits & 64) == 0;
     const int where = (bits >> 2) & 3;
     if (where == 3) throw std::runtime_error("set-up placement must be 0 (auto), 1 (host) or 2 (GPU)");
     opt.setup = where == 1 ? ganit::Setup::Host : where == 2 ? ganit::Setup::Device : ganit::Setup::Auto;
@@ -355,7 +356,8 @@ int ganit_batch_stats(const ganit_batch* h, double* st) {
                               (double)s.level_instances[2], (double)s.max_iterations, (double)s.total_iterations,
                               (double)s.autotune_pick, s.autotune_ms[0], s.autotune_ms[1], s.autotune_ms[2],
                               s.autotune_ms[3],
-                              reinterpret_cast<const ganit::BatchSolver*>(h)->setup_seconds(), 0.0, 0.0};
+                              reinterpret_cast<const ganit::BatchSolver*>(h)->setup_seconds(),
+                              (double)s.tail_iterations, 0.0};
         std::copy(v, v + 24, st);
     });
 }

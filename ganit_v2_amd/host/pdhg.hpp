@@ -52,6 +52,9 @@ struct Options {
     double polish_tol = 1e-9;
     bool scale_q = false;           // QPs: column scaling also equilibrates Q (scale_lp's q_columns); changes the
                                     // trajectory, so it is off unless asked for
+    bool tail_lanes = true;         // batches: once the running instances are few and small (launch-bound), iterate
+                                    // them lanes-per-row (row sums by a lane tree) instead of on row blocks (row
+                                    // order, the CPU's bits); false keeps the row blocks to the end
 };
 
 // DualInfeasible (an extension, only with Options::detect_unbounded) comes last so the reference's codes keep
@@ -158,6 +161,7 @@ struct BatchStats {
     long max_iterations = 0, total_iterations = 0;
     int autotune_pick = -1;               // chosen row-block partition candidate (-1: not tuned)
     double autotune_ms[8] = {0};          // ms per batched iteration of each candidate
+    long tail_iterations = 0;             // batched iterations run lanes-per-row (Options::tail_lanes), last run
 };
 
 class BatchSolver {

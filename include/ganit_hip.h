@@ -256,6 +256,11 @@ int ganit_hip_batch_phase(ganit_hip_batch* bt, int mdt, int vdt, const double* p
  * candidate (rows, nonzeros per workgroup) and keeps the fastest; ms[c] = ms per iteration of candidate c
  * (up to 8, may be NULL), *chosen = its index.  Advances the loaded iterate: load again afterwards. */
 int ganit_hip_batch_autotune(ganit_hip_batch* bt, int mdt, int vdt, int reps, double* ms, int* chosen);
+/* Launch-bound tail (no reference counterpart): while at most 8 instances run, with at most 2^18 nonzeros, the
+ * batched iteration kernels run 8, 16 or 32 lanes per row (a fixed lane tree per row) instead of the row blocks
+ * (each row summed in row order, the CPU's bits).  mode 1 (the default) allows it, 0 keeps the row blocks to
+ * the end, -1 leaves the mode; tail_iters (may be NULL) = iterations run that way so far. */
+int ganit_hip_batch_tail(ganit_hip_batch* bt, int mode, int64_t* tail_iters);
 /* FP64 KKT of every instance's FP64 iterate with matrix values mdt (the certification). */
 int ganit_hip_batch_certify(ganit_hip_batch* bt, int mdt);
 /* ints[5k..] = status (0 running, 1 converged, 2 infeasible, 3 iteration limit, 4 numerical, 5 stalled,

@@ -74,7 +74,10 @@ int ganit_lp_scaled_gpu(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, 
  *                      scaled problem and norm estimate either way (ignored by ganit_batch_*),
  *                      bit 4 polish (LPs: project an optimal point onto its active face on the GPU and keep it
  *                      when its FP64 KKT errors are all <= 1e-9; ignored by ganit_batch_*),
- *                      bit 5 scale_q (QPs: the column scaling also equilibrates Q, the block matrix [Q A'; A 0])
+ *                      bit 5 scale_q (QPs: the column scaling also equilibrates Q, the block matrix [Q A'; A 0]),
+ *                      bit 6 (ganit_batch_* only): no lanes-per-row tail -- a batch whose running instances are
+ *                      few and small (at most 8, 2^18 nonzeros) otherwise iterates them 8-32 lanes per row (row
+ *                      sums by a lane tree) instead of on row blocks (each row in row order, the CPU's bits)
  * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical,
  *                       5 dual infeasible -- only with detect_unbounded),
  *               pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,
@@ -129,7 +132,7 @@ void ganit_solver_free(ganit_solver* s);
  * stats[0..23] = rows, cols, nnz, row blocks of A, row blocks of A^T, upload_s, solve_s, certify_s,
  *                L1_s, L2_s, L3_s, instances in L1, L2, L3, max iterations, total iterations,
  *                autotuned partition (-1 none), ms per batched iteration of the 4 candidates, setup_s,
- *                2 reserved (0). */
+ *                batched iterations run lanes-per-row (opts bit 6), 1 reserved (0). */
 typedef struct ganit_batch ganit_batch;
 ganit_batch* ganit_batch_create(const ganit_lp* const* lps, int ninst, const double* opts);
 int ganit_batch_upload(ganit_batch* b);

@@ -269,15 +269,16 @@ def test_batch_adjoint_vs_active_set_gradient_and_single_solves():
 def test_batch_adjoint_instances_are_independent():
     """Each instance's GMRES in the batched adjoint runs on its own dot products, Hessenberg system and stopping
     test, so its result does not depend on the other instances: an instance solved beside others gives the same
-    bits as in a batch of its own (here beside a larger QP and an LP, whose empty Q rows the batch carries)."""
+    bits as in a batch of its own (here beside a larger QP and an LP, whose empty Q rows the batch carries).  The
+    forward keeps its row blocks to the end (tail_lanes=False), so it too is each instance's own, bit for bit."""
     small = g.Problem.generate("diff_qp", seed=7, n=400, m_eq=60, m_le=40, density=0.03, qdensity=0.003)
     big = g.Problem.generate("diff_qp", seed=8, n=900, m_eq=120, m_le=80, density=0.02, qdensity=0.002)
     lp = g.Problem.generate("random_lp", seed=3, m=120, n=300, m_eq=60)
     rng = np.random.default_rng(5)
     ws = [rng.normal(size=p.dims()["n"]) for p in (small, big, lp)]
-    together = g.Batch([small, big, lp], precision="double", tol=1e-8, scale_q=True)
+    together = g.Batch([small, big, lp], precision="double", tol=1e-8, scale_q=True, tail_lanes=False)
     r3 = together.run(return_solution=True)
-    alone = g.Batch([small], precision="double", tol=1e-8, scale_q=True)
+    alone = g.Batch([small], precision="double", tol=1e-8, scale_q=True, tail_lanes=False)
     r1 = alone.run(return_solution=True)
     assert r3[0]["status"] == r1[0]["status"] == "OPTIMAL" and r3[1]["status"] == "OPTIMAL"
     assert np.array_equal(r3[0]["x"], r1[0]["x"])  # the forward is per instance too

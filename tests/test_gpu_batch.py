@@ -1,8 +1,8 @@
 """Batched PDHG (many uneven LPs and QPs in one block-diagonal launch sequence) against the oracle, per instance.
 
-The batched kernels sum every row in the same order as the single-instance row-block kernels and the
-oracle's CPU row loop (rows here stay under 256 nonzeros), and each instance's decisions are the single
-solve's, taken on the device.  So each instance should follow the oracle's own trajectory: same status, and
+With tail_lanes=False the batched kernels sum every row in the same order as the single-instance row-block
+kernels and the oracle's CPU row loop (rows here stay under 256 nonzeros) to the end, and each instance's decisions
+are the single solve's, taken on the device.  So each instance should follow the oracle's own trajectory: same status, and
 in most instances the same iteration count and restart count (FP64 check sums are added in a different
 order, which can flip a discontinuous restart test now and then).  Where the iteration counts agree the
 objectives agree to round-off; elsewhere both points are 1e-4-accurate and agree to a tolerance-sized bound.
@@ -46,7 +46,7 @@ def _compare(res, refs, tol, same_iter_frac):
 
 def test_batch_fp64_matches_oracle_per_instance(oracle):
     probs = _problems(32)
-    b = g.Batch(probs, precision="double", tol=1e-4)
+    b = g.Batch(probs, precision="double", tol=1e-4, tail_lanes=False)
     res = b.run(return_solution=True)
     oracle.set_threads(4)
     refs = [oracle.OracleLP.from_export(p.export()).solve(precision="double", tol=1e-4) for p in probs]
@@ -65,7 +65,7 @@ def test_batch_mixed_matches_oracle_ladder(oracle):
     """FP32 matrix + FP32 vectors in L1, FP64 certification of every instance, L3 for those not certified:
     the oracle runs the same ladder per instance."""
     probs = _problems(32, seed0=100)
-    res = g.Batch(probs, precision="mixed", tol=1e-4).run()
+    res = g.Batch(probs, precision="mixed", tol=1e-4, tail_lanes=False).run()
     oracle.set_threads(4)
     refs = [oracle.OracleLP.from_export(p.export()).solve(precision="mixed", tol=1e-4) for p in probs]
     _compare(res, refs, 1e-4, 0.7)
@@ -99,7 +99,7 @@ def test_batch_with_qps_matches_oracle(oracle, precision):
     in the check, the QP step-size cap and the x'Qx objective terms of each instance's decisions -- each instance
     follows the oracle's own QP trajectory (same status; same iteration count for most; certified tolerance)."""
     probs = _qp_mix()
-    res = g.Batch(probs, precision=precision, tol=1e-4).run()
+    res = g.Batch(probs, precision=precision, tol=1e-4, tail_lanes=False).run()
     oracle.set_threads(4)
     refs = [oracle.OracleLP.from_export(p.export()).solve(precision=precision, tol=1e-4, time_limit=120)
             for p in probs]
@@ -115,6 +115,38 @@ def test_batch_is_deterministic():
         assert a["iters"] == c["iters"] and a["restarts"] == c["restarts"]
         assert a["pobj"] == c["pobj"]
         assert np.array_equal(a["x"], c["x"]) and np.array_equal(a["y"], c["y"])
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_batch_tail_lanes_against_row_order(precision):
+    """The launch-bound tail (tail_lanes, the default): once at most 8 instances run, with at most 2^18 nonzeros, the
+    iteration kernels run lanes-per-row.  Four small LPs are in that regime from the start; 48 C3-size LPs
+    (about 1.6M nonzeros) start on the row blocks and switch once their slow ones are left.  Against the same
+    batches kept on row blocks (row-order sums): same statuses, every optimal instance FP64-certified, objectives
+    within tolerance-sized bounds, and the iteration counts of most instances unchanged (the sums differ in the
+    last bits only)."""
+    for probs, whole in ((_problems(4, seed0=700), True),
+                         ([g.Problem.generate("batch_lp", seed=k) for k in range(48)], False)):
+        tb = g.Batch(probs, precision=precision, tol=1e-4)
+        tail = tb.run()
+        rb = g.Batch(probs, precision=precision, tol=1e-4, tail_lanes=False)
+        rows = rb.run()
+        st, st0 = tb.stats(), rb.stats()
+        print(f"{len(probs)} instances, nnz {st['nnz']:.0f}: tail iterations {st['tail_iters']:.0f} of "
+              f"{st['max_iters']:.0f}, {tail[0]['solve_s']:.4f} s against {rows[0]['solve_s']:.4f} s", flush=True)
+        assert st0["tail_iters"] == 0 and st["tail_iters"] > 0
+        if whole:
+            assert st["tail_iters"] >= 0.9 * st["max_iters"]
+        elif precision == "double":  # one phase: its batched iterations are the slowest instance's
+            assert st["tail_iters"] < st["max_iters"]
+        same = 0
+        for k, (a, b) in enumerate(zip(tail, rows)):
+            assert a["status"] == b["status"], k
+            if a["status"] == "OPTIMAL":
+                assert _certified(a, 1e-4), k
+                assert abs(a["pobj"] - b["pobj"]) <= 5e-4 * (1 + abs(b["pobj"])), k
+            same += a["iters"] == b["iters"]
+        assert same >= 0.75 * len(probs)
 
 
 def test_batch_with_netlib_and_infeasible_instances():
@@ -158,7 +190,7 @@ def test_batch_mixed_c3_outliers_follow_the_oracle_ladder(oracle):
     3.2e-3 from the FP64 optimum in objective: the LP is degenerate and its duals are large, so a 1e-4 KKT point
     can be that far off; the oracle's ladder lands on the same point."""
     probs = [g.Problem.generate("batch_lp", seed=s) for s in C3_OUTLIERS]
-    res = g.Batch(probs, precision="mixed", tol=1e-4).run()
+    res = g.Batch(probs, precision="mixed", tol=1e-4, tail_lanes=False).run()
     oracle.set_threads(8)
     refs = [oracle.OracleLP.from_export(p.export()).solve(precision="mixed", tol=1e-4, time_limit=120)
             for p in probs]
@@ -186,7 +218,7 @@ def test_batch_mixed_with_fp64_checks_matches_oracle(oracle):
     in L1 as with the reference's level-precision checks."""
     probs = _problems(16, seed0=300)
     opts = dict(precision="mixed", tol=1e-4, eval_fp64=True, stall_checks=60)
-    res = g.Batch(probs, **opts).run()
+    res = g.Batch(probs, tail_lanes=False, **opts).run()
     oracle.set_threads(4)
     refs = [oracle.OracleLP.from_export(p.export()).solve(**opts) for p in probs]
     _compare(res, refs, 1e-4, 0.7)
@@ -198,7 +230,7 @@ def test_batch_mixed_with_fp64_checks_matches_oracle(oracle):
         assert r["status"] == s["status"], k
         if s["status"] == "OPTIMAL":
             assert abs(r["pobj"] - s["pobj"]) <= 5e-4 * (1 + abs(s["pobj"])), k
-    plain = g.Batch(probs, precision="mixed", tol=1e-4).run()
+    plain = g.Batch(probs, precision="mixed", tol=1e-4, tail_lanes=False).run()
     l1_only = sum(r["fp32_iters"] == r["iters"] for r in res if r["status"] == "OPTIMAL")
     l1_only_plain = sum(r["fp32_iters"] == r["iters"] for r in plain if r["status"] == "OPTIMAL")
     print(f"finished in L1: {l1_only} with FP64 checks, {l1_only_plain} with level-precision checks", flush=True)
