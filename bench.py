@@ -213,6 +213,7 @@ def batched_lps(count=512):
         st = b.stats()
         res[name] = dict(solve_s=round(st["solve_s"], 4), wall_s=round(wall, 4), max_iters=st["max_iters"],
                          total_iters=st["total_iters"], l3_instances=st["l3_instances"],
+                         tail_lanes_iters=st["tail_iters"],
                          statuses={k: sum(1 for r in rs if r["status"] == k) for k in {r["status"] for r in rs}})
         res[name + "_results"] = rs
         print(f"[bench] batched {name}: {res[name]}", file=sys.stderr, flush=True)
