@@ -191,10 +191,11 @@ def portfolio_ladder():
     return out
 
 
-def batched_lps(count=512):
+def batched_lps(count=512, sequential=True):
     """512 uneven random LPs (node-relaxation shapes): the batched kernels (FP64; FP32 + FP64 certification with
     the reference's ladder rules; and the same with FP64 candidate checks and stall 60, the headline's
-    configuration) against sequential single-instance FP64 solves with each problem resident."""
+    configuration) against sequential single-instance FP64 solves with each problem resident (sequential=False:
+    the batched runs only)."""
     import numpy as np
     import ganit_v2_amd as g
     probs = [g.Problem.generate("batch_lp", seed=k) for k in range(count)]
@@ -217,6 +218,11 @@ def batched_lps(count=512):
                          statuses={k: sum(1 for r in rs if r["status"] == k) for k in {r["status"] for r in rs}})
         res[name + "_results"] = rs
         print(f"[bench] batched {name}: {res[name]}", file=sys.stderr, flush=True)
+    if not sequential:
+        for name, _ in configs:
+            res.pop(name + "_results")
+        out.update(res)
+        return out
     seq, tsum = [], 0.0
     t = time.perf_counter()
     for p in probs:
@@ -487,6 +493,8 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only time the fused iteration kernels (the launches the roofline figure uses), plus one "
                          "plain 1 GiB device copy as the counter calibration; for profiler runs")
+    ap.add_argument("--batched-only", type=int, default=0, metavar="N",
+                    help="only run the batched solves of N random LPs (no sequential comparison); for profiler runs")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU only (requested --gpus {a.gpus}); a single LP does not shard "
@@ -495,6 +503,9 @@ def main():
 
     import ganit_v2_amd as g
 
+    if a.batched_only > 0:
+        print(json.dumps(batched_lps(a.batched_only, sequential=False)), flush=True)
+        return
     t0 = time.time()
     if a.periods != WORKLOAD["periods"] and not a.kernels_only:
         print("--periods other than 365 is for --kernels-only runs", file=sys.stderr)
