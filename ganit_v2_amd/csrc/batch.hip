@@ -922,6 +922,120 @@ __global__ __launch_bounds__(kBlock) void k_ba_out(int N, int M_, const V* __res
     }
 }
 
+// ---- the GMRES bookkeeping of every instance on the device, one thread per instance (so the host does not wait
+// for each Arnoldi step): Hessenberg columns (kk x kk per instance, column-major), Givens rotations, the rotated
+// right-hand side, and the activity flags the host polls now and then.
+struct GmState {
+    double *H, *cs, *sn, *s;          // K kk kk, K kk, K kk, K (kk + 1)
+    double *bnorm, *prev_beta, *rel;  // K each
+    int *done, *act, *cols, *prods;   // K each
+};
+
+__global__ void k_gm_init(int K, const double* __restrict__ nr2, GmState g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const double b = sqrt(nr2[k]);
+    g.bnorm[k] = b;
+    g.rel[k] = b > 0 ? INFINITY : 0.0;
+    g.done[k] = !(b > 0);
+    g.prev_beta[k] = INFINITY;
+    g.prods[k] = 0;
+    g.act[k] = 0;
+    g.cols[k] = 0;
+}
+
+// A restart cycle begins: beta_k = ||r0_k||; an instance is done when converged or when a whole cycle gained less
+// than 1% (the working precision's floor); the others start a new system with s = (beta, 0, ...).
+__global__ void k_gm_cycle(int K, int kk, double tol, const double* __restrict__ nr2, GmState g) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    g.act[k] = 0;
+    g.cols[k] = 0;
+    if (g.done[k]) return;
+    const double beta = sqrt(nr2[k]);
+    const double rel = beta / g.bnorm[k];
+    g.rel[k] = rel;
+    if (!isfinite(rel) || rel <= tol || beta > 0.99 * g.prev_beta[k]) {
+        g.done[k] = 1;
+        return;
+    }
+    g.prev_beta[k] = beta;
+    g.act[k] = 1;
+    double* sk = g.s + (size_t)k * (kk + 1);
+    for (int i = 0; i <= kk; ++i) sk[i] = 0.0;
+    sk[0] = beta;
+}
+
+constexpr int kGmMax = 1024;  // longest restart cycle (GMRES columns per instance)
+
+// Arnoldi step j of every active instance (one wave per instance): column j of H from the two Gram-Schmidt passes,
+// the earlier rotations applied to it, a new rotation that zeroes the subdiagonal sqrt(nr2), the residual estimate
+// |s[j+1]| / ||b||.  The column and the rotations are staged in LDS by the wave; one lane then runs the (inherently
+// sequential) rotations with the running entry in a register, its loads independent of its arithmetic.
+__global__ __launch_bounds__(kWave) void k_gm_step(int K, int kk, int j, double tol, const double* __restrict__ h1,
+                                                   const double* __restrict__ h2, const double* __restrict__ nr2,
+                                                   GmState g) {
+    __shared__ double col[kGmMax + 1], cl[kGmMax], sl[kGmMax];
+    const int k = blockIdx.x;
+    if (!g.act[k]) return;  // the whole wave
+    double* __restrict__ colg = g.H + ((size_t)k * kk + j) * kk;
+    double* __restrict__ c = g.cs + (size_t)k * kk;
+    double* __restrict__ sn = g.sn + (size_t)k * kk;
+    for (int i = threadIdx.x; i <= j; i += kWave) col[i] = h1[(size_t)i * K + k] + h2[(size_t)i * K + k];
+    for (int i = threadIdx.x; i < j; i += kWave) {
+        cl[i] = c[i];
+        sl[i] = sn[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ++g.prods[k];
+        double* sk = g.s + (size_t)k * (kk + 1);
+        const double sub = sqrt(fmax(nr2[k], 0.0));
+        double cur = col[0];
+        for (int i = 0; i < j; ++i) {
+            const double nxt = col[i + 1], ci = cl[i], si = sl[i];
+            col[i] = ci * cur + si * nxt;
+            cur = -si * cur + ci * nxt;
+        }
+        const double den = hypot(cur, sub);
+        const double cj = den > 0 ? cur / den : 1.0, sj = den > 0 ? sub / den : 0.0;
+        c[j] = cj;
+        sn[j] = sj;
+        col[j] = den;
+        sk[j + 1] = -sj * sk[j];
+        sk[j] = cj * sk[j];
+        g.cols[k] = j + 1;
+        const double rel = fabs(sk[j + 1]) / g.bnorm[k];
+        g.rel[k] = rel;
+        if (!isfinite(rel) || rel <= tol || sub <= 1e-300 * g.bnorm[k]) g.act[k] = 0;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i <= j; i += kWave) colg[i] = col[i];
+}
+
+// End of a cycle (one workgroup per instance): y_k = R_k^{-1} s_k by columns -- y_l = s_l / R_ll, then s_i -= R_il y_l
+// for every i < l in parallel -- into yc[i * K + k] for i < ncols (0 past the instance's own columns), for the
+// update lambda_k += V y_k; an instance whose estimate is not finite is done.
+__global__ __launch_bounds__(kBlock) void k_gm_solve(int K, int kk, int ncols, GmState g, double* __restrict__ yc) {
+    __shared__ double sh[kGmMax];
+    const int k = blockIdx.x;
+    const int nc = g.cols[k];
+    const double* __restrict__ Hk = g.H + (size_t)k * kk * kk;
+    const double* __restrict__ sk = g.s + (size_t)k * (kk + 1);
+    for (int i = threadIdx.x; i < ncols; i += kBlock) {
+        if (i < nc) sh[i] = sk[i];
+        else yc[(size_t)i * K + k] = 0.0;
+    }
+    for (int l = nc - 1; l >= 0; --l) {
+        __syncthreads();
+        const double d = Hk[(size_t)l * kk + l];
+        const double yl = d != 0.0 ? sh[l] / d : 0.0;
+        for (int i = threadIdx.x; i < l; i += kBlock) sh[i] -= Hk[(size_t)l * kk + i] * yl;
+        if (threadIdx.x == 0) yc[(size_t)l * K + k] = yl;
+    }
+    if (threadIdx.x == 0 && !isfinite(g.rel[k])) g.done[k] = 1;
+}
+
 // A device buffer owned by one call.
 struct BtBuf {
     void* p = nullptr;
@@ -1474,7 +1588,7 @@ int ganit_hip_batch_adjoint(ganit_hip_batch* h, int mdt, int vdt, const double* 
         const int max_iter = (int)std::min(params[0], 2.0e9);
         const double tol = params[1];
         const int kk = (int)params[2];
-        GH_REQUIRE(max_iter > 0 && kk > 0 && kk <= 1024, "bad adjoint parameters (restart length 1..1024)");
+        GH_REQUIRE(max_iter > 0 && kk > 0 && kk <= kGmMax, "bad adjoint parameters (restart length 1..1024)");
         const int slot = vdt == GANIT_F64 ? 0 : 1;
         GH_REQUIRE(bt->on[0] && bt->on[slot], "vector dtype not resident");
         const int K = bt->ninst, N = bt->N, Mr = bt->Mr;
@@ -1550,58 +1664,37 @@ int ganit_hip_batch_adjoint(ganit_hip_batch* h, int mdt, int vdt, const double* 
                 hipLaunchKernelGGL(k_bgm_scale<V>, dim3(gl), dim3(kBlock), 0, st, N, L, bt->col_inst, bt->row_inst,
                                    src, nr2, dst);
             };
-            std::vector<double> hh1((size_t)(kk + 1) * K), hh2((size_t)(kk + 1) * K), hn(K);
-            auto fetch = [&](int ncols) {
-                if (ncols > 0) {
-                    GH_CHECK(hipMemcpyAsync(hh1.data(), h1, sizeof(double) * ncols * K, hipMemcpyDeviceToHost, st));
-                    GH_CHECK(hipMemcpyAsync(hh2.data(), h2, sizeof(double) * ncols * K, hipMemcpyDeviceToHost, st));
-                }
-                GH_CHECK(hipMemcpyAsync(hn.data(), nr2, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+            // per-instance GMRES bookkeeping on the device (k_gm_*); the host polls the activity flags once per
+            // restart cycle and every kPoll Arnoldi steps (steps past the last instance's convergence change nothing:
+            // an inactive instance's column count, rotations and estimate are frozen)
+            constexpr int kPoll = 8;
+            const size_t kk1 = (size_t)kk + 1;
+            BtBuf gH(8 * (size_t)K * kk * kk), gcs(8 * (size_t)K * kk), gsn(8 * (size_t)K * kk), gs(8 * (size_t)K * kk1),
+                gd(8 * 3 * (size_t)K), gi(4 * 4 * (size_t)K);
+            GmState gm{gH.as<double>(), gcs.as<double>(), gsn.as<double>(), gs.as<double>(), gd.as<double>(),
+                       gd.as<double>() + K, gd.as<double>() + 2 * (size_t)K, gi.as<int>(), gi.as<int>() + K,
+                       gi.as<int>() + 2 * (size_t)K, gi.as<int>() + 3 * (size_t)K};
+            const unsigned gk = (unsigned)((K + 255) / 256);
+            std::vector<int> acth(K);
+            auto running_now = [&]() {
+                GH_CHECK(hipMemcpyAsync(acth.data(), gm.act, sizeof(int) * K, hipMemcpyDeviceToHost, st));
                 GH_CHECK(hipStreamSynchronize(st));
+                return (int)std::count_if(acth.begin(), acth.end(), [](int a) { return a != 0; });
             };
             // ||(g_k, 0)|| per instance
             dots(g.as<V>(), 1, g.as<V>(), 1, nr2);
-            fetch(0);
-            std::vector<double> bnorm(K), prev_beta(K, INFINITY);
-            std::vector<char> done(K, 0), act(K, 0);
-            for (int k = 0; k < K; ++k) {
-                bnorm[k] = std::sqrt(hn[k]);
-                rel[k] = bnorm[k] > 0 ? INFINITY : 0.0;
-                done[k] = !(bnorm[k] > 0);
-            }
-            // per instance: Hessenberg (kk x kk, column-major), Givens rotations, rotated right-hand side
-            std::vector<double> H((size_t)K * kk * kk), cs((size_t)K * kk), sn((size_t)K * kk),
-                sv2((size_t)K * (kk + 1)), yc;
-            std::vector<int> cols(K, 0);
+            hipLaunchKernelGGL(k_gm_init, dim3(gk), dim3(256), 0, st, K, (const double*)nr2, gm);
             int iters = 0;
-            while (iters < max_iter && std::count(done.begin(), done.end(), (char)0) > 0) {
+            while (iters < max_iter) {
                 // r0 = b - (I - J^T) lambda into basis column 0; beta_k = ||r0_k||
                 apply(lx, W);
                 hipLaunchKernelGGL(k_bgm_rhs_minus<V>, dim3(gl), dim3(kBlock), 0, st, N, L, g.as<V>(), W, Vb);
                 dots(Vb, 1, Vb, 0, nr2);
-                fetch(0);
-                int running = 0;
-                for (int k = 0; k < K; ++k) {
-                    act[k] = 0;
-                    cols[k] = 0;
-                    if (done[k]) continue;
-                    const double beta = std::sqrt(hn[k]);
-                    rel[k] = beta / bnorm[k];
-                    // converged; or a whole restart cycle gained less than 1%: the working precision's floor
-                    if (!std::isfinite(rel[k]) || rel[k] <= tol || beta > 0.99 * prev_beta[k]) {
-                        done[k] = 1;
-                        continue;
-                    }
-                    prev_beta[k] = beta;
-                    act[k] = 1;
-                    ++running;
-                    double* s = &sv2[(size_t)k * (kk + 1)];
-                    std::fill(s, s + kk + 1, 0.0);
-                    s[0] = beta;
-                }
-                if (running == 0) break;
+                hipLaunchKernelGGL(k_gm_cycle, dim3(gk), dim3(256), 0, st, K, kk, tol, (const double*)nr2, gm);
+                if (running_now() == 0) break;
                 scale(Vb, Vb);
-                for (int j = 0; j < kk && iters < max_iter && running > 0; ++j) {
+                int steps = 0;
+                for (int j = 0; j < kk && iters < max_iter; ++j) {
                     V* vj1 = Vb + (size_t)(j + 1) * L;
                     apply(Vb + (size_t)j * L, W);
                     dots(Vb, j + 1, W, 0, h1);
@@ -1610,56 +1703,19 @@ int ganit_hip_batch_adjoint(ganit_hip_batch* h, int mdt, int vdt, const double* 
                     update(Vb, j + 1, h2, -1.0, W);
                     dots(W, 1, W, 0, nr2);
                     scale(W, vj1);
-                    fetch(j + 1);
+                    hipLaunchKernelGGL(k_gm_step, dim3(K), dim3(kWave), 0, st, K, kk, j, tol, (const double*)h1,
+                                       (const double*)h2, (const double*)nr2, gm);
                     ++iters;
-                    for (int k = 0; k < K; ++k) {
-                        if (!act[k]) continue;
-                        ++prods[k];
-                        double* col = &H[((size_t)k * kk + j) * kk];
-                        double* c = &cs[(size_t)k * kk];
-                        double* sn_k = &sn[(size_t)k * kk];
-                        double* s = &sv2[(size_t)k * (kk + 1)];
-                        for (int i = 0; i <= j; ++i) col[i] = hh1[(size_t)i * K + k] + hh2[(size_t)i * K + k];
-                        const double sub = std::sqrt(std::max(hn[k], 0.0));
-                        for (int i = 0; i < j; ++i) {
-                            const double t = c[i] * col[i] + sn_k[i] * col[i + 1];
-                            col[i + 1] = -sn_k[i] * col[i] + c[i] * col[i + 1];
-                            col[i] = t;
-                        }
-                        const double den = std::hypot(col[j], sub);
-                        c[j] = den > 0 ? col[j] / den : 1.0;
-                        sn_k[j] = den > 0 ? sub / den : 0.0;
-                        col[j] = den;
-                        s[j + 1] = -sn_k[j] * s[j];
-                        s[j] = c[j] * s[j];
-                        cols[k] = j + 1;
-                        rel[k] = std::fabs(s[j + 1]) / bnorm[k];
-                        if (!std::isfinite(rel[k]) || rel[k] <= tol || sub <= 1e-300 * bnorm[k]) {
-                            act[k] = 0;
-                            --running;
-                        }
-                    }
+                    ++steps;
+                    if ((j + 1) % kPoll == 0 && running_now() == 0) break;
                 }
-                // y_k = R_k^{-1} s_k, lambda_k += V y_k (coefficient c of instance k at yc[c * K + k])
-                const int cmax = *std::max_element(cols.begin(), cols.end());
-                if (cmax > 0) {
-                    yc.assign((size_t)cmax * K, 0.0);
-                    for (int k = 0; k < K; ++k) {
-                        const double* Hk = &H[(size_t)k * kk * kk];
-                        const double* s = &sv2[(size_t)k * (kk + 1)];
-                        for (int i = cols[k] - 1; i >= 0; --i) {
-                            double t = s[i];
-                            for (int l2 = i + 1; l2 < cols[k]; ++l2) t -= Hk[(size_t)l2 * kk + i] * yc[(size_t)l2 * K + k];
-                            yc[(size_t)i * K + k] = Hk[(size_t)i * kk + i] != 0.0 ? t / Hk[(size_t)i * kk + i] : 0.0;
-                        }
-                    }
-                    GH_CHECK(hipMemcpyAsync(h1, yc.data(), sizeof(double) * yc.size(), hipMemcpyHostToDevice, st));
-                    update(Vb, cmax, h1, 1.0, lx);
-                    GH_CHECK(hipStreamSynchronize(st));
-                }
-                for (int k = 0; k < K; ++k)
-                    if (!std::isfinite(rel[k])) done[k] = 1;
+                // y_k = R_k^{-1} s_k, lambda_k += V y_k (coefficient c of instance k at h1[c * K + k])
+                hipLaunchKernelGGL(k_gm_solve, dim3(K), dim3(kBlock), 0, st, K, kk, steps, gm, h1);
+                update(Vb, steps, h1, 1.0, lx);
             }
+            GH_CHECK(hipMemcpyAsync(prods.data(), gm.prods, sizeof(int) * K, hipMemcpyDeviceToHost, st));
+            GH_CHECK(hipMemcpyAsync(rel.data(), gm.rel, sizeof(double) * K, hipMemcpyDeviceToHost, st));
+            GH_CHECK(hipStreamSynchronize(st));
             // p and mu at the final lambda
             if (N > 0 && bt->AT->nblocks > 0)
                 hipLaunchKernelGGL((k_ba_at<M, V>), dim3(bt->AT->nblocks), dim3(kBlock), 0, st, bt->AT->view<M>(),
