@@ -174,7 +174,8 @@ struct SplitPlan {
     const int* __restrict__ first;
     double* part;
     int finish;   // 1: workgroup b folds split row b
-    // compact indices of the chunks: chunk c's columns = cbase[c] + idx16 (cbase[c] < 0 or null: 32-bit)
+    // compact indices of the chunks: chunk c's columns = cbase[c] + idx16; cbase[c] <= -2: the chunk's columns are
+    // consecutive from -2 - cbase[c] (no index is read); -1 or null: 32-bit
     const int* __restrict__ cbase = nullptr;
 };
 
@@ -244,8 +245,17 @@ __device__ __forceinline__ void stream_block_pf(const CsrView<M>& a, const int* 
 #pragma unroll
         for (int k = 0; k < NX; ++k) s[k] = V(0);
         // unrolled so that several index / value / gather loads are in flight; each thread still adds its
-        // products in increasing position order (the compact / plain index choice is outside the loops)
-        if (ccb >= 0) {
+        // products in increasing position order (the consecutive / compact / plain index choice is outside the
+        // loops)
+        if (ccb <= -2) {
+            const int c0 = -2 - ccb - p0;  // column of entry p: c0 + p
+#pragma unroll 8
+            for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
+                const V av = widen<V>(a.val[p]);
+#pragma unroll
+                for (int k = 0; k < NX; ++k) s[k] += av * gather(k, c0 + p);
+            }
+        } else if (ccb >= 0) {
 #pragma unroll 8
             for (int p = p0 + threadIdx.x; p < p1; p += kBlock) {
                 const V av = widen<V>(a.val[p]);

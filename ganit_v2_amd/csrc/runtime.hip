@@ -386,7 +386,12 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
         for (int p0 = ptr[i]; p0 < ptr[i + 1]; p0 += kChunk) {
             const int p1 = std::min(p0 + kChunk, ptr[i + 1]);
             int mn = 0;
-            if (span_of(p0, p1, mn) > 65535) {
+            bool dense = true;  // columns idx[p0], idx[p0] + 1, ...: no index needs reading
+            for (int p = p0 + 1; p < p1 && dense; ++p) dense = idx[p] == idx[p0] + (p - p0);
+            if (dense) {
+                cbase.push_back(-2 - idx[p0]);
+                fill(p0, p1, idx[p0]);  // (the compact offsets stay valid; the kernels do not read them)
+            } else if (span_of(p0, p1, mn) > 65535) {
                 cbase.push_back(-1);
                 n32 += p1 - p0;
             } else {
@@ -648,9 +653,14 @@ void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& l
     // timed -- building a partition is host work over every nonzero.
     constexpr int kManyBlocks = 1024;
     const bool coarse_enough = a->rb_rows == kBlock && a->rb_nnz == kTile && a->nblocks >= kManyBlocks;
+    // When most nonzeros sit in rows longer than kSplit (cut into chunks over many workgroups on the row blocks), a
+    // row group of at most 64 lanes cannot compete (one such row took 36 ms at 1 lane and 3 ms at 16 lanes, against
+    // 7 us split): those variants are not timed either.
+    const bool split_bound = a->nchunks > 0 && (int64_t)a->nchunks * kChunk >= a->nnz / 2;
     std::vector<char> skip(nc, 0);
     for (size_t c = 0; c < nc; ++c)
-        skip[c] = coarse_enough && cand[c].lanes == 0 && !(cand[c].rows == a->rb_rows && cand[c].nnz == a->rb_nnz);
+        skip[c] = (coarse_enough && cand[c].lanes == 0 && !(cand[c].rows == a->rb_rows && cand[c].nnz == a->rb_nnz)) ||
+                  (split_bound && cand[c].lanes != 0);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::vector<float> best(nc, 1e30f);
     size_t bi = nc, bl = nc;
@@ -675,10 +685,26 @@ void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& l
             GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
             return ms / (float)k;
         };
+        // a single launch first: a candidate already 8x slower than the lead is not screened further
+        auto probe_one = [&](size_t c) {
+            use(c);
+            GH_CHECK(hipEventRecord(e0, ctx->stream));
+            launch(1);
+            GH_CHECK(hipEventRecord(e1, ctx->stream));
+            GH_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return ms;
+        };
         const int screen = std::max(1, reps / 3);
         std::vector<float> first(nc, 1e30f);
-        for (size_t c = 0; c < nc; ++c)
-            if (!skip[c]) first[c] = time_one(c, screen);
+        float lead0 = 1e30f;
+        for (size_t c = 0; c < nc; ++c) {
+            if (skip[c]) continue;
+            const float one = lead0 < 1e30f ? probe_one(c) : 0.f;
+            first[c] = one > 8.0f * lead0 ? one : time_one(c, screen);
+            lead0 = std::min(lead0, first[c]);
+        }
         const float lead = *std::min_element(first.begin(), first.end());
         for (size_t c = 0; c < nc; ++c) skip[c] = skip[c] || first[c] > 2.0f * lead;
         for (int round = 0; round < 2; ++round)
