@@ -193,9 +193,9 @@ def portfolio_ladder():
 
 def batched_lps(count=512, sequential=True):
     """512 uneven random LPs (node-relaxation shapes): the batched kernels (FP64; FP32 + FP64 certification with
-    the reference's ladder rules; and the same with FP64 candidate checks and stall 60, the headline's
-    configuration) against sequential single-instance FP64 solves with each problem resident (sequential=False:
-    the batched runs only)."""
+    the reference's ladder rules; the same with FP64 candidate checks and stall 60, the headline's configuration;
+    and that with the FP32 level ended at the launch-bound tail) against sequential single-instance FP64 solves
+    with each problem resident (sequential=False: the batched runs only)."""
     import numpy as np
     import ganit_v2_amd as g
     probs = [g.Problem.generate("batch_lp", seed=k) for k in range(count)]
@@ -203,7 +203,10 @@ def batched_lps(count=512, sequential=True):
                         f"5% tightened bounds, seeds 0..{count - 1}, tol 1e-4")
     res = {}
     configs = (("batched_fp64", dict(precision="double")), ("batched_mixed", dict(precision="mixed")),
-               ("batched_mixed_fp64_checks", dict(precision="mixed", eval_fp64=True, stall_checks=60)))
+               ("batched_mixed_fp64_checks", dict(precision="mixed", eval_fp64=True, stall_checks=60)),
+               # a batch policy (not the reference's ladder rule): the FP32 level ends at the launch-bound tail
+               ("batched_mixed_fp64_checks_tail_fp64",
+                dict(precision="mixed", eval_fp64=True, stall_checks=60, tail_fp64=True)))
     for name, cfg in configs:
         b = g.Batch(probs, tol=1e-4, **cfg)
         b.upload()

@@ -263,9 +263,9 @@ SETUP = {"auto": 0, "host": 1, "gpu": 2}
 def _options(tol=1e-4, max_iter=10_000_000, time_limit=3600.0, check_every=64, ruiz_iters=10, pock_chambolle=True,
              verbose=0, infeas_tol=1e-8, precision="double", stall_checks=12, stall_checks_l2=24, values="f32",
              device=0, eval_fp64=False, kahan=False, detect_unbounded=False, setup="auto",
-             polish=False, scale_q=False, tail_lanes=True) -> np.ndarray:
+             polish=False, scale_q=False, tail_lanes=True, tail_fp64=False) -> np.ndarray:
     flags = (int(bool(kahan)) | (2 if detect_unbounded else 0) | (SETUP[setup] << 2) | (16 if polish else 0)
-             | (32 if scale_q else 0) | (0 if tail_lanes else 64))
+             | (32 if scale_q else 0) | (0 if tail_lanes else 64) | (128 if tail_fp64 else 0))
     return np.array([tol, max_iter, time_limit, check_every, ruiz_iters, int(bool(pock_chambolle)), verbose,
                      infeas_tol, PRECISION[precision], stall_checks, stall_checks_l2, VALUES[values], device,
                      int(bool(eval_fp64)), flags], dtype=np.float64)
@@ -408,8 +408,11 @@ class Batch:
     Options as Problem.solve_gpu (check_every must be even), plus tail_lanes (default True): once the running
     instances are few and small (at most 8 with 2^18 nonzeros: launch-bound), iterate them 8-32 lanes per row
     instead of on row blocks; tail_lanes=False keeps every row sum in row order (the CPU's bits) to the end.
-    run() returns one result dict per instance (solve_s is the whole batch's wall time) and stats() the
-    batch-level figures of that run (tail_iters: batched iterations run lanes-per-row)."""
+    tail_fp64 (default False; mixed / dynamic): the reduced-precision levels end once the running instances are
+    such a tail, and the ladder carries them on in FP64 (FP32 saves nothing on launch-bound iterations; a batch
+    policy, not the reference's ladder rule).  run() returns one result dict per instance (solve_s is the whole
+    batch's wall time) and stats() the batch-level figures of that run (tail_iters: batched iterations run
+    lanes-per-row)."""
 
     def __init__(self, problems, **opts):
         self._problems = list(problems)

@@ -455,7 +455,9 @@ struct Batch {
     int nact_a = 0, nact_at = 0;
     // launch-bound tail: when the listed instances hold at most kTailNnz nonzeros, the iteration kernels run
     // lanes-per-row over tail lists {first row, end row, instance} (the checks keep the row-block lists)
-    int tail_mode = 1;                 // 0: row blocks to the end (every row sum in row order)
+    // bit 0: lanes-per-row tail; bit 1: a reduced-precision phase ends once its running set is a tail (its instances
+    // leave as stalled, so the ladder carries them on in FP64); 0: row blocks to the end (row-order sums)
+    int tail_mode = 1;
     int tail_lpr_a = 0, tail_lpr_at = 0;  // lanes per row of the current tail lists; 0: no tail lists
     int4 *tlist_a = nullptr, *tlist_at = nullptr;
     size_t tcap_a = 0, tcap_at = 0;
@@ -470,7 +472,7 @@ struct Batch {
     // A captured block bakes in its kernels' arguments: the dtypes, the parameters passed by value to
     // k_bt_decide and the iterate buffers X[cur] / Y[cur].  All of them are in the key, so a phase with other
     // tolerances, limits or stall rules, or an iterate reloaded into the other buffer, captures its own block.
-    using GraphKey = std::tuple<int, int, int, int, double, double, int, int, int, int>;
+    using GraphKey = std::tuple<int, int, int, int, double, double, int, int, int, int, int>;
     std::map<GraphKey, hipGraphExec_t> graphs;
     void* alloc(size_t bytes) {
         void* p = nullptr;
@@ -507,9 +509,9 @@ constexpr int64_t kTailNnz = int64_t(1) << 18;
 // kTailNnz): per
 // matrix, lanes per row from the listed rows' nnz-weighted median length (8, 16 or 32), and one entry per
 // kBlock / lanes consecutive rows of an instance.  Otherwise none (tail_lpr_* = 0).
-void bt_tail_lists(Batch* bt, const std::vector<char>& on) {
-    bt->tail_lpr_a = bt->tail_lpr_at = 0;
-    if (!bt->tail_mode) return;
+// Nonzeros of A held by the instances flagged in `on`, when they are a tail (at most kTailInstances instances
+// with at most kTailNnz nonzeros, at least one); 0 otherwise.
+int64_t bt_tail_nnz(const Batch* bt, const std::vector<char>& on) {
     int64_t nz = 0;
     int count = 0;
     for (int k = 0; k < bt->ninst; ++k)
@@ -518,7 +520,14 @@ void bt_tail_lists(Batch* bt, const std::vector<char>& on) {
             nz += bt->aptr_h[in.row0 + in.m] - bt->aptr_h[in.row0];
             ++count;
         }
-    if (nz == 0 || nz > kTailNnz || count > kTailInstances) return;
+    return (nz == 0 || nz > kTailNnz || count > kTailInstances) ? 0 : nz;
+}
+
+void bt_tail_lists(Batch* bt, const std::vector<char>& on) {
+    bt->tail_lpr_a = bt->tail_lpr_at = 0;
+    if (!(bt->tail_mode & 1)) return;
+    const int64_t nz = bt_tail_nnz(bt, on);
+    if (nz == 0) return;
     auto build = [&](const std::vector<int>& ptr, bool rows_of_a, int4*& dev, size_t& cap, int& count) {
         std::vector<int> len;
         for (int k = 0; k < bt->ninst; ++k)
@@ -1332,6 +1341,7 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
         bt->P.check_every = (int)params[3];
         bt->P.stall_checks = (int)params[4];
         bt->P.stall_exit = params[5] != 0 ? 1 : 0;
+        bt->P.end_phase = 0;
         const double time_limit = params[6];
         // FP64 evaluation of a reduced-precision level's checks (the single solve's GANIT_CHECK_FP64)
         const bool e64 = params[7] != 0 && (mdt != GANIT_F64 || vdt != GANIT_F64);
@@ -1344,13 +1354,24 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
             using M = typename decltype(MT)::type;
             using V = typename decltype(VT)::type;
             bt_list(bt, bt_active(bt));  // launch only the selected instances' workgroups
+            // tail_mode bit 1: a reduced-precision phase ends once its listed instances are a tail (launch-bound:
+            // FP32 saves nothing there), its running instances leaving as stalled at the next check
+            const bool reduced = mdt != GANIT_F64 || vdt != GANIT_F64;
+            auto at_tail = [&]() {
+                return (bt->tail_mode & 2) && reduced && bt->P.stall_exit && bt_tail_nnz(bt, bt->listed) > 0;
+            };
+            bt->P.end_phase = at_tail() ? 1 : 0;
             auto check = [&](int avg, int mode) {
                 if (e64) bt_check<M, V, true>(bt, avg, mode);
                 else bt_check<M, V>(bt, avg, mode);
             };
             check(0, kInit);  // k_restart of every running instance
-            const Batch::GraphKey key{mdt, vdt, bt->cur, bt->P.check_every, bt->P.tol, bt->P.infeas_tol,
-                                      bt->P.max_iter, bt->P.stall_checks, bt->P.stall_exit, e64 ? 1 : 0};
+            auto make_key = [&]() {
+                return Batch::GraphKey{mdt, vdt, bt->cur, bt->P.check_every, bt->P.tol, bt->P.infeas_tol,
+                                       bt->P.max_iter, bt->P.stall_checks, bt->P.stall_exit, e64 ? 1 : 0,
+                                       bt->P.end_phase};
+            };
+            Batch::GraphKey key = make_key();
             auto block = [&]() {
                 for (int k = 0; k < bt->P.check_every; ++k) bt_iteration<M, V>(bt, mdt);
                 check(1, kCheck);
@@ -1412,6 +1433,10 @@ int ganit_hip_batch_phase(ganit_hip_batch* h, int mdt, int vdt, const double* pa
                 if (running > 0 && 2 * running <= listed) {
                     bt_list(bt, bt_active(bt));
                     listed = running;
+                    if (!bt->P.end_phase && at_tail()) {
+                        bt->P.end_phase = 1;
+                        key = make_key();
+                    }
                 }
             }
             if (running_left) *running_left = running;
@@ -1483,13 +1508,13 @@ int ganit_hip_batch_autotune(ganit_hip_batch* h, int mdt, int vdt, int reps, dou
     });
 }
 
-// Launch-bound tail switch: mode 1 (the default) lets the iteration kernels of a small running set run
-// lanes-per-row, 0 keeps the row blocks (row-order sums) to the end, -1 leaves it; tail_iters (may be null) =
-// iterations run on tail lists so far.
+// Launch-bound tail switch: mode bit 0 (the default) lets the iteration kernels of a small running set run
+// lanes-per-row (0: row blocks, row-order sums, to the end), bit 1 ends a reduced-precision phase once its running
+// set is such a tail; -1 leaves the mode; tail_iters (may be null) = iterations run on tail lists so far.
 int ganit_hip_batch_tail(ganit_hip_batch* h, int mode, int64_t* tail_iters) {
     return guarded([&] {
         auto* bt = reinterpret_cast<Batch*>(h);
-        GH_REQUIRE(bt && mode >= -1 && mode <= 1, "bad tail arguments");
+        GH_REQUIRE(bt && mode >= -1 && mode <= 3, "bad tail arguments");
         if (mode >= 0 && mode != bt->tail_mode) {
             GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
             bt->tail_mode = mode;

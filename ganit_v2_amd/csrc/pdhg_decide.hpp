@@ -32,6 +32,7 @@ struct BatchState {
 struct BatchParams {
     double tol, infeas_tol;
     int max_iter, check_every, stall_checks, stall_exit;
+    int end_phase;  // 1 (with stall_exit): every running instance leaves the phase at this check, as stalled
 };
 
 struct Kkt {
@@ -116,6 +117,9 @@ __device__ inline bool decide_one(const BatchInst& in, BatchState& s, const doub
             else if (++s.no_progress >= P.stall_checks && P.stall_exit) {
                 why = kStalled; best = kcand; best_avg = avg_better;
             }
+        }
+        if (why == kRunning && P.end_phase && P.stall_exit) {  // the batch's policy, not a stall test
+            why = kStalled; best = kcand; best_avg = avg_better;
         }
         if (why == kRunning) {
             const bool sufficient = kcand.err <= 0.2 * s.restart_err;

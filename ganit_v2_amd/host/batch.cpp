@@ -294,7 +294,7 @@ std::vector<Result> BatchSolver::run() {
     const int first_vdt = I.mixed ? GANIT_F32 : GANIT_F64;
     chk(ganit_hip_batch_load(bt, first_vdt, x0.data(), y0.data(), w.data()));
     int64_t tail0 = 0;
-    chk(ganit_hip_batch_tail(bt, opt.tail_lanes ? 1 : 0, &tail0));
+    chk(ganit_hip_batch_tail(bt, (opt.tail_lanes ? 1 : 0) | (opt.tail_fp64 ? 2 : 0), &tail0));
     S.upload_seconds = since(t_solve);
 
     std::vector<int> ints(5 * K);

@@ -200,13 +200,14 @@ ganit::Options options_from(const double* o) {
     opt.device = (int)o[12];
     opt.eval_fp64 = o[13] != 0;
     // bit 0: kahan, bit 1: detect_unbounded, bits 2-3: set-up (0 auto, 1 host, 2 GPU), bit 4: polish,
-    // bit 5: scale_q, bit 6: no lanes-per-row tail in batches
+    // bit 5: scale_q, bit 6: no lanes-per-row tail in batches, bit 7: batches end reduced levels at the tail
     const int bits = (int)o[14];
     opt.kahan = (bits & 1) != 0;
     opt.detect_unbounded = (bits & 2) != 0;
     opt.polish = (bits & 16) != 0;
     opt.scale_q = (bits & 32) != 0;
     opt.tail_lanes = (bits & 64) == 0;
+    opt.tail_fp64 = (bits & 128) != 0;
     const int where = (bits >> 2) & 3;
     if (where == 3) throw std::runtime_error("set-up placement must be 0 (auto), 1 (host) or 2 (GPU)");
     opt.setup = where == 1 ? ganit::Setup::Host : where == 2 ? ganit::Setup::Device : ganit::Setup::Auto;

@@ -55,6 +55,9 @@ struct Options {
     bool tail_lanes = true;         // batches: once the running instances are few and small (launch-bound), iterate
                                     // them lanes-per-row (row sums by a lane tree) instead of on row blocks (row
                                     // order, the CPU's bits); false keeps the row blocks to the end
+    bool tail_fp64 = false;         // batches: the reduced-precision levels end once the running instances are such
+                                    // a tail, which the ladder then carries on in FP64 (a batch policy: FP32 saves
+                                    // nothing on launch-bound iterations; not the reference's ladder rule)
 };
 
 // DualInfeasible (an extension, only with Options::detect_unbounded) comes last so the reference's codes keep

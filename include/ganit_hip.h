@@ -258,8 +258,10 @@ int ganit_hip_batch_phase(ganit_hip_batch* bt, int mdt, int vdt, const double* p
 int ganit_hip_batch_autotune(ganit_hip_batch* bt, int mdt, int vdt, int reps, double* ms, int* chosen);
 /* Launch-bound tail (no reference counterpart): while at most 8 instances run, with at most 2^18 nonzeros, the
  * batched iteration kernels run 8, 16 or 32 lanes per row (a fixed lane tree per row) instead of the row blocks
- * (each row summed in row order, the CPU's bits).  mode 1 (the default) allows it, 0 keeps the row blocks to
- * the end, -1 leaves the mode; tail_iters (may be NULL) = iterations run that way so far. */
+ * (each row summed in row order, the CPU's bits).  mode bit 0 (the default, mode 1) allows it -- 0 keeps the row
+ * blocks to the end; bit 1 also ends a reduced-precision phase (stall exit on) once its instances are such a
+ * tail: they leave as stalled at the next check, so the ladder carries them on in FP64 (a batch policy, not the
+ * reference's ladder rule); -1 leaves the mode.  tail_iters (may be NULL) = iterations run lanes-per-row so far. */
 int ganit_hip_batch_tail(ganit_hip_batch* bt, int mode, int64_t* tail_iters);
 /* FP64 KKT of every instance's FP64 iterate with matrix values mdt (the certification). */
 int ganit_hip_batch_certify(ganit_hip_batch* bt, int mdt);

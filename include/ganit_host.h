@@ -77,7 +77,10 @@ int ganit_lp_scaled_gpu(const ganit_lp* lp, int ruiz_iters, int pock_chambolle, 
  *                      bit 5 scale_q (QPs: the column scaling also equilibrates Q, the block matrix [Q A'; A 0]),
  *                      bit 6 (ganit_batch_* only): no lanes-per-row tail -- a batch whose running instances are
  *                      few and small (at most 8, 2^18 nonzeros) otherwise iterates them 8-32 lanes per row (row
- *                      sums by a lane tree) instead of on row blocks (each row in row order, the CPU's bits)
+ *                      sums by a lane tree) instead of on row blocks (each row in row order, the CPU's bits),
+ *                      bit 7 (ganit_batch_* only): the reduced-precision levels end once the running instances are
+ *                      such a tail, which then runs in FP64 (FP32 saves nothing when the launches bound it; a batch
+ *                      policy, not the reference's ladder rule)
  * out[0..17]  = status (0 optimal, 1 primal infeasible, 2 iteration limit, 3 time limit, 4 numerical,
  *                       5 dual infeasible -- only with detect_unbounded),
  *               pobj, dobj, rel_p, rel_d, rel_gap, iterations, fp32_iterations, l2_iterations, restarts,

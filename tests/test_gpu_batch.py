@@ -149,6 +149,32 @@ def test_batch_tail_lanes_against_row_order(precision):
         assert same >= 0.75 * len(probs)
 
 
+def test_batch_tail_fp64_policy():
+    """tail_fp64 (a batch policy, off by default): a reduced-precision level ends once the running instances are a
+    launch-bound tail, and the ladder carries them on in FP64.  On 48 C3-size LPs with the headline's ladder (FP32,
+    FP64 candidate checks, stall 60): the same statuses as without it, every optimal instance FP64-certified,
+    objectives within 5e-4, and at least as many instances reaching L3 (the tail's instances leave L1 early)."""
+    probs = [g.Problem.generate("batch_lp", seed=k) for k in range(48)]
+    opts = dict(precision="mixed", tol=1e-4, eval_fp64=True, stall_checks=60)
+    plain = g.Batch(probs, **opts)
+    rp = plain.run()
+    pol = g.Batch(probs, tail_fp64=True, **opts)
+    rt = pol.run()
+    sp, st = plain.stats(), pol.stats()
+    print(f"L3 instances {sp['l3_instances']:.0f} -> {st['l3_instances']:.0f}, max iterations {sp['max_iters']:.0f} -> "
+          f"{st['max_iters']:.0f}, {rp[0]['solve_s']:.4f} s -> {rt[0]['solve_s']:.4f} s", flush=True)
+    assert st["l3_instances"] >= sp["l3_instances"]
+    for k, (a, b) in enumerate(zip(rt, rp)):
+        assert a["status"] == b["status"], k
+        if a["status"] == "OPTIMAL":
+            assert _certified(a, 1e-4), k
+            assert abs(a["pobj"] - b["pobj"]) <= 5e-4 * (1 + abs(b["pobj"])), k
+    # FP64 runs have no reduced level to end: the policy changes nothing there
+    f0 = g.Batch(probs[:8], precision="double", tol=1e-4).run()
+    f1 = g.Batch(probs[:8], precision="double", tol=1e-4, tail_fp64=True).run()
+    assert [r["iters"] for r in f0] == [r["iters"] for r in f1]
+
+
 def test_batch_with_netlib_and_infeasible_instances():
     """Uneven mix: Netlib LPs (bgetam is primal infeasible) between random node relaxations."""
     names = ["afiro", "bgetam", "adlittle", "stair"]
