@@ -453,7 +453,7 @@ struct Batch {
     std::vector<int> first_a_h, first_at_h;
     int2 *list_a = nullptr, *list_at = nullptr;  // {row block, instance} of each launched workgroup
     int nact_a = 0, nact_at = 0;
-    // launch-bound tail: when the listed instances hold at most kTailNnz nonzeros, the iteration kernels run
+    // launch-bound tail: when the listed instances are few and small (bt_tail_nnz), the iteration kernels run
     // lanes-per-row over tail lists {first row, end row, instance} (the checks keep the row-block lists)
     // bit 0: lanes-per-row tail; bit 1: a reduced-precision phase ends once its running set is a tail (its instances
     // leave as stalled, so the ladder carries them on in FP64); 0: row blocks to the end (row-order sums)
