@@ -162,7 +162,7 @@ def test_compact_indices_match_plain(hipctx, oracle, monkeypatch, mdt, vdt):
     """16-bit column offsets (per staged block, long row and split-row chunk, each falling back to 32-bit
     indices where its span does not fit) give the same bits as the plain 32-bit indices, on one matrix with
     banded short rows, long rows (compact and gathered through an LDS window, and too wide) and split rows
-    (compact and too-wide chunks)."""
+    (compact, too-wide and consecutive-column chunks, the last reading no index at all)."""
     from ganit_v2_amd import hip
     rng = np.random.default_rng(60 + mdt)
     n = 300000
@@ -176,6 +176,9 @@ def test_compact_indices_match_plain(hipctx, oracle, monkeypatch, mdt, vdt):
     rows.append(np.sort(rng.choice(60000, 20000, replace=False)))       # split row, compact chunks
     rows.append(np.sort(rng.choice(n, 9000, replace=False)))            # split row: two too-wide chunks, one compact
     rows.append(np.sort(100000 + rng.choice(200000, 30000, replace=False)))  # split row, compact chunks
+    rows.append(np.arange(7000, 27000))                                 # split row, consecutive columns: no index
+    rows.append(np.concatenate([np.arange(1000, 5096),                  # split row: a consecutive chunk, then
+                                5096 + np.sort(rng.choice(200000, 6000, replace=False))]))  # compact ones
     lens = np.array([len(r) for r in rows])
     ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
     idx = np.concatenate(rows).astype(np.int32)
