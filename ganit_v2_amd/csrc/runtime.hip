@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -48,6 +49,21 @@ __global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const S* __restri
 unsigned ew_grid(int64_t n) { return std::min<unsigned>(grid_for(n), 8192u); }
 
 // ================================================================ SpMV and the two steps
+// FP32 and / or BF16 copies of FP64 matrix values, each rounded as bf16_bits_from_double / a C cast does on the
+// host: to nearest, ties to even (BF16 through FP32; NaN kept quiet).
+__global__ __launch_bounds__(kBlock) void k_round_values(int64_t n, const double* __restrict__ v64,
+                                                         float* __restrict__ v32, uint16_t* __restrict__ v16) {
+    for (int64_t p = (int64_t)blockIdx.x * kBlock + threadIdx.x; p < n; p += (int64_t)gridDim.x * kBlock) {
+        const float f = (float)v64[p];
+        if (v32) v32[p] = f;
+        if (v16) {
+            const uint32_t u = __float_as_uint(f);
+            v16[p] = (u & 0x7fffffffu) > 0x7f800000u ? (uint16_t)((u >> 16) | 0x40)
+                                                     : (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+        }
+    }
+}
+
 template <int LPR, class M, class V>
 __global__ __launch_bounds__(kBlock) void k_spmv(CsrView<M> a, const V* __restrict__ x, V* __restrict__ y) {
     const int64_t gt = (int64_t)xcd_block_id() * kBlock + threadIdx.x;
@@ -232,13 +248,15 @@ uint16_t bf16_bits_from_double(double d) {
 // nonzeros sit in rows no longer than it), rounded up to a power of two and clamped to one wavefront.
 int choose_lanes_from_lengths(const std::vector<int>& len, int64_t nnz) {
     if (nnz == 0) return 1;
-    std::vector<int> sorted(len);
-    std::sort(sorted.begin(), sorted.end());
+    // nonzeros per row length, lengths above one wavefront in one bin (they all give kWave lanes): the
+    // weighted median of the sorted lengths without sorting them
+    std::vector<int64_t> w(kWave + 2, 0);
+    for (int L : len) w[std::min(L, kWave + 1)] += L;
     int64_t acc = 0;
     int med = 1;
-    for (int L : sorted) {
-        acc += L;
-        if (2 * acc >= nnz) { med = L; break; }
+    for (int L = 0; L <= kWave + 1; ++L) {
+        acc += w[L];
+        if (w[L] > 0 && 2 * acc >= nnz) { med = L; break; }
     }
     int lpr = 1;
     while (lpr < med && lpr < kWave) lpr <<= 1;
@@ -351,34 +369,52 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
     auto fill = [&](int p0, int p1, int mn) {
         for (int p = p0; p < p1; ++p) c16[p] = (uint16_t)(idx[p] - mn);
     };
-    int64_t n32 = 0;
-    for (size_t q = 0; q < nb; ++q) {
-        const int r0 = rb[q], r1 = rb[q + 1], p0 = ptr[r0], p1 = ptr[r1];
-        if (p1 == p0) continue;
-        int mn = 0;
-        const int span = span_of(p0, p1, mn);
-        if (r1 - r0 == 1 && p1 - p0 > kLong) {
-            if (p1 - p0 > kSplit) continue;  // a split row: its chunks carry their own bases (below)
-            // a long row: compact when its span fits 16 bits, else 32-bit indices for this row only; windowed
-            // on the same terms as a staged block when it has at most kTile entries
-            if (span > 65535) {
-                bc[2 * q + 1] = -1;
-                n32 += p1 - p0;
-            } else {
-                bc[2 * q] = mn;
-                bc[2 * q + 1] = (windows && p1 - p0 <= kTile && span + 1 <= kWin && span + 1 <= 2 * (p1 - p0))
-                                    ? span + 1
-                                    : 0;
-                fill(p0, p1, mn);
+    // blocks are independent (each writes its own descriptor pair and offset range): a few host threads share them
+    const int nt = (int)std::max<int64_t>(1, std::min<int64_t>({8, (int64_t)std::max(1u, std::thread::hardware_concurrency()),
+                                                              a->nnz / 262144 + 1}));
+    std::vector<int64_t> n32t(nt, 0);
+    std::vector<char> too_wide(nt, 0);
+    auto blocks_of = [&](int t) {
+        for (size_t q = (size_t)t * nb / nt, qe = (size_t)(t + 1) * nb / nt; q < qe; ++q) {
+            const int r0 = rb[q], r1 = rb[q + 1], p0 = ptr[r0], p1 = ptr[r1];
+            if (p1 == p0) continue;
+            int mn = 0;
+            const int span = span_of(p0, p1, mn);
+            if (r1 - r0 == 1 && p1 - p0 > kLong) {
+                if (p1 - p0 > kSplit) continue;  // a split row: its chunks carry their own bases (below)
+                // a long row: compact when its span fits 16 bits, else 32-bit indices for this row only; windowed
+                // on the same terms as a staged block when it has at most kTile entries
+                if (span > 65535) {
+                    bc[2 * q + 1] = -1;
+                    n32t[t] += p1 - p0;
+                } else {
+                    bc[2 * q] = mn;
+                    bc[2 * q + 1] = (windows && p1 - p0 <= kTile && span + 1 <= kWin && span + 1 <= 2 * (p1 - p0))
+                                        ? span + 1
+                                        : 0;
+                    fill(p0, p1, mn);
+                }
+                continue;
             }
-            continue;
+            if (span > 65535) {  // a staged block too wide: keep the 32-bit path for this matrix
+                too_wide[t] = 1;
+                return;
+            }
+            bc[2 * q] = mn;
+            // windowed when the span fits the LDS window and loading it costs at most two entries per nonzero
+            bc[2 * q + 1] = (windows && span + 1 <= kWin && span + 1 <= 2 * (p1 - p0)) ? span + 1 : 0;
+            fill(p0, p1, mn);
         }
-        if (span > 65535) return;  // a staged block too wide: keep the 32-bit path for this matrix
-        bc[2 * q] = mn;
-        // windowed when the span fits the LDS window and loading it costs at most two entries per nonzero
-        bc[2 * q + 1] = (windows && span + 1 <= kWin && span + 1 <= 2 * (p1 - p0)) ? span + 1 : 0;
-        fill(p0, p1, mn);
+    };
+    {
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; ++t) th.emplace_back(blocks_of, t);
+        blocks_of(0);
+        for (auto& x : th) x.join();
     }
+    if (std::count(too_wide.begin(), too_wide.end(), (char)1) > 0) return;
+    int64_t n32 = 0;
+    for (int64_t v : n32t) n32 += v;
     // chunks of the split rows, enumerated as the chunk plan enumerates them (rows ascending, kChunk apart)
     std::vector<int> cbase;
     for (int i = 0; i < a->rows; ++i) {
@@ -473,19 +509,33 @@ Csr* csr_create(Ctx* ctx, int rows, int cols, const int* ptr, const int* idx, co
             GH_CHECK(hipMalloc(&a->v64, sizeof(double) * nz));
             if (a->nnz) GH_CHECK(hipMemcpyAsync(a->v64, val, sizeof(double) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
         }
+        // the reduced-precision copies are rounded on the device from the FP64 values when those are resident
+        // (the same rounding as on the host: nearest, ties to even), else on the host
+        const bool on_device = a->v64 && a->nnz > 0;
         if (mask & (1u << GANIT_F32)) {
-            std::vector<float> h(val, val + a->nnz);
             GH_CHECK(hipMalloc(&a->v32, sizeof(float) * nz));
-            if (a->nnz) GH_CHECK(hipMemcpyAsync(a->v32, h.data(), sizeof(float) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
-            GH_CHECK(hipStreamSynchronize(ctx->stream));
+            if (on_device) {
+                hipLaunchKernelGGL(k_round_values, dim3(grid_for(a->nnz)), dim3(kBlock), 0, ctx->stream, a->nnz,
+                                   (const double*)a->v64, a->v32, (uint16_t*)nullptr);
+            } else if (a->nnz) {
+                std::vector<float> h(val, val + a->nnz);
+                GH_CHECK(hipMemcpyAsync(a->v32, h.data(), sizeof(float) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
+                GH_CHECK(hipStreamSynchronize(ctx->stream));
+            }
         }
         if (mask & (1u << GANIT_BF16)) {
-            std::vector<uint16_t> h(a->nnz);
-            for (int64_t p = 0; p < a->nnz; ++p) h[p] = bf16_bits_from_double(val[p]);
             GH_CHECK(hipMalloc(&a->v16, sizeof(uint16_t) * nz));
-            if (a->nnz) GH_CHECK(hipMemcpyAsync(a->v16, h.data(), sizeof(uint16_t) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
-            GH_CHECK(hipStreamSynchronize(ctx->stream));
+            if (on_device) {
+                hipLaunchKernelGGL(k_round_values, dim3(grid_for(a->nnz)), dim3(kBlock), 0, ctx->stream, a->nnz,
+                                   (const double*)a->v64, (float*)nullptr, reinterpret_cast<uint16_t*>(a->v16));
+            } else if (a->nnz) {
+                std::vector<uint16_t> h(a->nnz);
+                for (int64_t p = 0; p < a->nnz; ++p) h[p] = bf16_bits_from_double(val[p]);
+                GH_CHECK(hipMemcpyAsync(a->v16, h.data(), sizeof(uint16_t) * a->nnz, hipMemcpyHostToDevice, ctx->stream));
+                GH_CHECK(hipStreamSynchronize(ctx->stream));
+            }
         }
+        GH_CHECK(hipGetLastError());
         GH_CHECK(hipStreamSynchronize(ctx->stream));
     } catch (...) {
         csr_destroy(a);
@@ -685,7 +735,8 @@ void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& l
             GH_CHECK(hipEventElapsedTime(&ms, e0, e1));
             return ms / (float)k;
         };
-        // a single launch first: a candidate already 8x slower than the lead is not screened further
+        // a single launch first: a candidate already 1.3x slower than the lead's average is not screened further
+        // (to be kept, a row-group width has to beat the row blocks by kRowGroupMargin anyway)
         auto probe_one = [&](size_t c) {
             use(c);
             GH_CHECK(hipEventRecord(e0, ctx->stream));
@@ -702,7 +753,11 @@ void autotune_with(Ctx* ctx, Csr* a, int reps, const std::function<void(int)>& l
         for (size_t c = 0; c < nc; ++c) {
             if (skip[c]) continue;
             const float one = lead0 < 1e30f ? probe_one(c) : 0.f;
-            first[c] = one > 8.0f * lead0 ? one : time_one(c, screen);
+            if (one > 1.3f * lead0) {
+                skip[c] = 1;
+                continue;
+            }
+            first[c] = time_one(c, screen);
             lead0 = std::min(lead0, first[c]);
         }
         const float lead = *std::min_element(first.begin(), first.end());
