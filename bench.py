@@ -257,7 +257,7 @@ def batched_lps(count=512, sequential=True):
     return out
 
 
-def diff_qps(count, threads=32, batched=True, scale_q=True, cd_h=1e-4):
+def diff_qps(count, threads=32, batched=True, scale_q=True, cd_h=1e-6):
     """Differentiable QPs (n=2000, 300 equality + 200 <= rows): the FP32 path -- forward (mixed ladder, tol 1e-6)
     + FP32 adjoint -- timed as one batch, then the FP64 path -- forward (tol 1e-9) + FP64 adjoint -- and each
     instance's FP32 gradient error against it; then each instance's FP64 gradient against FP64 central differences
