@@ -105,45 +105,83 @@ __global__ __launch_bounds__(kBlock) void k_bt_dual(CsrView<M> A, const int* __r
         lds);
 }
 
+// A launch-bound tail: at most kTailInstances instances listed, with at most kTailNnz nonzeros between them.  (Sixteen
+// small LPs with 130K nonzeros ran 7% slower lanes-per-row; a few slow instances left of 512 run 25% faster.)
+constexpr int kTailInstances = 8;
+constexpr int64_t kTailNnz = int64_t(1) << 18;
+
+// The rows of a tail, passed by value: instance q's rows r0[q] .. r1[q]-1 go to workgroups wg0[q] .. wg0[q+1]-1,
+// kBlock / LPR consecutive rows each.  A workgroup finds its rows from its index and these arguments, so no list
+// entry is loaded ahead of the rows' own loads.
+struct TailArgs {
+    int n;                        // instances (each with at least one row)
+    int wg0[kTailInstances + 1];  // wg0[n] = the grid
+    int r0[kTailInstances], r1[kTailInstances], inst[kTailInstances];
+};
+
+struct TailRows {
+    int row, end, inst;
+};
+
+// This workgroup's instance and row group (uniform: a select chain over constant indices, kept in scalar registers).
+template <int LPR>
+__device__ __forceinline__ TailRows tail_rows(const TailArgs& t) {
+    const int w = blockIdx.x;
+    int r0 = t.r0[0], r1 = t.r1[0], w0 = t.wg0[0], in = t.inst[0];
+#pragma unroll
+    for (int q = 1; q < kTailInstances; ++q)
+        if (q < t.n && w >= t.wg0[q]) {
+            r0 = t.r0[q];
+            r1 = t.r1[q];
+            w0 = t.wg0[q];
+            in = t.inst[q];
+        }
+    return TailRows{r0 + (w - w0) * (kBlock / LPR) + (int)threadIdx.x / LPR, r1, in};
+}
+
 // The same two kernels for a launch-bound tail: LPR lanes per row (row_dot), a workgroup over kBlock / LPR
-// consecutive rows of one instance, list entry {first row, end row, instance}.  Rows are summed by a fixed
-// lane tree, not in row order (kernels over a few small instances are bound by their launches and dependent
-// loads, which the row-block staging lengthens).  A stopped instance's workgroups load but store nothing, so
-// the state load overlaps the row's loads.
+// consecutive rows of one instance (TailArgs).  Rows are summed by a fixed lane tree, not in row order (kernels
+// over a few small instances are bound by their launches and dependent loads, which the row-block staging
+// lengthens).  The instance state and the row's own operands are loaded first, beside the row's pointer loads,
+// so the dependent chain is pointers, entries, gathered values.  A stopped instance's workgroups load but store
+// nothing.
 template <int LPR, class M, class V, bool HASQ>
-__global__ __launch_bounds__(kBlock) void k_bt_primal_lpr(CsrView<M> AT, const int4* __restrict__ list, CsrView<M> Q,
+__global__ __launch_bounds__(kBlock) void k_bt_primal_lpr(CsrView<M> AT, TailArgs t, CsrView<M> Q,
                                                           const BatchState* __restrict__ st,
                                                           const V* __restrict__ y, const V* __restrict__ x,
                                                           const V* __restrict__ c, const V* __restrict__ l,
                                                           const V* __restrict__ u, V* __restrict__ xn,
                                                           V* __restrict__ xbar, V* __restrict__ sumx) {
-    const int4 e = list[blockIdx.x];
-    const int j = e.x + (int)threadIdx.x / LPR, lane = (int)threadIdx.x % LPR;
-    if (j >= e.y) return;  // whole row groups leave together
-    const BatchState& s = st[e.z];
+    const TailRows r = tail_rows<LPR>(t);
+    const int j = r.row, lane = (int)threadIdx.x % LPR;
+    if (j >= r.end) return;  // whole row groups leave together
+    const int active = st[r.inst].active;
+    const V tau = V(st[r.inst].tau);
+    const V xj = x[j], cj = c[j], lj = l[j], uj = u[j], sj = sumx[j];
     const V aty = row_dot<LPR>(AT, j, lane, y);
     const V qx = HASQ ? row_dot<LPR>(Q, j, lane, x) : V(0);
-    if (lane == 0 && s.active) {
-        V sm = sumx[j];
-        primal_update<V>(x[j], aty, qx, c[j], l[j], u[j], V(s.tau), xn[j], xbar[j], sm);
+    if (lane == 0 && active) {
+        V sm = sj;
+        primal_update<V>(xj, aty, qx, cj, lj, uj, tau, xn[j], xbar[j], sm);
         sumx[j] = sm;
     }
 }
 
 template <int LPR, class M, class V>
-__global__ __launch_bounds__(kBlock) void k_bt_dual_lpr(CsrView<M> A, const int4* __restrict__ list,
-                                                        const BatchState* __restrict__ st, const V* __restrict__ xbar,
-                                                        const V* __restrict__ y, const V* __restrict__ lo,
-                                                        const V* __restrict__ hi, V* __restrict__ yn,
-                                                        V* __restrict__ sumy) {
-    const int4 e = list[blockIdx.x];
-    const int i = e.x + (int)threadIdx.x / LPR, lane = (int)threadIdx.x % LPR;
-    if (i >= e.y) return;
-    const BatchState& s = st[e.z];
+__global__ __launch_bounds__(kBlock) void k_bt_dual_lpr(CsrView<M> A, TailArgs t, const BatchState* __restrict__ st,
+                                                        const V* __restrict__ xbar, const V* __restrict__ y,
+                                                        const V* __restrict__ lo, const V* __restrict__ hi,
+                                                        V* __restrict__ yn, V* __restrict__ sumy) {
+    const TailRows r = tail_rows<LPR>(t);
+    const int i = r.row, lane = (int)threadIdx.x % LPR;
+    if (i >= r.end) return;
+    const int active = st[r.inst].active;
+    const V sigma = V(st[r.inst].sigma);
+    const V yi = y[i], li = lo[i], hi_ = hi[i], si = sumy[i];
     const V axb = row_dot<LPR>(A, i, lane, xbar);
-    if (lane == 0 && s.active) {
-        V sm = sumy[i];
-        dual_update<V>(y[i], axb, lo[i], hi[i], V(s.sigma), yn[i], sm);
+    if (lane == 0 && active) {
+        V sm = si;
+        dual_update<V>(yi, axb, li, hi_, sigma, yn[i], sm);
         sumy[i] = sm;
     }
 }
@@ -454,14 +492,13 @@ struct Batch {
     int2 *list_a = nullptr, *list_at = nullptr;  // {row block, instance} of each launched workgroup
     int nact_a = 0, nact_at = 0;
     // launch-bound tail: when the listed instances are few and small (bt_tail_nnz), the iteration kernels run
-    // lanes-per-row over tail lists {first row, end row, instance} (the checks keep the row-block lists)
+    // lanes-per-row over the tail's rows (TailArgs; the checks keep the row-block lists)
     // bit 0: lanes-per-row tail; bit 1: a reduced-precision phase ends once its running set is a tail (its instances
     // leave as stalled, so the ladder carries them on in FP64); 0: row blocks to the end (row-order sums)
     int tail_mode = 1;
     int tail_lpr_a = 0, tail_lpr_at = 0;  // lanes per row of the current tail lists; 0: no tail lists
-    int4 *tlist_a = nullptr, *tlist_at = nullptr;
-    size_t tcap_a = 0, tcap_at = 0;
-    int tact_a = 0, tact_at = 0;
+    TailArgs targ_a{}, targ_at{};  // the tail's rows of A and of A^T
+    int tact_a = 0, tact_at = 0;    // their grids
     int64_t tail_iters = 0;  // iterations run on tail lists (all phases)
     std::vector<char> listed;  // instances in the current lists
     BatchVecs vs[2];
@@ -488,8 +525,7 @@ struct Batch {
         if (ctx) (void)hipStreamSynchronize(ctx->stream);
         drop_graphs();
         for (void* p : owned) (void)hipFree(p);
-        for (void* p : {(void*)first_a, (void*)first_at, (void*)part_n, (void*)part_m, (void*)list_a, (void*)list_at,
-                        (void*)tlist_a, (void*)tlist_at})
+        for (void* p : {(void*)first_a, (void*)first_at, (void*)part_n, (void*)part_m, (void*)list_a, (void*)list_at})
             if (p) (void)hipFree(p);
         if (host_running) (void)hipHostFree(host_running);
         csr_destroy(A);
@@ -500,15 +536,6 @@ struct Batch {
 
 unsigned bt_grid(int64_t n) { return std::min<unsigned>(grid_for(n), 8192u); }
 
-// A launch-bound tail: at most kTailInstances instances listed, with at most kTailNnz nonzeros between them.  (Sixteen
-// small LPs with 130K nonzeros ran 7% slower lanes-per-row; a few slow instances left of 512 run 25% faster.)
-constexpr int kTailInstances = 8;
-constexpr int64_t kTailNnz = int64_t(1) << 18;
-
-// Tail lists for the instances flagged in `on` when they are few and small enough (tail_mode, kTailInstances,
-// kTailNnz): per
-// matrix, lanes per row from the listed rows' nnz-weighted median length (8, 16 or 32), and one entry per
-// kBlock / lanes consecutive rows of an instance.  Otherwise none (tail_lpr_* = 0).
 // Nonzeros of A held by the instances flagged in `on`, when they are a tail (at most kTailInstances instances
 // with at most kTailNnz nonzeros, at least one); 0 otherwise.
 int64_t bt_tail_nnz(const Batch* bt, const std::vector<char>& on) {
@@ -523,12 +550,15 @@ int64_t bt_tail_nnz(const Batch* bt, const std::vector<char>& on) {
     return (nz == 0 || nz > kTailNnz || count > kTailInstances) ? 0 : nz;
 }
 
+// Tail rows for the instances flagged in `on` when they are few and small enough (tail_mode, bt_tail_nnz): per
+// matrix, lanes per row from the listed rows' nnz-weighted median length (8, 16 or 32), and each instance's rows
+// in groups of kBlock / lanes, one workgroup each (TailArgs).  Otherwise none (tail_lpr_* = 0).
 void bt_tail_lists(Batch* bt, const std::vector<char>& on) {
     bt->tail_lpr_a = bt->tail_lpr_at = 0;
     if (!(bt->tail_mode & 1)) return;
     const int64_t nz = bt_tail_nnz(bt, on);
     if (nz == 0) return;
-    auto build = [&](const std::vector<int>& ptr, bool rows_of_a, int4*& dev, size_t& cap, int& count) {
+    auto build = [&](const std::vector<int>& ptr, bool rows_of_a, TailArgs& t, int& grid) {
         std::vector<int> len;
         for (int k = 0; k < bt->ninst; ++k)
             if (on[k]) {
@@ -538,27 +568,26 @@ void bt_tail_lists(Batch* bt, const std::vector<char>& on) {
             }
         const int lpr = std::min(32, std::max(8, choose_lanes_from_lengths(len, nz)));
         const int per = kBlock / lpr;
-        std::vector<int4> l;
+        t = TailArgs{};
+        grid = 0;
         for (int k = 0; k < bt->ninst; ++k)
             if (on[k]) {
                 const BatchInst& in = bt->inst_h[k];
-                const int r0 = rows_of_a ? in.row0 : in.col0, r1 = r0 + (rows_of_a ? in.m : in.n);
-                for (int r = r0; r < r1; r += per) l.push_back(make_int4(r, std::min(r + per, r1), k, 0));
+                const int r0 = rows_of_a ? in.row0 : in.col0, nr = rows_of_a ? in.m : in.n;
+                if (nr == 0) continue;
+                GH_REQUIRE(t.n < kTailInstances, "tail holds too many instances");
+                t.r0[t.n] = r0;
+                t.r1[t.n] = r0 + nr;
+                t.inst[t.n] = k;
+                t.wg0[t.n] = grid;
+                grid += (nr + per - 1) / per;
+                ++t.n;
             }
-        count = (int)l.size();
-        if (l.size() > cap) {
-            if (dev) GH_CHECK(hipFree(dev));
-            dev = nullptr;
-            GH_CHECK(hipMalloc(&dev, sizeof(int4) * l.size()));
-            cap = l.size();
-        }
-        if (count > 0)
-            GH_CHECK(hipMemcpyAsync(dev, l.data(), sizeof(int4) * l.size(), hipMemcpyHostToDevice, bt->ctx->stream));
-        GH_CHECK(hipStreamSynchronize(bt->ctx->stream));
+        t.wg0[t.n] = grid;
         return lpr;
     };
-    bt->tail_lpr_a = build(bt->aptr_h, true, bt->tlist_a, bt->tcap_a, bt->tact_a);
-    bt->tail_lpr_at = build(bt->atptr_h, false, bt->tlist_at, bt->tcap_at, bt->tact_at);
+    bt->tail_lpr_a = build(bt->aptr_h, true, bt->targ_a, bt->tact_a);
+    bt->tail_lpr_at = build(bt->atptr_h, false, bt->targ_at, bt->tact_at);
 }
 
 // Launch lists for the instances flagged in `on`: the workgroups of instances that have stopped are left out
@@ -667,18 +696,18 @@ void bt_primal_lpr(Batch* bt, BatchVecs& v, int c) {
     const CsrView<M> q = bt->Q ? bt->Q->view<M>() : bt->AT->view<M>();
     if (bt->Q)
         hipLaunchKernelGGL((k_bt_primal_lpr<LPR, M, V, true>), dim3(bt->tact_at), dim3(kBlock), 0, s,
-                           bt->AT->view<M>(), bt->tlist_at, q, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
+                           bt->AT->view<M>(), bt->targ_at, q, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
                            (const V*)v.c, (const V*)v.l, (const V*)v.u, (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
     else
         hipLaunchKernelGGL((k_bt_primal_lpr<LPR, M, V, false>), dim3(bt->tact_at), dim3(kBlock), 0, s,
-                           bt->AT->view<M>(), bt->tlist_at, q, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
+                           bt->AT->view<M>(), bt->targ_at, q, bt->st, (const V*)v.Y[c], (const V*)v.X[c],
                            (const V*)v.c, (const V*)v.l, (const V*)v.u, (V*)v.X[1 - c], (V*)v.xbar, (V*)v.sumx);
 }
 template <int LPR, class M, class V>
 void bt_dual_lpr(Batch* bt, BatchVecs& v, int c) {
     if (bt->tact_a == 0) return;
     hipLaunchKernelGGL((k_bt_dual_lpr<LPR, M, V>), dim3(bt->tact_a), dim3(kBlock), 0, bt->ctx->stream,
-                       bt->A->view<M>(), bt->tlist_a, bt->st, (const V*)v.xbar, (const V*)v.Y[c], (const V*)v.lo,
+                       bt->A->view<M>(), bt->targ_a, bt->st, (const V*)v.xbar, (const V*)v.Y[c], (const V*)v.lo,
                        (const V*)v.hi, (V*)v.Y[1 - c], (V*)v.sumy);
 }
 

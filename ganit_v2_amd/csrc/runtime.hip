@@ -408,7 +408,12 @@ void build_compact_indices(Ctx* ctx, Csr* a, const int* ptr, const int* idx, con
     };
     {
         std::vector<std::thread> th;
-        for (int t = 1; t < nt; ++t) th.emplace_back(blocks_of, t);
+        int started = 1;  // ranges 1 .. started-1 have a thread; a thread that cannot be created leaves the rest here
+        try {
+            for (; started < nt; ++started) th.emplace_back(blocks_of, started);
+        } catch (...) {
+        }
+        for (int t = started; t < nt; ++t) blocks_of(t);
         blocks_of(0);
         for (auto& x : th) x.join();
     }
