@@ -1,7 +1,7 @@
 """Edge cases of the problem data on the GPU path, each against the oracle on the same input: no constraints,
 empty rows and columns, maximisation, ranged and free rows, free and one-sided variables, an objective
-constant, a QP with a zero Hessian row, an infeasible system, and ragged row lengths that exercise every SpMV
-path (staged blocks, long rows, split rows) in one matrix."""
+constant, a QP with a zero Hessian row, an infeasible system, ragged row lengths that exercise every SpMV
+path (staged blocks, long rows, split rows) in one matrix, and the LP cases together as one batch."""
 import numpy as np
 import pytest
 
@@ -144,3 +144,40 @@ def test_detect_unbounded_leaves_bounded_solves_unchanged(oracle, precision, mon
         b = p.solve_gpu(tol=1e-6, time_limit=60, precision=precision, detect_unbounded=True)
         assert a["status"] == b["status"] != "DUAL_INFEASIBLE"
         assert a["iters"] == b["iters"] and a["pobj"] == b["pobj"]
+
+
+@pytest.mark.parametrize("tail_lanes", [True, False])
+def test_batch_of_edge_cases(oracle, tail_lanes):
+    """The edge cases above as one batch -- an instance with no constraints, one with an empty row and column,
+    a maximisation with ranged and free rows, free and one-sided variables, an infeasible system -- each
+    instance against the oracle's solve of it alone.  Five tiny instances are a launch-bound tail from the first
+    iteration, so tail_lanes=True runs them lanes-per-row throughout (the m = 0 instance has no rows of A there)
+    and tail_lanes=False on row blocks."""
+    probs = [
+        _lp(0, 4, [], [1.0, -2.0, 0.5, -1.0], [0, 0, -1, 0], [3, 5, 2, 1], [], []),
+        _lp(3, 4, [(0, 0, 1.0), (0, 1, 1.0), (2, 1, 1.0), (2, 3, 2.0)], [1, 2, -1, 1], [0, 0, 0, 0],
+            [10, 10, 7, 10], [1, 0, 2], [INF, 2, 6]),
+        _lp(3, 3, [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, -1.0), (2, 0, 3.0), (2, 2, 1.0)], [1.0, 2.0, -0.5],
+            [0, 0, 0], [3, 3, 10], [1, -INF, 2], [4, INF, INF], obj_const=5.0, maximize=True),
+        _lp(2, 3, [(0, 0, 1.0), (0, 1, 1.0), (0, 2, 1.0), (1, 0, 1.0), (1, 2, -1.0)], [1.0, 1.0, -1.0],
+            [-INF, 0, -INF], [INF, INF, 4], [2, 0], [2, 0]),
+        _lp(2, 2, [(0, 0, 1.0), (0, 1, 1.0), (1, 0, 1.0), (1, 1, 1.0)], [1.0, 1.0], [0, 0], [10, 10], [1, 3],
+            [1, 3]),
+    ]
+    tol = 1e-6
+    b = g.Batch(probs, tol=tol, precision="double", time_limit=60, tail_lanes=tail_lanes)
+    b.upload()
+    rs = b.run()
+    oracle.set_threads(2)
+    for k, (p, r) in enumerate(zip(probs, rs)):
+        o = oracle.OracleLP.from_export(p.export()).solve(tol=tol, time_limit=60)
+        print(f"instance {k}: batch {r['status']} {r['pobj']:.10g} it {r['iters']} | oracle {o['status']} "
+              f"{o['pobj']:.10g} it {o['iters']}", flush=True)
+        assert r["status"] == o["status"]
+        if not tail_lanes:  # row-order sums: the oracle's trajectory
+            assert r["iters"] == o["iters"]
+        if o["status"] == "OPTIMAL":
+            assert abs(r["pobj"] - o["pobj"]) <= 100 * tol * (1 + abs(o["pobj"]))
+            assert max(r["rel_p"], r["rel_d"], r["rel_gap"]) <= tol * (1 + 1e-7)
+    if tail_lanes:
+        assert b.stats()["tail_iters"] > 0
